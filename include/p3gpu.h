@@ -1,0 +1,176 @@
+/*
+ * p3gpu.h — C ABI of the MI355X (gfx950) lifted-PCS library, libp3gpu.so.
+ *
+ * Drop-in boundary for the data-parallel core of the p3-miden lifted-STARK
+ * prover over Goldilocks: coset LDE (TwoAdicSubgroupDft), LMCS commit/open
+ * (Lmcs / LmcsTree), DEEP + FRI opening (open_with_channel) and the transcript
+ * (ProverChannel).  Plain pointers and sizes only; every call returns a status
+ * code (P3G_OK = 0) and never throws across the boundary.  Field elements are
+ * canonical little-endian u64 Goldilocks values; extension elements are two
+ * consecutive words (c0, c1) of F[x]/(x^2 - 7).  Citations are to files of
+ * the reference repository (adr1anh/p3-miden).
+ */
+#ifndef P3GPU_H
+#define P3GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define P3G_OK 0
+#define P3G_E_INVAL 1      /* bad argument */
+#define P3G_E_NOT_POW2 2   /* height not a power of two */
+#define P3G_E_UNSORTED 3   /* matrices not sorted by ascending height */
+#define P3G_E_NOMEM 4      /* device allocation failed */
+#define P3G_E_DEVICE 5     /* HIP runtime error */
+#define P3G_E_EMPTY 6      /* empty batch */
+#define P3G_E_TRANSCRIPT 7 /* transcript misuse */
+#define P3G_E_RANGE 8      /* index / size out of range */
+
+/* ---- hash and challenger kinds ---- */
+#define P3G_HASH_POSEIDON2_GL12 0 /* StatefulSponge<Poseidon2Goldilocks<12>,12,8,4> + TruncatedPermutation */
+#define P3G_HASH_KECCAK_GL 1      /* SerializingStatefulSponge<StatefulSponge<KeccakF,25,17,4>> + PaddingFreeSponge compress */
+#define P3G_CHALLENGER_DUPLEX_P2 0 /* DuplexChallenger<Goldilocks, Poseidon2GL12, 12, 8> */
+#define P3G_CHALLENGER_KECCAK 1    /* SerializingChallenger64<Goldilocks, HashChallenger<u8, Keccak256, 32>> */
+
+/* ---- flags ---- */
+#define P3G_DEVICE_PTRS 1u /* pointers are device (HBM) addresses */
+#define P3G_BITREV_ROWS 2u /* coset LDE: permute output rows by bit reversal */
+#define P3G_TAKE_OWNERSHIP 4u /* lmcs_build with device pointers: the tree frees them */
+
+typedef struct p3g_ctx p3g_ctx;
+typedef struct p3g_tree p3g_tree;
+typedef struct p3g_transcript p3g_transcript;
+
+/* Row-major height x width matrix of canonical Goldilocks words. */
+typedef struct {
+  const uint64_t* data;
+  uint64_t height;
+  uint64_t width;
+} p3g_matrix;
+
+/* PcsParams::new(log_blowup, log_folding_arity, log_final_degree, folding_pow_bits,
+ * deep_pow_bits, num_queries, query_pow_bits)  (p3-miden-lifted-fri/src/params.rs:170-204) */
+typedef struct {
+  uint32_t log_blowup;
+  uint32_t log_folding_arity;
+  uint32_t log_final_degree;
+  uint32_t folding_pow_bits;
+  uint32_t deep_pow_bits;
+  uint32_t num_queries;
+  uint32_t query_pow_bits;
+} p3g_pcs_params;
+
+const char* p3g_status_string(int status);
+const char* p3g_last_error(void); /* thread-local detail of the last failure */
+
+/* ---- context (one per device; all work runs on its stream) ---- */
+int p3g_ctx_create(int device, p3g_ctx** out);
+void p3g_ctx_destroy(p3g_ctx* ctx);
+int p3g_ctx_synchronize(p3g_ctx* ctx);
+int p3g_ctx_trim(p3g_ctx* ctx); /* release cached device blocks */
+
+/* ---- per-kernel timing (instrumentation; no reference counterpart) ----
+ * When enabled, HIP events are recorded on the context's stream around each
+ * launch of a named kernel family ("lde_inverse_pass", "lde_forward_pass",
+ * "p2_leaf", "keccak_leaf", "compress_layer", "compress_top", "deep_weights",
+ * "deep_colsum", "deep_quotient", "fri_fold"). */
+int p3g_profile_enable(p3g_ctx* ctx, int on);
+int p3g_profile_reset(p3g_ctx* ctx);
+int p3g_profile_read(p3g_ctx* ctx, const char* name, double* total_ms, uint64_t* launches);
+
+/* ---- device memory (resident inputs, benchmarks, serving) ---- */
+int p3g_alloc(p3g_ctx* ctx, uint64_t bytes, void** dptr);
+int p3g_free(p3g_ctx* ctx, void* dptr);
+int p3g_upload(p3g_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int p3g_download(p3g_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* RowMajorMatrix::rand(SmallRng::seed_from_u64(seed), n/w, w) as n canonical words, on the host. */
+int p3g_fill_random(uint64_t seed, uint64_t n, uint64_t* host_out);
+
+/* ---- TwoAdicSubgroupDft ----
+ * coset_lde_batch(mat, added_bits, shift) (+ bit_reverse_rows with P3G_BITREV_ROWS);
+ * replaces the call in p3-miden-lifted-stark/src/prover/commit.rs:217-221.
+ * out holds (n << added_bits) x w words.  Without P3G_BITREV_ROWS the natural
+ * order is produced on the host side of the copy (host pointers only). */
+int p3g_coset_lde_batch(p3g_ctx* ctx, const uint64_t* trace, uint64_t n, uint64_t w, uint32_t added_bits,
+                        uint64_t shift, uint32_t flags, uint64_t* out);
+
+/* ---- Lmcs ----
+ * LmcsConfig::build_tree (alignment = 1) / build_aligned_tree (alignment = p3g_lmcs_alignment)
+ * (p3-miden-lmcs/src/lmcs.rs:102-130, lifted_tree.rs:244-310).  Matrices sorted by
+ * ascending power-of-two height.  Host matrices are copied to the device; device
+ * matrices are referenced (and freed by the tree with P3G_TAKE_OWNERSHIP). */
+uint32_t p3g_lmcs_alignment(uint32_t hash_kind);
+int p3g_lmcs_build(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* mats, uint64_t k, uint32_t alignment,
+                   uint32_t flags, p3g_tree** out);
+/* Lmcs::hash / Lmcs::compress (lib.rs:203-209), host side. */
+int p3g_lmcs_hash(uint32_t hash_kind, const uint64_t* elems, const uint64_t* row_widths, uint64_t nrows,
+                  uint64_t out[4]);
+int p3g_lmcs_compress(uint32_t hash_kind, const uint64_t left[4], const uint64_t right[4], uint64_t out[4]);
+
+/* ---- LmcsTree (lib.rs:262-295) ---- */
+int p3g_tree_root(const p3g_tree* t, uint64_t out[4]);
+uint64_t p3g_tree_height(const p3g_tree* t);
+uint32_t p3g_tree_alignment(const p3g_tree* t);
+uint64_t p3g_tree_num_matrices(const p3g_tree* t);
+/* widths(): aligned widths, one per matrix */
+int p3g_tree_widths(const p3g_tree* t, uint64_t* out);
+/* rows(index): upsampled rows padded to alignment (lifted_tree.rs:127-138); *len receives the count */
+int p3g_tree_rows(p3g_tree* t, uint64_t index, uint64_t* out, uint64_t cap, uint64_t* len);
+/* digest layer l (0 = leaves): (height >> l) digests of 4 words */
+int p3g_tree_layer(p3g_tree* t, uint32_t layer, uint64_t* out);
+/* device address of committed matrix i (the bit-reversed LDE for commit_traces trees) */
+int p3g_tree_matrix(p3g_tree* t, uint64_t i, const uint64_t** dptr, uint64_t* height, uint64_t* width);
+/* prove_batch(indices, channel) (lifted_tree.rs:157-224) */
+int p3g_tree_prove_batch(p3g_tree* t, const uint64_t* indices, uint64_t n, p3g_transcript* ch);
+void p3g_tree_free(p3g_tree* t);
+
+/* ---- commit_traces (p3-miden-lifted-stark/src/prover/commit.rs:175-228) ----
+ * Per trace: coset LDE on the nested shift GENERATOR^(2^(log n_max - log n_j)),
+ * bit-reversed rows, then one aligned LMCS tree over all LDEs. */
+int p3g_commit_traces(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* traces, uint64_t k, uint32_t log_blowup,
+                      uint32_t flags, p3g_tree** out);
+
+/* ---- transcript (ProverChannel, p3-miden-transcript/src/prover.rs:53-145) ---- */
+int p3g_transcript_create(uint32_t challenger_kind, p3g_transcript** out);
+void p3g_transcript_free(p3g_transcript* ch);
+int p3g_transcript_clone(const p3g_transcript* ch, p3g_transcript** out);
+int p3g_transcript_observe(p3g_transcript* ch, const uint64_t* values, uint64_t n);       /* CanObserve<F> */
+int p3g_transcript_observe_commitment(p3g_transcript* ch, const uint64_t c[4]);          /* CanObserve<Hash> */
+int p3g_transcript_sample(p3g_transcript* ch, uint64_t* out);                             /* sample F */
+int p3g_transcript_sample_ext(p3g_transcript* ch, uint64_t out[2]);                       /* sample_algebra_element */
+int p3g_transcript_sample_bits(p3g_transcript* ch, uint32_t bits, uint64_t* out);
+int p3g_transcript_send_fields(p3g_transcript* ch, const uint64_t* values, uint64_t n);
+int p3g_transcript_send_commitment(p3g_transcript* ch, const uint64_t c[4]);
+int p3g_transcript_hint_fields(p3g_transcript* ch, const uint64_t* values, uint64_t n);
+int p3g_transcript_hint_commitment(p3g_transcript* ch, const uint64_t c[4]);
+int p3g_transcript_grind(p3g_transcript* ch, uint32_t bits, uint64_t* witness);
+/* TranscriptData { fields, commitments } (data.rs:11-14) */
+uint64_t p3g_transcript_num_fields(const p3g_transcript* ch);
+uint64_t p3g_transcript_num_commitments(const p3g_transcript* ch);
+int p3g_transcript_data(const p3g_transcript* ch, uint64_t* fields, uint64_t* commitments);
+
+/* ---- PCS opening ----
+ * open_with_channel(params, lmcs, log_lde_height, eval_points, trace_trees, channel)
+ * (p3-miden-lifted-fri/src/prover.rs:35-116): DEEP quotient, FRI commit phase,
+ * query grinding and sampling, batch openings of every trace tree and FRI round.
+ * eval_points: n_points extension elements (2 words each), 1 <= n_points <= 4. */
+int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t log_lde_height,
+                          const uint64_t* eval_points, uint32_t n_points, p3g_tree* const* trees, uint64_t ntrees,
+                          p3g_transcript* ch);
+
+/* ---- one-shot proof (the pcs_trace harness flow, p3-miden-lifted-fri/benches/pcs_trace.rs:60-100,
+ * with commit_traces semantics): commit the traces, observe the root, sample two
+ * extension points, open.  root receives the commitment, eval_points_out (4 words)
+ * the sampled points; the transcript holds the proof. */
+int p3g_prove(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* params, const p3g_matrix* traces, uint64_t k,
+              uint32_t flags, p3g_transcript* ch, uint64_t root[4], uint64_t eval_points_out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

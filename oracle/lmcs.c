@@ -24,16 +24,27 @@
 #include "oracle.h"
 #include "tx_internal.h"
 
-size_t orc_hash_width(int kind) { return kind == ORC_HASH_P2 ? 12 : 25; }
-size_t orc_hash_rate(int kind) { return kind == ORC_HASH_P2 ? 8 : 17; }
+/* Mock permutation of the reference's sponge tests
+ * (p3-miden-stateful-hasher/src/testing.rs:21-52): every lane becomes
+ * sum_i state[i] * (i + 1) with wrapping u64 arithmetic. */
+static size_t mock_width = 4, mock_rate = 2;
+void orc_mock_config(size_t width, size_t rate) { mock_width = width; mock_rate = rate; }
+
+size_t orc_hash_width(int kind) { return kind == ORC_HASH_P2 ? 12 : kind == ORC_HASH_KECCAK ? 25 : mock_width; }
+size_t orc_hash_rate(int kind) { return kind == ORC_HASH_P2 ? 8 : kind == ORC_HASH_KECCAK ? 17 : mock_rate; }
 size_t orc_hash_alignment(int kind) {
-    /* P2: RATE.  Keccak serializing: lcm(8, 8*17)/8 = 17. */
-    return kind == ORC_HASH_P2 ? 8 : 17;
+    /* P2: RATE.  Keccak serializing: lcm(8, 8*17)/8 = 17.  Mock: RATE. */
+    return kind == ORC_HASH_P2 ? 8 : kind == ORC_HASH_KECCAK ? 17 : mock_rate;
 }
 
 static void permute(int kind, uint64_t* s) {
     if (kind == ORC_HASH_P2) orc_p2_permute(s);
-    else orc_keccakf(s);
+    else if (kind == ORC_HASH_KECCAK) orc_keccakf(s);
+    else {
+        uint64_t acc = 0;
+        for (size_t i = 0; i < mock_width; i++) acc += s[i] * (uint64_t)(i + 1);
+        for (size_t i = 0; i < mock_width; i++) s[i] = acc;
+    }
 }
 
 void orc_sponge_absorb(int kind, uint64_t* state, const uint64_t* in, size_t len) {
@@ -57,7 +68,7 @@ void orc_sponge_absorb(int kind, uint64_t* state, const uint64_t* in, size_t len
 }
 
 void orc_compress(int kind, const uint64_t* left, const uint64_t* right, uint64_t* out) {
-    uint64_t s[25];
+    uint64_t s[64];
     memset(s, 0, sizeof(s));
     memcpy(s, left, 4 * sizeof(uint64_t));
     memcpy(s + 4, right, 4 * sizeof(uint64_t));
@@ -66,7 +77,7 @@ void orc_compress(int kind, const uint64_t* left, const uint64_t* right, uint64_
 }
 
 void orc_hash_rows(int kind, const uint64_t* elems, size_t n, uint64_t* out4) {
-    uint64_t s[25];
+    uint64_t s[64];
     memset(s, 0, sizeof(s));
     orc_sponge_absorb(kind, s, elems, n);
     memcpy(out4, s, 4 * sizeof(uint64_t));
@@ -251,7 +262,7 @@ int orc_lmcs_open_batch(int kind, const uint64_t* root, const size_t* widths, si
         if (uniq[q] >= max_h) { rc = 1; goto done; }
         if (orc_tx_receive_hint_fields(tx, total, elems)) { rc = 3; goto done; }
         if (rows_out) memcpy(rows_out + q * total, elems, total * sizeof(uint64_t));
-        uint64_t st[25];
+        uint64_t st[64];
         memset(st, 0, sizeof(st));
         size_t off = 0;
         for (size_t i = 0; i < nw; i++) { orc_sponge_absorb(kind, st, elems + off, widths[i]); off += widths[i]; }

@@ -53,6 +53,8 @@ void orc_idft_ext(ef2* v, size_t n);
 /* ---- sponge / compression ---- */
 #define ORC_HASH_P2 0
 #define ORC_HASH_KECCAK 1
+#define ORC_HASH_MOCK 2
+void orc_mock_config(size_t width, size_t rate);
 size_t orc_hash_width(int kind);
 size_t orc_hash_rate(int kind);
 size_t orc_hash_alignment(int kind);

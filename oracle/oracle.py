@@ -23,6 +23,7 @@ P = 0xFFFFFFFF00000001
 GENERATOR = 7
 HASH_P2 = 0
 HASH_KECCAK = 1
+HASH_MOCK = 2
 CH_DUPLEX_P2 = 0
 CH_KECCAK = 1
 
@@ -63,6 +64,7 @@ def _load():
         "orc_bit_reverse_rows": (None, [_u64p, C.c_size_t, C.c_size_t]),
         "orc_naive_coset_dft": (None, [_u64p, C.c_size_t, C.c_size_t, C.c_uint64, _u64p]),
         "orc_hash_alignment": (C.c_size_t, [C.c_int]),
+        "orc_mock_config": (None, [C.c_size_t, C.c_size_t]),
         "orc_sponge_absorb": (None, [C.c_int, _u64p, _u64p, C.c_size_t]),
         "orc_compress": (None, [C.c_int, _u64p, _u64p, _u64p]),
         "orc_hash_rows": (None, [C.c_int, _u64p, C.c_size_t, _u64p]),
@@ -221,6 +223,11 @@ def keccak256(data: bytes) -> bytes:
     out = C.create_string_buffer(32)
     lib.orc_keccak256(data, len(data), out)
     return out.raw
+
+
+def mock_config(width: int, rate: int) -> None:
+    """Configure the mock permutation (width, rate) used by HASH_MOCK."""
+    lib.orc_mock_config(width, rate)
 
 
 def hash_alignment(kind: int) -> int:

@@ -1,0 +1,539 @@
+// extern "C" entry points declared in include/p3gpu.h.  Every call catches
+// library and HIP failures and turns them into status codes; nothing throws
+// across the boundary.
+#include <cstring>
+#include <string>
+
+#include "../../include/p3gpu.h"
+#include "internal.hpp"
+
+using namespace p3g;
+
+struct p3g_ctx {
+  Ctx c;
+};
+struct p3g_tree {
+  Tree* t;
+};
+struct p3g_transcript {
+  Transcript tx;
+};
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    f();
+    return P3G_OK;
+  } catch (const ApiError& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const DeviceError& e) {
+    g_last_error = std::string(e.what) + ": " + hipGetErrorString(e.err);
+    return P3G_E_DEVICE;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return P3G_E_NOMEM;
+  } catch (...) {
+    g_last_error = "unknown failure";
+    return P3G_E_INVAL;
+  }
+}
+
+extern "C" {
+
+const char* p3g_status_string(int status) {
+  switch (status) {
+    case P3G_OK: return "ok";
+    case P3G_E_INVAL: return "invalid argument";
+    case P3G_E_NOT_POW2: return "height is not a power of two";
+    case P3G_E_UNSORTED: return "matrices are not sorted by height";
+    case P3G_E_NOMEM: return "out of device memory";
+    case P3G_E_DEVICE: return "device error";
+    case P3G_E_EMPTY: return "empty batch";
+    case P3G_E_TRANSCRIPT: return "transcript error";
+    case P3G_E_RANGE: return "index or size out of range";
+    default: return "unknown status";
+  }
+}
+
+const char* p3g_last_error(void) { return g_last_error.c_str(); }
+
+int p3g_ctx_create(int device, p3g_ctx** out) {
+  if (!out) return P3G_E_INVAL;
+  *out = nullptr;
+  p3g_ctx* c = new p3g_ctx();
+  int rc = guard([&] {
+    c->c.device = device;
+    P3G_CHECK_HIP(hipSetDevice(device));
+    P3G_CHECK_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+    p2_generate_constants(2025, &c->c.host_p2);
+    P3G_CHECK_HIP(hipMalloc(&c->c.dev_p2, sizeof(P2Consts)));
+    P3G_CHECK_HIP(hipMemcpy(c->c.dev_p2, &c->c.host_p2, sizeof(P2Consts), hipMemcpyHostToDevice));
+  });
+  if (rc != P3G_OK) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return P3G_OK;
+}
+
+void p3g_ctx_destroy(p3g_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  ctx->c.trim();
+  for (auto& kv : ctx->c.live_blocks) (void)hipFree(kv.first);
+  for (auto& kv : ctx->c.pow_tables) (void)hipFree(kv.second);
+  if (ctx->c.dev_p2) (void)hipFree(ctx->c.dev_p2);
+  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+int p3g_ctx_synchronize(p3g_ctx* ctx) {
+  if (!ctx) return P3G_E_INVAL;
+  return guard([&] { P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+int p3g_ctx_trim(p3g_ctx* ctx) {
+  if (!ctx) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+    ctx->c.trim();
+  });
+}
+
+int p3g_profile_enable(p3g_ctx* ctx, int on) {
+  if (!ctx) return P3G_E_INVAL;
+  ctx->c.profiling = on != 0;
+  return P3G_OK;
+}
+
+int p3g_profile_reset(p3g_ctx* ctx) {
+  if (!ctx) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+    std::lock_guard<std::mutex> g(ctx->c.mu);
+    for (auto& kv : ctx->c.prof)
+      for (auto& ev : kv.second) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+      }
+    ctx->c.prof.clear();
+  });
+}
+
+int p3g_profile_read(p3g_ctx* ctx, const char* name, double* total_ms, uint64_t* launches) {
+  if (!ctx || !name || !total_ms || !launches) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+    std::lock_guard<std::mutex> g(ctx->c.mu);
+    double tot = 0;
+    uint64_t n = 0;
+    auto it = ctx->c.prof.find(name);
+    if (it != ctx->c.prof.end())
+      for (auto& ev : it->second) {
+        float ms = 0;
+        P3G_CHECK_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
+        tot += ms;
+        n++;
+      }
+    *total_ms = tot;
+    *launches = n;
+  });
+}
+
+int p3g_alloc(p3g_ctx* ctx, uint64_t bytes, void** dptr) {
+  if (!ctx || !dptr) return P3G_E_INVAL;
+  return guard([&] { *dptr = ctx->c.alloc(bytes); });
+}
+
+int p3g_free(p3g_ctx* ctx, void* dptr) {
+  if (!ctx) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+    ctx->c.release(dptr);
+  });
+}
+
+int p3g_upload(p3g_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int p3g_download(p3g_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+    P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int p3g_fill_random(uint64_t seed, uint64_t n, uint64_t* host_out) {
+  if (!host_out && n) return P3G_E_INVAL;
+  SmallRng rng(seed);
+  for (uint64_t i = 0; i < n; i++) host_out[i] = rng.next_gl();
+  return P3G_OK;
+}
+
+// ---- helpers for host/device matrix inputs
+static uint64_t* to_device(Ctx& c, const uint64_t* src, uint64_t words) {
+  uint64_t* d = (uint64_t*)c.alloc(words * sizeof(uint64_t));
+  if (words) {
+    P3G_CHECK_HIP(hipMemcpyAsync(d, src, words * 8, hipMemcpyHostToDevice, c.stream));
+    P3G_CHECK_HIP(hipStreamSynchronize(c.stream));
+  }
+  return d;
+}
+
+int p3g_coset_lde_batch(p3g_ctx* ctx, const uint64_t* trace, uint64_t n, uint64_t w, uint32_t added_bits,
+                        uint64_t shift, uint32_t flags, uint64_t* out) {
+  if (!ctx || !trace || !out) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    if (shift >= P) throw ApiError(P3G_E_INVAL, "shift is not a canonical field element");
+    const bool dev = flags & P3G_DEVICE_PTRS;
+    const uint64_t N = n << added_bits;
+    const uint64_t* dtrace = dev ? trace : to_device(c, trace, n * w);
+    uint64_t* dout = dev ? out : (uint64_t*)c.alloc(N * w * 8);
+    uint64_t* tmp = nullptr;
+    try {
+      lde_coset_batch(c, dtrace, n, w, added_bits, shift, dout, c.stream);
+      if (!(flags & P3G_BITREV_ROWS)) {
+        // natural order: undo the bit reversal into a temporary, then swap back
+        tmp = (uint64_t*)c.alloc(N * w * 8);
+        bit_reverse_rows(c, dout, tmp, N, w, c.stream);
+        P3G_CHECK_HIP(hipMemcpyAsync(dout, tmp, N * w * 8, hipMemcpyDeviceToDevice, c.stream));
+      }
+      if (!dev) P3G_CHECK_HIP(hipMemcpyAsync(out, dout, N * w * 8, hipMemcpyDeviceToHost, c.stream));
+      P3G_CHECK_HIP(hipStreamSynchronize(c.stream));
+    } catch (...) {
+      if (!dev) {
+        c.release((void*)dtrace);
+        c.release(dout);
+      }
+      c.release(tmp);
+      throw;
+    }
+    c.release(tmp);
+    if (!dev) {
+      c.release((void*)dtrace);
+      c.release(dout);
+    }
+  });
+}
+
+uint32_t p3g_lmcs_alignment(uint32_t hash_kind) { return hash_alignment(hash_kind); }
+
+static std::vector<DevMat> import_mats(Ctx& c, const p3g_matrix* mats, uint64_t k, uint32_t flags) {
+  std::vector<DevMat> v;
+  const bool dev = flags & P3G_DEVICE_PTRS;
+  try {
+    for (uint64_t i = 0; i < k; i++) {
+      if (!mats[i].data && mats[i].height * mats[i].width) throw ApiError(P3G_E_INVAL, "null matrix data");
+      DevMat m;
+      m.height = mats[i].height;
+      m.width = mats[i].width;
+      if (dev) {
+        m.ptr = const_cast<uint64_t*>(mats[i].data);
+        m.owned = (flags & P3G_TAKE_OWNERSHIP) != 0;
+      } else {
+        m.ptr = to_device(c, mats[i].data, m.height * m.width);
+        m.owned = true;
+      }
+      v.push_back(m);
+    }
+  } catch (...) {
+    for (auto& m : v)
+      if (!dev) c.release(m.ptr);
+    throw;
+  }
+  return v;
+}
+
+static void check_hash(uint32_t h) {
+  if (h != HASH_P2 && h != HASH_KECCAK) throw ApiError(P3G_E_INVAL, "unknown hash kind");
+}
+
+int p3g_lmcs_build(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* mats, uint64_t k, uint32_t alignment,
+                   uint32_t flags, p3g_tree** out) {
+  if (!ctx || !out || (!mats && k)) return P3G_E_INVAL;
+  *out = nullptr;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    check_hash(hash_kind);
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    if (k == 0) throw ApiError(P3G_E_EMPTY, "cannot commit empty batch");
+    std::vector<DevMat> dm = import_mats(c, mats, k, flags);
+    Tree* t = nullptr;
+    try {
+      t = lmcs_build(c, hash_kind, dm, alignment, c.stream);
+    } catch (...) {
+      for (auto& m : dm)
+        if (m.owned && !(flags & P3G_DEVICE_PTRS)) c.release(m.ptr);
+      throw;
+    }
+    *out = new p3g_tree{t};
+  });
+}
+
+int p3g_lmcs_hash(uint32_t hash_kind, const uint64_t* elems, const uint64_t* row_widths, uint64_t nrows,
+                  uint64_t out[4]) {
+  if (!out || (nrows && !row_widths)) return P3G_E_INVAL;
+  return guard([&] {
+    check_hash(hash_kind);
+    uint64_t st[25] = {};
+    uint64_t off = 0;
+    for (uint64_t r = 0; r < nrows; r++) {
+      host_sponge_absorb(hash_kind, st, elems + off, row_widths[r]);
+      off += row_widths[r];
+    }
+    memcpy(out, st, 32);
+  });
+}
+
+int p3g_lmcs_compress(uint32_t hash_kind, const uint64_t left[4], const uint64_t right[4], uint64_t out[4]) {
+  if (!left || !right || !out) return P3G_E_INVAL;
+  return guard([&] {
+    check_hash(hash_kind);
+    host_compress(hash_kind, left, right, out);
+  });
+}
+
+int p3g_tree_root(const p3g_tree* t, uint64_t out[4]) {
+  if (!t || !out) return P3G_E_INVAL;
+  memcpy(out, t->t->root, 32);
+  return P3G_OK;
+}
+
+uint64_t p3g_tree_height(const p3g_tree* t) { return t ? t->t->height : 0; }
+uint32_t p3g_tree_alignment(const p3g_tree* t) { return t ? t->t->alignment : 0; }
+uint64_t p3g_tree_num_matrices(const p3g_tree* t) { return t ? t->t->mats.size() : 0; }
+
+int p3g_tree_widths(const p3g_tree* t, uint64_t* out) {
+  if (!t || !out) return P3G_E_INVAL;
+  for (size_t i = 0; i < t->t->mats.size(); i++) out[i] = aligned_len(t->t->mats[i].width, t->t->alignment);
+  return P3G_OK;
+}
+
+int p3g_tree_rows(p3g_tree* t, uint64_t index, uint64_t* out, uint64_t cap, uint64_t* len) {
+  if (!t || !len) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipSetDevice(t->t->ctx->device));
+    std::vector<uint64_t> rows;
+    tree_rows(*t->t, index, rows);
+    *len = rows.size();
+    if (out) {
+      if (cap < rows.size()) throw ApiError(P3G_E_RANGE, "output buffer too small");
+      memcpy(out, rows.data(), rows.size() * 8);
+    }
+  });
+}
+
+int p3g_tree_layer(p3g_tree* t, uint32_t layer, uint64_t* out) {
+  if (!t || !out) return P3G_E_INVAL;
+  return guard([&] {
+    Tree& tr = *t->t;
+    if (layer > tr.depth) throw ApiError(P3G_E_RANGE, "layer out of range");
+    P3G_CHECK_HIP(hipSetDevice(tr.ctx->device));
+    P3G_CHECK_HIP(hipMemcpyAsync(out, tr.digests + tr.layer_off[layer] * 4, (tr.height >> layer) * 32,
+                                 hipMemcpyDeviceToHost, tr.ctx->stream));
+    P3G_CHECK_HIP(hipStreamSynchronize(tr.ctx->stream));
+  });
+}
+
+int p3g_tree_matrix(p3g_tree* t, uint64_t i, const uint64_t** dptr, uint64_t* height, uint64_t* width) {
+  if (!t || i >= t->t->mats.size()) return P3G_E_INVAL;
+  if (dptr) *dptr = t->t->mats[i].ptr;
+  if (height) *height = t->t->mats[i].height;
+  if (width) *width = t->t->mats[i].width;
+  return P3G_OK;
+}
+
+int p3g_tree_prove_batch(p3g_tree* t, const uint64_t* indices, uint64_t n, p3g_transcript* ch) {
+  if (!t || !ch || (n && !indices)) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipSetDevice(t->t->ctx->device));
+    std::vector<uint64_t> idx(indices, indices + n);
+    tree_prove_batch(*t->t, idx, ch->tx, t->t->ctx->stream);
+  });
+}
+
+void p3g_tree_free(p3g_tree* t) {
+  if (!t) return;
+  (void)hipSetDevice(t->t->ctx->device);
+  (void)hipStreamSynchronize(t->t->ctx->stream);
+  delete t->t;
+  delete t;
+}
+
+int p3g_commit_traces(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* traces, uint64_t k, uint32_t log_blowup,
+                      uint32_t flags, p3g_tree** out) {
+  if (!ctx || !out || (!traces && k)) return P3G_E_INVAL;
+  *out = nullptr;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    check_hash(hash_kind);
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    if (k == 0) throw ApiError(P3G_E_EMPTY, "at least one trace required");
+    std::vector<DevMat> dm = import_mats(c, traces, k, flags & P3G_DEVICE_PTRS);
+    Tree* t = nullptr;
+    try {
+      t = commit_traces(c, hash_kind, dm, log_blowup, c.stream);
+    } catch (...) {
+      if (!(flags & P3G_DEVICE_PTRS))
+        for (auto& m : dm) c.release(m.ptr);
+      throw;
+    }
+    if (!(flags & P3G_DEVICE_PTRS))
+      for (auto& m : dm) c.release(m.ptr);
+    *out = new p3g_tree{t};
+  });
+}
+
+int p3g_transcript_create(uint32_t kind, p3g_transcript** out) {
+  if (!out) return P3G_E_INVAL;
+  if (kind != CH_DUPLEX_P2 && kind != CH_KECCAK) return P3G_E_INVAL;
+  *out = new p3g_transcript();
+  (*out)->tx.kind = kind;
+  return P3G_OK;
+}
+
+void p3g_transcript_free(p3g_transcript* ch) { delete ch; }
+
+int p3g_transcript_clone(const p3g_transcript* ch, p3g_transcript** out) {
+  if (!ch || !out) return P3G_E_INVAL;
+  *out = new p3g_transcript(*ch);
+  return P3G_OK;
+}
+
+static bool canonical(const uint64_t* v, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++)
+    if (v[i] >= P) return false;
+  return true;
+}
+
+int p3g_transcript_observe(p3g_transcript* ch, const uint64_t* values, uint64_t n) {
+  if (!ch || (n && !values) || !canonical(values, n)) return P3G_E_INVAL;
+  for (uint64_t i = 0; i < n; i++) ch->tx.observe(values[i]);
+  return P3G_OK;
+}
+
+int p3g_transcript_observe_commitment(p3g_transcript* ch, const uint64_t c[4]) {
+  if (!ch || !c) return P3G_E_INVAL;
+  ch->tx.observe_commitment(c);
+  return P3G_OK;
+}
+
+int p3g_transcript_sample(p3g_transcript* ch, uint64_t* out) {
+  if (!ch || !out) return P3G_E_INVAL;
+  *out = ch->tx.sample();
+  return P3G_OK;
+}
+
+int p3g_transcript_sample_ext(p3g_transcript* ch, uint64_t out[2]) {
+  if (!ch || !out) return P3G_E_INVAL;
+  ef2 e = ch->tx.sample_ef();
+  out[0] = e.c0;
+  out[1] = e.c1;
+  return P3G_OK;
+}
+
+int p3g_transcript_sample_bits(p3g_transcript* ch, uint32_t bits, uint64_t* out) {
+  if (!ch || !out || bits >= 32) return P3G_E_INVAL;
+  *out = ch->tx.sample_bits(bits);
+  return P3G_OK;
+}
+
+int p3g_transcript_send_fields(p3g_transcript* ch, const uint64_t* values, uint64_t n) {
+  if (!ch || (n && !values) || !canonical(values, n)) return P3G_E_INVAL;
+  ch->tx.send_fields(values, n);
+  return P3G_OK;
+}
+
+int p3g_transcript_send_commitment(p3g_transcript* ch, const uint64_t c[4]) {
+  if (!ch || !c) return P3G_E_INVAL;
+  ch->tx.send_commitment(c);
+  return P3G_OK;
+}
+
+int p3g_transcript_hint_fields(p3g_transcript* ch, const uint64_t* values, uint64_t n) {
+  if (!ch || (n && !values)) return P3G_E_INVAL;
+  ch->tx.hint_fields(values, n);
+  return P3G_OK;
+}
+
+int p3g_transcript_hint_commitment(p3g_transcript* ch, const uint64_t c[4]) {
+  if (!ch || !c) return P3G_E_INVAL;
+  ch->tx.hint_commitment(c);
+  return P3G_OK;
+}
+
+int p3g_transcript_grind(p3g_transcript* ch, uint32_t bits, uint64_t* witness) {
+  if (!ch || bits >= 32) return P3G_E_INVAL;
+  uint64_t w = ch->tx.grind(bits);
+  if (witness) *witness = w;
+  return P3G_OK;
+}
+
+uint64_t p3g_transcript_num_fields(const p3g_transcript* ch) { return ch ? ch->tx.fields.size() : 0; }
+uint64_t p3g_transcript_num_commitments(const p3g_transcript* ch) { return ch ? ch->tx.comms.size() / 4 : 0; }
+
+int p3g_transcript_data(const p3g_transcript* ch, uint64_t* fields, uint64_t* commitments) {
+  if (!ch) return P3G_E_INVAL;
+  if (fields && !ch->tx.fields.empty()) memcpy(fields, ch->tx.fields.data(), ch->tx.fields.size() * 8);
+  if (commitments && !ch->tx.comms.empty()) memcpy(commitments, ch->tx.comms.data(), ch->tx.comms.size() * 8);
+  return P3G_OK;
+}
+
+int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t log_lde_height,
+                          const uint64_t* eval_points, uint32_t n_points, p3g_tree* const* trees, uint64_t ntrees,
+                          p3g_transcript* ch) {
+  if (!ctx || !params || !eval_points || !ch || (!trees && ntrees)) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    PcsParams p{params->log_blowup,   params->log_folding_arity, params->log_final_degree, params->folding_pow_bits,
+                params->deep_pow_bits, params->num_queries,      params->query_pow_bits};
+    if (!canonical(eval_points, 2ull * n_points)) throw ApiError(P3G_E_INVAL, "non-canonical evaluation point");
+    std::vector<ef2> z(n_points);
+    for (uint32_t j = 0; j < n_points; j++) z[j] = ef2{eval_points[2 * j], eval_points[2 * j + 1]};
+    std::vector<Tree*> tv;
+    for (uint64_t i = 0; i < ntrees; i++) {
+      if (!trees[i]) throw ApiError(P3G_E_INVAL, "null tree");
+      tv.push_back(trees[i]->t);
+    }
+    open_with_channel(c, p, log_lde_height, z.data(), (int)n_points, tv, ch->tx, c.stream);
+  });
+}
+
+int p3g_prove(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* params, const p3g_matrix* traces, uint64_t k,
+              uint32_t flags, p3g_transcript* ch, uint64_t root[4], uint64_t eval_points_out[4]) {
+  if (!ctx || !params || !ch || (!traces && k)) return P3G_E_INVAL;
+  p3g_tree* tree = nullptr;
+  int rc = p3g_commit_traces(ctx, hash_kind, traces, k, params->log_blowup, flags, &tree);
+  if (rc != P3G_OK) return rc;
+  uint64_t r[4], z[4];
+  p3g_tree_root(tree, r);
+  ch->tx.observe_commitment(r);
+  ef2 z0 = ch->tx.sample_ef();
+  ef2 z1 = ch->tx.sample_ef();
+  z[0] = z0.c0;
+  z[1] = z0.c1;
+  z[2] = z1.c0;
+  z[3] = z1.c1;
+  uint32_t logL = log2_exact(p3g_tree_height(tree));
+  rc = p3g_open_with_channel(ctx, params, logL, z, 2, &tree, 1, ch);
+  p3g_tree_free(tree);
+  if (root) memcpy(root, r, 32);
+  if (eval_points_out) memcpy(eval_points_out, z, 32);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// Goldilocks field (p = 2^64 - 2^32 + 1) and its quadratic extension
+// EF = F[x]/(x^2 - 7), for host and gfx950 device code.
+//
+// Matches the field the reference instantiates: Goldilocks with
+// BinomialExtensionField<Goldilocks, 2> (p3-miden-dev-utils/src/configs.rs:394-407).
+// All values are kept canonical (< p) so that every stored word equals the
+// reference's canonical serialisation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define P3G_HD __host__ __device__ __forceinline__
+
+namespace p3g {
+
+constexpr uint64_t P = 0xFFFFFFFF00000001ull;
+constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
+constexpr uint64_t GENERATOR = 7;
+constexpr uint64_t OMEGA32 = 1753635133440165772ull;  // 7^((p-1)/2^32)
+constexpr uint64_t EXT_W = 7;
+
+P3G_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+P3G_HD uint64_t gl_add(uint64_t a, uint64_t b) {
+  uint64_t s = a + b;
+  uint64_t c = s < a ? EPS : 0;  // 2^64 == EPS (mod p)
+  s += c;
+  return s >= P ? s - P : s;
+}
+
+P3G_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
+  uint64_t d = a - b;
+  return a < b ? d - EPS : d;  // borrow: -2^64 == -EPS
+}
+
+P3G_HD uint64_t gl_neg(uint64_t a) { return a ? P - a : 0; }
+
+// x = hi*2^64 + lo reduced with 2^64 = eps and 2^96 = -1.
+P3G_HD uint64_t gl_reduce(uint64_t lo, uint64_t hi) {
+  uint64_t hh = hi >> 32;
+  uint64_t hl = hi & EPS;
+  uint64_t t0 = lo - hh;
+  if (lo < hh) t0 -= EPS;
+  uint64_t t1 = (hl << 32) - hl;  // hl * eps
+  uint64_t r = t0 + t1;
+  if (r < t1) r += EPS;
+  return r >= P ? r - P : r;
+}
+
+P3G_HD uint64_t gl_mul(uint64_t a, uint64_t b) { return gl_reduce(a * b, mulhi64(a, b)); }
+P3G_HD uint64_t gl_sqr(uint64_t a) { return gl_mul(a, a); }
+
+P3G_HD uint64_t gl_pow(uint64_t a, uint64_t e) {
+  uint64_t r = 1;
+  while (e) {
+    if (e & 1) r = gl_mul(r, a);
+    a = gl_mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+
+P3G_HD uint64_t gl_inv(uint64_t a) { return gl_pow(a, P - 2); }
+
+P3G_HD uint64_t gl_two_adic_generator(unsigned bits) {
+  uint64_t g = OMEGA32;
+  for (unsigned i = bits; i < 32; i++) g = gl_mul(g, g);
+  return g;
+}
+
+P3G_HD uint64_t gl_inv_pow2(unsigned k) { return gl_inv(gl_pow(2, k)); }
+
+struct ef2 {
+  uint64_t c0, c1;
+};
+
+P3G_HD ef2 ef_make(uint64_t a, uint64_t b) { return ef2{a, b}; }
+P3G_HD ef2 ef_from(uint64_t a) { return ef2{a, 0}; }
+P3G_HD ef2 ef_add(ef2 a, ef2 b) { return ef2{gl_add(a.c0, b.c0), gl_add(a.c1, b.c1)}; }
+P3G_HD ef2 ef_sub(ef2 a, ef2 b) { return ef2{gl_sub(a.c0, b.c0), gl_sub(a.c1, b.c1)}; }
+P3G_HD ef2 ef_neg(ef2 a) { return ef2{gl_neg(a.c0), gl_neg(a.c1)}; }
+P3G_HD ef2 ef_mul(ef2 a, ef2 b) {
+  uint64_t t = gl_mul(gl_mul(a.c1, b.c1), EXT_W);
+  return ef2{gl_add(gl_mul(a.c0, b.c0), t), gl_add(gl_mul(a.c0, b.c1), gl_mul(a.c1, b.c0))};
+}
+P3G_HD ef2 ef_mul_base(ef2 a, uint64_t b) { return ef2{gl_mul(a.c0, b), gl_mul(a.c1, b)}; }
+P3G_HD ef2 ef_sqr(ef2 a) { return ef_mul(a, a); }
+P3G_HD bool ef_eq(ef2 a, ef2 b) { return a.c0 == b.c0 && a.c1 == b.c1; }
+P3G_HD ef2 ef_inv(ef2 a) {
+  uint64_t norm = gl_sub(gl_sqr(a.c0), gl_mul(EXT_W, gl_sqr(a.c1)));
+  uint64_t ni = gl_inv(norm);
+  return ef2{gl_mul(a.c0, ni), gl_neg(gl_mul(a.c1, ni))};
+}
+P3G_HD ef2 ef_pow(ef2 a, uint64_t e) {
+  ef2 r = ef_from(1);
+  while (e) {
+    if (e & 1) r = ef_mul(r, a);
+    a = ef_sqr(a);
+    e >>= 1;
+  }
+  return r;
+}
+
+P3G_HD uint32_t bitrev32(uint32_t x, unsigned bits) {
+  if (bits == 0) return 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __brev(x) >> (32 - bits);
+#else
+  uint32_t r = 0;
+  for (unsigned i = 0; i < bits; i++) {
+    r = (r << 1) | (x & 1);
+    x >>= 1;
+  }
+  return r;
+#endif
+}
+
+inline unsigned log2_exact(uint64_t n) {
+  unsigned l = 0;
+  while ((1ull << l) < n) l++;
+  return l;
+}
+
+}  // namespace p3g

@@ -1,0 +1,189 @@
+// Internal declarations shared by the HIP library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "gl.hpp"
+#include "hash.hpp"
+
+#define P3G_CHECK_HIP(expr)                                  \
+  do {                                                       \
+    hipError_t e_ = (expr);                                  \
+    if (e_ != hipSuccess) throw p3g::DeviceError(e_, #expr); \
+  } while (0)
+
+namespace p3g {
+
+enum Status : int {
+  OK = 0,
+  E_INVAL = 1,
+  E_NOT_POW2 = 2,
+  E_UNSORTED = 3,
+  E_NOMEM = 4,
+  E_DEVICE = 5,
+  E_EMPTY = 6,
+  E_TRANSCRIPT = 7,
+  E_RANGE = 8,
+};
+
+struct DeviceError {
+  hipError_t err;
+  const char* what;
+  DeviceError(hipError_t e, const char* w) : err(e), what(w) {}
+};
+
+struct ApiError {
+  int code;
+  std::string msg;
+  ApiError(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+enum HashKind : uint32_t { HASH_P2 = 0, HASH_KECCAK = 1 };
+enum ChallengerKind : uint32_t { CH_DUPLEX_P2 = 0, CH_KECCAK = 1 };
+
+inline uint32_t hash_rate(uint32_t h) { return h == HASH_P2 ? 8 : 17; }
+inline uint32_t hash_width(uint32_t h) { return h == HASH_P2 ? 12 : 25; }
+// Alignable::ALIGNMENT: RATE for the field sponge; lcm(8, 8*17)/8 = 17 for the
+// u64-serialising Keccak sponge (serializing_sponge.rs:166-203).
+inline uint32_t hash_alignment(uint32_t h) { return h == HASH_P2 ? 8 : 17; }
+inline uint64_t aligned_len(uint64_t n, uint64_t a) { return a <= 1 ? n : (n + a - 1) / a * a; }
+
+// ------------------------------------------------------------------ context
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  P2Consts host_p2{};
+  P2Consts* dev_p2 = nullptr;
+  std::mutex mu;
+  // two-level power tables keyed by (base, exponent bits, lo multiplier)
+  std::map<std::tuple<uint64_t, uint32_t, uint64_t>, uint64_t*> pow_tables;
+  // size-bucketed cache of device allocations
+  std::multimap<size_t, void*> free_blocks;
+  std::map<void*, size_t> live_blocks;
+  size_t cached_bytes = 0;
+
+  // per-kernel timing with HIP events on this context's stream (off by default)
+  bool profiling = false;
+  std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof;
+
+  void* alloc(size_t bytes);
+  void release(void* p);
+  void trim();
+  const uint64_t* pow_table(uint64_t base, uint32_t bits, uint64_t lo_mul = 1);
+};
+
+// Records a start/stop event pair around the launches in its scope when the
+// context has profiling enabled.
+struct ProfScope {
+  Ctx& ctx;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  const char* name;
+  ProfScope(Ctx& c, const char* n, hipStream_t st) : ctx(c), s(st), name(n) {
+    if (ctx.profiling) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, s);
+    }
+  }
+  ~ProfScope() {
+    if (a) {
+      (void)hipEventRecord(b, s);
+      std::lock_guard<std::mutex> g(ctx.mu);
+      ctx.prof[name].emplace_back(a, b);
+    }
+  }
+};
+
+// Table layout: [lo: 2048 entries base^i * mul][hi: base^(2048 i)], so that
+// base^e * mul = lo[e & 2047] * hi[e >> 11].
+constexpr uint32_t POW_LO = 2048;
+inline uint64_t pow_table_hi_len(uint32_t bits) { return bits > 11 ? (1ull << (bits - 11)) : 1; }
+
+// ------------------------------------------------------------------ matrices and trees
+struct DevMat {
+  uint64_t* ptr = nullptr;
+  uint64_t height = 0, width = 0;
+  bool owned = false;
+};
+
+struct Tree {
+  Ctx* ctx = nullptr;
+  uint32_t hash = HASH_P2;
+  uint32_t alignment = 1;
+  std::vector<DevMat> mats;
+  uint64_t height = 0;
+  unsigned depth = 0;
+  uint64_t* digests = nullptr;  // all layers, layer l at layer_off[l] (in 4-word digests)
+  std::vector<uint64_t> layer_off;
+  uint64_t root[4] = {0, 0, 0, 0};
+  ~Tree();
+};
+
+// ------------------------------------------------------------------ transcript
+struct Transcript {
+  uint32_t kind = CH_DUPLEX_P2;
+  uint64_t st[12] = {};
+  uint64_t in[8] = {};
+  int in_len = 0;
+  uint64_t out[8] = {};
+  int out_len = 0;
+  std::vector<uint8_t> kin;
+  uint8_t kout[32] = {};
+  int kout_len = 0;
+  std::vector<uint64_t> fields;
+  std::vector<uint64_t> comms;  // 4 words each
+
+  void observe(uint64_t v);
+  void observe_commitment(const uint64_t* c4);
+  uint64_t sample();
+  ef2 sample_ef();
+  uint64_t sample_bits(unsigned bits);
+  void send_fields(const uint64_t* v, size_t n);
+  void send_commitment(const uint64_t* c4);
+  void hint_fields(const uint64_t* v, size_t n);
+  void hint_commitment(const uint64_t* c4);
+  uint64_t grind(unsigned bits);
+  bool check_witness(unsigned bits, uint64_t w);
+
+ private:
+  void duplex();
+  void k_observe_u64(uint64_t v);
+  uint8_t k_sample_byte();
+};
+
+const P2Consts& host_p2_consts();
+void keccak256_host(const uint8_t* in, size_t len, uint8_t* out32);
+
+// ------------------------------------------------------------------ PCS parameters
+struct PcsParams {
+  uint32_t log_blowup, log_arity, log_final_degree, folding_pow_bits, deep_pow_bits, num_queries, query_pow_bits;
+};
+size_t fri_num_rounds(const PcsParams& p, unsigned log_domain);
+size_t fri_final_degree(const PcsParams& p, unsigned log_domain);
+
+// ------------------------------------------------------------------ launchers
+void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
+                     uint64_t shift, uint64_t* d_out, hipStream_t s);
+void bit_reverse_rows(Ctx& ctx, const uint64_t* src, uint64_t* dst, uint64_t h, uint64_t w, hipStream_t s);
+
+Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s);
+void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out);
+void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Transcript& tx, hipStream_t s);
+void gather_to_host(Ctx& ctx, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces,
+                    std::vector<uint64_t>& out, hipStream_t s);
+
+void host_sponge_absorb(uint32_t hash, uint64_t* state, const uint64_t* in, size_t len);
+void host_compress(uint32_t hash, const uint64_t* l, const uint64_t* r, uint64_t* out);
+
+Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, unsigned log_blowup, hipStream_t s);
+
+void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned log_lde_height, const ef2* z, int npts,
+                       const std::vector<Tree*>& trees, Transcript& tx, hipStream_t s);
+
+}  // namespace p3g

@@ -1,0 +1,271 @@
+// Coset low-degree extension with bit-reversed output rows, for gfx950.
+//
+// Replaces `TwoAdicSubgroupDft::coset_lde_batch(trace, added_bits, shift)` followed
+// by `.bit_reverse_rows()` (p3-miden-lifted-stark/src/prover/commit.rs:217-221).
+//
+// Layout: the trace is row-major n x w (u64 words); the output is row-major
+// (n*B) x w.  Output row r holds f(shift * omega_{nB}^{bitrev(r)}) per column.
+//
+// Method (one permutation, no zero padding):
+//   1. inverse NTT of size n, decimation in time: rows are gathered in
+//      bit-reversed order straight from the trace (each row is a contiguous
+//      w-word segment, so the gather stays coalesced) and the natural-order
+//      coefficients land in output block 0;
+//   2. the LDE over nB points splits into B cosets c_t * H_n with
+//      c_t = shift * omega_{nB}^t.  For each t the coefficients are scaled by
+//      c_t^k / n and a size-n decimation-in-frequency NTT produces the
+//      bit-reversed evaluations, which are exactly output rows
+//      [bitrev_b(t)*n, (bitrev_b(t)+1)*n).  The first forward pass reads the
+//      coefficient tile once and emits all B cosets.
+// Each pass runs up to NTT_K butterfly stages on a 2^K x C tile staged in LDS;
+// twiddles come from two-level power tables (one multiply per twiddle),
+// computed once per workgroup into LDS.
+#include "internal.hpp"
+
+namespace p3g {
+
+constexpr int NTT_K = 8;
+constexpr int NTT_C = 16;
+constexpr int NTT_T = 256;
+
+enum NttMode : uint32_t { INV_FIRST = 0, INV = 1, FWD_FIRST = 2, FWD = 3 };
+
+struct NttArgs {
+  const uint64_t* src;
+  uint64_t* dst;
+  const uint64_t* tw;     // power table of omega_n^{-1} (inverse) or omega_n (forward)
+  const uint64_t* scale;  // FWD_FIRST: B power tables of c_t with 1/n folded into lo
+  uint64_t scale_stride;  // words per scale table
+  uint64_t n;
+  uint32_t logn, s0, K, w, C, ncoltiles, logB, mode;
+};
+
+__device__ __forceinline__ uint64_t pw(const uint64_t* __restrict__ tab, uint64_t e) {
+  return gl_mul(tab[e & (POW_LO - 1)], tab[POW_LO + (e >> 11)]);
+}
+
+__global__ void __launch_bounds__(NTT_T) ntt_pass_kernel(NttArgs a) {
+  __shared__ uint64_t tile[(1 << NTT_K) * NTT_C];
+  __shared__ uint64_t tws[1 << NTT_K];
+  const uint32_t K = a.K, C = a.C, tid = threadIdx.x;
+  const bool inverse = a.mode <= INV;
+  uint64_t bid = blockIdx.x;
+  const uint32_t ct = (uint32_t)(bid % a.ncoltiles);
+  bid /= a.ncoltiles;
+  // unit decomposition
+  uint32_t lb;  // number of low index bits below the unit's butterfly bits
+  if (inverse) lb = a.s0;
+  else lb = a.logn - a.s0 - K;
+  const uint64_t nunits = a.n >> K;
+  const uint64_t u = bid % nunits;
+  const uint64_t blk = bid / nunits;  // FWD: which of the B transforms
+  const uint64_t lo = u & ((1ull << lb) - 1);
+  const uint64_t hi = u >> lb;
+  const uint64_t base = hi << (lb + K) | lo;  // index of x = 0
+  const uint64_t xstride = 1ull << lb;
+  const uint32_t col0 = ct * C;
+  const uint32_t nel = (1u << K) * C;
+
+  // twiddles for this unit
+  for (uint32_t e = tid; e + 1 < (1u << K); e += NTT_T) {
+    // e = (2^t - 1) + m
+    uint32_t t = 31 - __clz(e + 1);
+    uint32_t m = e + 1 - (1u << t);
+    uint64_t ex;
+    if (inverse) {
+      // DIT stage t: j = m*2^s0 + lo, exponent j * 2^(logn - s0 - t - 1)
+      ex = ((uint64_t)m << a.s0 | lo) << (a.logn - a.s0 - t - 1);
+    } else {
+      // DIF stage with q = t here (table slot indexed by q): j = m*2^lb + lo,
+      // exponent j * 2^(s0 + K - 1 - q)
+      ex = ((uint64_t)m << lb | lo) << (a.s0 + K - 1 - t);
+    }
+    tws[e] = pw(a.tw, ex);
+  }
+
+  constexpr int PER = (1 << NTT_K) * NTT_C / NTT_T;
+  uint64_t regs[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    uint32_t e = tid + r * NTT_T;
+    if (e < nel) {
+      uint32_t x = e / C, c = e % C;
+      uint32_t col = col0 + c;
+      uint64_t idx = base + (uint64_t)x * xstride;
+      uint64_t v = 0;
+      if (col < a.w) {
+        if (a.mode == INV_FIRST) {
+          uint64_t srow = bitrev32((uint32_t)idx, a.logn);
+          v = a.src[srow * a.w + col];
+        } else {
+          uint64_t row = (a.mode == FWD ? blk * a.n : 0) + idx;
+          v = a.dst[row * a.w + col];
+        }
+      }
+      regs[r] = v;
+    }
+  }
+
+  const uint32_t nbf = (1u << (K ? K - 1 : 0)) * C;
+  const uint32_t nrep = a.mode == FWD_FIRST ? (1u << a.logB) : 1;
+  for (uint32_t t_rep = 0; t_rep < nrep; t_rep++) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      uint32_t e = tid + r * NTT_T;
+      if (e < nel) {
+        uint64_t v = regs[r];
+        if (a.mode == FWD_FIRST) {
+          uint32_t x = e / C;
+          uint64_t k = base + (uint64_t)x * xstride;
+          v = gl_mul(v, pw(a.scale + t_rep * a.scale_stride, k));
+        }
+        tile[e] = v;
+      }
+    }
+    __syncthreads();
+    for (uint32_t t = 0; t < K; t++) {
+      for (uint32_t b = tid; b < nbf; b += NTT_T) {
+        uint32_t p = b / C, c = b % C;
+        if (inverse) {
+          uint32_t mlo = p & ((1u << t) - 1);
+          uint32_t x0 = ((p >> t) << (t + 1)) | mlo;
+          uint32_t x1 = x0 | (1u << t);
+          uint64_t twv = tws[(1u << t) - 1 + mlo];
+          uint64_t uu = tile[x0 * C + c];
+          uint64_t vv = gl_mul(tile[x1 * C + c], twv);
+          tile[x0 * C + c] = gl_add(uu, vv);
+          tile[x1 * C + c] = gl_sub(uu, vv);
+        } else {
+          uint32_t q = K - 1 - t;
+          uint32_t mlo = p & ((1u << q) - 1);
+          uint32_t x0 = ((p >> q) << (q + 1)) | mlo;
+          uint32_t x1 = x0 | (1u << q);
+          uint64_t twv = tws[(1u << q) - 1 + mlo];
+          uint64_t uu = tile[x0 * C + c];
+          uint64_t vv = tile[x1 * C + c];
+          tile[x0 * C + c] = gl_add(uu, vv);
+          tile[x1 * C + c] = gl_mul(gl_sub(uu, vv), twv);
+        }
+      }
+      __syncthreads();
+    }
+    uint64_t out_block;
+    if (a.mode == FWD_FIRST) out_block = bitrev32(t_rep, a.logB);
+    else if (a.mode == FWD) out_block = blk;
+    else out_block = 0;
+    for (uint32_t e = tid; e < nel; e += NTT_T) {
+      uint32_t x = e / C, c = e % C;
+      uint32_t col = col0 + c;
+      if (col < a.w) {
+        uint64_t row = out_block * a.n + base + (uint64_t)x * xstride;
+        a.dst[row * a.w + col] = tile[e];
+      }
+    }
+  }
+}
+
+void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
+                     uint64_t shift, uint64_t* d_out, hipStream_t s) {
+  if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
+  if (w == 0) return;
+  const unsigned logn = log2_exact(n);
+  if (logn + log_blowup > 32) throw ApiError(E_RANGE, "LDE height exceeds the field two-adicity");
+  const uint32_t C = w < (uint64_t)NTT_C ? (uint32_t)w : (uint32_t)NTT_C;
+  const uint32_t ncol = (uint32_t)((w + C - 1) / C);
+  const uint32_t B = 1u << log_blowup;
+  const uint32_t twbits = logn ? logn - 1 : 0;
+
+  const uint64_t* tw_inv = ctx.pow_table(gl_inv(gl_two_adic_generator(logn)), twbits);
+  const uint64_t* tw_fwd = ctx.pow_table(gl_two_adic_generator(logn), twbits);
+  // scale tables for the B cosets, stacked in one allocation
+  const uint64_t ninv = gl_inv_pow2(logn);
+  const uint64_t stride = POW_LO + pow_table_hi_len(logn);
+  uint64_t* scale = (uint64_t*)ctx.alloc(stride * B * sizeof(uint64_t));
+  {
+    std::vector<uint64_t> host(stride * B);
+    const uint64_t om = gl_two_adic_generator(logn + log_blowup);
+    uint64_t ct = shift;
+    for (uint32_t t = 0; t < B; t++) {
+      uint64_t* tab = host.data() + t * stride;
+      uint64_t v = ninv;
+      for (uint32_t i = 0; i < POW_LO; i++) {
+        tab[i] = v;
+        v = gl_mul(v, ct);
+      }
+      uint64_t step = gl_pow(ct, POW_LO), h = 1;
+      for (uint64_t i = 0; i < pow_table_hi_len(logn); i++) {
+        tab[POW_LO + i] = h;
+        h = gl_mul(h, step);
+      }
+      ct = gl_mul(ct, om);
+    }
+    P3G_CHECK_HIP(hipMemcpyAsync(scale, host.data(), host.size() * 8, hipMemcpyHostToDevice, s));
+    // the host vector dies at scope end: make the copy complete first
+    P3G_CHECK_HIP(hipStreamSynchronize(s));
+  }
+
+  auto launch = [&](uint32_t mode, uint32_t s0, uint32_t K, uint64_t nblk) {
+    NttArgs a{};
+    a.src = d_trace;
+    a.dst = d_out;
+    a.tw = (mode <= INV) ? tw_inv : tw_fwd;
+    a.scale = scale;
+    a.scale_stride = stride;
+    a.n = n;
+    a.logn = logn;
+    a.s0 = s0;
+    a.K = K;
+    a.w = (uint32_t)w;
+    a.C = C;
+    a.ncoltiles = ncol;
+    a.logB = log_blowup;
+    a.mode = mode;
+    uint64_t grid = (n >> K) * ncol * nblk;
+    ProfScope ps(ctx, mode <= INV ? "lde_inverse_pass" : "lde_forward_pass", s);
+    hipLaunchKernelGGL(ntt_pass_kernel, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a);
+    P3G_CHECK_HIP(hipGetLastError());
+  };
+
+  // inverse (DIT), first pass gathers bit-reversed rows from the trace
+  {
+    uint32_t K0 = logn < (unsigned)NTT_K ? logn : NTT_K;
+    launch(INV_FIRST, 0, K0, 1);
+    for (uint32_t s0 = K0; s0 < logn; s0 += NTT_K) {
+      uint32_t K = (logn - s0) < (unsigned)NTT_K ? logn - s0 : NTT_K;
+      launch(INV, s0, K, 1);
+    }
+  }
+  // forward (DIF) per coset
+  {
+    uint32_t K0 = logn < (unsigned)NTT_K ? logn : NTT_K;
+    launch(FWD_FIRST, 0, K0, 1);
+    for (uint32_t s0 = K0; s0 < logn; s0 += NTT_K) {
+      uint32_t K = (logn - s0) < (unsigned)NTT_K ? logn - s0 : NTT_K;
+      launch(FWD, s0, K, B);
+    }
+  }
+  ctx.release(scale);
+}
+
+}  // namespace p3g
+
+namespace p3g {
+
+__global__ void bitrev_rows_kernel(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, uint64_t h,
+                                   unsigned logh, uint64_t w) {
+  const uint64_t r = blockIdx.x;
+  const uint64_t sr = bitrev32((uint32_t)r, logh);
+  for (uint64_t c = threadIdx.x; c < w; c += blockDim.x) dst[r * w + c] = src[sr * w + c];
+  (void)h;
+}
+
+// Out-of-place row bit reversal (bit_reverse_rows of p3-matrix).
+void bit_reverse_rows(Ctx& ctx, const uint64_t* src, uint64_t* dst, uint64_t h, uint64_t w, hipStream_t s) {
+  (void)ctx;
+  if (h == 0 || w == 0) return;
+  hipLaunchKernelGGL(bitrev_rows_kernel, dim3((uint32_t)h), dim3(64), 0, s, src, dst, h, log2_exact(h), w);
+  P3G_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace p3g

@@ -1,0 +1,478 @@
+// Lifted Merkle tree (LMCS) on gfx950: leaf sponge hashing with lifting,
+// 2-to-1 compression layers, row openings and batch-opening hints.
+//
+// Replaces LiftedMerkleTree::build_with_alignment
+// (p3-miden-lmcs/src/lifted_tree.rs:244-310), build_leaf_states_upsampled
+// (:385-437), absorb_matrix (:447-483), compress_uniform (:494-539),
+// rows (:127-138) and prove_batch (:157-224).
+//
+// Lifting without materialising upsampled rows: matrices are processed in
+// height groups (ascending).  The kernel for a group of height h runs one lane
+// per leaf state i < h; it starts from the previous group's state at index
+// i >> log2(h / h_prev) — the reference's state duplication, done as an index
+// map — absorbs row i of every matrix of that height, and writes the state
+// (or, for the last group, the 4-word digest).  Every LDE element is read
+// once and no state is duplicated in memory.
+//
+// Sponge: overwrite mode, rate 8 of width 12 for Poseidon2 (StatefulSponge,
+// p3-miden-stateful-hasher/src/field_sponge.rs:164-187), rate 17 of width 25 for
+// the u64-serialising Keccak sponge; a partial final chunk is zero-filled.
+// Compression: TruncatedPermutation (perm(l || r || 0)[0..4]) for Poseidon2 and
+// a single-block PaddingFreeSponge over KeccakF for Keccak (both leave the
+// remaining lanes zero).
+#include <algorithm>
+#include <cstring>
+#include <set>
+
+#include "internal.hpp"
+
+namespace p3g {
+
+constexpr int LEAF_T = 128;
+
+struct LeafMat {
+  const uint64_t* ptr;
+  uint64_t width;
+};
+
+struct LeafArgs {
+  const LeafMat* mats;
+  uint32_t nmats;
+  const uint64_t* prev_states;  // nullptr for the first group
+  uint32_t prev_shift;
+  uint64_t* out_states;   // non-final groups
+  uint64_t* out_digests;  // final group
+  uint64_t h;
+  const P2Consts* p2;
+};
+
+__global__ void __launch_bounds__(LEAF_T) p2_leaf_kernel(LeafArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * LEAF_T + threadIdx.x;
+  if (i >= a.h) return;
+  uint64_t st[12];
+  if (a.prev_states) {
+    const uint64_t* ps = a.prev_states + (i >> a.prev_shift) * 12;
+#pragma unroll
+    for (int k = 0; k < 12; k++) st[k] = ps[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) st[k] = 0;
+  }
+  for (uint32_t m = 0; m < a.nmats; m++) {
+    const uint64_t w = a.mats[m].width;
+    const uint64_t* row = a.mats[m].ptr + i * w;
+    uint64_t j = 0;
+    for (; j + 8 <= w; j += 8) {
+      const uint4* r4 = reinterpret_cast<const uint4*>(row + j);
+      if ((((uintptr_t)(row + j)) & 15) == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          uint4 v = r4[k];
+          st[2 * k] = ((uint64_t)v.y << 32) | v.x;
+          st[2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) st[k] = row[j + k];
+      }
+      p2_permute(st, a.p2);
+    }
+    if (j < w) {
+      uint64_t rem = w - j;
+#pragma unroll
+      for (int k = 0; k < 8; k++) st[k] = (uint64_t)k < rem ? row[j + k] : 0;
+      p2_permute(st, a.p2);
+    }
+  }
+  if (a.out_digests) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.out_digests[i * 4 + k] = st[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) a.out_states[i * 12 + k] = st[k];
+  }
+}
+
+__global__ void __launch_bounds__(LEAF_T) keccak_leaf_kernel(LeafArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * LEAF_T + threadIdx.x;
+  if (i >= a.h) return;
+  uint64_t st[25];
+  if (a.prev_states) {
+    const uint64_t* ps = a.prev_states + (i >> a.prev_shift) * 25;
+#pragma unroll
+    for (int k = 0; k < 25; k++) st[k] = ps[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 25; k++) st[k] = 0;
+  }
+  for (uint32_t m = 0; m < a.nmats; m++) {
+    const uint64_t w = a.mats[m].width;
+    const uint64_t* row = a.mats[m].ptr + i * w;
+    uint64_t j = 0;
+    for (; j + 17 <= w; j += 17) {
+#pragma unroll
+      for (int k = 0; k < 17; k++) st[k] = row[j + k];
+      keccak_f(st);
+    }
+    if (j < w) {
+      uint64_t rem = w - j;
+#pragma unroll
+      for (int k = 0; k < 17; k++) st[k] = (uint64_t)k < rem ? row[j + k] : 0;
+      keccak_f(st);
+    }
+  }
+  if (a.out_digests) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.out_digests[i * 4 + k] = st[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 25; k++) a.out_states[i * 25 + k] = st[k];
+  }
+}
+
+constexpr int CMP_T = 128;
+
+__device__ __forceinline__ void compress_one(uint32_t hash, const uint64_t* l, const uint64_t* r, uint64_t* out,
+                                             const P2Consts* p2) {
+  if (hash == HASH_P2) {
+    uint64_t s[12];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      s[k] = l[k];
+      s[4 + k] = r[k];
+      s[8 + k] = 0;
+    }
+    p2_permute(s, p2);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = s[k];
+  } else {
+    uint64_t s[25];
+#pragma unroll
+    for (int k = 0; k < 25; k++) s[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      s[k] = l[k];
+      s[4 + k] = r[k];
+    }
+    keccak_f(s);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = s[k];
+  }
+}
+
+template <uint32_t HASH>
+__global__ void __launch_bounds__(CMP_T) compress_layer_kernel(const uint64_t* __restrict__ prev,
+                                                               uint64_t* __restrict__ next, uint64_t len,
+                                                               const P2Consts* p2) {
+  const uint64_t i = (uint64_t)blockIdx.x * CMP_T + threadIdx.x;
+  if (i >= len) return;
+  uint64_t l[4], r[4], o[4];
+  const uint64_t* src = prev + i * 8;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    l[k] = src[k];
+    r[k] = src[4 + k];
+  }
+  compress_one(HASH, l, r, o, p2);
+#pragma unroll
+  for (int k = 0; k < 4; k++) next[i * 4 + k] = o[k];
+}
+
+// Remaining levels once a layer has at most 2*TOP_T digests: one workgroup,
+// levels staged through LDS, every level written out.
+constexpr int TOP_T = 256;
+template <uint32_t HASH>
+__global__ void __launch_bounds__(TOP_T) compress_top_kernel(uint64_t* digests, uint64_t off, uint64_t len,
+                                                             const P2Consts* p2) {
+  __shared__ uint64_t buf[2][TOP_T * 2 * 4];
+  const uint32_t tid = threadIdx.x;
+  for (uint64_t e = tid; e < len * 4; e += TOP_T) buf[0][e] = digests[off * 4 + e];
+  __syncthreads();
+  int cur = 0;
+  while (len > 1) {
+    uint64_t nl = len / 2;
+    uint64_t noff = off + len;
+    if (tid < nl) {
+      uint64_t o[4];
+      compress_one(HASH, &buf[cur][tid * 8], &buf[cur][tid * 8 + 4], o, p2);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        buf[cur ^ 1][tid * 4 + k] = o[k];
+        digests[(noff + tid) * 4 + k] = o[k];
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+    off = noff;
+    len = nl;
+  }
+}
+
+__global__ void gather_kernel(const uint64_t* const* __restrict__ srcs, const uint64_t* __restrict__ lens,
+                              const uint64_t* __restrict__ offs, uint64_t* __restrict__ out) {
+  const uint32_t b = blockIdx.x;
+  const uint64_t* s = srcs[b];
+  uint64_t n = lens[b], o = offs[b];
+  for (uint64_t e = threadIdx.x; e < n; e += blockDim.x) out[o + e] = s[e];
+}
+
+// ------------------------------------------------------------------ host side
+
+static void launch_leaf(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& group, const uint64_t* prev,
+                        uint32_t prev_shift, uint64_t* out_states, uint64_t* out_digests, uint64_t h,
+                        hipStream_t s) {
+  std::vector<LeafMat> hm(group.size());
+  for (size_t i = 0; i < group.size(); i++) hm[i] = LeafMat{group[i].ptr, group[i].width};
+  LeafMat* dm = (LeafMat*)ctx.alloc(hm.size() * sizeof(LeafMat));
+  P3G_CHECK_HIP(hipMemcpyAsync(dm, hm.data(), hm.size() * sizeof(LeafMat), hipMemcpyHostToDevice, s));
+  LeafArgs a{};
+  a.mats = dm;
+  a.nmats = (uint32_t)hm.size();
+  a.prev_states = prev;
+  a.prev_shift = prev_shift;
+  a.out_states = out_states;
+  a.out_digests = out_digests;
+  a.h = h;
+  a.p2 = ctx.dev_p2;
+  uint32_t grid = (uint32_t)((h + LEAF_T - 1) / LEAF_T);
+  {
+  ProfScope ps(ctx, hash == HASH_P2 ? "p2_leaf" : "keccak_leaf", s);
+  if (hash == HASH_P2) hipLaunchKernelGGL(p2_leaf_kernel, dim3(grid), dim3(LEAF_T), 0, s, a);
+  else hipLaunchKernelGGL(keccak_leaf_kernel, dim3(grid), dim3(LEAF_T), 0, s, a);
+  P3G_CHECK_HIP(hipGetLastError());
+  }
+  // the descriptor copy is from pageable host memory: complete it before hm dies
+  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx.release(dm);
+}
+
+static void build_layers(Ctx& ctx, Tree& t, hipStream_t s) {
+  uint64_t len = t.height;
+  unsigned l = 0;
+  while (len > 2 * TOP_T) {
+    uint64_t nl = len / 2;
+    uint32_t grid = (uint32_t)((nl + CMP_T - 1) / CMP_T);
+    const uint64_t* prev = t.digests + t.layer_off[l] * 4;
+    uint64_t* next = t.digests + t.layer_off[l + 1] * 4;
+    ProfScope ps(ctx, "compress_layer", s);
+    if (t.hash == HASH_P2)
+      hipLaunchKernelGGL(compress_layer_kernel<HASH_P2>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, ctx.dev_p2);
+    else
+      hipLaunchKernelGGL(compress_layer_kernel<HASH_KECCAK>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl,
+                         ctx.dev_p2);
+    P3G_CHECK_HIP(hipGetLastError());
+    len = nl;
+    l++;
+  }
+  if (len > 1) {
+    ProfScope ps(ctx, "compress_top", s);
+    if (t.hash == HASH_P2)
+      hipLaunchKernelGGL(compress_top_kernel<HASH_P2>, dim3(1), dim3(TOP_T), 0, s, t.digests, t.layer_off[l], len,
+                         ctx.dev_p2);
+    else
+      hipLaunchKernelGGL(compress_top_kernel<HASH_KECCAK>, dim3(1), dim3(TOP_T), 0, s, t.digests, t.layer_off[l],
+                         len, ctx.dev_p2);
+    P3G_CHECK_HIP(hipGetLastError());
+  }
+}
+
+Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s) {
+  if (mats.empty()) throw ApiError(E_EMPTY, "cannot commit empty batch");
+  uint64_t prevh = 0;
+  for (auto& m : mats) {
+    if (m.height == 0 || (m.height & (m.height - 1))) throw ApiError(E_NOT_POW2, "matrix height must be a power of two");
+    if (m.height < prevh) throw ApiError(E_UNSORTED, "matrices must be sorted by height");
+    prevh = m.height;
+  }
+  Tree* t = new Tree();
+  t->ctx = &ctx;
+  t->hash = hash;
+  t->alignment = alignment < 1 ? 1 : alignment;
+  t->mats = mats;
+  t->height = mats.back().height;
+  t->depth = log2_exact(t->height);
+  t->layer_off.resize(t->depth + 1);
+  uint64_t off = 0;
+  for (unsigned l = 0; l <= t->depth; l++) {
+    t->layer_off[l] = off;
+    off += t->height >> l;
+  }
+  t->digests = (uint64_t*)ctx.alloc(off * 4 * sizeof(uint64_t));
+
+  const uint32_t W = hash_width(hash);
+  uint64_t* states[2] = {nullptr, nullptr};
+  int cur = -1;
+  uint64_t cur_h = 0;
+  size_t i = 0;
+  while (i < mats.size()) {
+    size_t j = i;
+    while (j < mats.size() && mats[j].height == mats[i].height) j++;
+    std::vector<DevMat> group(mats.begin() + i, mats.begin() + j);
+    uint64_t h = mats[i].height;
+    bool final = (j == mats.size());
+    const uint64_t* prev = cur >= 0 ? states[cur] : nullptr;
+    uint32_t shift = cur >= 0 ? log2_exact(h / cur_h) : 0;
+    int nxt = cur >= 0 ? cur ^ 1 : 0;
+    uint64_t* out_states = nullptr;
+    if (!final) {
+      if (!states[nxt]) states[nxt] = (uint64_t*)ctx.alloc(t->height * W * sizeof(uint64_t));
+      out_states = states[nxt];
+    }
+    launch_leaf(ctx, hash, group, prev, shift, out_states, final ? t->digests : nullptr, h, s);
+    cur = nxt;
+    cur_h = h;
+    i = j;
+  }
+  build_layers(ctx, *t, s);
+  P3G_CHECK_HIP(hipMemcpyAsync(t->root, t->digests + t->layer_off[t->depth] * 4, 32, hipMemcpyDeviceToHost, s));
+  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx.release(states[0]);
+  ctx.release(states[1]);
+  return t;
+}
+
+void gather_to_host(Ctx& ctx, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces,
+                    std::vector<uint64_t>& out, hipStream_t s) {
+  out.clear();
+  if (pieces.empty()) return;
+  std::vector<const uint64_t*> srcs(pieces.size());
+  std::vector<uint64_t> lens(pieces.size()), offs(pieces.size());
+  uint64_t total = 0;
+  for (size_t i = 0; i < pieces.size(); i++) {
+    srcs[i] = pieces[i].first;
+    lens[i] = pieces[i].second;
+    offs[i] = total;
+    total += pieces[i].second;
+  }
+  out.resize(total);
+  if (total == 0) return;
+  size_t np = pieces.size();
+  // one staging block: [srcs | lens | offs | data]
+  uint64_t* stage = (uint64_t*)ctx.alloc((3 * np + total) * sizeof(uint64_t));
+  std::vector<uint64_t> hdr(3 * np);
+  memcpy(hdr.data(), srcs.data(), np * 8);
+  memcpy(hdr.data() + np, lens.data(), np * 8);
+  memcpy(hdr.data() + 2 * np, offs.data(), np * 8);
+  P3G_CHECK_HIP(hipMemcpyAsync(stage, hdr.data(), 3 * np * 8, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(gather_kernel, dim3((uint32_t)np), dim3(64), 0, s, (const uint64_t* const*)stage,
+                     stage + np, stage + 2 * np, stage + 3 * np);
+  P3G_CHECK_HIP(hipGetLastError());
+  P3G_CHECK_HIP(hipMemcpyAsync(out.data(), stage + 3 * np, total * 8, hipMemcpyDeviceToHost, s));
+  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx.release(stage);
+}
+
+void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out) {
+  if (index >= t.height) throw ApiError(E_RANGE, "leaf index out of range");
+  std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+  for (auto& m : t.mats) {
+    uint64_t row = index >> log2_exact(t.height / m.height);
+    pieces.emplace_back(m.ptr + row * m.width, m.width);
+  }
+  std::vector<uint64_t> raw;
+  gather_to_host(*t.ctx, pieces, raw, t.ctx->stream);
+  out.clear();
+  size_t off = 0;
+  for (auto& m : t.mats) {
+    out.insert(out.end(), raw.begin() + off, raw.begin() + off + m.width);
+    out.resize(out.size() + (aligned_len(m.width, t.alignment) - m.width), 0);
+    off += m.width;
+  }
+}
+
+void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Transcript& tx, hipStream_t s) {
+  std::set<uint64_t> uniq(indices.begin(), indices.end());
+  for (uint64_t idx : uniq)
+    if (idx >= t.height) throw ApiError(E_RANGE, "query index out of range");
+  // 1. rows of every matrix at every unique index, in leaf order
+  std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+  for (uint64_t idx : uniq)
+    for (auto& m : t.mats) {
+      uint64_t row = idx >> log2_exact(t.height / m.height);
+      pieces.emplace_back(m.ptr + row * m.width, m.width);
+    }
+  // 2. missing siblings, level by level, left to right
+  std::vector<uint64_t> known(uniq.begin(), uniq.end());
+  size_t nrows = pieces.size();
+  for (unsigned layer = 0; layer < t.depth; layer++) {
+    std::vector<uint64_t> parents;
+    for (size_t q = 0; q < known.size(); q++) {
+      uint64_t parent = known[q] / 2;
+      if (!parents.empty() && parents.back() == parent) continue;
+      parents.push_back(parent);
+      uint64_t left = parent * 2, right = left + 1;
+      bool hl = std::binary_search(known.begin(), known.end(), left);
+      bool hr = std::binary_search(known.begin(), known.end(), right);
+      if (hl && !hr) pieces.emplace_back(t.digests + (t.layer_off[layer] + right) * 4, 4);
+      else if (!hl && hr) pieces.emplace_back(t.digests + (t.layer_off[layer] + left) * 4, 4);
+    }
+    known.swap(parents);
+  }
+  std::vector<uint64_t> raw;
+  gather_to_host(*t.ctx, pieces, raw, s);
+  size_t off = 0;
+  std::vector<uint64_t> padded;
+  for (size_t p = 0; p < nrows; p++) {
+    uint64_t w = pieces[p].second;
+    uint64_t aw = aligned_len(w, t.alignment);
+    padded.assign(raw.begin() + off, raw.begin() + off + w);
+    padded.resize(aw, 0);
+    tx.hint_fields(padded.data(), aw);
+    off += w;
+  }
+  for (size_t p = nrows; p < pieces.size(); p++) {
+    tx.hint_commitment(raw.data() + off);
+    off += 4;
+  }
+}
+
+void host_sponge_absorb(uint32_t hash, uint64_t* state, const uint64_t* in, size_t len) {
+  const size_t rate = hash_rate(hash);
+  size_t j = 0;
+  while (j < len) {
+    size_t rem = len - j;
+    for (size_t k = 0; k < rate; k++) state[k] = k < rem ? in[j + k] : 0;
+    if (hash == HASH_P2) p2_permute(state, &host_p2_consts());
+    else keccak_f(state);
+    j += rate < rem ? rate : rem;
+  }
+}
+
+void host_compress(uint32_t hash, const uint64_t* l, const uint64_t* r, uint64_t* out) {
+  uint64_t s[25] = {};
+  memcpy(s, l, 32);
+  memcpy(s + 4, r, 32);
+  if (hash == HASH_P2) p2_permute(s, &host_p2_consts());
+  else keccak_f(s);
+  memcpy(out, s, 32);
+}
+
+Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, unsigned log_blowup, hipStream_t s) {
+  if (traces.empty()) throw ApiError(E_EMPTY, "at least one trace required");
+  for (size_t i = 1; i < traces.size(); i++)
+    if (traces[i].height < traces[i - 1].height) throw ApiError(E_UNSORTED, "traces must be sorted by height");
+  for (auto& tr : traces)
+    if (tr.height == 0 || (tr.height & (tr.height - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
+  const unsigned lmax = log2_exact(traces.back().height);
+  std::vector<DevMat> ldes;
+  try {
+    for (auto& tr : traces) {
+      unsigned lt = log2_exact(tr.height);
+      uint64_t shift = GENERATOR;
+      for (unsigned k = lt; k < lmax; k++) shift = gl_mul(shift, shift);
+      DevMat m;
+      m.height = tr.height << log_blowup;
+      m.width = tr.width;
+      m.owned = true;
+      m.ptr = (uint64_t*)ctx.alloc(m.height * m.width * sizeof(uint64_t));
+      ldes.push_back(m);
+      lde_coset_batch(ctx, tr.ptr, tr.height, tr.width, log_blowup, shift, m.ptr, s);
+    }
+    return lmcs_build(ctx, hash, ldes, hash_alignment(hash), s);
+  } catch (...) {
+    for (auto& m : ldes) ctx.release(m.ptr);
+    throw;
+  }
+}
+
+}  // namespace p3g

@@ -1,0 +1,579 @@
+// DEEP quotient, FRI commit/fold and the query phase on gfx950, sequenced by
+// the host exactly as `open_with_channel` (p3-miden-lifted-fri/src/prover.rs:35-116).
+//
+// DEEP (deep/interpolate.rs:81-201, deep/prover.rs:230-409):
+//   deep_weights_kernel   barycentric weights x_i / (z_j - x_i) over the first
+//                         L/B coset points x_i = g * omega_L^{bitrev(i)};
+//                         1/(z - x) = (z0 - x, -z1) / N(z - x) with the base-field
+//                         norm inverted by a per-thread Montgomery batch;
+//   fold_weights_kernel   chunk sums for lifted (shorter) matrices;
+//   colsum kernels        per-column dot products with the weights (lanes over
+//                         columns, coalesced row reads), then a block reduce;
+//   deep_quotient_kernel  Q(x_i) = sum_j beta^j (f_red(z_j) + acc_i) / (z_j - x_i)
+//                         with acc_i = sum_c -alpha^(W-1-c) f_c(x_i) over the
+//                         lifted rows (row i >> log2(L/h) of each matrix).
+// FRI (fri/prover.rs:190-356, fold/arity{2,4,8}.rs): each round commits the
+//   current EF vector viewed as (m/a) x 2a base-field rows with an unaligned
+//   LMCS tree, then fri_fold_kernel interpolates each bit-reversed coset row
+//   and evaluates at beta / s (s^-1 = omega_L^{-bitrev(k) * L/m} from a
+//   two-level table).  The final polynomial is recovered on the host.
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "internal.hpp"
+
+namespace p3g {
+
+constexpr int MAXPTS = 4;
+
+struct Points {
+  ef2 z[MAXPTS];
+};
+
+__device__ __forceinline__ uint64_t pwt(const uint64_t* __restrict__ tab, uint64_t e) {
+  return gl_mul(tab[e & (POW_LO - 1)], tab[POW_LO + (e >> 11)]);
+}
+
+constexpr int DW_T = 256;
+constexpr int DW_CH = 8;
+
+template <int N>
+__global__ void __launch_bounds__(DW_T) deep_weights_kernel(const uint64_t* __restrict__ powL, unsigned logL,
+                                                            Points zp, uint64_t d, ef2* __restrict__ W) {
+  const uint64_t i0 = ((uint64_t)blockIdx.x * DW_T + threadIdx.x) * DW_CH;
+  if (i0 >= d) return;
+  const int cnt = (int)((d - i0) < (uint64_t)DW_CH ? (d - i0) : DW_CH);
+  uint64_t xs[DW_CH], d0[DW_CH][N], nrm[DW_CH][N], pre[DW_CH][N];
+  uint64_t acc = 1;
+#pragma unroll
+  for (int r = 0; r < DW_CH; r++) {
+    if (r < cnt) {
+      uint64_t i = i0 + r;
+      uint64_t x = gl_mul(GENERATOR, pwt(powL, bitrev32((uint32_t)i, logL)));
+      xs[r] = x;
+#pragma unroll
+      for (int j = 0; j < N; j++) {
+        uint64_t a0 = gl_sub(zp.z[j].c0, x);
+        d0[r][j] = a0;
+        uint64_t n = gl_sub(gl_sqr(a0), gl_mul(EXT_W, gl_sqr(zp.z[j].c1)));
+        nrm[r][j] = n;
+        pre[r][j] = acc;
+        acc = gl_mul(acc, n);
+      }
+    }
+  }
+  uint64_t inv = gl_inv(acc);
+#pragma unroll
+  for (int r = DW_CH - 1; r >= 0; r--) {
+    if (r < cnt) {
+#pragma unroll
+      for (int j = N - 1; j >= 0; j--) {
+        uint64_t ni = gl_mul(inv, pre[r][j]);
+        inv = gl_mul(inv, nrm[r][j]);
+        // q = (d0 * ni, -z1 * ni); weight = x * q
+        uint64_t q0 = gl_mul(d0[r][j], ni);
+        uint64_t q1 = gl_neg(gl_mul(zp.z[j].c1, ni));
+        W[(i0 + r) * N + j] = ef2{gl_mul(q0, xs[r]), gl_mul(q1, xs[r])};
+      }
+    }
+  }
+}
+
+template <int N>
+__global__ void fold_weights_kernel(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t len, uint64_t cs) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    ef2 s = ef_from(0);
+    for (uint64_t c = 0; c < cs; c++) s = ef_add(s, in[(i * cs + c) * N + j]);
+    out[i * N + j] = s;
+  }
+}
+
+constexpr int CS_COLS = 64;
+constexpr int CS_SLICES = 4;
+constexpr int CS_T = CS_COLS * CS_SLICES;
+
+template <int N>
+__global__ void __launch_bounds__(CS_T) deep_colsum_kernel(const uint64_t* __restrict__ M, uint64_t w, uint64_t rows,
+                                                           const ef2* __restrict__ W, uint64_t rows_per_block,
+                                                           ef2* __restrict__ partial) {
+  __shared__ ef2 red[CS_SLICES][CS_COLS][N];
+  const uint32_t cl = threadIdx.x % CS_COLS, rs = threadIdx.x / CS_COLS;
+  const uint64_t col = (uint64_t)blockIdx.x * CS_COLS + cl;
+  const uint64_t r0 = (uint64_t)blockIdx.y * rows_per_block;
+  const uint64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  ef2 acc[N];
+#pragma unroll
+  for (int j = 0; j < N; j++) acc[j] = ef_from(0);
+  if (col < w) {
+    for (uint64_t r = r0 + rs; r < r1; r += CS_SLICES) {
+      uint64_t v = M[r * w + col];
+#pragma unroll
+      for (int j = 0; j < N; j++) acc[j] = ef_add(acc[j], ef_mul_base(W[r * N + j], v));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < N; j++) red[rs][cl][j] = acc[j];
+  __syncthreads();
+  if (rs == 0 && col < w) {
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+      ef2 s = red[0][cl][j];
+      for (int k = 1; k < CS_SLICES; k++) s = ef_add(s, red[k][cl][j]);
+      partial[((uint64_t)blockIdx.y * w + col) * N + j] = s;
+    }
+  }
+}
+
+template <int N>
+__global__ void colsum_reduce_kernel(const ef2* __restrict__ partial, uint64_t nblocks, uint64_t w,
+                                     ef2* __restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= w * N) return;
+  ef2 s = ef_from(0);
+  for (uint64_t b = 0; b < nblocks; b++) s = ef_add(s, partial[b * w * N + t]);
+  out[t] = s;
+}
+
+struct QMat {
+  const uint64_t* ptr;
+  uint64_t width;
+  uint32_t shift;
+  uint32_t coeff_off;
+};
+
+struct QuotArgs {
+  const QMat* mats;
+  uint32_t nmats;
+  const ef2* coeffs;
+  uint64_t L;
+  unsigned logL;
+  const uint64_t* powL;
+  Points z;
+  ef2 fred[MAXPTS];
+  ef2 bpow[MAXPTS];
+  ef2* out;
+};
+
+constexpr int DQ_T = 128;
+constexpr int DQ_CH = 4;
+
+template <int N>
+__global__ void __launch_bounds__(DQ_T) deep_quotient_kernel(QuotArgs a) {
+  const uint64_t i0 = ((uint64_t)blockIdx.x * DQ_T + threadIdx.x) * DQ_CH;
+  if (i0 >= a.L) return;
+  const int cnt = (int)((a.L - i0) < (uint64_t)DQ_CH ? (a.L - i0) : DQ_CH);
+  ef2 acc[DQ_CH];
+  uint64_t xs[DQ_CH];
+#pragma unroll
+  for (int r = 0; r < DQ_CH; r++) acc[r] = ef_from(0);
+  for (uint32_t m = 0; m < a.nmats; m++) {
+    const QMat mm = a.mats[m];
+    const ef2* cf = a.coeffs + mm.coeff_off;
+#pragma unroll
+    for (int r = 0; r < DQ_CH; r++) {
+      if (r < cnt) {
+        const uint64_t* row = mm.ptr + ((i0 + r) >> mm.shift) * mm.width;
+        ef2 s = acc[r];
+        for (uint64_t c = 0; c < mm.width; c++) s = ef_add(s, ef_mul_base(cf[c], row[c]));
+        acc[r] = s;
+      }
+    }
+  }
+  uint64_t d0[DQ_CH][N], nrm[DQ_CH][N], pre[DQ_CH][N];
+  uint64_t run = 1;
+#pragma unroll
+  for (int r = 0; r < DQ_CH; r++) {
+    if (r < cnt) {
+      uint64_t x = gl_mul(GENERATOR, pwt(a.powL, bitrev32((uint32_t)(i0 + r), a.logL)));
+      xs[r] = x;
+#pragma unroll
+      for (int j = 0; j < N; j++) {
+        uint64_t a0 = gl_sub(a.z.z[j].c0, x);
+        d0[r][j] = a0;
+        uint64_t n = gl_sub(gl_sqr(a0), gl_mul(EXT_W, gl_sqr(a.z.z[j].c1)));
+        nrm[r][j] = n;
+        pre[r][j] = run;
+        run = gl_mul(run, n);
+      }
+    }
+  }
+  uint64_t inv = gl_inv(run);
+  ef2 res[DQ_CH];
+#pragma unroll
+  for (int r = 0; r < DQ_CH; r++) res[r] = ef_from(0);
+#pragma unroll
+  for (int r = DQ_CH - 1; r >= 0; r--) {
+    if (r < cnt) {
+#pragma unroll
+      for (int j = N - 1; j >= 0; j--) {
+        uint64_t ni = gl_mul(inv, pre[r][j]);
+        inv = gl_mul(inv, nrm[r][j]);
+        ef2 q = ef2{gl_mul(d0[r][j], ni), gl_neg(gl_mul(a.z.z[j].c1, ni))};
+        ef2 t = ef_mul(q, ef_add(a.fred[j], acc[r]));
+        if (j > 0) t = ef_mul(a.bpow[j], t);
+        res[r] = ef_add(res[r], t);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < DQ_CH; r++)
+    if (r < cnt) a.out[i0 + r] = res[r];
+}
+
+struct FoldTw {
+  uint64_t w[8];  // inverse twiddles for the in-row inverse DFT (stage-major)
+};
+
+template <int LA>
+__global__ void fri_fold_kernel(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t nrows,
+                                unsigned log_rows, const uint64_t* __restrict__ powLinv, unsigned exp_shift,
+                                ef2 beta, FoldTw tw, uint64_t ainv) {
+  constexpr int A = 1 << LA;
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrows) return;
+  ef2 v[A];
+#pragma unroll
+  for (int p = 0; p < A; p++) v[p] = in[k * A + p];
+  // bit-reversed input + decimation in time = natural-order unscaled inverse DFT
+  int twi = 0;
+#pragma unroll
+  for (int s = 1; s <= LA; s++) {
+    const int m = 1 << s, h = m >> 1;
+#pragma unroll
+    for (int j = 0; j < h; j++) {
+      const uint64_t t = tw.w[twi + j];
+#pragma unroll
+      for (int b = 0; b < A; b += m) {
+        ef2 uu = v[b + j];
+        ef2 vv = ef_mul_base(v[b + j + h], t);
+        v[b + j] = ef_add(uu, vv);
+        v[b + j + h] = ef_sub(uu, vv);
+      }
+    }
+    twi += h;
+  }
+  const uint64_t e = (uint64_t)bitrev32((uint32_t)k, log_rows) << exp_shift;
+  const uint64_t s_inv = pwt(powLinv, e);
+  const ef2 x = ef_mul_base(beta, s_inv);
+  ef2 r = v[A - 1];
+#pragma unroll
+  for (int j = A - 2; j >= 0; j--) r = ef_add(ef_mul(r, x), v[j]);
+  out[k] = ef_mul_base(r, ainv);
+}
+
+// ------------------------------------------------------------------ host driver
+
+template <int N>
+static void run_deep_kernels(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z,
+                             const std::vector<DevMat>& mats, std::vector<ef2>& evals_host, hipStream_t s) {
+  const uint64_t L = 1ull << logL;
+  const unsigned logd = logL - p.log_blowup;
+  const uint64_t d = 1ull << logd;
+  Points zp{};
+  for (int j = 0; j < N; j++) zp.z[j] = z[j];
+  const uint64_t* powL = ctx.pow_table(gl_two_adic_generator(logL), logL);
+  std::map<uint64_t, ef2*> wts;
+  ef2* top = (ef2*)ctx.alloc(d * N * sizeof(ef2));
+  {
+    uint64_t threads = (d + DW_CH - 1) / DW_CH;
+    ProfScope ps(ctx, "deep_weights", s);
+    hipLaunchKernelGGL(deep_weights_kernel<N>, dim3((uint32_t)((threads + DW_T - 1) / DW_T)), dim3(DW_T), 0, s,
+                       powL, logL, zp, d, top);
+    P3G_CHECK_HIP(hipGetLastError());
+  }
+  wts[d] = top;
+  std::vector<uint64_t> degs;
+  for (auto& m : mats) degs.push_back(m.height >> p.log_blowup);
+  std::sort(degs.begin(), degs.end());
+  degs.erase(std::unique(degs.begin(), degs.end()), degs.end());
+  if (degs.back() != d) throw ApiError(E_INVAL, "no matrix spans the full LDE height");
+  uint64_t prev_len = d;
+  ef2* prev = top;
+  for (auto it = degs.rbegin(); it != degs.rend(); ++it) {
+    uint64_t len = *it;
+    if (len == d) continue;
+    ef2* nw = (ef2*)ctx.alloc(len * N * sizeof(ef2));
+    hipLaunchKernelGGL(fold_weights_kernel<N>, dim3((uint32_t)((len + 127) / 128)), dim3(128), 0, s, prev, nw, len,
+                       prev_len / len);
+    P3G_CHECK_HIP(hipGetLastError());
+    wts[len] = nw;
+    prev = nw;
+    prev_len = len;
+  }
+  uint64_t totw = 0;
+  for (auto& m : mats) totw += m.width;
+  ef2* evals_dev = (ef2*)ctx.alloc((totw ? totw : 1) * N * sizeof(ef2));
+  uint64_t off = 0;
+  for (auto& m : mats) {
+    if (m.width == 0) continue;
+    uint64_t rows = m.height >> p.log_blowup;
+    // keep the y grid dimension within 65535 blocks
+    uint64_t rpb = rows > 1024ull * 65535 ? (rows + 65534) / 65535 : 1024;
+    uint64_t nblk = (rows + rpb - 1) / rpb;
+    uint64_t ncb = (m.width + CS_COLS - 1) / CS_COLS;
+    ef2* partial = (ef2*)ctx.alloc(nblk * m.width * N * sizeof(ef2));
+    {
+    ProfScope ps(ctx, "deep_colsum", s);
+    hipLaunchKernelGGL(deep_colsum_kernel<N>, dim3((uint32_t)ncb, (uint32_t)nblk), dim3(CS_T), 0, s, m.ptr, m.width,
+                       rows, wts[rows], rpb, partial);
+    P3G_CHECK_HIP(hipGetLastError());
+    }
+    uint64_t nt = m.width * N;
+    hipLaunchKernelGGL(colsum_reduce_kernel<N>, dim3((uint32_t)((nt + 127) / 128)), dim3(128), 0, s, partial, nblk,
+                       m.width, evals_dev + off * N);
+    P3G_CHECK_HIP(hipGetLastError());
+    ctx.release(partial);
+    off += m.width;
+  }
+  evals_host.resize(totw * N);
+  if (totw)
+    P3G_CHECK_HIP(hipMemcpyAsync(evals_host.data(), evals_dev, totw * N * sizeof(ef2), hipMemcpyDeviceToHost, s));
+  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx.release(evals_dev);
+  for (auto& kv : wts) ctx.release(kv.second);
+  // barycentric scaling s_j = ((z_j / g)^d - 1) / d
+  const uint64_t ginv = gl_inv(GENERATOR);
+  const uint64_t dinv = gl_inv_pow2(logd);
+  ef2 sc[N];
+  for (int j = 0; j < N; j++) {
+    ef2 t = ef_mul_base(z[j], ginv);
+    for (unsigned k = 0; k < logd; k++) t = ef_sqr(t);
+    sc[j] = ef_mul_base(ef_sub(t, ef_from(1)), dinv);
+  }
+  for (uint64_t c = 0; c < totw; c++)
+    for (int j = 0; j < N; j++) evals_host[c * N + j] = ef_mul(evals_host[c * N + j], sc[j]);
+  (void)L;
+}
+
+template <int N>
+static void run_quotient(Ctx& ctx, unsigned logL, const ef2* z, const std::vector<DevMat>& mats,
+                         const std::vector<uint64_t>& awidths, const std::vector<ef2>& coeffs, const ef2* fred,
+                         ef2 beta, ef2* out, hipStream_t s) {
+  const uint64_t L = 1ull << logL;
+  std::vector<QMat> qm;
+  uint64_t aoff = 0;
+  for (size_t i = 0; i < mats.size(); i++) {
+    qm.push_back(QMat{mats[i].ptr, mats[i].width, log2_exact(L / mats[i].height), (uint32_t)aoff});
+    aoff += awidths[i];
+  }
+  QMat* dq = (QMat*)ctx.alloc(qm.size() * sizeof(QMat));
+  ef2* dc = (ef2*)ctx.alloc((coeffs.size() ? coeffs.size() : 1) * sizeof(ef2));
+  P3G_CHECK_HIP(hipMemcpyAsync(dq, qm.data(), qm.size() * sizeof(QMat), hipMemcpyHostToDevice, s));
+  if (!coeffs.empty())
+    P3G_CHECK_HIP(hipMemcpyAsync(dc, coeffs.data(), coeffs.size() * sizeof(ef2), hipMemcpyHostToDevice, s));
+  QuotArgs a{};
+  a.mats = dq;
+  a.nmats = (uint32_t)qm.size();
+  a.coeffs = dc;
+  a.L = L;
+  a.logL = logL;
+  a.powL = ctx.pow_table(gl_two_adic_generator(logL), logL);
+  for (int j = 0; j < N; j++) {
+    a.z.z[j] = z[j];
+    a.fred[j] = fred[j];
+  }
+  a.bpow[0] = ef_from(1);
+  for (int j = 1; j < N; j++) a.bpow[j] = ef_mul(a.bpow[j - 1], beta);
+  a.out = out;
+  uint64_t threads = (L + DQ_CH - 1) / DQ_CH;
+  {
+  ProfScope ps(ctx, "deep_quotient", s);
+  hipLaunchKernelGGL(deep_quotient_kernel<N>, dim3((uint32_t)((threads + DQ_T - 1) / DQ_T)), dim3(DQ_T), 0, s, a);
+  P3G_CHECK_HIP(hipGetLastError());
+  }
+  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx.release(dq);
+  ctx.release(dc);
+}
+
+static void host_idft_ext(std::vector<ef2>& v) {
+  const size_t n = v.size();
+  if (n <= 1) return;
+  const unsigned ln = log2_exact(n);
+  const uint64_t winv = gl_inv(gl_two_adic_generator(ln));
+  const uint64_t ninv = gl_inv_pow2(ln);
+  std::vector<ef2> out(n);
+  for (size_t k = 0; k < n; k++) {
+    ef2 acc = ef_from(0);
+    uint64_t step = gl_pow(winv, k), x = 1;
+    for (size_t i = 0; i < n; i++) {
+      acc = ef_add(acc, ef_mul_base(v[i], x));
+      x = gl_mul(x, step);
+    }
+    out[k] = ef_mul_base(acc, ninv);
+  }
+  v.swap(out);
+}
+
+void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z, int npts,
+                       const std::vector<Tree*>& trees, Transcript& tx, hipStream_t s) {
+  if (trees.empty()) throw ApiError(E_EMPTY, "at least one trace tree required");
+  if (npts < 1 || npts > MAXPTS) throw ApiError(E_INVAL, "1 to 4 evaluation points supported");
+  if (p.log_arity < 1 || p.log_arity > 3) throw ApiError(E_INVAL, "invalid folding arity");
+  if (p.log_blowup == 0 || p.num_queries == 0) throw ApiError(E_INVAL, "invalid PCS parameters");
+  const uint64_t L = 1ull << logL;
+  for (auto* t : trees)
+    if (t->height != L) throw ApiError(E_INVAL, "mixed LDE heights are not supported");
+  const uint32_t alignment = trees[0]->alignment;
+  for (auto* t : trees)
+    if (t->alignment != alignment) throw ApiError(E_INVAL, "mixed trace tree alignments are not supported");
+  if (logL < p.log_blowup) throw ApiError(E_INVAL, "LDE height below blowup");
+  const uint32_t hash = trees[0]->hash;
+
+  std::vector<DevMat> mats;
+  std::vector<uint64_t> awidths;
+  for (auto* t : trees)
+    for (auto& m : t->mats) {
+      mats.push_back(m);
+      awidths.push_back(aligned_len(m.width, alignment));
+    }
+
+  // ---- DEEP: evaluations at the OOD points
+  std::vector<ef2> evals;  // [column][point]
+  switch (npts) {
+    case 1: run_deep_kernels<1>(ctx, p, logL, z, mats, evals, s); break;
+    case 2: run_deep_kernels<2>(ctx, p, logL, z, mats, evals, s); break;
+    case 3: run_deep_kernels<3>(ctx, p, logL, z, mats, evals, s); break;
+    default: run_deep_kernels<4>(ctx, p, logL, z, mats, evals, s); break;
+  }
+  uint64_t tota = 0;
+  for (auto a : awidths) tota += a;
+  std::vector<ef2> aligned((size_t)tota * npts, ef_from(0));  // [point][aligned column]
+  {
+    uint64_t off = 0, aoff = 0;
+    for (size_t m = 0; m < mats.size(); m++) {
+      for (uint64_t c = 0; c < mats[m].width; c++)
+        for (int j = 0; j < npts; j++) aligned[j * tota + aoff + c] = evals[(off + c) * npts + j];
+      off += mats[m].width;
+      aoff += awidths[m];
+    }
+  }
+  for (int j = 0; j < npts; j++) tx.send_fields(reinterpret_cast<const uint64_t*>(aligned.data() + j * tota), 2 * tota);
+  tx.grind(p.deep_pow_bits);
+  const ef2 alpha = tx.sample_ef();
+  const ef2 beta = tx.sample_ef();
+  ef2 fred[MAXPTS];
+  for (int j = 0; j < npts; j++) {
+    ef2 acc = ef_from(0);
+    for (uint64_t c = 0; c < tota; c++) acc = ef_add(ef_mul(acc, alpha), aligned[j * tota + c]);
+    fred[j] = acc;
+  }
+  std::vector<ef2> coeffs(tota);
+  {
+    ef2 pw = ef_neg(ef_from(1));
+    for (uint64_t i = tota; i-- > 0;) {
+      coeffs[i] = pw;
+      pw = ef_mul(pw, alpha);
+    }
+  }
+  ef2* Q = (ef2*)ctx.alloc(L * sizeof(ef2));
+  switch (npts) {
+    case 1: run_quotient<1>(ctx, logL, z, mats, awidths, coeffs, fred, beta, Q, s); break;
+    case 2: run_quotient<2>(ctx, logL, z, mats, awidths, coeffs, fred, beta, Q, s); break;
+    case 3: run_quotient<3>(ctx, logL, z, mats, awidths, coeffs, fred, beta, Q, s); break;
+    default: run_quotient<4>(ctx, logL, z, mats, awidths, coeffs, fred, beta, Q, s); break;
+  }
+
+  // ---- FRI commit phase
+  const unsigned la = p.log_arity;
+  const uint64_t arity = 1ull << la;
+  const size_t fdeg = fri_final_degree(p, logL);
+  const uint64_t fdomain = (uint64_t)fdeg << p.log_blowup;
+  const unsigned tbits = logL > la ? logL - la : 0;
+  const uint64_t* powLinv = ctx.pow_table(gl_inv(gl_two_adic_generator(logL)), tbits);
+  FoldTw tw{};
+  {
+    int twi = 0;
+    for (unsigned st = 1; st <= la; st++) {
+      uint64_t wm = gl_inv(gl_two_adic_generator(st));
+      uint64_t h = 1ull << (st - 1), x = 1;
+      for (uint64_t j = 0; j < h; j++) {
+        tw.w[twi + j] = x;
+        x = gl_mul(x, wm);
+      }
+      twi += (int)h;
+    }
+  }
+  const uint64_t ainv = gl_inv_pow2(la);
+  std::vector<Tree*> ftrees;
+  std::vector<ef2*> bufs;
+  ef2* cur = Q;
+  uint64_t domain = L;
+  try {
+    while (domain > fdomain) {
+      const uint64_t fd = domain >> la;
+      DevMat m;
+      m.ptr = reinterpret_cast<uint64_t*>(cur);
+      m.height = fd;
+      m.width = 2 * arity;
+      m.owned = false;
+      Tree* t = lmcs_build(ctx, hash, {m}, 1, s);
+      ftrees.push_back(t);
+      bufs.push_back(cur);
+      tx.send_commitment(t->root);
+      tx.grind(p.folding_pow_bits);
+      const ef2 b = tx.sample_ef();
+      ef2* nxt = (ef2*)ctx.alloc(fd * sizeof(ef2));
+      const unsigned log_rows = log2_exact(fd);
+      const unsigned exp_shift = logL - log2_exact(domain);
+      const uint32_t grid = (uint32_t)((fd + 255) / 256);
+      ProfScope ps(ctx, "fri_fold", s);
+      switch (la) {
+        case 1:
+          hipLaunchKernelGGL(fri_fold_kernel<1>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                             exp_shift, b, tw, ainv);
+          break;
+        case 2:
+          hipLaunchKernelGGL(fri_fold_kernel<2>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                             exp_shift, b, tw, ainv);
+          break;
+        default:
+          hipLaunchKernelGGL(fri_fold_kernel<3>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                             exp_shift, b, tw, ainv);
+          break;
+      }
+      P3G_CHECK_HIP(hipGetLastError());
+      cur = nxt;
+      domain = fd;
+    }
+    // final polynomial: first fdeg evaluations, natural order, inverse DFT, descending
+    std::vector<ef2> fin(fdeg);
+    P3G_CHECK_HIP(hipMemcpyAsync(fin.data(), cur, fdeg * sizeof(ef2), hipMemcpyDeviceToHost, s));
+    P3G_CHECK_HIP(hipStreamSynchronize(s));
+    {
+      const unsigned lf = log2_exact(fdeg);
+      std::vector<ef2> nat(fdeg);
+      for (size_t i = 0; i < fdeg; i++) nat[bitrev32((uint32_t)i, lf)] = fin[i];
+      host_idft_ext(nat);
+      std::reverse(nat.begin(), nat.end());
+      tx.send_fields(reinterpret_cast<const uint64_t*>(nat.data()), 2 * fdeg);
+    }
+    ctx.release(cur);
+    cur = nullptr;
+
+    // ---- query phase
+    tx.grind(p.query_pow_bits);
+    std::vector<uint64_t> idx(p.num_queries);
+    for (uint32_t q = 0; q < p.num_queries; q++) idx[q] = bitrev32((uint32_t)tx.sample_bits(logL), logL);
+    for (auto* t : trees) tree_prove_batch(*t, idx, tx, s);
+    std::vector<uint64_t> ri(idx.size());
+    for (size_t r = 0; r < ftrees.size(); r++) {
+      unsigned sh = la * (unsigned)(r + 1);
+      for (size_t q = 0; q < idx.size(); q++) ri[q] = idx[q] >> sh;
+      tree_prove_batch(*ftrees[r], ri, tx, s);
+    }
+  } catch (...) {
+    for (auto* t : ftrees) delete t;
+    for (auto* b : bufs) ctx.release(b);
+    if (cur) ctx.release(cur);
+    throw;
+  }
+  for (auto* t : ftrees) delete t;
+  for (auto* b : bufs) ctx.release(b);
+}
+
+}  // namespace p3g

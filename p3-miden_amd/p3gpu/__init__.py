@@ -1,0 +1,489 @@
+"""Python host mirror of the reference's lifted-PCS interfaces, over the C ABI
+of libp3gpu.so (include/p3gpu.h).
+
+The names follow the reference crates so that callers and tests read like the
+reference's own code:
+
+* ``Dft.coset_lde_batch``            — p3-dft TwoAdicSubgroupDft (commit.rs:217-221)
+* ``LmcsConfig.build_tree`` / ``build_aligned_tree`` / ``hash`` / ``compress``
+                                     — p3-miden-lmcs/src/lmcs.rs:102-149
+* ``LiftedMerkleTree.root/height/rows/widths/prove_batch``
+                                     — p3-miden-lmcs/src/lifted_tree.rs:102-224
+* ``ProverTranscript``               — p3-miden-transcript/src/prover.rs:15-145
+* ``PcsParams``                      — p3-miden-lifted-fri/src/params.rs:170-204
+* ``open_with_channel``              — p3-miden-lifted-fri/src/prover.rs:35-116
+* ``commit_traces``                  — p3-miden-lifted-stark/src/prover/commit.rs:175-228
+
+Every computation runs in the HIP library; there is no CPU fallback.  If the
+library is missing or no GPU is visible, the constructors raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(_PKG, "libp3gpu.so")
+
+P = 0xFFFFFFFF00000001
+GENERATOR = 7
+HASH_POSEIDON2 = 0
+HASH_KECCAK = 1
+CHALLENGER_DUPLEX_P2 = 0
+CHALLENGER_KECCAK = 1
+DEVICE_PTRS = 1
+BITREV_ROWS = 2
+TAKE_OWNERSHIP = 4
+
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+
+
+class P3GError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        self.code = code
+        detail = _lib.p3g_last_error().decode() if _lib is not None else ""
+        msg = _lib.p3g_status_string(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{where}: {msg} ({code}){': ' + detail if detail else ''}")
+
+
+class _Matrix(C.Structure):
+    _fields_ = [("data", _u64p), ("height", C.c_uint64), ("width", C.c_uint64)]
+
+
+class PcsParams(C.Structure):
+    """PcsParams::new(log_blowup, log_folding_arity, log_final_degree, folding_pow_bits,
+    deep_pow_bits, num_queries, query_pow_bits) with the reference's validation
+    (p3-miden-lifted-fri/src/params.rs:170-204)."""
+
+    _fields_ = [(n, C.c_uint32) for n in ("log_blowup", "log_folding_arity", "log_final_degree",
+                                           "folding_pow_bits", "deep_pow_bits", "num_queries", "query_pow_bits")]
+
+    def __init__(self, log_blowup, log_folding_arity, log_final_degree, folding_pow_bits=0, deep_pow_bits=0,
+                 num_queries=1, query_pow_bits=0):
+        if log_folding_arity not in (1, 2, 3):
+            raise ValueError(f"invalid folding arity: log_arity {log_folding_arity} (must be 1, 2, or 3)")
+        if log_blowup == 0:
+            raise ValueError("log_blowup must be > 0")
+        if log_final_degree + log_blowup > 64:
+            raise ValueError("log_final_degree + log_blowup exceeds 64")
+        if num_queries == 0:
+            raise ValueError("num_queries must be > 0")
+        super().__init__(log_blowup, log_folding_arity, log_final_degree, folding_pow_bits, deep_pow_bits,
+                         num_queries, query_pow_bits)
+
+    def num_rounds(self, log_domain: int) -> int:
+        lmf = self.log_final_degree + self.log_blowup
+        diff = max(0, log_domain - lmf)
+        return -(-diff // self.log_folding_arity)
+
+    def final_poly_degree(self, log_domain: int) -> int:
+        lfs = log_domain - self.num_rounds(log_domain) * self.log_folding_arity
+        return 1 << max(0, lfs - self.log_blowup)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        return None
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "p3g_status_string": (C.c_char_p, [C.c_int]),
+        "p3g_last_error": (C.c_char_p, []),
+        "p3g_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+        "p3g_ctx_destroy": (None, [_vp]),
+        "p3g_ctx_synchronize": (C.c_int, [_vp]),
+        "p3g_ctx_trim": (C.c_int, [_vp]),
+        "p3g_profile_enable": (C.c_int, [_vp, C.c_int]),
+        "p3g_profile_reset": (C.c_int, [_vp]),
+        "p3g_profile_read": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), _u64p]),
+        "p3g_alloc": (C.c_int, [_vp, C.c_uint64, C.POINTER(_vp)]),
+        "p3g_free": (C.c_int, [_vp, _vp]),
+        "p3g_upload": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
+        "p3g_download": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
+        "p3g_fill_random": (C.c_int, [C.c_uint64, C.c_uint64, _u64p]),
+        "p3g_coset_lde_batch": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, _u64p]),
+        "p3g_lmcs_alignment": (C.c_uint32, [C.c_uint32]),
+        "p3g_lmcs_build": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+        "p3g_lmcs_hash": (C.c_int, [C.c_uint32, _u64p, _u64p, C.c_uint64, _u64p]),
+        "p3g_lmcs_compress": (C.c_int, [C.c_uint32, _u64p, _u64p, _u64p]),
+        "p3g_tree_root": (C.c_int, [_vp, _u64p]),
+        "p3g_tree_height": (C.c_uint64, [_vp]),
+        "p3g_tree_alignment": (C.c_uint32, [_vp]),
+        "p3g_tree_num_matrices": (C.c_uint64, [_vp]),
+        "p3g_tree_widths": (C.c_int, [_vp, _u64p]),
+        "p3g_tree_rows": (C.c_int, [_vp, C.c_uint64, _u64p, C.c_uint64, _u64p]),
+        "p3g_tree_layer": (C.c_int, [_vp, C.c_uint32, _u64p]),
+        "p3g_tree_matrix": (C.c_int, [_vp, C.c_uint64, C.POINTER(_vp), _u64p, _u64p]),
+        "p3g_tree_prove_batch": (C.c_int, [_vp, _u64p, C.c_uint64, _vp]),
+        "p3g_tree_free": (None, [_vp]),
+        "p3g_commit_traces": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+        "p3g_transcript_create": (C.c_int, [C.c_uint32, C.POINTER(_vp)]),
+        "p3g_transcript_free": (None, [_vp]),
+        "p3g_transcript_clone": (C.c_int, [_vp, C.POINTER(_vp)]),
+        "p3g_transcript_observe": (C.c_int, [_vp, _u64p, C.c_uint64]),
+        "p3g_transcript_observe_commitment": (C.c_int, [_vp, _u64p]),
+        "p3g_transcript_sample": (C.c_int, [_vp, _u64p]),
+        "p3g_transcript_sample_ext": (C.c_int, [_vp, _u64p]),
+        "p3g_transcript_sample_bits": (C.c_int, [_vp, C.c_uint32, _u64p]),
+        "p3g_transcript_send_fields": (C.c_int, [_vp, _u64p, C.c_uint64]),
+        "p3g_transcript_send_commitment": (C.c_int, [_vp, _u64p]),
+        "p3g_transcript_hint_fields": (C.c_int, [_vp, _u64p, C.c_uint64]),
+        "p3g_transcript_hint_commitment": (C.c_int, [_vp, _u64p]),
+        "p3g_transcript_grind": (C.c_int, [_vp, C.c_uint32, _u64p]),
+        "p3g_transcript_num_fields": (C.c_uint64, [_vp]),
+        "p3g_transcript_num_commitments": (C.c_uint64, [_vp]),
+        "p3g_transcript_data": (C.c_int, [_vp, _u64p, _u64p]),
+        "p3g_open_with_channel": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, _vp]),
+        "p3g_prove": (C.c_int, [_vp, C.c_uint32, C.POINTER(PcsParams), C.POINTER(_Matrix), C.c_uint64, C.c_uint32, _vp, _u64p, _u64p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = _load()
+
+
+def lib():
+    if _lib is None:
+        raise ImportError(f"libp3gpu.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    return _lib
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != 0:
+        raise P3GError(rc, where)
+
+
+def u64(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_u64p)
+
+
+def fill_random(seed: int, n: int) -> np.ndarray:
+    """RowMajorMatrix::rand values from SmallRng::seed_from_u64(seed), flat."""
+    out = np.empty(n, np.uint64)
+    _check(lib().p3g_fill_random(seed, n, _p(out)), "fill_random")
+    return out
+
+
+class Device:
+    """One context per GPU: stream, constants, cached tables and device memory."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().p3g_ctx_create(device, C.byref(h)), "ctx_create")
+        self.h = h
+        self.index = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().p3g_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().p3g_ctx_synchronize(self.h), "synchronize")
+
+    def trim(self):
+        _check(lib().p3g_ctx_trim(self.h), "trim")
+
+    def profile(self, on: bool = True):
+        """Record HIP events around every launch of the named kernel families."""
+        _check(lib().p3g_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_reset(self):
+        _check(lib().p3g_profile_reset(self.h), "profile_reset")
+
+    def profile_read(self, name: str):
+        """(total milliseconds, launches) of one kernel family since the last reset."""
+        ms = C.c_double(0)
+        n = np.zeros(1, np.uint64)
+        _check(lib().p3g_profile_read(self.h, name.encode(), C.byref(ms), _p(n)), "profile_read")
+        return float(ms.value), int(n[0])
+
+    def upload(self, arr: np.ndarray) -> "DeviceMatrix":
+        a = u64(arr)
+        ptr = _vp()
+        _check(lib().p3g_alloc(self.h, max(8, a.nbytes), C.byref(ptr)), "alloc")
+        _check(lib().p3g_upload(self.h, ptr, a.ctypes.data_as(_vp), a.nbytes), "upload")
+        shape = a.shape if a.ndim == 2 else (a.size, 1)
+        return DeviceMatrix(self, ptr, shape[0], shape[1])
+
+
+class DeviceMatrix:
+    """A row-major matrix resident in HBM (owned by this object unless handed to a tree)."""
+
+    def __init__(self, dev: Device, ptr, height: int, width: int, owned: bool = True):
+        self.dev, self.ptr, self.height, self.width, self.owned = dev, ptr, height, width, owned
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.height * self.width, np.uint64)
+        _check(lib().p3g_download(self.dev.h, out.ctypes.data_as(_vp), self.ptr, out.nbytes), "download")
+        return out.reshape(self.height, self.width)
+
+    def free(self):
+        if self.owned and self.ptr:
+            _check(lib().p3g_free(self.dev.h, self.ptr), "free")
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _matrices(mats):
+    """Build a p3g_matrix array; returns (array, keepalive, device_flag)."""
+    arr = (_Matrix * max(1, len(mats)))()
+    keep = []
+    dev = None
+    for i, m in enumerate(mats):
+        if isinstance(m, DeviceMatrix):
+            isdev = True
+            arr[i] = _Matrix(C.cast(m.ptr, _u64p), m.height, m.width)
+        else:
+            isdev = False
+            a = u64(m)
+            if a.ndim != 2:
+                raise ValueError("matrices must be 2-D")
+            keep.append(a)
+            arr[i] = _Matrix(_p(a), a.shape[0], a.shape[1])
+        if dev is None:
+            dev = isdev
+        elif dev != isdev:
+            raise ValueError("mix of host and device matrices")
+    return arr, keep, bool(dev)
+
+
+class Dft:
+    """TwoAdicSubgroupDft on the GPU."""
+
+    def __init__(self, device: Device):
+        self.dev = device
+
+    def coset_lde_batch(self, mat, added_bits: int, shift: int, bit_reverse_rows: bool = True) -> np.ndarray:
+        a = u64(mat)
+        n, w = a.shape
+        out = np.empty((n << added_bits) * w, np.uint64)
+        flags = BITREV_ROWS if bit_reverse_rows else 0
+        _check(lib().p3g_coset_lde_batch(self.dev.h, _p(a), n, w, added_bits, shift, flags, _p(out)), "coset_lde_batch")
+        return out.reshape(n << added_bits, w)
+
+
+class ProverTranscript:
+    """ProverTranscript over DuplexChallenger (Poseidon2) or the Keccak challenger."""
+
+    def __init__(self, kind: int = CHALLENGER_DUPLEX_P2, _h=None):
+        self.kind = kind
+        if _h is None:
+            _h = _vp()
+            _check(lib().p3g_transcript_create(kind, C.byref(_h)), "transcript_create")
+        self.h = _h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().p3g_transcript_free(self.h)
+            self.h = None
+
+    def clone(self) -> "ProverTranscript":
+        h = _vp()
+        _check(lib().p3g_transcript_clone(self.h, C.byref(h)), "transcript_clone")
+        return ProverTranscript(self.kind, h)
+
+    def observe(self, values):
+        a = u64(np.atleast_1d(values))
+        _check(lib().p3g_transcript_observe(self.h, _p(a), a.size), "observe")
+
+    def observe_commitment(self, c):
+        a = u64(c)
+        _check(lib().p3g_transcript_observe_commitment(self.h, _p(a)), "observe_commitment")
+
+    def sample(self) -> int:
+        o = np.zeros(1, np.uint64)
+        _check(lib().p3g_transcript_sample(self.h, _p(o)), "sample")
+        return int(o[0])
+
+    def sample_algebra_element(self):
+        o = np.zeros(2, np.uint64)
+        _check(lib().p3g_transcript_sample_ext(self.h, _p(o)), "sample_ext")
+        return (int(o[0]), int(o[1]))
+
+    def sample_bits(self, bits: int) -> int:
+        o = np.zeros(1, np.uint64)
+        _check(lib().p3g_transcript_sample_bits(self.h, bits, _p(o)), "sample_bits")
+        return int(o[0])
+
+    def send_field_slice(self, values):
+        a = u64(values)
+        _check(lib().p3g_transcript_send_fields(self.h, _p(a), a.size), "send_fields")
+
+    def send_commitment(self, c):
+        a = u64(c)
+        _check(lib().p3g_transcript_send_commitment(self.h, _p(a)), "send_commitment")
+
+    def hint_field_slice(self, values):
+        a = u64(values)
+        _check(lib().p3g_transcript_hint_fields(self.h, _p(a), a.size), "hint_fields")
+
+    def hint_commitment(self, c):
+        a = u64(c)
+        _check(lib().p3g_transcript_hint_commitment(self.h, _p(a)), "hint_commitment")
+
+    def grind(self, bits: int) -> int:
+        o = np.zeros(1, np.uint64)
+        _check(lib().p3g_transcript_grind(self.h, bits, _p(o)), "grind")
+        return int(o[0])
+
+    def data(self):
+        """TranscriptData: (fields [n], commitments [m, 4])."""
+        nf = lib().p3g_transcript_num_fields(self.h)
+        nc = lib().p3g_transcript_num_commitments(self.h)
+        f = np.zeros(max(1, nf), np.uint64)
+        c = np.zeros(max(1, 4 * nc), np.uint64)
+        _check(lib().p3g_transcript_data(self.h, _p(f), _p(c)), "transcript_data")
+        return f[:nf].copy(), c[: 4 * nc].reshape(nc, 4).copy()
+
+
+def proof_bytes(fields: np.ndarray, commitments: np.ndarray) -> bytes:
+    """Canonical proof encoding: field stream then commitment stream, each as little-endian u64."""
+    return u64(fields).astype("<u8").tobytes() + u64(commitments).ravel().astype("<u8").tobytes()
+
+
+class LiftedMerkleTree:
+    def __init__(self, h, hash_kind: int):
+        self.h = h
+        self.hash_kind = hash_kind
+
+    def __del__(self):
+        self.free()
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().p3g_tree_free(self.h)
+            self.h = None
+
+    def root(self) -> np.ndarray:
+        o = np.zeros(4, np.uint64)
+        _check(lib().p3g_tree_root(self.h, _p(o)), "tree_root")
+        return o
+
+    def height(self) -> int:
+        return int(lib().p3g_tree_height(self.h))
+
+    def alignment(self) -> int:
+        return int(lib().p3g_tree_alignment(self.h))
+
+    def widths(self):
+        n = lib().p3g_tree_num_matrices(self.h)
+        o = np.zeros(max(1, n), np.uint64)
+        _check(lib().p3g_tree_widths(self.h, _p(o)), "tree_widths")
+        return [int(x) for x in o[:n]]
+
+    def rows(self, index: int) -> np.ndarray:
+        ln = np.zeros(1, np.uint64)
+        _check(lib().p3g_tree_rows(self.h, index, None, 0, _p(ln)), "tree_rows")
+        o = np.zeros(max(1, int(ln[0])), np.uint64)
+        _check(lib().p3g_tree_rows(self.h, index, _p(o), o.size, _p(ln)), "tree_rows")
+        return o[: int(ln[0])]
+
+    def layer(self, l: int) -> np.ndarray:
+        n = self.height() >> l
+        o = np.zeros(4 * n, np.uint64)
+        _check(lib().p3g_tree_layer(self.h, l, _p(o)), "tree_layer")
+        return o.reshape(n, 4)
+
+    def matrix(self, i: int):
+        ptr = _vp()
+        h = np.zeros(1, np.uint64)
+        w = np.zeros(1, np.uint64)
+        _check(lib().p3g_tree_matrix(self.h, i, C.byref(ptr), _p(h), _p(w)), "tree_matrix")
+        return ptr, int(h[0]), int(w[0])
+
+    def prove_batch(self, indices, channel: ProverTranscript):
+        a = u64(list(indices))
+        _check(lib().p3g_tree_prove_batch(self.h, _p(a) if a.size else None, a.size, channel.h), "prove_batch")
+
+
+class LmcsConfig:
+    """LmcsConfig over Poseidon2-Goldilocks-12 or the u64-serialising Keccak sponge."""
+
+    def __init__(self, device: Device, hash_kind: int = HASH_POSEIDON2):
+        self.dev = device
+        self.hash_kind = hash_kind
+
+    def alignment(self) -> int:
+        return int(lib().p3g_lmcs_alignment(self.hash_kind))
+
+    def _build(self, mats, alignment, take_ownership=False):
+        arr, keep, dev = _matrices(mats)
+        flags = (DEVICE_PTRS if dev else 0) | (TAKE_OWNERSHIP if (dev and take_ownership) else 0)
+        h = _vp()
+        _check(lib().p3g_lmcs_build(self.dev.h, self.hash_kind, arr, len(mats), alignment, flags, C.byref(h)), "lmcs_build")
+        if dev and take_ownership:
+            for m in mats:
+                m.owned = False
+        return LiftedMerkleTree(h, self.hash_kind)
+
+    def build_tree(self, mats, take_ownership=False) -> LiftedMerkleTree:
+        return self._build(mats, 1, take_ownership)
+
+    def build_aligned_tree(self, mats, take_ownership=False) -> LiftedMerkleTree:
+        return self._build(mats, self.alignment(), take_ownership)
+
+    def hash(self, rows) -> np.ndarray:
+        rows = [u64(r).ravel() for r in rows]
+        flat = u64(np.concatenate(rows)) if rows else np.zeros(1, np.uint64)
+        ws = u64([r.size for r in rows]) if rows else np.zeros(1, np.uint64)
+        o = np.zeros(4, np.uint64)
+        _check(lib().p3g_lmcs_hash(self.hash_kind, _p(flat), _p(ws), len(rows), _p(o)), "lmcs_hash")
+        return o
+
+    def compress(self, left, right) -> np.ndarray:
+        a, b, o = u64(left), u64(right), np.zeros(4, np.uint64)
+        _check(lib().p3g_lmcs_compress(self.hash_kind, _p(a), _p(b), _p(o)), "lmcs_compress")
+        return o
+
+
+def commit_traces(lmcs: LmcsConfig, traces, log_blowup: int) -> LiftedMerkleTree:
+    arr, keep, dev = _matrices(traces)
+    h = _vp()
+    _check(lib().p3g_commit_traces(lmcs.dev.h, lmcs.hash_kind, arr, len(traces), log_blowup,
+                                   DEVICE_PTRS if dev else 0, C.byref(h)), "commit_traces")
+    return LiftedMerkleTree(h, lmcs.hash_kind)
+
+
+def open_with_channel(params: PcsParams, lmcs: LmcsConfig, log_lde_height: int, eval_points, trace_trees,
+                      channel: ProverTranscript) -> None:
+    z = u64(np.asarray(eval_points, dtype=np.uint64).ravel())
+    hs = (_vp * len(trace_trees))(*[t.h for t in trace_trees])
+    _check(lib().p3g_open_with_channel(lmcs.dev.h, C.byref(params), log_lde_height, _p(z), z.size // 2, hs,
+                                       len(trace_trees), channel.h), "open_with_channel")
+
+
+def prove(lmcs: LmcsConfig, params: PcsParams, traces, channel: ProverTranscript = None):
+    """pcs_trace flow: commit_traces, observe root, sample two points, open.
+    Returns (root, eval_points, fields, commitments)."""
+    if channel is None:
+        channel = ProverTranscript(CHALLENGER_DUPLEX_P2 if lmcs.hash_kind == HASH_POSEIDON2 else CHALLENGER_KECCAK)
+    arr, keep, dev = _matrices(traces)
+    root = np.zeros(4, np.uint64)
+    z = np.zeros(4, np.uint64)
+    _check(lib().p3g_prove(lmcs.dev.h, lmcs.hash_kind, C.byref(params), arr, len(traces),
+                           DEVICE_PTRS if dev else 0, channel.h, _p(root), _p(z)), "prove")
+    fields, comms = channel.data()
+    return root, ((int(z[0]), int(z[1])), (int(z[2]), int(z[3]))), fields, comms

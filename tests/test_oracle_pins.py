@@ -1,0 +1,471 @@
+"""Pins of the CPU oracle against what the reference and the mathematics fix.
+
+Nothing here compares the oracle with itself: every check is a closed form,
+a brute-force computation in Python big integers, a published test vector, a
+known-answer test taken from the reference's own tests, or a library routine
+(hashlib's SHA3).  Each test names the reference test or fact it restates.
+"""
+import hashlib
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+P = O.P
+
+
+# ---------------------------------------------------------------- field (closed forms)
+def test_field_ops_match_bigint():
+    rng = random.Random(1)
+    edge = [0, 1, 2, P - 1, P - 2, 2**32, 2**32 - 1, 2**63, P >> 1]
+    vals = edge + [rng.randrange(P) for _ in range(200)]
+    for a in vals:
+        for b in edge + [rng.randrange(P) for _ in range(5)]:
+            assert O.gl_mul(a, b) == a * b % P
+            assert O.gl_add(a, b) == (a + b) % P
+            assert O.gl_sub(a, b) == (a - b) % P
+        if a:
+            assert O.gl_inv(a) == pow(a, P - 2, P)
+
+
+def test_two_adic_generators_closed_form():
+    # omega_32 = 7^((p-1)/2^32) has order exactly 2^32; omega_k = omega_32^(2^(32-k))
+    w32 = pow(7, (P - 1) >> 32, P)
+    assert w32 == 1753635133440165772
+    assert pow(w32, 1 << 32, P) == 1 and pow(w32, 1 << 31, P) != 1
+    for k in range(0, 33):
+        assert O.two_adic_generator(k) == pow(w32, 1 << (32 - k), P)
+
+
+def test_seven_generates_multiplicative_group():
+    # p - 1 = 2^32 * 3 * 5 * 17 * 257 * 65537
+    assert (1 << 32) * 3 * 5 * 17 * 257 * 65537 == P - 1
+    for q in (2, 3, 5, 17, 257, 65537):
+        assert pow(7, (P - 1) // q, P) != 1
+
+
+def _ef_mul(a, b):
+    return ((a[0] * b[0] + 7 * a[1] * b[1]) % P, (a[0] * b[1] + a[1] * b[0]) % P)
+
+
+def test_extension_field():
+    # x^2 - 7 is irreducible: 7 is a quadratic non-residue
+    assert pow(7, (P - 1) // 2, P) == P - 1
+    rng = random.Random(2)
+    for _ in range(100):
+        a = (rng.randrange(P), rng.randrange(P))
+        b = (rng.randrange(P), rng.randrange(P))
+        assert O.ef_mul(a, b) == _ef_mul(a, b)
+        if a != (0, 0):
+            assert _ef_mul(O.ef_inv(a), a) == (1, 0)
+
+
+# ---------------------------------------------------------------- generator (published vectors)
+def test_xoshiro256pp_reference_vector():
+    # Xoshiro256++ from state (1, 2, 3, 4): values of the reference C implementation
+    # (as checked by rand_xoshiro's `reference` test).  Derived by hand for the
+    # first two: rotl(1+4, 23) + 1 = 41943041, then 58720359.
+    import ctypes as C
+
+    class Rng(C.Structure):
+        _fields_ = [("s", C.c_uint64 * 4)]
+
+    r = Rng()
+    r.s[:] = [1, 2, 3, 4]
+    O.lib.orc_rng_next.restype = C.c_uint64
+    O.lib.orc_rng_next.argtypes = [C.POINTER(Rng)]
+    expected = [41943041, 58720359, 3588806011781223, 3591011842654386, 9228616714210784205,
+                9973669472204895162, 14011001112246962877, 12406186145184390807, 15849039046786891736,
+                10450023813501588000]
+    got = [int(O.lib.orc_rng_next(C.byref(r))) for _ in expected]
+    assert got == expected
+
+
+def test_splitmix_seeding_and_rejection_sampling():
+    # seed_from_u64: four SplitMix64 outputs; Goldilocks samples reject u64 >= p
+    def splitmix(seed, n):
+        out, st = [], seed
+        for _ in range(n):
+            st = (st + 0x9E3779B97F4A7C15) % 2**64
+            z = st
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) % 2**64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) % 2**64
+            out.append(z ^ (z >> 31))
+        return out
+
+    def rotl(x, k):
+        return ((x << k) | (x >> (64 - k))) % 2**64
+
+    s = splitmix(2025, 4)
+    vals = []
+    while len(vals) < 64:
+        res = (rotl((s[0] + s[3]) % 2**64, 23) + s[0]) % 2**64
+        t = (s[1] << 17) % 2**64
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45)
+        if res < P:
+            vals.append(res)
+    assert O.rand_matrix(2025, 8, 8).ravel().tolist() == vals
+
+
+# ---------------------------------------------------------------- DFT (brute force)
+def _naive_dft(v, root):
+    n = len(v)
+    return [sum(v[k] * pow(root, i * k, P) for k in range(n)) % P for i in range(n)]
+
+
+@pytest.mark.parametrize("logn", [0, 1, 2, 3, 5])
+def test_ntt_matches_naive(logn):
+    n = 1 << logn
+    rng = random.Random(logn)
+    v = [rng.randrange(P) for _ in range(n)]
+    w = pow(7, (P - 1) >> logn, P) if logn else 1
+    assert O.ntt(v).tolist() == _naive_dft(v, O.two_adic_generator(logn))
+    winv = pow(O.two_adic_generator(logn), P - 2, P)
+    ninv = pow(n, P - 2, P)
+    assert O.ntt(v, inverse=True).tolist() == [x * ninv % P for x in _naive_dft(v, winv)]
+    del w
+
+
+@pytest.mark.parametrize("logn,bits,w", [(0, 1, 2), (1, 1, 3), (2, 2, 1), (3, 1, 4), (4, 3, 2)])
+def test_coset_lde_matches_interpolation(logn, bits, w):
+    # coset_lde_batch + bit_reverse_rows == evaluations of the interpolant at
+    # shift * omega_{nB}^{bitrev(i)} (commit.rs:217-221), by brute force.
+    n = 1 << logn
+    rng = random.Random(100 + logn)
+    trace = np.array([[rng.randrange(P) for _ in range(w)] for _ in range(n)], dtype=np.uint64)
+    shift = pow(7, 1 + logn, P)
+    lde = O.coset_lde_batch(trace, bits, shift)
+    N = n << bits
+    om_n_inv = pow(O.two_adic_generator(logn), P - 2, P)
+    om_N = O.two_adic_generator(logn + bits)
+    ninv = pow(n, P - 2, P)
+    for c in range(w):
+        col = [int(x) for x in trace[:, c]]
+        coeffs = [x * ninv % P for x in _naive_dft(col, om_n_inv)]
+        for r in range(N):
+            i = int(format(r, f"0{logn + bits}b")[::-1], 2) if logn + bits else 0
+            x = shift * pow(om_N, i, P) % P
+            assert int(lde[r, c]) == sum(ck * pow(x, k, P) for k, ck in enumerate(coeffs)) % P
+
+
+def test_bit_reverse_rows_kat():
+    # commit.rs:269-290: an 8x2 matrix with row i = [2i, 2i+1]; natural row i
+    # holds the bit-reversed row bitrev(i)
+    m = np.arange(16, dtype=np.uint64).reshape(8, 2)
+    br = O.bit_reverse_rows(m)
+    for i in range(8):
+        b = int(format(i, "03b")[::-1], 2)
+        assert br[i].tolist() == [2 * b, 2 * b + 1]
+
+
+# ---------------------------------------------------------------- sponge (reference KATs)
+def test_stateful_sponge_mock_kat():
+    # field_sponge.rs:203-230: WIDTH 4, RATE 2, mock permutation, input [1..5] -> [327, 327]
+    O.mock_config(4, 2)
+    st = O.sponge_absorb(O.HASH_MOCK, np.zeros(4, np.uint64), [1, 2, 3, 4, 5])
+    assert st[:2].tolist() == [327, 327]
+
+
+@pytest.mark.parametrize("width,rate", [(4, 2), (6, 3), (8, 4)])
+def test_sponge_implicit_padding_equals_explicit_zeros(width, rate):
+    # field_sponge.rs:232-266
+    O.mock_config(width, rate)
+    for n in range(1, 3 * rate + 1):
+        inp = list(range(1, n + 1))
+        pad = inp + [0] * ((-n) % rate)
+        a = O.sponge_absorb(O.HASH_MOCK, np.zeros(width, np.uint64), inp)
+        b = O.sponge_absorb(O.HASH_MOCK, np.zeros(width, np.uint64), pad)
+        assert a[:2].tolist() == b[:2].tolist()
+
+
+def test_hash_alignments():
+    # build_aligned_tree alignment: RATE for the field sponge; the u64-serialising
+    # Keccak sponge uses lcm(field_bytes, 8*17)/field_bytes = 17 (serializing_sponge.rs:166-203)
+    from math import gcd
+    assert O.hash_alignment(O.HASH_P2) == 8
+    assert O.hash_alignment(O.HASH_KECCAK) == (8 * 17) // gcd(8, 8 * 17)
+
+
+# ---------------------------------------------------------------- Keccak (library routine)
+def _sha3_block(msg: bytes):
+    rate = 136
+    blk = bytearray(msg) + bytes(rate - len(msg))
+    blk[len(msg)] ^= 0x06
+    blk[rate - 1] ^= 0x80
+    st = np.zeros(25, np.uint64)
+    for i in range(rate // 8):
+        st[i] = int.from_bytes(blk[8 * i: 8 * i + 8], "little")
+    return st
+
+
+@pytest.mark.parametrize("msg", [b"", b"abc", bytes(range(100)), b"x" * 135])
+def test_keccak_f_matches_hashlib_sha3(msg):
+    st = O.keccakf(_sha3_block(msg))
+    digest = b"".join(int(st[i]).to_bytes(8, "little") for i in range(4))
+    assert digest == hashlib.sha3_256(msg).digest()
+
+
+def test_keccak256_empty_vector():
+    # Keccak-256 with the original 0x01 padding: the well-known empty-string digest
+    assert O.keccak256(b"").hex() == "c5d2460186f7233c927e7db2dcc703c0e500b653ca82273b7bfad8045d85a470"
+    # multi-block input: 136-byte message needs a second (padding) block
+    assert len(O.keccak256(b"a" * 136)) == 32 and O.keccak256(b"a" * 136) != O.keccak256(b"a" * 135)
+
+
+# ---------------------------------------------------------------- Poseidon2 structure
+def test_poseidon2_linear_layers_structure():
+    # external layer: M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] per block, then
+    # circ(2 M4, M4, M4); internal layer: diag(d) + all-ones (Poseidon2 paper, Plonky3 layout).
+    M4 = [[2, 3, 1, 1], [1, 2, 3, 1], [1, 1, 2, 3], [3, 1, 1, 2]]
+    rng = random.Random(5)
+    x = [rng.randrange(P) for _ in range(12)]
+    ext = np.array(x, dtype=np.uint64)
+    O.lib.orc_p2_external_linear(O._p(ext))
+    Mext = [[(2 if (i // 4) == (j // 4) else 1) * M4[i % 4][j % 4] for j in range(12)] for i in range(12)]
+    assert ext.tolist() == [sum(Mext[i][j] * x[j] for j in range(12)) % P for i in range(12)]
+    d = [int(v) for v in O.p2_diag()]
+    inn = np.array(x, dtype=np.uint64)
+    O.lib.orc_p2_internal_linear(O._p(inn))
+    assert inn.tolist() == [(d[i] * x[i] + sum(x)) % P for i in range(12)]
+
+
+def test_poseidon2_round_counts_and_constant_draw_order():
+    # new_from_rng_128 draws 4 x [F;12] initial, 4 x [F;12] terminal, 22 x F internal
+    # constants from SmallRng(2025), in that order (parity unpinned: Plonky3 is not vendored).
+    c = O.p2_constants()
+    assert c.size == 118
+    assert c.tolist() == O.rand_matrix(2025, 1, 118).ravel().tolist()
+    assert all(int(v) < P for v in c)
+
+
+# ---------------------------------------------------------------- LMCS structure
+def _scenarios(rate, pack=4):
+    return [
+        [(1, 1)], [(1, rate - 1)], [(2, 3), (4, 5), (8, rate)], [(1, 5), (1, 3), (2, 7), (4, 1), (8, rate + 1)],
+        [(pack // 2, rate - 1), (pack, rate), (pack * 2, rate + 3)], [(pack, rate + 5), (pack * 2, 25)],
+        [(1, rate * 2), (pack // 2, rate * 2 - 1), (pack, rate * 2), (pack * 2, rate * 3 - 2)],
+        [(4, rate - 1), (4, rate), (8, rate + 3), (8, rate * 2)], [(pack * 2, rate - 1)],
+    ]
+
+
+@pytest.mark.parametrize("kind", [O.HASH_P2, O.HASH_KECCAK])
+def test_lifted_equals_upsampled_equals_concatenated(kind):
+    # lifted_tree.rs:587-617 (fixtures.rs:54-81 matrix_scenarios)
+    rate = 8 if kind == O.HASH_P2 else 17
+    seed = 42
+    for sc in _scenarios(rate):
+        mats = []
+        for (h, w) in sc:
+            mats.append(O.rand_matrix(seed, h, w))
+            seed += 1
+        H = sc[-1][0]
+        t_lift = O.Tree(kind, mats)
+        ups = [np.repeat(m, H // m.shape[0], axis=0) for m in mats]
+        t_up = O.Tree(kind, ups)
+        assert np.array_equal(t_lift.layer(0), t_up.layer(0))
+        cat = np.concatenate([np.pad(u, ((0, 0), (0, (-u.shape[1]) % rate))) for u in ups], axis=1)
+        t_cat = O.Tree(kind, [cat])
+        assert np.array_equal(t_lift.layer(0), t_cat.layer(0))
+        assert np.array_equal(t_lift.root(), t_cat.root())
+
+
+def test_tree_layers_are_pairwise_compressions():
+    mats = [O.rand_matrix(7, 16, 5)]
+    t = O.Tree(O.HASH_P2, mats)
+    for r in range(16):
+        assert t.layer(0)[r].tolist() == O.hash_rows(O.HASH_P2, [mats[0][r]]).tolist()
+    for l in range(1, 5):
+        prev, cur = t.layer(l - 1), t.layer(l)
+        for i in range(cur.shape[0]):
+            assert cur[i].tolist() == O.compress(O.HASH_P2, prev[2 * i], prev[2 * i + 1]).tolist()
+    # TruncatedPermutation: parent = perm(l || r || 0)[0..4]
+    l, r = prev[0], prev[1]
+    st = O.p2_permute(np.concatenate([l, r, np.zeros(4, np.uint64)]))
+    assert st[:4].tolist() == cur[0].tolist()
+
+
+def test_alignment_changes_widths_not_root():
+    # tests.rs:279-321
+    mats = [O.rand_matrix(3, 4, 3), O.rand_matrix(4, 8, 5)]
+    a = O.Tree(O.HASH_P2, mats, alignment=1)
+    b = O.Tree(O.HASH_P2, mats, alignment=8)
+    assert np.array_equal(a.root(), b.root())
+    assert a.rows(5).size == 8 and b.rows(5).size == 16
+
+
+def test_lmcs_roundtrip_duplicates_and_rejections():
+    # tests.rs:123-201, 241-277
+    for seed, shapes, nq in [(1, [(8, 4)], 1), (42, [(4, 3), (8, 5), (16, 7)], 4), (99, [(32, 2)], 8)]:
+        rng = random.Random(seed)
+        mats = [O.rand_matrix(seed + i, h, w) for i, (h, w) in enumerate(shapes)]
+        t = O.Tree(O.HASH_P2, mats)
+        H = t.height()
+        idx = [rng.randrange(H) for _ in range(nq)] + ([rng.randrange(H)] * 2)
+        tx = O.Transcript()
+        t.prove_batch(idx, tx)
+        f, c = tx.data()
+        v = O.Transcript()
+        v.load(f, c)
+        rc, rows = O.open_batch(O.HASH_P2, t.root(), [w for _, w in shapes], H.bit_length() - 1, idx, v)
+        assert rc == 0
+        for k, i in enumerate(sorted(set(idx))):
+            assert rows[k].tolist() == t.rows(i).tolist()
+        assert v.is_empty()
+        # empty indices and out-of-range indices are invalid
+        v2 = O.Transcript(); v2.load(f, c)
+        assert O.open_batch(O.HASH_P2, t.root(), [w for _, w in shapes], H.bit_length() - 1, [], v2)[0] == 1
+        v3 = O.Transcript(); v3.load(f, c)
+        assert O.open_batch(O.HASH_P2, t.root(), [w for _, w in shapes], H.bit_length() - 1, [H], v3)[0] != 0
+        # tampering with an opened value changes the root
+        f2 = f.copy(); f2[0] = (int(f2[0]) + 1) % P
+        v4 = O.Transcript(); v4.load(f2, c)
+        assert O.open_batch(O.HASH_P2, t.root(), [w for _, w in shapes], H.bit_length() - 1, idx, v4)[0] == 2
+
+
+# ---------------------------------------------------------------- DEEP (closed forms)
+def _ef_add(a, b):
+    return ((a[0] + b[0]) % P, (a[1] + b[1]) % P)
+
+
+def _ef_pow(a, e):
+    r = (1, 0)
+    while e:
+        if e & 1:
+            r = _ef_mul(r, a)
+        a = _ef_mul(a, a)
+        e >>= 1
+    return r
+
+
+def test_batch_eval_lifted_matches_polynomial_evaluation():
+    # interpolate.rs:227-298: f(z^r) for every column of LDE matrices of several heights
+    log_blowup, logL = 2, 7
+    rng = random.Random(9)
+    z = [(rng.randrange(P), rng.randrange(P)), (rng.randrange(P), rng.randrange(P))]
+    traces, polys = [], []
+    for lt, w in [(2, 2), (4, 1), (5, 3)]:
+        n = 1 << lt
+        tr = O.rand_matrix(1000 + lt, n, w)
+        traces.append(tr)
+        winv = pow(O.two_adic_generator(lt), P - 2, P)
+        ninv = pow(n, P - 2, P)
+        polys.append([[x * ninv % P for x in _naive_dft([int(v) for v in tr[:, c]], winv)] for c in range(w)])
+    tree = O.commit_traces(O.HASH_P2, traces, log_blowup)
+    total = sum(t.shape[1] for t in traces)
+    ev = O.batch_eval_lifted_widths([tree], total, z, log_blowup)
+    col = 0
+    for tr, cs in zip(traces, polys):
+        r = (1 << (logL - log_blowup)) // tr.shape[0]
+        for coeffs in cs:
+            for j in range(2):
+                zr = _ef_pow(z[j], r)
+                # the trace is evaluated on the subgroup H_n: f(z^r) with f = interpolant over H_n
+                val = (0, 0)
+                for k, ck in enumerate(coeffs):
+                    t = _ef_pow(zr, k)
+                    val = _ef_add(val, ((t[0] * ck) % P, (t[1] * ck) % P))
+                assert tuple(int(v) for v in ev[col, j]) == val
+            col += 1
+
+
+def test_point_quotients_are_inverses():
+    xs = O.bit_reversed_coset_points(4)
+    z = [(123, 456), (789, 1011)]
+    q = O.point_quotients(z, xs)
+    for i, x in enumerate(xs):
+        for j in range(2):
+            assert _ef_mul(tuple(int(v) for v in q[i, j]), ((z[j][0] - int(x)) % P, z[j][1])) == (1, 0)
+    # bit-reversed coset: adjacent negation and squaring prefix (utils.rs:310-323)
+    assert all((int(xs[2 * i]) + int(xs[2 * i + 1])) % P == 0 for i in range(8))
+
+
+# ---------------------------------------------------------------- FRI (closed forms)
+@pytest.mark.parametrize("log_arity", [1, 2, 3])
+def test_fold_of_low_degree_is_evaluation_at_beta(log_arity):
+    # fold/mod.rs:212-282: for deg f < arity, fold(row on s<w>) == f(beta)
+    a = 1 << log_arity
+    rng = random.Random(log_arity)
+    coeffs = [(rng.randrange(P), rng.randrange(P)) for _ in range(a)]
+    s = pow(7, 12345, P)
+    w = O.two_adic_generator(log_arity)
+    row = []
+    for p_ in range(a):
+        k = int(format(p_, f"0{log_arity}b")[::-1], 2) if log_arity else 0
+        x = s * pow(w, k, P) % P
+        v = (0, 0)
+        for e, c in enumerate(coeffs):
+            xe = pow(x, e, P)
+            v = _ef_add(v, (c[0] * xe % P, c[1] * xe % P))
+        row.extend(v)
+    beta = (rng.randrange(P), rng.randrange(P))
+    got = O.fri_fold_row(log_arity, row, pow(s, P - 2, P), beta)
+    fb = (0, 0)
+    for e, c in enumerate(coeffs):
+        fb = _ef_add(fb, _ef_mul(c, _ef_pow(beta, e)))
+    assert got == fb
+
+
+def test_round_counts_for_configs():
+    # fri/mod.rs:66-96: C1 2^15 domain -> 11 rounds, final degree 8; C2 2^24 -> 9 rounds, degree 16
+    c1 = O.PcsParams(1, 1, 3, 0, 0, 64, 0)
+    assert c1.num_rounds(15) == 11 and c1.final_degree(15) == 8
+    c2 = O.PcsParams(2, 2, 4, 0, 0, 100, 0)
+    assert c2.num_rounds(24) == 9 and c2.final_degree(24) == 16
+    with pytest.raises(ValueError):
+        O.PcsParams(0, 1, 3, 0, 0, 1, 0)
+    with pytest.raises(ValueError):
+        O.PcsParams(1, 4, 3, 0, 0, 1, 0)
+
+
+# ---------------------------------------------------------------- PCS round trips
+def _prove_verify(params, kind, shapes, seed=2025):
+    traces = [O.rand_matrix(seed + i, h, w) for i, (h, w) in enumerate(shapes)]
+    root, z, f, c, tree = O.pcs_trace_prove(params, kind, traces)
+    logL = tree.height().bit_length() - 1
+    widths = [w for _, w in shapes]
+    return root, f, c, logL, widths
+
+
+@pytest.mark.parametrize("la,lb", [(1, 1), (2, 2), (3, 1)])
+def test_prove_verify_accepts_and_rejects_tampering(la, lb):
+    params = O.PcsParams(lb, la, 2, 0, 0, 12, 0)
+    shapes = [(16, 3), (64, 5), (64, 9)]
+    root, f, c, logL, widths = _prove_verify(params, O.HASH_P2, shapes)
+    assert O.pcs_trace_verify(params, O.HASH_P2, root, widths, logL, f, c) == 0
+    rng = random.Random(la)
+    for pos in [0, len(f) // 3, len(f) // 2, len(f) - 1] + [rng.randrange(len(f)) for _ in range(3)]:
+        f2 = f.copy()
+        f2[pos] = (int(f2[pos]) + 1) % P
+        assert O.pcs_trace_verify(params, O.HASH_P2, root, widths, logL, f2, c) != 0
+    c2 = c.copy()
+    c2[0, 0] = (int(c2[0, 0]) + 1) % P
+    assert O.pcs_trace_verify(params, O.HASH_P2, root, widths, logL, f, c2) != 0
+    # trailing data is rejected (transcript finalize)
+    assert O.pcs_trace_verify(params, O.HASH_P2, root, widths, logL, np.append(f, np.uint64(5)), c) != 0
+
+
+def test_prove_verify_keccak_and_grinding():
+    params = O.PcsParams(1, 1, 1, 2, 3, 6, 4)
+    root, f, c, logL, widths = _prove_verify(params, O.HASH_KECCAK, [(32, 20)])
+    assert O.pcs_trace_verify(params, O.HASH_KECCAK, root, widths, logL, f, c) == 0
+    params = O.PcsParams(1, 2, 1, 2, 3, 6, 4)
+    root, f, c, logL, widths = _prove_verify(params, O.HASH_P2, [(32, 4)])
+    assert O.pcs_trace_verify(params, O.HASH_P2, root, widths, logL, f, c) == 0
+
+
+def test_grind_finds_smallest_witness():
+    tx = O.Transcript()
+    tx.observe(17)
+    probe = tx.clone()
+    w = tx.grind(6)
+    for i in range(w):
+        t = probe.clone()
+        assert not t.check_witness(6, i)
+    assert probe.clone().check_witness(6, w)
+    # zero bits: witness 0 and no observation
+    a, b = O.Transcript(), O.Transcript()
+    a.observe(3); b.observe(3)
+    assert a.grind(0) == 0
+    assert a.sample() == b.sample()
