@@ -25,6 +25,7 @@
 #include <set>
 
 #include "internal.hpp"
+#include "p2_fast.hpp"
 
 namespace p3g {
 
@@ -58,31 +59,33 @@ __global__ void __launch_bounds__(LEAF_T) p2_leaf_kernel(LeafArgs a) {
 #pragma unroll
     for (int k = 0; k < 12; k++) st[k] = 0;
   }
-  for (uint32_t m = 0; m < a.nmats; m++) {
+  // one chunk of up to 8 words per iteration: overwrite the rate, zero-fill a
+  // partial chunk, permute (a single permutation call site keeps the code small)
+  uint32_t m = 0;
+  uint64_t j = 0;
+  while (m < a.nmats) {
     const uint64_t w = a.mats[m].width;
-    const uint64_t* row = a.mats[m].ptr + i * w;
-    uint64_t j = 0;
-    for (; j + 8 <= w; j += 8) {
-      const uint4* r4 = reinterpret_cast<const uint4*>(row + j);
-      if ((((uintptr_t)(row + j)) & 15) == 0) {
+    if (j >= w) {
+      m++;
+      j = 0;
+      continue;
+    }
+    const uint64_t* row = a.mats[m].ptr + i * w + j;
+    const uint64_t rem = w - j;
+    if (rem >= 8 && ((((uintptr_t)row) & 15) == 0)) {
+      const uint4* r4 = reinterpret_cast<const uint4*>(row);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          uint4 v = r4[k];
-          st[2 * k] = ((uint64_t)v.y << 32) | v.x;
-          st[2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 8; k++) st[k] = row[j + k];
+      for (int k = 0; k < 4; k++) {
+        uint4 v = r4[k];
+        st[2 * k] = ((uint64_t)v.y << 32) | v.x;
+        st[2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
       }
-      p2_permute(st, a.p2);
-    }
-    if (j < w) {
-      uint64_t rem = w - j;
+    } else {
 #pragma unroll
-      for (int k = 0; k < 8; k++) st[k] = (uint64_t)k < rem ? row[j + k] : 0;
-      p2_permute(st, a.p2);
+      for (int k = 0; k < 8; k++) st[k] = (uint64_t)k < rem ? row[k] : 0;
     }
+    p2_permute_fast(st, a.p2);
+    j += 8;
   }
   if (a.out_digests) {
 #pragma unroll
@@ -142,7 +145,7 @@ __device__ __forceinline__ void compress_one(uint32_t hash, const uint64_t* l, c
       s[4 + k] = r[k];
       s[8 + k] = 0;
     }
-    p2_permute(s, p2);
+    p2_permute_fast(s, p2);
 #pragma unroll
     for (int k = 0; k < 4; k++) out[k] = s[k];
   } else {

@@ -1,0 +1,144 @@
+// Poseidon2-Goldilocks-12 with lazy reduction, for the hashing kernels.
+//
+// Same permutation as p2_permute in hash.hpp (the reference configuration of
+// p3-miden-dev-utils/src/configs.rs:394-407), computed on non-canonical u64
+// representatives: products are reduced to some value < 2^64 without the
+// final conditional subtraction, the linear layers accumulate sums as
+// (64-bit low word, carry count) pairs and reduce once per output, and the
+// internal layer folds "+ sum" into the 128-bit product before reducing.
+// Every state word is canonicalised at the end, so the output equals the
+// canonical permutation bit for bit.
+#pragma once
+#include "hash.hpp"
+
+namespace p3g {
+
+P3G_HD void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t p00 = (uint64_t)a0 * b0;
+  const uint64_t p01 = (uint64_t)a0 * b1 + (p00 >> 32);
+  const uint64_t p10 = (uint64_t)a1 * b0 + (uint32_t)p01;
+  hi = (uint64_t)a1 * b1 + (p01 >> 32) + (p10 >> 32);
+  lo = (p10 << 32) | (uint32_t)p00;
+}
+
+// (hi * 2^64 + lo) mod p as some value < 2^64 (not necessarily < p).
+P3G_HD uint64_t red_nc(uint64_t lo, uint64_t hi) {
+  const uint64_t hh = hi >> 32;
+  const uint64_t hl = (uint32_t)hi;
+  uint64_t t0;
+  const bool borrow = __builtin_sub_overflow(lo, hh, &t0);
+  t0 -= borrow ? EPS : 0;
+  const uint64_t t1 = (hl << 32) - hl;
+  uint64_t r;
+  const bool carry = __builtin_add_overflow(t0, t1, &r);
+  return r + (carry ? EPS : 0);
+}
+
+P3G_HD uint64_t mul_nc(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  mul128(a, b, lo, hi);
+  return red_nc(lo, hi);
+}
+
+// a (any u64) + c (canonical, < p): one carry fold suffices.
+P3G_HD uint64_t add_canon_nc(uint64_t a, uint64_t c) {
+  uint64_t s;
+  const bool carry = __builtin_add_overflow(a, c, &s);
+  return s + (carry ? EPS : 0);
+}
+
+// Sum of up to 2^32 words: low 64 bits and carry count.
+struct Acc {
+  uint64_t lo;
+  uint32_t c;
+};
+
+P3G_HD Acc acc_of(uint64_t x) { return Acc{x, 0}; }
+P3G_HD Acc acc_add(Acc a, uint64_t x) {
+  uint64_t s;
+  const bool carry = __builtin_add_overflow(a.lo, x, &s);
+  return Acc{s, a.c + (carry ? 1u : 0u)};
+}
+P3G_HD Acc acc_add(Acc a, Acc b) {
+  uint64_t s;
+  const bool carry = __builtin_add_overflow(a.lo, b.lo, &s);
+  return Acc{s, a.c + b.c + (carry ? 1u : 0u)};
+}
+// lo + c * 2^64 == lo + c * eps (mod p), as some value < 2^64
+P3G_HD uint64_t acc_fold(Acc a) {
+  const uint64_t t = ((uint64_t)a.c << 32) - a.c;
+  uint64_t r;
+  const bool carry = __builtin_add_overflow(a.lo, t, &r);
+  return r + (carry ? EPS : 0);
+}
+
+P3G_HD uint64_t canon(uint64_t x) { return x >= P ? x - P : x; }
+
+P3G_HD uint64_t sbox_nc(uint64_t x) {
+  const uint64_t x2 = mul_nc(x, x);
+  const uint64_t x3 = mul_nc(x2, x);
+  const uint64_t x4 = mul_nc(x2, x2);
+  return mul_nc(x3, x4);
+}
+
+// External layer: y = circ(2 M4, M4, M4) x with M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]].
+P3G_HD void ext_linear_nc(uint64_t* s) {
+  Acc o[12];
+#pragma unroll
+  for (int j = 0; j < 12; j += 4) {
+    const uint64_t a = s[j], b = s[j + 1], c = s[j + 2], d = s[j + 3];
+    const Acc ab = acc_add(acc_of(a), b);
+    const Acc cd = acc_add(acc_of(c), d);
+    const Acc t = acc_add(ab, cd);       // a+b+c+d
+    const Acc tb = acc_add(t, b);        // a+2b+c+d
+    const Acc td = acc_add(t, d);        // a+b+c+2d
+    o[j] = acc_add(tb, ab);              // 2a+3b+c+d
+    o[j + 1] = acc_add(acc_add(tb, c), c);  // a+2b+3c+d
+    o[j + 2] = acc_add(td, cd);             // a+b+2c+3d
+    o[j + 3] = acc_add(acc_add(td, a), a);  // 3a+b+c+2d
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const Acc sum = acc_add(acc_add(o[k], o[4 + k]), o[8 + k]);
+    s[k] = acc_fold(acc_add(o[k], sum));
+    s[4 + k] = acc_fold(acc_add(o[4 + k], sum));
+    s[8 + k] = acc_fold(acc_add(o[8 + k], sum));
+  }
+}
+
+template <bool UNROLL_INTERNAL = true>
+P3G_HD void p2_permute_fast(uint64_t* s, const P2Consts* __restrict__ c) {
+  ext_linear_nc(s);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+#pragma unroll
+    for (int i = 0; i < 12; i++) s[i] = sbox_nc(add_canon_nc(s[i], c->rc_init[r][i]));
+    ext_linear_nc(s);
+  }
+#pragma unroll
+  for (int r = 0; r < 22; r++) {
+    s[0] = sbox_nc(add_canon_nc(s[0], c->rc_int[r]));
+    Acc sum = acc_of(s[0]);
+#pragma unroll
+    for (int i = 1; i < 12; i++) sum = acc_add(sum, s[i]);
+    const uint64_t sm = acc_fold(sum);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+      uint64_t lo, hi;
+      mul128(s[i], c->diag[i], lo, hi);
+      const bool carry = __builtin_add_overflow(lo, sm, &lo);
+      s[i] = red_nc(lo, hi + (carry ? 1 : 0));
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+#pragma unroll
+    for (int i = 0; i < 12; i++) s[i] = sbox_nc(add_canon_nc(s[i], c->rc_term[r][i]));
+    ext_linear_nc(s);
+  }
+#pragma unroll
+  for (int i = 0; i < 12; i++) s[i] = canon(s[i]);
+}
+
+}  // namespace p3g

@@ -311,8 +311,9 @@ static void run_deep_kernels(Ctx& ctx, const PcsParams& p, unsigned logL, const 
   for (auto& m : mats) {
     if (m.width == 0) continue;
     uint64_t rows = m.height >> p.log_blowup;
-    // keep the y grid dimension within 65535 blocks
-    uint64_t rpb = rows > 1024ull * 65535 ? (rows + 65534) / 65535 : 1024;
+    // at most 1024 row blocks: enough blocks to stream the matrix, few enough
+    // partials for the reduction kernel
+    uint64_t rpb = rows > 1024ull * 1024 ? (rows + 1023) / 1024 : 1024;
     uint64_t nblk = (rows + rpb - 1) / rpb;
     uint64_t ncb = (m.width + CS_COLS - 1) / CS_COLS;
     ef2* partial = (ef2*)ctx.alloc(nblk * m.width * N * sizeof(ef2));
