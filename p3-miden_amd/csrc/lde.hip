@@ -38,13 +38,14 @@ struct NttArgs {
   uint64_t scale_stride;  // words per scale table
   uint64_t n;
   uint32_t logn, s0, K, w, C, ncoltiles, logB, mode;
+  uint32_t logC;  // C = 2^logC
 };
 
 __device__ __forceinline__ uint64_t pw(const uint64_t* __restrict__ tab, uint64_t e) {
   return gl_mul(tab[e & (POW_LO - 1)], tab[POW_LO + (e >> 11)]);
 }
 
-__global__ void __launch_bounds__(NTT_T) ntt_pass_kernel(NttArgs a) {
+__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4))) ntt_pass_kernel(NttArgs a) {
   __shared__ uint64_t tile[(1 << NTT_K) * NTT_C];
   __shared__ uint64_t tws[1 << NTT_K];
   const uint32_t K = a.K, C = a.C, tid = threadIdx.x;
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(NTT_T) ntt_pass_kernel(NttArgs a) {
   for (int r = 0; r < PER; r++) {
     uint32_t e = tid + r * NTT_T;
     if (e < nel) {
-      uint32_t x = e / C, c = e % C;
+      uint32_t x = e >> a.logC, c = e & (C - 1);
       uint32_t col = col0 + c;
       uint64_t idx = base + (uint64_t)x * xstride;
       uint64_t v = 0;
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(NTT_T) ntt_pass_kernel(NttArgs a) {
       if (e < nel) {
         uint64_t v = regs[r];
         if (a.mode == FWD_FIRST) {
-          uint32_t x = e / C;
+          uint32_t x = e >> a.logC;
           uint64_t k = base + (uint64_t)x * xstride;
           v = gl_mul(v, pw(a.scale + t_rep * a.scale_stride, k));
         }
@@ -126,7 +127,7 @@ __global__ void __launch_bounds__(NTT_T) ntt_pass_kernel(NttArgs a) {
     __syncthreads();
     for (uint32_t t = 0; t < K; t++) {
       for (uint32_t b = tid; b < nbf; b += NTT_T) {
-        uint32_t p = b / C, c = b % C;
+        uint32_t p = b >> a.logC, c = b & (C - 1);
         if (inverse) {
           uint32_t mlo = p & ((1u << t) - 1);
           uint32_t x0 = ((p >> t) << (t + 1)) | mlo;
@@ -155,11 +156,160 @@ __global__ void __launch_bounds__(NTT_T) ntt_pass_kernel(NttArgs a) {
     else if (a.mode == FWD) out_block = blk;
     else out_block = 0;
     for (uint32_t e = tid; e < nel; e += NTT_T) {
-      uint32_t x = e / C, c = e % C;
+      uint32_t x = e >> a.logC, c = e & (C - 1);
       uint32_t col = col0 + c;
       if (col < a.w) {
         uint64_t row = out_block * a.n + base + (uint64_t)x * xstride;
         a.dst[row * a.w + col] = tile[e];
+      }
+    }
+  }
+}
+
+
+// Full 8-stage pass on a 256-row x 16-column tile, register blocked: each
+// thread owns one column and 16 rows, performs four butterfly stages in
+// registers, exchanges through an XOR-swizzled LDS tile once, and performs the
+// other four stages in registers.  Layout A: thread (g, c) holds rows
+// x = 16 g + k; layout B: rows x = g + 16 k (k = 0..15).
+__device__ __forceinline__ uint32_t swz(uint32_t x, uint32_t c) { return x * 16 + (c ^ ((x >> 4) & 15)); }
+
+template <uint32_t MODE>
+__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4))) ntt8_kernel(NttArgs a) {
+  __shared__ uint64_t tile[256 * 16];
+  __shared__ uint64_t tws[256];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t c = tid & 15, g = tid >> 4;
+  constexpr bool inverse = MODE <= INV;
+  uint64_t bid = blockIdx.x;
+  const uint32_t ct = (uint32_t)(bid % a.ncoltiles);
+  bid /= a.ncoltiles;
+  const uint32_t lb = inverse ? a.s0 : a.logn - a.s0 - 8;
+  const uint64_t nunits = a.n >> 8;
+  const uint64_t u = bid % nunits;
+  const uint64_t blk = bid / nunits;
+  const uint64_t lo = u & ((1ull << lb) - 1);
+  const uint64_t hi = u >> lb;
+  const uint64_t base = hi << (lb + 8) | lo;
+  const uint64_t xstride = 1ull << lb;
+  const uint32_t col = ct * 16 + c;
+  const bool live = col < a.w;
+
+  for (uint32_t e = tid; e < 255; e += NTT_T) {
+    uint32_t t = 31 - __clz(e + 1);
+    uint32_t m = e + 1 - (1u << t);
+    uint64_t ex;
+    if (inverse) ex = ((uint64_t)m << a.s0 | lo) << (a.logn - a.s0 - t - 1);
+    else ex = ((uint64_t)m << lb | lo) << (a.s0 + 8 - 1 - t);
+    tws[e] = pw(a.tw, ex);
+  }
+
+  __syncthreads();  // twiddles ready
+  const int nrep = MODE == FWD_FIRST ? (1 << a.logB) : 1;
+  // cosets in descending order: coset 0 writes output block 0, which holds the
+  // coefficients, so it must come last
+  for (int rep = nrep - 1; rep >= 0; rep--) {
+    uint64_t r[16];
+    // inverse passes start in layout A, forward passes in layout B; the first
+    // forward pass re-reads its coefficient tile for every coset (L2 hits)
+    // FWD_FIRST: coefficient k_glob = base + (g + 16 k) xstride is scaled by
+    // c_t^k_glob / n, built as f0 * step^k (the table's lo half carries 1/n)
+    uint64_t f = 0, step = 0;
+    if (MODE == FWD_FIRST) {
+      const uint64_t* sc = a.scale + rep * a.scale_stride;
+      f = pw(sc, base + (uint64_t)g * xstride);
+      step = gl_mul(pw(sc, 16 * xstride), a.n % P);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const uint32_t x = inverse ? 16 * g + k : g + 16 * k;
+      const uint64_t idx = base + (uint64_t)x * xstride;
+      uint64_t val = 0;
+      if (live) {
+        if (MODE == INV_FIRST) val = a.src[(uint64_t)bitrev32((uint32_t)idx, a.logn) * a.w + col];
+        else val = a.dst[((MODE == FWD ? blk * a.n : 0) + idx) * a.w + col];
+      }
+      if (MODE == FWD_FIRST) {
+        val = gl_mul(val, f);
+        f = gl_mul(f, step);
+      }
+      r[k] = val;
+    }
+    if (inverse) {
+      // stages on x bits 0..3 (k bits), layout A
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0++) {
+          if (k0 & (1 << t)) continue;
+          const int k1 = k0 | (1 << t);
+          const uint64_t tw = tws[(1 << t) - 1 + (k0 & ((1 << t) - 1))];
+          const uint64_t x = r[k0], y = gl_mul(r[k1], tw);
+          r[k0] = gl_add(x, y);
+          r[k1] = gl_sub(x, y);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 16; k++) tile[swz(16 * g + k, c)] = r[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; k++) r[k] = tile[swz(g + 16 * k, c)];
+      // stages on x bits 4..7, layout B
+#pragma unroll
+      for (int t = 4; t < 8; t++) {
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0++) {
+          if (k0 & (1 << (t - 4))) continue;
+          const int k1 = k0 | (1 << (t - 4));
+          const uint64_t tw = tws[(1 << t) - 1 + g + 16 * (k0 & ((1 << (t - 4)) - 1))];
+          const uint64_t x = r[k0], y = gl_mul(r[k1], tw);
+          r[k0] = gl_add(x, y);
+          r[k1] = gl_sub(x, y);
+        }
+      }
+      if (live) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) a.dst[(base + (uint64_t)(g + 16 * k) * xstride) * a.w + col] = r[k];
+      }
+    } else {
+      // DIF: x bits 7..4 first (layout B, k bits 3..0)
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int q = 7 - t;
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0++) {
+          if (k0 & (1 << (q - 4))) continue;
+          const int k1 = k0 | (1 << (q - 4));
+          const uint64_t tw = tws[(1 << q) - 1 + g + 16 * (k0 & ((1 << (q - 4)) - 1))];
+          const uint64_t x = r[k0], y = r[k1];
+          r[k0] = gl_add(x, y);
+          r[k1] = gl_mul(gl_sub(x, y), tw);
+        }
+      }
+      __syncthreads();  // previous repetition's readers are done with the tile
+#pragma unroll
+      for (int k = 0; k < 16; k++) tile[swz(g + 16 * k, c)] = r[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; k++) r[k] = tile[swz(16 * g + k, c)];
+      // x bits 3..0, layout A
+#pragma unroll
+      for (int t = 4; t < 8; t++) {
+        const int q = 7 - t;
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0++) {
+          if (k0 & (1 << q)) continue;
+          const int k1 = k0 | (1 << q);
+          const uint64_t tw = tws[(1 << q) - 1 + (k0 & ((1 << q) - 1))];
+          const uint64_t x = r[k0], y = r[k1];
+          r[k0] = gl_add(x, y);
+          r[k1] = gl_mul(gl_sub(x, y), tw);
+        }
+      }
+      const uint64_t ob = MODE == FWD_FIRST ? bitrev32((uint32_t)rep, a.logB) : (MODE == FWD ? blk : 0);
+      if (live) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) a.dst[(ob * a.n + base + (uint64_t)(16 * g + k) * xstride) * a.w + col] = r[k];
       }
     }
   }
@@ -171,7 +321,9 @@ void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, 
   if (w == 0) return;
   const unsigned logn = log2_exact(n);
   if (logn + log_blowup > 32) throw ApiError(E_RANGE, "LDE height exceeds the field two-adicity");
-  const uint32_t C = w < (uint64_t)NTT_C ? (uint32_t)w : (uint32_t)NTT_C;
+  uint32_t logC = 0;
+  while ((2u << logC) <= NTT_C && (2ull << logC) <= w) logC++;
+  const uint32_t C = 1u << logC;
   const uint32_t ncol = (uint32_t)((w + C - 1) / C);
   const uint32_t B = 1u << log_blowup;
   const uint32_t twbits = logn ? logn - 1 : 0;
@@ -218,12 +370,22 @@ void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, 
     a.K = K;
     a.w = (uint32_t)w;
     a.C = C;
+    a.logC = logC;
     a.ncoltiles = ncol;
     a.logB = log_blowup;
     a.mode = mode;
     uint64_t grid = (n >> K) * ncol * nblk;
     ProfScope ps(ctx, mode <= INV ? "lde_inverse_pass" : "lde_forward_pass", s);
-    hipLaunchKernelGGL(ntt_pass_kernel, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a);
+    if (K == 8 && C == 16) {
+      switch (mode) {
+        case INV_FIRST: hipLaunchKernelGGL(ntt8_kernel<INV_FIRST>, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a); break;
+        case INV: hipLaunchKernelGGL(ntt8_kernel<INV>, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a); break;
+        case FWD_FIRST: hipLaunchKernelGGL(ntt8_kernel<FWD_FIRST>, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a); break;
+        default: hipLaunchKernelGGL(ntt8_kernel<FWD>, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a); break;
+      }
+    }
+    else
+      hipLaunchKernelGGL(ntt_pass_kernel, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a);
     P3G_CHECK_HIP(hipGetLastError());
   };
 

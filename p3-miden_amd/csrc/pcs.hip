@@ -158,47 +158,62 @@ struct QuotArgs {
   ef2* out;
 };
 
-constexpr int DQ_T = 128;
-constexpr int DQ_CH = 4;
+constexpr int DQ_T = 256;
+constexpr int DQ_CH = 4;  // rows per thread: tid + 256 r, r < DQ_CH
+constexpr int DQ_ROWS = DQ_T * DQ_CH;
 
+// Each workgroup owns DQ_ROWS consecutive LDE rows.  Matrix rows are staged
+// through LDS in 256-row x 16-column tiles with coalesced loads (16 lanes per
+// 128-byte row segment); each thread then reduces its own row from LDS with
+// the (uniform, scalar-loaded) -alpha^(W-1-c) coefficients.
 template <int N>
-__global__ void __launch_bounds__(DQ_T) deep_quotient_kernel(QuotArgs a) {
-  const uint64_t i0 = ((uint64_t)blockIdx.x * DQ_T + threadIdx.x) * DQ_CH;
-  if (i0 >= a.L) return;
-  const int cnt = (int)((a.L - i0) < (uint64_t)DQ_CH ? (a.L - i0) : DQ_CH);
+__global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4))) deep_quotient_kernel(QuotArgs a) {
+  __shared__ uint64_t tile[256 * 16];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t wbase = (uint64_t)blockIdx.x * DQ_ROWS;
   ef2 acc[DQ_CH];
-  uint64_t xs[DQ_CH];
 #pragma unroll
-  for (int r = 0; r < DQ_CH; r++) acc[r] = ef_from(0);
-  for (uint32_t m = 0; m < a.nmats; m++) {
-    const QMat mm = a.mats[m];
-    const ef2* cf = a.coeffs + mm.coeff_off;
-#pragma unroll
-    for (int r = 0; r < DQ_CH; r++) {
-      if (r < cnt) {
-        const uint64_t* row = mm.ptr + ((i0 + r) >> mm.shift) * mm.width;
-        ef2 s = acc[r];
-        for (uint64_t c = 0; c < mm.width; c++) s = ef_add(s, ef_mul_base(cf[c], row[c]));
-        acc[r] = s;
+  for (int r = 0; r < DQ_CH; r++) {
+    ef2 s = ef_from(0);
+#pragma unroll 1
+    for (uint32_t m = 0; m < a.nmats; m++) {
+      const QMat mm = a.mats[m];
+      const ef2* cf = a.coeffs + mm.coeff_off;
+#pragma unroll 1
+      for (uint64_t c0 = 0; c0 < mm.width; c0 += 16) {
+        const uint32_t nc = (uint32_t)((mm.width - c0) < 16 ? (mm.width - c0) : 16);
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < 16; k++) {
+          const uint32_t e = tid + 256 * k;
+          const uint32_t rl = e >> 4, cc = e & 15;
+          const uint64_t row = wbase + 256 * r + rl;
+          uint64_t v = 0;
+          if (cc < nc && row < a.L) v = mm.ptr[(row >> mm.shift) * mm.width + c0 + cc];
+          tile[rl * 16 + (cc ^ (rl & 15))] = v;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (uint32_t cc = 0; cc < nc; cc++)
+          s = ef_add(s, ef_mul_base(cf[c0 + cc], tile[tid * 16 + (cc ^ (tid & 15))]));
       }
     }
+    acc[r] = s;
   }
   uint64_t d0[DQ_CH][N], nrm[DQ_CH][N], pre[DQ_CH][N];
   uint64_t run = 1;
 #pragma unroll
   for (int r = 0; r < DQ_CH; r++) {
-    if (r < cnt) {
-      uint64_t x = gl_mul(GENERATOR, pwt(a.powL, bitrev32((uint32_t)(i0 + r), a.logL)));
-      xs[r] = x;
+    const uint64_t i = wbase + tid + 256 * r;
+    const uint64_t x = gl_mul(GENERATOR, pwt(a.powL, bitrev32((uint32_t)(i < a.L ? i : 0), a.logL)));
 #pragma unroll
-      for (int j = 0; j < N; j++) {
-        uint64_t a0 = gl_sub(a.z.z[j].c0, x);
-        d0[r][j] = a0;
-        uint64_t n = gl_sub(gl_sqr(a0), gl_mul(EXT_W, gl_sqr(a.z.z[j].c1)));
-        nrm[r][j] = n;
-        pre[r][j] = run;
-        run = gl_mul(run, n);
-      }
+    for (int j = 0; j < N; j++) {
+      uint64_t a0 = gl_sub(a.z.z[j].c0, x);
+      d0[r][j] = a0;
+      uint64_t n = gl_sub(gl_sqr(a0), gl_mul(EXT_W, gl_sqr(a.z.z[j].c1)));
+      nrm[r][j] = n;
+      pre[r][j] = run;
+      run = gl_mul(run, n);
     }
   }
   uint64_t inv = gl_inv(run);
@@ -207,21 +222,21 @@ __global__ void __launch_bounds__(DQ_T) deep_quotient_kernel(QuotArgs a) {
   for (int r = 0; r < DQ_CH; r++) res[r] = ef_from(0);
 #pragma unroll
   for (int r = DQ_CH - 1; r >= 0; r--) {
-    if (r < cnt) {
 #pragma unroll
-      for (int j = N - 1; j >= 0; j--) {
-        uint64_t ni = gl_mul(inv, pre[r][j]);
-        inv = gl_mul(inv, nrm[r][j]);
-        ef2 q = ef2{gl_mul(d0[r][j], ni), gl_neg(gl_mul(a.z.z[j].c1, ni))};
-        ef2 t = ef_mul(q, ef_add(a.fred[j], acc[r]));
-        if (j > 0) t = ef_mul(a.bpow[j], t);
-        res[r] = ef_add(res[r], t);
-      }
+    for (int j = N - 1; j >= 0; j--) {
+      uint64_t ni = gl_mul(inv, pre[r][j]);
+      inv = gl_mul(inv, nrm[r][j]);
+      ef2 q = ef2{gl_mul(d0[r][j], ni), gl_neg(gl_mul(a.z.z[j].c1, ni))};
+      ef2 t = ef_mul(q, ef_add(a.fred[j], acc[r]));
+      if (j > 0) t = ef_mul(a.bpow[j], t);
+      res[r] = ef_add(res[r], t);
     }
   }
 #pragma unroll
-  for (int r = 0; r < DQ_CH; r++)
-    if (r < cnt) a.out[i0 + r] = res[r];
+  for (int r = 0; r < DQ_CH; r++) {
+    const uint64_t i = wbase + tid + 256 * r;
+    if (i < a.L) a.out[i] = res[r];
+  }
 }
 
 struct FoldTw {
@@ -380,10 +395,9 @@ static void run_quotient(Ctx& ctx, unsigned logL, const ef2* z, const std::vecto
   a.bpow[0] = ef_from(1);
   for (int j = 1; j < N; j++) a.bpow[j] = ef_mul(a.bpow[j - 1], beta);
   a.out = out;
-  uint64_t threads = (L + DQ_CH - 1) / DQ_CH;
   {
   ProfScope ps(ctx, "deep_quotient", s);
-  hipLaunchKernelGGL(deep_quotient_kernel<N>, dim3((uint32_t)((threads + DQ_T - 1) / DQ_T)), dim3(DQ_T), 0, s, a);
+  hipLaunchKernelGGL(deep_quotient_kernel<N>, dim3((uint32_t)((L + DQ_ROWS - 1) / DQ_ROWS)), dim3(DQ_T), 0, s, a);
   P3G_CHECK_HIP(hipGetLastError());
   }
   P3G_CHECK_HIP(hipStreamSynchronize(s));
