@@ -27,33 +27,65 @@ P3G_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
 #endif
 }
 
+// Canonical add: a + b if a < p - b, else a - (p - b); 32-bit carry chains.
 P3G_HD uint64_t gl_add(uint64_t a, uint64_t b) {
-  uint64_t s = a + b;
-  uint64_t c = s < a ? EPS : 0;  // 2^64 == EPS (mod p)
-  s += c;
-  return s >= P ? s - P : s;
+  unsigned int b0, b1, bt0, bt1;
+  const uint32_t n0 = __builtin_subc(1u, (uint32_t)b, 0u, &b0);  // p - b, low word
+  const uint32_t n1 = __builtin_subc(0xFFFFFFFFu, (uint32_t)(b >> 32), b0, &b1);
+  const uint32_t t0 = __builtin_subc((uint32_t)a, n0, 0u, &bt0);
+  const uint32_t t1 = __builtin_subc((uint32_t)(a >> 32), n1, bt0, &bt1);
+  (void)b1;
+  return bt1 ? a + b : (((uint64_t)t1 << 32) | t0);
 }
 
 P3G_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
-  uint64_t d = a - b;
-  return a < b ? d - EPS : d;  // borrow: -2^64 == -EPS
+  unsigned int c0, c1;
+  const uint32_t d0 = __builtin_subc((uint32_t)a, (uint32_t)b, 0u, &c0);
+  const uint32_t d1 = __builtin_subc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c0, &c1);
+  const uint64_t d = ((uint64_t)d1 << 32) | d0;
+  return c1 ? d - EPS : d;  // borrow: -2^64 == -EPS
 }
 
 P3G_HD uint64_t gl_neg(uint64_t a) { return a ? P - a : 0; }
 
-// x = hi*2^64 + lo reduced with 2^64 = eps and 2^96 = -1.
+// x = hi*2^64 + lo with hi = h1*2^32 + h0: since 2^64 == 2^32 - 1 and
+// 2^96 == -1, x == lo + h0*2^32 - (h0 + h1).  Computed on 32-bit limbs with
+// carry chains; the final carry (+2^64) and borrow (-2^64) fold as +-eps.
+// Returns a value < 2^64 that is not necessarily < p.
+P3G_HD uint64_t gl_reduce_nc(uint64_t lo, uint64_t hi) {
+  const uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32);
+  const uint32_t h0 = (uint32_t)hi, h1 = (uint32_t)(hi >> 32);
+  unsigned int sc, b0, b1, c1;
+  const uint32_t s0 = __builtin_addc(h0, h1, 0u, &sc);
+  const uint32_t t0 = __builtin_subc(l0, s0, 0u, &b0);
+  const uint32_t t1 = __builtin_subc(l1, sc, b0, &b1);
+  const uint32_t u1 = __builtin_addc(t1, h0, 0u, &c1);
+  uint64_t u = ((uint64_t)u1 << 32) | t0;
+  u += c1 ? EPS : 0;
+  u -= b1 ? EPS : 0;
+  return u;
+}
+
 P3G_HD uint64_t gl_reduce(uint64_t lo, uint64_t hi) {
-  uint64_t hh = hi >> 32;
-  uint64_t hl = hi & EPS;
-  uint64_t t0 = lo - hh;
-  if (lo < hh) t0 -= EPS;
-  uint64_t t1 = (hl << 32) - hl;  // hl * eps
-  uint64_t r = t0 + t1;
-  if (r < t1) r += EPS;
+  const uint64_t r = gl_reduce_nc(lo, hi);
   return r >= P ? r - P : r;
 }
 
-P3G_HD uint64_t gl_mul(uint64_t a, uint64_t b) { return gl_reduce(a * b, mulhi64(a, b)); }
+// 64 x 64 -> 128-bit product from four 32 x 32 -> 64 multiply-adds.
+P3G_HD void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t p00 = (uint64_t)a0 * b0;
+  const uint64_t p01 = (uint64_t)a0 * b1 + (p00 >> 32);
+  const uint64_t p10 = (uint64_t)a1 * b0 + (uint32_t)p01;
+  hi = (uint64_t)a1 * b1 + (p01 >> 32) + (p10 >> 32);
+  lo = (p10 << 32) | (uint32_t)p00;
+}
+
+P3G_HD uint64_t gl_mul(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  mul128(a, b, lo, hi);
+  return gl_reduce(lo, hi);
+}
 P3G_HD uint64_t gl_sqr(uint64_t a) { return gl_mul(a, a); }
 
 P3G_HD uint64_t gl_pow(uint64_t a, uint64_t e) {

@@ -13,26 +13,15 @@
 
 namespace p3g {
 
-P3G_HD void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
-  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-  const uint64_t p00 = (uint64_t)a0 * b0;
-  const uint64_t p01 = (uint64_t)a0 * b1 + (p00 >> 32);
-  const uint64_t p10 = (uint64_t)a1 * b0 + (uint32_t)p01;
-  hi = (uint64_t)a1 * b1 + (p01 >> 32) + (p10 >> 32);
-  lo = (p10 << 32) | (uint32_t)p00;
-}
-
 // (hi * 2^64 + lo) mod p as some value < 2^64 (not necessarily < p).
-P3G_HD uint64_t red_nc(uint64_t lo, uint64_t hi) {
-  const uint64_t hh = hi >> 32;
-  const uint64_t hl = (uint32_t)hi;
-  uint64_t t0;
-  const bool borrow = __builtin_sub_overflow(lo, hh, &t0);
-  t0 -= borrow ? EPS : 0;
-  const uint64_t t1 = (hl << 32) - hl;
-  uint64_t r;
-  const bool carry = __builtin_add_overflow(t0, t1, &r);
-  return r + (carry ? EPS : 0);
+P3G_HD uint64_t red_nc(uint64_t lo, uint64_t hi) { return gl_reduce_nc(lo, hi); }
+
+// 64-bit add on 32-bit limbs; returns the carry out.
+P3G_HD uint64_t add64c(uint64_t a, uint64_t b, unsigned int& carry) {
+  unsigned int c0;
+  const uint32_t r0 = __builtin_addc((uint32_t)a, (uint32_t)b, 0u, &c0);
+  const uint32_t r1 = __builtin_addc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c0, &carry);
+  return ((uint64_t)r1 << 32) | r0;
 }
 
 P3G_HD uint64_t mul_nc(uint64_t a, uint64_t b) {
@@ -43,8 +32,8 @@ P3G_HD uint64_t mul_nc(uint64_t a, uint64_t b) {
 
 // a (any u64) + c (canonical, < p): one carry fold suffices.
 P3G_HD uint64_t add_canon_nc(uint64_t a, uint64_t c) {
-  uint64_t s;
-  const bool carry = __builtin_add_overflow(a, c, &s);
+  unsigned int carry;
+  const uint64_t s = add64c(a, c, carry);
   return s + (carry ? EPS : 0);
 }
 
@@ -56,20 +45,20 @@ struct Acc {
 
 P3G_HD Acc acc_of(uint64_t x) { return Acc{x, 0}; }
 P3G_HD Acc acc_add(Acc a, uint64_t x) {
-  uint64_t s;
-  const bool carry = __builtin_add_overflow(a.lo, x, &s);
-  return Acc{s, a.c + (carry ? 1u : 0u)};
+  unsigned int carry, dummy;
+  const uint64_t s = add64c(a.lo, x, carry);
+  return Acc{s, __builtin_addc(a.c, 0u, carry, &dummy)};
 }
 P3G_HD Acc acc_add(Acc a, Acc b) {
-  uint64_t s;
-  const bool carry = __builtin_add_overflow(a.lo, b.lo, &s);
-  return Acc{s, a.c + b.c + (carry ? 1u : 0u)};
+  unsigned int carry, dummy;
+  const uint64_t s = add64c(a.lo, b.lo, carry);
+  return Acc{s, __builtin_addc(a.c, b.c, carry, &dummy)};
 }
 // lo + c * 2^64 == lo + c * eps (mod p), as some value < 2^64
 P3G_HD uint64_t acc_fold(Acc a) {
   const uint64_t t = ((uint64_t)a.c << 32) - a.c;
-  uint64_t r;
-  const bool carry = __builtin_add_overflow(a.lo, t, &r);
+  unsigned int carry;
+  const uint64_t r = add64c(a.lo, t, carry);
   return r + (carry ? EPS : 0);
 }
 
@@ -127,8 +116,9 @@ P3G_HD void p2_permute_fast(uint64_t* s, const P2Consts* __restrict__ c) {
     for (int i = 0; i < 12; i++) {
       uint64_t lo, hi;
       mul128(s[i], c->diag[i], lo, hi);
-      const bool carry = __builtin_add_overflow(lo, sm, &lo);
-      s[i] = red_nc(lo, hi + (carry ? 1 : 0));
+      unsigned int carry;
+      lo = add64c(lo, sm, carry);
+      s[i] = red_nc(lo, hi + carry);
     }
   }
 #pragma unroll
