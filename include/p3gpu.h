@@ -163,6 +163,55 @@ int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t l
                           const uint64_t* eval_points, uint32_t n_points, p3g_tree* const* trees, uint64_t ntrees,
                           p3g_transcript* ch);
 
+/* ---- caller-owned channel ----
+ * The same operations driven through the caller's ProverChannel (for example a
+ * Rust ProverTranscript over its own DuplexChallenger): the library calls back
+ * for every send / hint / grind / sample, in exactly the order the reference's
+ * open_with_channel and LmcsTree::prove_batch perform them.  Callbacks return 0
+ * on success; any other value aborts the call with P3G_E_TRANSCRIPT. */
+typedef struct {
+  void* user;
+  int (*send_fields)(void* user, const uint64_t* values, uint64_t n);  /* send_field_slice (observed) */
+  int (*send_commitment)(void* user, const uint64_t c[4]);            /* send_commitment (observed) */
+  int (*hint_fields)(void* user, const uint64_t* values, uint64_t n);  /* hint_field_slice */
+  int (*hint_commitment)(void* user, const uint64_t c[4]);            /* hint_commitment */
+  int (*grind)(void* user, uint32_t bits, uint64_t* witness);          /* grind */
+  int (*sample)(void* user, uint64_t* out);                           /* Channel::sample */
+  int (*sample_bits)(void* user, uint32_t bits, uint64_t* out);       /* Channel::sample_bits */
+} p3g_channel;
+
+/* prove_batch(indices, channel) (lifted_tree.rs:157-224) into a caller channel */
+int p3g_tree_prove_batch_cb(p3g_tree* t, const uint64_t* indices, uint64_t n, const p3g_channel* ch);
+/* open_with_channel (p3-miden-lifted-fri/src/prover.rs:35-116) into a caller channel */
+int p3g_open_with_callbacks(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t log_lde_height,
+                            const uint64_t* eval_points, uint32_t n_points, p3g_tree* const* trees, uint64_t ntrees,
+                            const p3g_channel* ch);
+
+/* ---- batched commit plans (no reference counterpart: the planned, graph-captured
+ * form of commit_traces for many traces of uneven heights) ----
+ * A plan fixes the shapes (heights ascending, powers of two), owns the device
+ * input buffers (fill them through p3g_commit_plan_input), and on every run
+ * recomputes the LDEs and the lifted LMCS tree exactly as p3g_commit_traces
+ * would.  Every NTT pass index is one grouped launch over all traces; with
+ * P3G_PLAN_GRAPH the launch sequence is captured once into a hipGraph and
+ * replayed; with P3G_PLAN_AUTOTUNE the NTT stage grouping and the leaf-hash
+ * workgroup size are picked by timing candidates on this shape.  The tree
+ * returned by run is a view valid until the next run or destroy. */
+#define P3G_PLAN_GRAPH 1u
+#define P3G_PLAN_AUTOTUNE 2u
+typedef struct p3g_commit_plan p3g_commit_plan;
+int p3g_commit_plan_create(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* heights, const uint64_t* widths,
+                           uint64_t k, uint32_t log_blowup, uint32_t flags, p3g_commit_plan** out);
+int p3g_commit_plan_input(p3g_commit_plan* plan, uint64_t i, uint64_t** dptr);
+int p3g_commit_plan_info(const p3g_commit_plan* plan, uint32_t* kmax, uint32_t* leaf_threads, uint32_t* launches);
+int p3g_commit_plan_run(p3g_commit_plan* plan, p3g_tree** view);
+void p3g_commit_plan_destroy(p3g_commit_plan* plan);
+/* run the plan, observe the root, sample two points, open (the pcs_trace flow of p3g_prove) */
+int p3g_prove_planned(p3g_ctx* ctx, p3g_commit_plan* plan, const p3g_pcs_params* params, p3g_transcript* ch,
+                      uint64_t root[4], uint64_t eval_points_out[4]);
+/* synthetic canonical words generated on the device (counter-based SplitMix64; not the SmallRng stream) */
+int p3g_fill_random_device(p3g_ctx* ctx, uint64_t seed, uint64_t n, uint64_t* dptr);
+
 /* ---- one-shot proof (the pcs_trace harness flow, p3-miden-lifted-fri/benches/pcs_trace.rs:60-100,
  * with commit_traces semantics): commit the traces, observe the root, sample two
  * extension points, open.  root receives the commitment, eval_points_out (4 words)

@@ -6,6 +6,7 @@
 
 #include "../../include/p3gpu.h"
 #include "internal.hpp"
+#include "batch.hpp"
 
 using namespace p3g;
 
@@ -18,6 +19,13 @@ struct p3g_tree {
 struct p3g_transcript {
   Transcript tx;
 };
+struct p3g_commit_plan {
+  CommitPlan* p;
+};
+
+namespace p3g {
+void fill_random_device(Ctx& ctx, uint64_t seed, uint64_t n, uint64_t* out, hipStream_t s);
+}
 
 static thread_local std::string g_last_error;
 
@@ -511,6 +519,156 @@ int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t l
     }
     open_with_channel(c, p, log_lde_height, z.data(), (int)n_points, tv, ch->tx, c.stream);
   });
+}
+
+namespace {
+// ProverChannel implemented by the caller through function pointers.
+struct CallbackChannel : Channel {
+  const p3g_channel* cb;
+  explicit CallbackChannel(const p3g_channel* c) : cb(c) {}
+  static void ok(int rc) {
+    if (rc != 0) throw ApiError(P3G_E_TRANSCRIPT, "channel callback failed");
+  }
+  void send_fields(const uint64_t* v, size_t n) override { ok(cb->send_fields(cb->user, v, n)); }
+  void send_commitment(const uint64_t* c4) override { ok(cb->send_commitment(cb->user, c4)); }
+  void hint_fields(const uint64_t* v, size_t n) override { ok(cb->hint_fields(cb->user, v, n)); }
+  void hint_commitment(const uint64_t* c4) override { ok(cb->hint_commitment(cb->user, c4)); }
+  uint64_t grind(unsigned bits) override {
+    uint64_t w = 0;
+    ok(cb->grind(cb->user, bits, &w));
+    return w;
+  }
+  uint64_t sample() override {
+    uint64_t v = 0;
+    ok(cb->sample(cb->user, &v));
+    return v;
+  }
+  uint64_t sample_bits(unsigned bits) override {
+    uint64_t v = 0;
+    ok(cb->sample_bits(cb->user, bits, &v));
+    return v;
+  }
+};
+
+bool channel_complete(const p3g_channel* c) {
+  return c && c->send_fields && c->send_commitment && c->hint_fields && c->hint_commitment && c->grind &&
+         c->sample && c->sample_bits;
+}
+}  // namespace
+
+int p3g_tree_prove_batch_cb(p3g_tree* t, const uint64_t* indices, uint64_t n, const p3g_channel* ch) {
+  if (!t || !channel_complete(ch) || (n && !indices)) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipSetDevice(t->t->ctx->device));
+    std::vector<uint64_t> idx(indices, indices + n);
+    CallbackChannel cc(ch);
+    tree_prove_batch(*t->t, idx, cc, t->t->ctx->stream);
+  });
+}
+
+int p3g_open_with_callbacks(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t log_lde_height,
+                            const uint64_t* eval_points, uint32_t n_points, p3g_tree* const* trees, uint64_t ntrees,
+                            const p3g_channel* ch) {
+  if (!ctx || !params || !eval_points || !channel_complete(ch) || (!trees && ntrees)) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    PcsParams p{params->log_blowup,   params->log_folding_arity, params->log_final_degree, params->folding_pow_bits,
+                params->deep_pow_bits, params->num_queries,      params->query_pow_bits};
+    if (!canonical(eval_points, 2ull * n_points)) throw ApiError(P3G_E_INVAL, "non-canonical evaluation point");
+    std::vector<ef2> z(n_points);
+    for (uint32_t j = 0; j < n_points; j++) z[j] = ef2{eval_points[2 * j], eval_points[2 * j + 1]};
+    std::vector<Tree*> tv;
+    for (uint64_t i = 0; i < ntrees; i++) {
+      if (!trees[i]) throw ApiError(P3G_E_INVAL, "null tree");
+      tv.push_back(trees[i]->t);
+    }
+    CallbackChannel cc(ch);
+    open_with_channel(c, p, log_lde_height, z.data(), (int)n_points, tv, cc, c.stream);
+  });
+}
+
+int p3g_fill_random_device(p3g_ctx* ctx, uint64_t seed, uint64_t n, uint64_t* dptr) {
+  if (!ctx || (!dptr && n)) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
+    fill_random_device(ctx->c, seed, n, dptr, ctx->c.stream);
+    P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int p3g_commit_plan_create(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* heights, const uint64_t* widths,
+                           uint64_t k, uint32_t log_blowup, uint32_t flags, p3g_commit_plan** out) {
+  if (!ctx || !out || (k && (!heights || !widths))) return P3G_E_INVAL;
+  *out = nullptr;
+  return guard([&] {
+    check_hash(hash_kind);
+    P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
+    std::vector<uint64_t> h(heights, heights + k), w(widths, widths + k);
+    CommitPlan* p = make_commit_plan(ctx->c, hash_kind, h, w, log_blowup, (flags & P3G_PLAN_GRAPH) != 0,
+                                     (flags & P3G_PLAN_AUTOTUNE) != 0);
+    *out = new p3g_commit_plan{p};
+  });
+}
+
+int p3g_commit_plan_input(p3g_commit_plan* plan, uint64_t i, uint64_t** dptr) {
+  if (!plan || !dptr || i >= plan->p->traces.size()) return P3G_E_INVAL;
+  *dptr = plan->p->traces[i].ptr;
+  return P3G_OK;
+}
+
+int p3g_commit_plan_info(const p3g_commit_plan* plan, uint32_t* kmax, uint32_t* leaf_threads, uint32_t* launches) {
+  if (!plan) return P3G_E_INVAL;
+  if (kmax) *kmax = plan->p->kmax;
+  if (leaf_threads) *leaf_threads = (uint32_t)plan->p->leaf_threads;
+  if (launches) *launches = (uint32_t)plan->p->launches.size();
+  return P3G_OK;
+}
+
+int p3g_commit_plan_run(p3g_commit_plan* plan, p3g_tree** view) {
+  if (!plan || !view) return P3G_E_INVAL;
+  *view = nullptr;
+  return guard([&] {
+    CommitPlan& p = *plan->p;
+    P3G_CHECK_HIP(hipSetDevice(p.ctx->device));
+    {
+      ProfScope ps(*p.ctx, "batch_commit", p.ctx->stream);
+      p.run(p.ctx->stream);
+    }
+    *view = new p3g_tree{p.tree_view()};
+  });
+}
+
+void p3g_commit_plan_destroy(p3g_commit_plan* plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->p->ctx->device);
+  (void)hipStreamSynchronize(plan->p->ctx->stream);
+  delete plan->p;
+  delete plan;
+}
+
+int p3g_prove_planned(p3g_ctx* ctx, p3g_commit_plan* plan, const p3g_pcs_params* params, p3g_transcript* ch,
+                      uint64_t root[4], uint64_t eval_points_out[4]) {
+  if (!ctx || !plan || !params || !ch) return P3G_E_INVAL;
+  if (params->log_blowup != plan->p->log_blowup) return P3G_E_INVAL;
+  p3g_tree* tree = nullptr;
+  int rc = p3g_commit_plan_run(plan, &tree);
+  if (rc != P3G_OK) return rc;
+  uint64_t r[4], z[4];
+  p3g_tree_root(tree, r);
+  ch->tx.observe_commitment(r);
+  ef2 z0 = ch->tx.sample_ef();
+  ef2 z1 = ch->tx.sample_ef();
+  z[0] = z0.c0;
+  z[1] = z0.c1;
+  z[2] = z1.c0;
+  z[3] = z1.c1;
+  uint32_t logL = log2_exact(p3g_tree_height(tree));
+  rc = p3g_open_with_channel(ctx, params, logL, z, 2, &tree, 1, ch);
+  p3g_tree_free(tree);
+  if (root) memcpy(root, r, 32);
+  if (eval_points_out) memcpy(eval_points_out, z, 32);
+  return rc;
 }
 
 int p3g_prove(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* params, const p3g_matrix* traces, uint64_t k,

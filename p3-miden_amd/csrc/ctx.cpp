@@ -85,7 +85,7 @@ Tree::~Tree() {
   if (ctx) {
     for (auto& m : mats)
       if (m.owned) ctx->release(m.ptr);
-    ctx->release(digests);
+    if (owns_digests) ctx->release(digests);
   }
 }
 

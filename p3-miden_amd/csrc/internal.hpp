@@ -120,13 +120,32 @@ struct Tree {
   uint64_t height = 0;
   unsigned depth = 0;
   uint64_t* digests = nullptr;  // all layers, layer l at layer_off[l] (in 4-word digests)
+  bool owns_digests = true;
   std::vector<uint64_t> layer_off;
   uint64_t root[4] = {0, 0, 0, 0};
   ~Tree();
 };
 
 // ------------------------------------------------------------------ transcript
-struct Transcript {
+// The prover-channel operations the PCS driver performs, in the reference's
+// ProverChannel vocabulary (p3-miden-transcript/src/prover.rs:53-95).
+struct Channel {
+  virtual ~Channel() = default;
+  virtual void send_fields(const uint64_t* v, size_t n) = 0;
+  virtual void send_commitment(const uint64_t* c4) = 0;
+  virtual void hint_fields(const uint64_t* v, size_t n) = 0;
+  virtual void hint_commitment(const uint64_t* c4) = 0;
+  virtual uint64_t grind(unsigned bits) = 0;
+  virtual uint64_t sample() = 0;
+  virtual uint64_t sample_bits(unsigned bits) = 0;
+  ef2 sample_ef() {
+    const uint64_t a = sample();
+    const uint64_t b = sample();
+    return ef2{a, b};
+  }
+};
+
+struct Transcript : Channel {
   uint32_t kind = CH_DUPLEX_P2;
   uint64_t st[12] = {};
   uint64_t in[8] = {};
@@ -141,14 +160,13 @@ struct Transcript {
 
   void observe(uint64_t v);
   void observe_commitment(const uint64_t* c4);
-  uint64_t sample();
-  ef2 sample_ef();
-  uint64_t sample_bits(unsigned bits);
-  void send_fields(const uint64_t* v, size_t n);
-  void send_commitment(const uint64_t* c4);
-  void hint_fields(const uint64_t* v, size_t n);
-  void hint_commitment(const uint64_t* c4);
-  uint64_t grind(unsigned bits);
+  uint64_t sample() override;
+  uint64_t sample_bits(unsigned bits) override;
+  void send_fields(const uint64_t* v, size_t n) override;
+  void send_commitment(const uint64_t* c4) override;
+  void hint_fields(const uint64_t* v, size_t n) override;
+  void hint_commitment(const uint64_t* c4) override;
+  uint64_t grind(unsigned bits) override;
   bool check_witness(unsigned bits, uint64_t w);
 
  private:
@@ -174,7 +192,7 @@ void bit_reverse_rows(Ctx& ctx, const uint64_t* src, uint64_t* dst, uint64_t h, 
 
 Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s);
 void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out);
-void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Transcript& tx, hipStream_t s);
+void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s);
 void gather_to_host(Ctx& ctx, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces,
                     std::vector<uint64_t>& out, hipStream_t s);
 
@@ -184,6 +202,6 @@ void host_compress(uint32_t hash, const uint64_t* l, const uint64_t* r, uint64_t
 Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, unsigned log_blowup, hipStream_t s);
 
 void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned log_lde_height, const ef2* z, int npts,
-                       const std::vector<Tree*>& trees, Transcript& tx, hipStream_t s);
+                       const std::vector<Tree*>& trees, Channel& tx, hipStream_t s);
 
 }  // namespace p3g

@@ -20,6 +20,8 @@
 // Each pass runs up to NTT_K butterfly stages on a 2^K x C tile staged in LDS;
 // twiddles come from two-level power tables (one multiply per twiddle),
 // computed once per workgroup into LDS.
+#include <cstring>
+
 #include "internal.hpp"
 
 namespace p3g {
@@ -45,12 +47,11 @@ __device__ __forceinline__ uint64_t pw(const uint64_t* __restrict__ tab, uint64_
   return gl_mul(tab[e & (POW_LO - 1)], tab[POW_LO + (e >> 11)]);
 }
 
-__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4))) ntt_pass_kernel(NttArgs a) {
+__device__ __forceinline__ void ntt_pass_body(const NttArgs& a, uint64_t bid) {
   __shared__ uint64_t tile[(1 << NTT_K) * NTT_C];
   __shared__ uint64_t tws[1 << NTT_K];
   const uint32_t K = a.K, C = a.C, tid = threadIdx.x;
   const bool inverse = a.mode <= INV;
-  uint64_t bid = blockIdx.x;
   const uint32_t ct = (uint32_t)(bid % a.ncoltiles);
   bid /= a.ncoltiles;
   // unit decomposition
@@ -175,13 +176,12 @@ __global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4)))
 __device__ __forceinline__ uint32_t swz(uint32_t x, uint32_t c) { return x * 16 + (c ^ ((x >> 4) & 15)); }
 
 template <uint32_t MODE>
-__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4))) ntt8_kernel(NttArgs a) {
+__device__ __forceinline__ void ntt8_body(const NttArgs& a, uint64_t bid) {
   __shared__ uint64_t tile[256 * 16];
   __shared__ uint64_t tws[256];
   const uint32_t tid = threadIdx.x;
   const uint32_t c = tid & 15, g = tid >> 4;
   constexpr bool inverse = MODE <= INV;
-  uint64_t bid = blockIdx.x;
   const uint32_t ct = (uint32_t)(bid % a.ncoltiles);
   bid /= a.ncoltiles;
   const uint32_t lb = inverse ? a.s0 : a.logn - a.s0 - 8;
@@ -315,6 +315,38 @@ __global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
+__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4))) ntt_pass_kernel(NttArgs a) {
+  ntt_pass_body(a, blockIdx.x);
+}
+
+template <uint32_t MODE>
+__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4))) ntt8_kernel(NttArgs a) {
+  ntt8_body<MODE>(a, blockIdx.x);
+}
+
+// One launch for the same pass of several transforms (the traces of a batch):
+// block b belongs to descriptor k with starts[k] <= b < starts[k + 1].
+__device__ __forceinline__ uint32_t find_desc(const uint64_t* __restrict__ starts, uint32_t nd, uint64_t b) {
+  uint32_t k = 0;
+  while (k + 1 < nd && starts[k + 1] <= b) k++;
+  return k;
+}
+
+__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4)))
+ntt_pass_grouped_kernel(const NttArgs* __restrict__ d, const uint64_t* __restrict__ starts, uint32_t nd) {
+  const uint32_t k = find_desc(starts, nd, blockIdx.x);
+  const NttArgs a = d[k];
+  ntt_pass_body(a, blockIdx.x - starts[k]);
+}
+
+template <uint32_t MODE>
+__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4)))
+ntt8_grouped_kernel(const NttArgs* __restrict__ d, const uint64_t* __restrict__ starts, uint32_t nd) {
+  const uint32_t k = find_desc(starts, nd, blockIdx.x);
+  const NttArgs a = d[k];
+  ntt8_body<MODE>(a, blockIdx.x - starts[k]);
+}
+
 void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
                      uint64_t shift, uint64_t* d_out, hipStream_t s) {
   if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
@@ -427,6 +459,53 @@ void bit_reverse_rows(Ctx& ctx, const uint64_t* src, uint64_t* dst, uint64_t h, 
   (void)ctx;
   if (h == 0 || w == 0) return;
   hipLaunchKernelGGL(bitrev_rows_kernel, dim3((uint32_t)h), dim3(64), 0, s, src, dst, h, log2_exact(h), w);
+  P3G_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace p3g
+
+namespace p3g {
+
+size_t ntt_args_size() { return sizeof(NttArgs); }
+
+void make_ntt_args(void* out, const uint64_t* src, uint64_t* dst, const uint64_t* tw, const uint64_t* scale,
+                   uint64_t scale_stride, uint64_t n, uint32_t logn, uint32_t s0, uint32_t K, uint32_t w, uint32_t logC,
+                   uint32_t ncol, uint32_t logB, uint32_t mode) {
+  NttArgs a{};
+  a.src = src;
+  a.dst = dst;
+  a.tw = tw;
+  a.scale = scale;
+  a.scale_stride = scale_stride;
+  a.n = n;
+  a.logn = logn;
+  a.s0 = s0;
+  a.K = K;
+  a.w = w;
+  a.C = 1u << logC;
+  a.logC = logC;
+  a.ncoltiles = ncol;
+  a.logB = logB;
+  a.mode = mode;
+  memcpy(out, &a, sizeof(a));
+}
+
+// Grouped launch of one pass over several transforms (no host interaction:
+// safe inside stream capture).
+void launch_ntt_grouped(uint32_t mode, bool ntt8, const void* descs, const uint64_t* starts, uint32_t nd,
+                        uint64_t blocks, hipStream_t s) {
+  const NttArgs* d = reinterpret_cast<const NttArgs*>(descs);
+  const dim3 grid((uint32_t)blocks), block(NTT_T);
+  if (ntt8) {
+    switch (mode) {
+      case INV_FIRST: hipLaunchKernelGGL(ntt8_grouped_kernel<INV_FIRST>, grid, block, 0, s, d, starts, nd); break;
+      case INV: hipLaunchKernelGGL(ntt8_grouped_kernel<INV>, grid, block, 0, s, d, starts, nd); break;
+      case FWD_FIRST: hipLaunchKernelGGL(ntt8_grouped_kernel<FWD_FIRST>, grid, block, 0, s, d, starts, nd); break;
+      default: hipLaunchKernelGGL(ntt8_grouped_kernel<FWD>, grid, block, 0, s, d, starts, nd); break;
+    }
+  } else {
+    hipLaunchKernelGGL(ntt_pass_grouped_kernel, grid, block, 0, s, d, starts, nd);
+  }
   P3G_CHECK_HIP(hipGetLastError());
 }
 

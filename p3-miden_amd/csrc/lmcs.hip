@@ -29,7 +29,7 @@
 
 namespace p3g {
 
-constexpr int LEAF_T = 128;
+constexpr int LEAF_T = 128;  // default lanes per workgroup for leaf hashing
 
 struct LeafMat {
   const uint64_t* ptr;
@@ -47,8 +47,9 @@ struct LeafArgs {
   const P2Consts* p2;
 };
 
-__global__ void __launch_bounds__(LEAF_T) p2_leaf_kernel(LeafArgs a) {
-  const uint64_t i = (uint64_t)blockIdx.x * LEAF_T + threadIdx.x;
+template <int T>
+__global__ void __launch_bounds__(T) p2_leaf_kernel(LeafArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x;
   if (i >= a.h) return;
   uint64_t st[12];
   if (a.prev_states) {
@@ -96,8 +97,9 @@ __global__ void __launch_bounds__(LEAF_T) p2_leaf_kernel(LeafArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(LEAF_T) keccak_leaf_kernel(LeafArgs a) {
-  const uint64_t i = (uint64_t)blockIdx.x * LEAF_T + threadIdx.x;
+template <int T>
+__global__ void __launch_bounds__(T) keccak_leaf_kernel(LeafArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x;
   if (i >= a.h) return;
   uint64_t st[25];
   if (a.prev_states) {
@@ -240,8 +242,8 @@ static void launch_leaf(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& grou
   uint32_t grid = (uint32_t)((h + LEAF_T - 1) / LEAF_T);
   {
   ProfScope ps(ctx, hash == HASH_P2 ? "p2_leaf" : "keccak_leaf", s);
-  if (hash == HASH_P2) hipLaunchKernelGGL(p2_leaf_kernel, dim3(grid), dim3(LEAF_T), 0, s, a);
-  else hipLaunchKernelGGL(keccak_leaf_kernel, dim3(grid), dim3(LEAF_T), 0, s, a);
+  if (hash == HASH_P2) hipLaunchKernelGGL(p2_leaf_kernel<LEAF_T>, dim3(grid), dim3(LEAF_T), 0, s, a);
+  else hipLaunchKernelGGL(keccak_leaf_kernel<LEAF_T>, dim3(grid), dim3(LEAF_T), 0, s, a);
   P3G_CHECK_HIP(hipGetLastError());
   }
   // the descriptor copy is from pageable host memory: complete it before hm dies
@@ -249,34 +251,86 @@ static void launch_leaf(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& grou
   ctx.release(dm);
 }
 
-static void build_layers(Ctx& ctx, Tree& t, hipStream_t s) {
-  uint64_t len = t.height;
+// All compression layers of a tree whose leaf digests are in place.  Only
+// kernel launches (safe inside stream capture); ctx may be null (no timing).
+static void compress_layers(Ctx* ctx, uint32_t hash, uint64_t* digests, const std::vector<uint64_t>& layer_off,
+                            uint64_t height, const P2Consts* p2, hipStream_t s) {
+  uint64_t len = height;
   unsigned l = 0;
   while (len > 2 * TOP_T) {
     uint64_t nl = len / 2;
     uint32_t grid = (uint32_t)((nl + CMP_T - 1) / CMP_T);
-    const uint64_t* prev = t.digests + t.layer_off[l] * 4;
-    uint64_t* next = t.digests + t.layer_off[l + 1] * 4;
-    ProfScope ps(ctx, "compress_layer", s);
-    if (t.hash == HASH_P2)
-      hipLaunchKernelGGL(compress_layer_kernel<HASH_P2>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, ctx.dev_p2);
-    else
-      hipLaunchKernelGGL(compress_layer_kernel<HASH_KECCAK>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl,
-                         ctx.dev_p2);
+    const uint64_t* prev = digests + layer_off[l] * 4;
+    uint64_t* next = digests + layer_off[l + 1] * 4;
+    if (ctx) {
+      ProfScope ps(*ctx, "compress_layer", s);
+      if (hash == HASH_P2) hipLaunchKernelGGL(compress_layer_kernel<HASH_P2>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
+      else hipLaunchKernelGGL(compress_layer_kernel<HASH_KECCAK>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
+    } else {
+      if (hash == HASH_P2) hipLaunchKernelGGL(compress_layer_kernel<HASH_P2>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
+      else hipLaunchKernelGGL(compress_layer_kernel<HASH_KECCAK>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
+    }
     P3G_CHECK_HIP(hipGetLastError());
     len = nl;
     l++;
   }
   if (len > 1) {
-    ProfScope ps(ctx, "compress_top", s);
-    if (t.hash == HASH_P2)
-      hipLaunchKernelGGL(compress_top_kernel<HASH_P2>, dim3(1), dim3(TOP_T), 0, s, t.digests, t.layer_off[l], len,
-                         ctx.dev_p2);
-    else
-      hipLaunchKernelGGL(compress_top_kernel<HASH_KECCAK>, dim3(1), dim3(TOP_T), 0, s, t.digests, t.layer_off[l],
-                         len, ctx.dev_p2);
+    auto top = [&] {
+      if (hash == HASH_P2)
+        hipLaunchKernelGGL(compress_top_kernel<HASH_P2>, dim3(1), dim3(TOP_T), 0, s, digests, layer_off[l], len, p2);
+      else
+        hipLaunchKernelGGL(compress_top_kernel<HASH_KECCAK>, dim3(1), dim3(TOP_T), 0, s, digests, layer_off[l], len, p2);
+    };
+    if (ctx) {
+      ProfScope ps(*ctx, "compress_top", s);
+      top();
+    } else {
+      top();
+    }
     P3G_CHECK_HIP(hipGetLastError());
   }
+}
+
+static void build_layers(Ctx& ctx, Tree& t, hipStream_t s) {
+  compress_layers(&ctx, t.hash, t.digests, t.layer_off, t.height, ctx.dev_p2, s);
+}
+
+void launch_compress_layers(uint32_t hash, uint64_t* digests, const std::vector<uint64_t>& layer_off,
+                            uint64_t height, const P2Consts* p2, hipStream_t s) {
+  compress_layers(nullptr, hash, digests, layer_off, height, p2, s);
+}
+
+size_t leaf_mat_size() { return sizeof(LeafMat); }
+void make_leaf_mat(void* out, const uint64_t* ptr, uint64_t width) {
+  LeafMat m{ptr, width};
+  memcpy(out, &m, sizeof(m));
+}
+
+// Leaf hashing of one height group with device-resident matrix descriptors
+// (no host interaction: safe inside stream capture).
+void launch_leaf_raw(uint32_t hash, int threads, const void* mats, uint32_t nmats, const uint64_t* prev,
+                     uint32_t prev_shift, uint64_t* out_states, uint64_t* out_digests, uint64_t h,
+                     const P2Consts* p2, hipStream_t s) {
+  LeafArgs a{};
+  a.mats = reinterpret_cast<const LeafMat*>(mats);
+  a.nmats = nmats;
+  a.prev_states = prev;
+  a.prev_shift = prev_shift;
+  a.out_states = out_states;
+  a.out_digests = out_digests;
+  a.h = h;
+  a.p2 = p2;
+  const uint32_t grid = (uint32_t)((h + threads - 1) / threads);
+  if (hash == HASH_P2) {
+    if (threads == 64) hipLaunchKernelGGL(p2_leaf_kernel<64>, dim3(grid), dim3(64), 0, s, a);
+    else if (threads == 256) hipLaunchKernelGGL(p2_leaf_kernel<256>, dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(p2_leaf_kernel<128>, dim3(grid), dim3(128), 0, s, a);
+  } else {
+    if (threads == 64) hipLaunchKernelGGL(keccak_leaf_kernel<64>, dim3(grid), dim3(64), 0, s, a);
+    else if (threads == 256) hipLaunchKernelGGL(keccak_leaf_kernel<256>, dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(keccak_leaf_kernel<128>, dim3(grid), dim3(128), 0, s, a);
+  }
+  P3G_CHECK_HIP(hipGetLastError());
 }
 
 Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s) {
@@ -383,7 +437,7 @@ void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out) {
   }
 }
 
-void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Transcript& tx, hipStream_t s) {
+void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s) {
   std::set<uint64_t> uniq(indices.begin(), indices.end());
   for (uint64_t idx : uniq)
     if (idx >= t.height) throw ApiError(E_RANGE, "query index out of range");

@@ -425,7 +425,7 @@ static void host_idft_ext(std::vector<ef2>& v) {
 }
 
 void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z, int npts,
-                       const std::vector<Tree*>& trees, Transcript& tx, hipStream_t s) {
+                       const std::vector<Tree*>& trees, Channel& tx, hipStream_t s) {
   if (trees.empty()) throw ApiError(E_EMPTY, "at least one trace tree required");
   if (npts < 1 || npts > MAXPTS) throw ApiError(E_INVAL, "1 to 4 evaluation points supported");
   if (p.log_arity < 1 || p.log_arity > 3) throw ApiError(E_INVAL, "invalid folding arity");

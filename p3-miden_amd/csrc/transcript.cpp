@@ -104,12 +104,6 @@ uint64_t Transcript::sample() {
   }
 }
 
-ef2 Transcript::sample_ef() {
-  uint64_t a = sample();
-  uint64_t b = sample();
-  return ef2{a, b};
-}
-
 uint64_t Transcript::sample_bits(unsigned bits) {
   uint64_t v = sample();
   return v & ((1ull << bits) - 1);

@@ -85,6 +85,50 @@ class PcsParams(C.Structure):
         return 1 << max(0, lfs - self.log_blowup)
 
 
+_SEND = C.CFUNCTYPE(C.c_int, C.c_void_p, _u64p, C.c_uint64)
+_COMM = C.CFUNCTYPE(C.c_int, C.c_void_p, _u64p)
+_GRIND = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, _u64p)
+_SAMPLE = C.CFUNCTYPE(C.c_int, C.c_void_p, _u64p)
+_SBITS = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, _u64p)
+
+
+class _Channel(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("send_fields", _SEND), ("send_commitment", _COMM), ("hint_fields", _SEND),
+                ("hint_commitment", _COMM), ("grind", _GRIND), ("sample", _SAMPLE), ("sample_bits", _SBITS)]
+
+
+class CallbackChannel:
+    """Adapts a caller-side prover channel (any object with send_field_slice,
+    send_commitment, hint_field_slice, hint_commitment, grind, sample and
+    sample_bits) to the library's p3g_channel callbacks."""
+
+    def __init__(self, channel):
+        self.channel = channel
+        ch = channel
+
+        def arr(ptr, n):
+            return [int(ptr[i]) for i in range(n)]
+
+        def wrap(fn):
+            def inner(*args):
+                try:
+                    return fn(*args) or 0
+                except Exception:
+                    return 1
+            return inner
+
+        self._fns = (
+            _SEND(wrap(lambda u, p, n: ch.send_field_slice(arr(p, n)))),
+            _COMM(wrap(lambda u, p: ch.send_commitment(arr(p, 4)))),
+            _SEND(wrap(lambda u, p, n: ch.hint_field_slice(arr(p, n)))),
+            _COMM(wrap(lambda u, p: ch.hint_commitment(arr(p, 4)))),
+            _GRIND(wrap(lambda u, b, out: out.__setitem__(0, ch.grind(b)))),
+            _SAMPLE(wrap(lambda u, out: out.__setitem__(0, ch.sample()))),
+            _SBITS(wrap(lambda u, b, out: out.__setitem__(0, ch.sample_bits(b)))),
+        )
+        self.struct = _Channel(None, *self._fns)
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         return None
@@ -137,6 +181,15 @@ def _load():
         "p3g_transcript_num_commitments": (C.c_uint64, [_vp]),
         "p3g_transcript_data": (C.c_int, [_vp, _u64p, _u64p]),
         "p3g_open_with_channel": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, _vp]),
+        "p3g_tree_prove_batch_cb": (C.c_int, [_vp, _u64p, C.c_uint64, C.POINTER(_Channel)]),
+        "p3g_open_with_callbacks": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, C.POINTER(_Channel)]),
+        "p3g_commit_plan_create": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+        "p3g_commit_plan_input": (C.c_int, [_vp, C.c_uint64, C.POINTER(_vp)]),
+        "p3g_commit_plan_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "p3g_commit_plan_run": (C.c_int, [_vp, C.POINTER(_vp)]),
+        "p3g_commit_plan_destroy": (None, [_vp]),
+        "p3g_prove_planned": (C.c_int, [_vp, _vp, C.POINTER(PcsParams), _vp, _u64p, _u64p]),
+        "p3g_fill_random_device": (C.c_int, [_vp, C.c_uint64, C.c_uint64, _vp]),
         "p3g_prove": (C.c_int, [_vp, C.c_uint32, C.POINTER(PcsParams), C.POINTER(_Matrix), C.c_uint64, C.c_uint32, _vp, _u64p, _u64p]),
     }
     for name, (res, args) in sig.items():
@@ -414,9 +467,14 @@ class LiftedMerkleTree:
         _check(lib().p3g_tree_matrix(self.h, i, C.byref(ptr), _p(h), _p(w)), "tree_matrix")
         return ptr, int(h[0]), int(w[0])
 
-    def prove_batch(self, indices, channel: ProverTranscript):
+    def prove_batch(self, indices, channel):
         a = u64(list(indices))
-        _check(lib().p3g_tree_prove_batch(self.h, _p(a) if a.size else None, a.size, channel.h), "prove_batch")
+        if isinstance(channel, ProverTranscript):
+            _check(lib().p3g_tree_prove_batch(self.h, _p(a) if a.size else None, a.size, channel.h), "prove_batch")
+        else:
+            cb = channel if isinstance(channel, CallbackChannel) else CallbackChannel(channel)
+            _check(lib().p3g_tree_prove_batch_cb(self.h, _p(a) if a.size else None, a.size, C.byref(cb.struct)),
+                   "prove_batch_cb")
 
 
 class LmcsConfig:
@@ -468,11 +526,18 @@ def commit_traces(lmcs: LmcsConfig, traces, log_blowup: int) -> LiftedMerkleTree
 
 
 def open_with_channel(params: PcsParams, lmcs: LmcsConfig, log_lde_height: int, eval_points, trace_trees,
-                      channel: ProverTranscript) -> None:
+                      channel) -> None:
+    """Runs on the library's own transcript when given a ProverTranscript; any other
+    channel object is driven through callbacks (CallbackChannel)."""
     z = u64(np.asarray(eval_points, dtype=np.uint64).ravel())
     hs = (_vp * len(trace_trees))(*[t.h for t in trace_trees])
-    _check(lib().p3g_open_with_channel(lmcs.dev.h, C.byref(params), log_lde_height, _p(z), z.size // 2, hs,
-                                       len(trace_trees), channel.h), "open_with_channel")
+    if isinstance(channel, ProverTranscript):
+        _check(lib().p3g_open_with_channel(lmcs.dev.h, C.byref(params), log_lde_height, _p(z), z.size // 2, hs,
+                                           len(trace_trees), channel.h), "open_with_channel")
+    else:
+        cb = channel if isinstance(channel, CallbackChannel) else CallbackChannel(channel)
+        _check(lib().p3g_open_with_callbacks(lmcs.dev.h, C.byref(params), log_lde_height, _p(z), z.size // 2, hs,
+                                             len(trace_trees), C.byref(cb.struct)), "open_with_callbacks")
 
 
 def prove(lmcs: LmcsConfig, params: PcsParams, traces, channel: ProverTranscript = None):
@@ -487,3 +552,73 @@ def prove(lmcs: LmcsConfig, params: PcsParams, traces, channel: ProverTranscript
                            DEVICE_PTRS if dev else 0, channel.h, _p(root), _p(z)), "prove")
     fields, comms = channel.data()
     return root, ((int(z[0]), int(z[1])), (int(z[2]), int(z[3]))), fields, comms
+
+
+PLAN_GRAPH = 1
+PLAN_AUTOTUNE = 2
+
+
+class CommitPlan:
+    """Batched commit of many traces of fixed shapes: grouped NTT launches over
+    all traces, optional hipGraph replay and autotuned launch shapes.  Results
+    are identical to commit_traces on the same traces."""
+
+    def __init__(self, lmcs: LmcsConfig, shapes, log_blowup: int, graph: bool = True, autotune: bool = False):
+        self.lmcs = lmcs
+        self.shapes = [(int(h), int(w)) for h, w in shapes]
+        self.log_blowup = log_blowup
+        hs = u64([h for h, _ in self.shapes]) if shapes else np.zeros(1, np.uint64)
+        ws = u64([w for _, w in self.shapes]) if shapes else np.zeros(1, np.uint64)
+        h = _vp()
+        flags = (PLAN_GRAPH if graph else 0) | (PLAN_AUTOTUNE if autotune else 0)
+        _check(lib().p3g_commit_plan_create(lmcs.dev.h, lmcs.hash_kind, _p(hs), _p(ws), len(self.shapes), log_blowup,
+                                            flags, C.byref(h)), "commit_plan_create")
+        self.h = h
+
+    def __del__(self):
+        self.free()
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().p3g_commit_plan_destroy(self.h)
+            self.h = None
+
+    def input_ptr(self, i: int) -> int:
+        ptr = _vp()
+        _check(lib().p3g_commit_plan_input(self.h, i, C.byref(ptr)), "commit_plan_input")
+        return ptr.value
+
+    def set_input(self, i: int, mat: np.ndarray) -> None:
+        a = u64(mat)
+        assert a.shape == self.shapes[i], (a.shape, self.shapes[i])
+        _check(lib().p3g_upload(self.lmcs.dev.h, self.input_ptr(i), _p(a), a.size * 8), "upload")
+
+    def fill_random(self, seed: int) -> None:
+        for i, (h, w) in enumerate(self.shapes):
+            _check(lib().p3g_fill_random_device(self.lmcs.dev.h, seed + i, h * w, self.input_ptr(i)),
+                   "fill_random_device")
+
+    def info(self):
+        k, t, n = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(lib().p3g_commit_plan_info(self.h, C.byref(k), C.byref(t), C.byref(n)), "commit_plan_info")
+        return {"kmax": k.value, "leaf_threads": t.value, "launches": n.value}
+
+    def run(self) -> LiftedMerkleTree:
+        """Recompute LDEs and tree; the returned view is valid until the next run."""
+        t = _vp()
+        _check(lib().p3g_commit_plan_run(self.h, C.byref(t)), "commit_plan_run")
+        return LiftedMerkleTree(t, self.lmcs.hash_kind)
+
+    def prove(self, params: PcsParams, channel: ProverTranscript = None):
+        if channel is None:
+            channel = ProverTranscript(CHALLENGER_DUPLEX_P2 if self.lmcs.hash_kind == HASH_POSEIDON2 else CHALLENGER_KECCAK)
+        root = np.zeros(4, np.uint64)
+        z = np.zeros(4, np.uint64)
+        _check(lib().p3g_prove_planned(self.lmcs.dev.h, self.h, C.byref(params), channel.h, _p(root), _p(z)),
+               "prove_planned")
+        fields, comms = channel.data()
+        return root, ((int(z[0]), int(z[1])), (int(z[2]), int(z[3]))), fields, comms
+
+
+def fill_random_device(device: Device, seed: int, n: int, ptr) -> None:
+    _check(lib().p3g_fill_random_device(device.h, seed, n, ptr), "fill_random_device")

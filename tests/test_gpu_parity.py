@@ -168,3 +168,125 @@ def test_repeated_proofs_are_identical(device):
     a = G.prove(cfg, p, tr)
     b = G.prove(cfg, p, tr)
     assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+
+
+class _OracleChannel:
+    """A caller-owned prover channel (the oracle's transcript) in ProverChannel vocabulary."""
+
+    def __init__(self, tx):
+        self.tx = tx
+
+    def send_field_slice(self, v): self.tx.send_fields(v)
+    def send_commitment(self, c): self.tx.send_commitment(c)
+    def hint_field_slice(self, v): self.tx.hint_fields(v)
+    def hint_commitment(self, c): self.tx.hint_commitment(c)
+    def grind(self, bits): return self.tx.grind(bits)
+    def sample(self): return self.tx.sample()
+    def sample_bits(self, bits): return self.tx.sample_bits(bits)
+
+
+@pytest.mark.parametrize("kind", [G.HASH_POSEIDON2, G.HASH_KECCAK])
+def test_open_through_caller_channel_callbacks(device, kind):
+    # the drop-in path: the caller's own channel is driven by the library's callbacks
+    cfg = G.LmcsConfig(device, kind)
+    ch_kind = 0 if kind == G.HASH_POSEIDON2 else 1
+    tr = _mats(21, [(128, 9), (256, 20)])
+    params = (1, 2, 2, 1, 2, 20, 3)
+    gt = G.commit_traces(cfg, tr, 1)
+    ot = O.commit_traces(kind, tr, 1)
+    z = [(101, 202), (303, 404)]
+    via_cb = O.Transcript(ch_kind)
+    G.open_with_channel(G.PcsParams(*params), cfg, 9, z, [gt], _OracleChannel(via_cb))
+    want = O.Transcript(ch_kind)
+    O.open_with_channel(O.PcsParams(*params), kind, [ot], 9, z, want)
+    a, b = via_cb.data(), want.data()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # a failing callback aborts with a transcript status
+    class Broken(_OracleChannel):
+        def sample(self):
+            raise RuntimeError("channel closed")
+    with pytest.raises(G.P3GError) as e:
+        G.open_with_channel(G.PcsParams(*params), cfg, 9, z, [gt], Broken(O.Transcript(ch_kind)))
+    assert e.value.code == 7
+
+
+# ---------------------------------------------------------------- batched commit plans
+PLAN_CASES = [
+    (G.HASH_POSEIDON2, [(1 << 10, 8)], 1, True, False),
+    (G.HASH_POSEIDON2, [(16, 3), (256, 10), (1024, 50), (1024, 16)], 2, True, True),
+    (G.HASH_POSEIDON2, [(2, 1), (8, 20), (4096, 33)], 1, False, False),
+    (G.HASH_POSEIDON2, [(1 << 12, 8), (1 << 13, 100), (1 << 14, 17)], 1, True, True),
+    (G.HASH_KECCAK, [(64, 40), (256, 17), (256, 2)], 1, True, False),
+]
+
+
+@pytest.mark.parametrize("kind,shapes,lb,graph,autotune", PLAN_CASES)
+def test_commit_plan_matches_oracle(device, kind, shapes, lb, graph, autotune):
+    cfg = G.LmcsConfig(device, kind)
+    plan = G.CommitPlan(cfg, shapes, lb, graph=graph, autotune=autotune)
+    for rnd in range(2):  # the second run replays the same plan on new inputs
+        tr = _mats(500 + 17 * rnd, shapes)
+        for i, m in enumerate(tr):
+            plan.set_input(i, m)
+        got = plan.run()
+        want = O.commit_traces(kind, tr, lb)
+        assert got.root().tolist() == want.root().tolist(), (rnd, plan.info())
+        H = got.height()
+        for l in (0, 1, H.bit_length() - 2):
+            assert np.array_equal(got.layer(l), want.layer(l)), (rnd, l)
+        assert got.rows(H - 3).tolist() == want.rows(H - 3).tolist()
+        got.free()
+
+
+@pytest.mark.parametrize("kind,shapes,params", [
+    (G.HASH_POSEIDON2, [(8, 2), (64, 11), (256, 7), (256, 30)], (1, 1, 1, 0, 0, 40, 0)),
+    (G.HASH_POSEIDON2, [(1 << 10, 16)], (2, 2, 4, 0, 0, 100, 0)),
+    (G.HASH_KECCAK, [(32, 18), (128, 17)], (2, 2, 1, 0, 1, 25, 2)),
+])
+def test_prove_planned_matches_oracle(device, kind, shapes, params):
+    tr = _mats(2025, shapes)
+    op = O.PcsParams(*params)
+    root_o, z_o, f_o, c_o, _ = O.pcs_trace_prove(op, kind, tr)
+    plan = G.CommitPlan(G.LmcsConfig(device, kind), shapes, params[0], graph=True)
+    for i, m in enumerate(tr):
+        plan.set_input(i, m)
+    for _ in range(2):  # graph replays give the same proof
+        root_g, z_g, f_g, c_g = plan.prove(G.PcsParams(*params))
+        assert root_g.tolist() == root_o.tolist() and z_g == tuple(z_o)
+        assert np.array_equal(f_g, f_o) and np.array_equal(c_g, c_o)
+
+
+def test_commit_plan_rejects_bad_shapes(device):
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    for shapes, code in [([], 6), ([(8, 3), (4, 5)], 3), ([(6, 3)], 2), ([(8, 0)], 1)]:
+        with pytest.raises(G.P3GError) as e:
+            G.CommitPlan(cfg, shapes, 1)
+        assert e.value.code == code, shapes
+    plan = G.CommitPlan(cfg, [(8, 3)], 1)
+    with pytest.raises(G.P3GError):
+        plan.input_ptr(1)
+    with pytest.raises(G.P3GError):
+        plan.prove(G.PcsParams(2, 1, 0, 0, 0, 4, 0))  # log_blowup differs from the plan's
+
+
+def _splitmix_fill(seed, n):
+    golden, p, m = 0x9E3779B97F4A7C15, (1 << 64) - (1 << 32) + 1, (1 << 64) - 1
+    out = []
+    for i in range(n):
+        z = (seed + (i + 1) * golden) & m
+        while True:
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+            z ^= z >> 31
+            if z < p:
+                break
+            z = (z + golden) & m
+        out.append(z)
+    return out
+
+
+def test_fill_random_device(device):
+    d = device.upload(np.zeros((1, 1000), np.uint64))
+    G.fill_random_device(device, 99, 1000, d.ptr)
+    got = d.download().ravel().tolist()
+    assert got == _splitmix_fill(99, 1000)
