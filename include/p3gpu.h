@@ -70,6 +70,7 @@ const char* p3g_last_error(void); /* thread-local detail of the last failure */
 
 /* ---- context (one per device; all work runs on its stream) ---- */
 int p3g_ctx_create(int device, p3g_ctx** out);
+/* frees all device memory of the context: free its trees and plans first (they refer to it) */
 void p3g_ctx_destroy(p3g_ctx* ctx);
 int p3g_ctx_synchronize(p3g_ctx* ctx);
 int p3g_ctx_trim(p3g_ctx* ctx); /* release cached device blocks */

@@ -20,6 +20,7 @@ library is missing or no GPU is visible, the constructors raise.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -236,9 +237,20 @@ class Device:
         _check(lib().p3g_ctx_create(device, C.byref(h)), "ctx_create")
         self.h = h
         self.index = device
+        # trees, plans and matrices living in this context: freed before the context is
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
+        return obj
 
     def close(self):
         if getattr(self, "h", None):
+            for c in list(self._children):
+                try:
+                    c.free()
+                except Exception:
+                    pass
             lib().p3g_ctx_destroy(self.h)
             self.h = None
 
@@ -274,7 +286,7 @@ class Device:
         _check(lib().p3g_alloc(self.h, max(8, a.nbytes), C.byref(ptr)), "alloc")
         _check(lib().p3g_upload(self.h, ptr, a.ctypes.data_as(_vp), a.nbytes), "upload")
         shape = a.shape if a.ndim == 2 else (a.size, 1)
-        return DeviceMatrix(self, ptr, shape[0], shape[1])
+        return self._adopt(DeviceMatrix(self, ptr, shape[0], shape[1]))
 
 
 class DeviceMatrix:
@@ -289,7 +301,7 @@ class DeviceMatrix:
         return out.reshape(self.height, self.width)
 
     def free(self):
-        if self.owned and self.ptr:
+        if self.owned and self.ptr and self.dev.h:
             _check(lib().p3g_free(self.dev.h, self.ptr), "free")
         self.ptr = None
 
@@ -418,16 +430,19 @@ def proof_bytes(fields: np.ndarray, commitments: np.ndarray) -> bytes:
 
 
 class LiftedMerkleTree:
-    def __init__(self, h, hash_kind: int):
+    def __init__(self, h, hash_kind: int, dev: "Device"):
         self.h = h
         self.hash_kind = hash_kind
+        self.dev = dev
+        dev._adopt(self)
 
     def __del__(self):
         self.free()
 
     def free(self):
         if getattr(self, "h", None):
-            lib().p3g_tree_free(self.h)
+            if self.dev.h:
+                lib().p3g_tree_free(self.h)
             self.h = None
 
     def root(self) -> np.ndarray:
@@ -495,7 +510,7 @@ class LmcsConfig:
         if dev and take_ownership:
             for m in mats:
                 m.owned = False
-        return LiftedMerkleTree(h, self.hash_kind)
+        return LiftedMerkleTree(h, self.hash_kind, self.dev)
 
     def build_tree(self, mats, take_ownership=False) -> LiftedMerkleTree:
         return self._build(mats, 1, take_ownership)
@@ -522,7 +537,7 @@ def commit_traces(lmcs: LmcsConfig, traces, log_blowup: int) -> LiftedMerkleTree
     h = _vp()
     _check(lib().p3g_commit_traces(lmcs.dev.h, lmcs.hash_kind, arr, len(traces), log_blowup,
                                    DEVICE_PTRS if dev else 0, C.byref(h)), "commit_traces")
-    return LiftedMerkleTree(h, lmcs.hash_kind)
+    return LiftedMerkleTree(h, lmcs.hash_kind, lmcs.dev)
 
 
 def open_with_channel(params: PcsParams, lmcs: LmcsConfig, log_lde_height: int, eval_points, trace_trees,
@@ -574,13 +589,15 @@ class CommitPlan:
         _check(lib().p3g_commit_plan_create(lmcs.dev.h, lmcs.hash_kind, _p(hs), _p(ws), len(self.shapes), log_blowup,
                                             flags, C.byref(h)), "commit_plan_create")
         self.h = h
+        lmcs.dev._adopt(self)
 
     def __del__(self):
         self.free()
 
     def free(self):
         if getattr(self, "h", None):
-            lib().p3g_commit_plan_destroy(self.h)
+            if self.lmcs.dev.h:
+                lib().p3g_commit_plan_destroy(self.h)
             self.h = None
 
     def input_ptr(self, i: int) -> int:
@@ -607,7 +624,7 @@ class CommitPlan:
         """Recompute LDEs and tree; the returned view is valid until the next run."""
         t = _vp()
         _check(lib().p3g_commit_plan_run(self.h, C.byref(t)), "commit_plan_run")
-        return LiftedMerkleTree(t, self.lmcs.hash_kind)
+        return LiftedMerkleTree(t, self.lmcs.hash_kind, self.lmcs.dev)
 
     def prove(self, params: PcsParams, channel: ProverTranscript = None):
         if channel is None:
