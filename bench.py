@@ -1,22 +1,34 @@
-"""Benchmark: plain single-trace lifted-PCS proving on one MI355X.
+"""Benchmark of the lifted-PCS prover core on one MI355X.
 
-Workload (the single-GPU prove configuration): a Goldilocks trace of 2^22 rows
-x 64 columns, uniform random from SmallRng(2025); log_blowup 2 (LDE of 2^24
-rows), Poseidon2-Goldilocks-12 LMCS, DEEP at 2 extension points, FRI arity 4
-down to degree 2^4, 100 queries.  One step = one full proof with the trace
-already resident in HBM: commit_traces (coset LDE + LMCS tree), observe the
-root, sample two points, open_with_channel (DEEP, FRI commit phase, query
+Headline (`value`): plain single-trace proving of a Goldilocks trace of 2^22
+rows x 64 columns, uniform random from SmallRng(2025); log_blowup 2 (LDE of
+2^24 rows), Poseidon2-Goldilocks-12 LMCS, DEEP at 2 extension points, FRI
+arity 4 down to degree 2^4, 100 queries.  One step = one full proof with the
+trace already resident in HBM: commit_traces (coset LDE + LMCS tree), observe
+the root, sample two points, open_with_channel (DEEP, FRI commit phase, query
 openings); the proof (TranscriptData) is copied back to the host.
 
-Prints one JSON line.  `roofline` describes the kernel family with the most
-device time in the timed region, measured with HIP events on the library's
-stream.  `cpu_baseline` times the oracle (the C restatement of the reference
-CPU prover, OpenMP) on a bounded sample of the same workload.
+Other sections of the same JSON line:
+  batched  - 12 traces of mixed heights (2^12..2^22) and widths (8..256),
+             log_blowup 1, one lifted proof: the batched commit plan (grouped
+             NTT launches, hipGraph replay, autotuned shapes) against the
+             per-trace launch path, with the two proofs compared word for word;
+  keccak   - 2^16 x 2600 bit-structured trace, Keccak LMCS, arity 2, 100 queries;
+  serving  - Poisson arrivals of prove requests (1-8 traces of 2^14..2^22 rows,
+             16..128 columns, log_blowup 1..3) into the prove-request server;
+             saturation throughput, then p50/p99 latency at 0.5x, 0.8x and 1.0x
+             of it, against the same requests proven one after another.
+
+`roofline` describes the kernel family with the most device time in the
+headline proof, timed with HIP events on the library's stream.
+`cpu_baseline` times the oracle (the C restatement of the reference CPU prover,
+OpenMP) on a bounded sample of the headline workload.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -25,8 +37,6 @@ for sub in ("p3-miden_amd", "oracle"):
 
 import numpy as np  # noqa: E402
 
-LOG_N, WIDTH, LOG_BLOWUP = 22, 64, 2
-PARAMS = (2, 2, 4, 0, 0, 100, 0)  # PcsParams(log_blowup, log_arity, log_final_degree, fold/deep pow, queries, query pow)
 SEED = 2025
 
 # MI355X published peaks
@@ -38,19 +48,319 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic(n, w, b):
-    """Algorithmic bytes / permutations per launch of each kernel family, for one proof."""
-    L = n << b
-    trace = 8 * n * w
-    lde = 8 * L * w
+def lde_commit_bytes(shapes, log_blowup):
+    """Algorithmic bytes of LDE + commit: read each trace, write its LDE, write all digest layers."""
+    B = 1 << log_blowup
+    L = max(h for h, _ in shapes) * B
+    return sum(8 * h * w * (1 + B) for h, w in shapes) + 32 * (2 * L - 1)
+
+
+def leaf_perms(shapes, log_blowup, rate):
+    B = 1 << log_blowup
+    return sum(h * B * -(-w // rate) for h, w in shapes)
+
+
+def read_profile_constants():
+    """Per-permutation VALU work and measured HBM traffic of the leaf kernel (rocprofv3 PMC runs)."""
+    out = {}
+    path = os.path.join(ROOT, "profiles", "p2_leaf_c2.md")
+    if os.path.exists(path):
+        for line in open(path):
+            for key in ("valu_lane_ops_per_perm", "hbm_bytes_per_launch"):
+                if line.startswith(key + ":"):
+                    out[key] = float(line.split(":")[1])
+    return out
+
+
+def timed(fn, steps, dev):
+    dev.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    dev.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, out
+
+
+# ------------------------------------------------------------------ headline: plain single-trace proof
+def bench_c2(G, W, dev, steps, warmup):
+    n, w, b = 1 << W.C2_LOG_ROWS, W.C2_WIDTH, W.C2_PARAMS[0]
+    log(f"[c2] generating trace 2^{W.C2_LOG_ROWS} x {w} from SmallRng({SEED})")
+    host = G.fill_random(SEED, n * w).reshape(n, w)
+    trace = dev.upload(host)
+    del host
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    params = G.PcsParams(*W.C2_PARAMS)
+
+    def step():
+        return G.prove(cfg, params, [trace])
+
+    log(f"[c2] warmup x{warmup}")
+    first = None
+    for _ in range(warmup):
+        first = step()
+    dev.profile(True)
+    dev.profile_reset()
+    log(f"[c2] timing x{steps}")
+    ms, out = timed(step, steps, dev)
+    if first is not None:
+        assert np.array_equal(first[2], out[2]) and np.array_equal(first[3], out[3]), "proofs differ between runs"
+    fams = ["lde_inverse_pass", "lde_forward_pass", "p2_leaf", "compress_layer", "compress_top", "deep_weights",
+            "deep_colsum", "deep_quotient", "fri_fold"]
+    per = {}
+    for f in fams:
+        tot, cnt = dev.profile_read(f)
+        per[f] = (tot / steps, cnt // max(1, steps))
+    dev.profile(False)
+    dom = max(per, key=lambda k: per[k][0])
+
+    log("[c2] timing the commit phase (LDE + LMCS) alone")
+    dev.profile(True)
+    dev.profile_reset()
+    reps = max(3, steps // 2)
+
+    def commit():
+        t = G.commit_traces(cfg, [trace], b)
+        t.free()
+
+    commit_ms, _ = timed(commit, reps, dev)
+    leaf_ms, leaf_n = dev.profile_read("p2_leaf")
+    leaf_ms /= max(1, leaf_n)
+    lde_ms = (dev.profile_read("lde_inverse_pass")[0] + dev.profile_read("lde_forward_pass")[0]) / reps
+    dev.profile(False)
+    trace.free()
+    dev.trim()
+
+    shapes = [(n, w)]
+    perms = leaf_perms(shapes, b, 8)
+    lde_bytes = 8 * n * w * (1 + (1 << b))
+    pc = read_profile_constants()
+    roof = {
+        "kernel": "p2_leaf (Poseidon2-GL12 LMCS leaf hashing of the 2^24 x 64 LDE: 2^24 leaves x 8 permutations)",
+        "bound": "valu",
+        "achieved": None,
+        "peak": round(VALU_PEAK_TOPS, 2),
+        "unit": "TOP/s",
+        "frac": None,
+        "traffic": pc.get("hbm_bytes_per_launch"),
+        "launch_ms": round(leaf_ms, 3),
+        "permutations_per_s": round(perms / (leaf_ms * 1e-3), 1) if leaf_ms else None,
+        "hbm_view": {"algorithmic_bytes": 8 * (n << b) * w + 32 * (n << b),
+                     "achieved": round((8 * (n << b) * w + 32 * (n << b)) / (leaf_ms * 1e-3) / 1e9, 1) if leaf_ms else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+    }
+    if pc.get("valu_lane_ops_per_perm") and leaf_ms:
+        ach = perms * pc["valu_lane_ops_per_perm"] / (leaf_ms * 1e-3) / 1e12
+        roof["achieved"] = round(ach, 2)
+        roof["frac"] = round(ach / VALU_PEAK_TOPS, 3)
+    others = {
+        "lde (inverse + forward passes)": {"ms": round(lde_ms, 3), "bytes": lde_bytes,
+                                           "gbps": round(lde_bytes / (lde_ms * 1e-3) / 1e9, 1) if lde_ms else None,
+                                           "peak": HBM_PEAK_GBS},
+    }
+    dq_ms = per["deep_quotient"][0]
+    if dq_ms:
+        dq_bytes = 8 * (n << b) * w + 16 * (n << b)
+        others["deep_quotient"] = {"ms": round(dq_ms, 3), "bytes": dq_bytes,
+                                   "gbps": round(dq_bytes / (dq_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS}
     return {
-        # LDE: read trace, write LDE (counted once over all passes of one trace)
-        "lde": trace + lde,
-        # leaf hashing: read every LDE element once, write one 32-byte digest per leaf
-        "p2_leaf_bytes": lde + 32 * L,
-        "p2_leaf_perms": L * -(-w // 8),
-        # LDE + commit (the headline GB/s): trace in, LDE resident, all digest layers out
-        "lde_commit": trace + lde + 32 * (2 * L - 1),
+        "ms": ms,
+        "lde_commit_gbps": round(lde_commit_bytes(shapes, b) / (commit_ms * 1e-3) / 1e9, 1),
+        "commit_ms": round(commit_ms, 3),
+        "kernel_ms_per_proof": {k: round(v[0], 3) for k, v in per.items()},
+        "dominant_kernel": dom,
+        "roofline": roof,
+        "other_kernels_hbm": others,
+    }
+
+
+# ------------------------------------------------------------------ batched multi-trace proof
+def bench_c3(G, W, dev, steps, warmup):
+    shapes = W.c3_shapes(SEED)
+    params = G.PcsParams(*W.c3_params())
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    log(f"[c3] planning {len(shapes)} traces {shapes} (autotune)")
+    plan = G.CommitPlan(cfg, shapes, 1, graph=True, autotune=True)
+    plan.fill_random(SEED)
+    inputs = [G.DeviceMatrix(dev, plan.input_ptr(i), h, w, owned=False) for i, (h, w) in enumerate(shapes)]
+
+    def plain():
+        return G.prove(cfg, params, inputs)
+
+    def batched():
+        return plan.prove(params)
+
+    a, bt = plain(), batched()
+    bit_exact = (a[0].tolist() == bt[0].tolist() and np.array_equal(a[2], bt[2]) and np.array_equal(a[3], bt[3]))
+    for _ in range(max(0, warmup - 1)):
+        plain()
+        batched()
+    log(f"[c3] timing x{steps} plain, x{steps} batched")
+    plain_ms, _ = timed(plain, steps, dev)
+    batched_ms, _ = timed(batched, steps, dev)
+
+    def commit_plain():
+        G.commit_traces(cfg, inputs, 1).free()
+
+    def commit_batched():
+        plan.run().free()
+
+    cp_ms, _ = timed(commit_plain, steps, dev)
+    cb_ms, _ = timed(commit_batched, steps, dev)
+    info = plan.info()
+    plan.free()
+    dev.trim()
+    nbytes = lde_commit_bytes(shapes, 1)
+    return {
+        "workload": f"12 traces, heights 2^12..2^22 log-uniform, widths 8..256 (seed {SEED}), log_blowup 1, "
+                    "one lifted proof (Poseidon2 LMCS, arity 2 to degree 8, 100 queries)",
+        "shapes": shapes,
+        "plain_proofs_per_s": round(1000.0 / plain_ms, 3),
+        "batched_proofs_per_s": round(1000.0 / batched_ms, 3),
+        "speedup": round(plain_ms / batched_ms, 3),
+        "bit_exact_batched_vs_plain": bool(bit_exact),
+        "plain_commit_ms": round(cp_ms, 3),
+        "batched_commit_ms": round(cb_ms, 3),
+        "batched_lde_commit_gbps": round(nbytes / (cb_ms * 1e-3) / 1e9, 1),
+        "plan": info,
+    }
+
+
+# ------------------------------------------------------------------ Keccak wide trace
+def bench_c4(G, W, dev, steps, warmup):
+    log("[c4] generating bit-structured 2^16 x 2600 trace")
+    host = W.c4_trace(SEED)
+    trace = dev.upload(host)
+    del host
+    cfg = G.LmcsConfig(dev, G.HASH_KECCAK)
+    params = G.PcsParams(*W.C4_PARAMS)
+
+    def step():
+        return G.prove(cfg, params, [trace])
+
+    first = None
+    for _ in range(max(1, warmup)):
+        first = step()
+    dev.profile(True)
+    dev.profile_reset()
+    ms, out = timed(step, steps, dev)
+    assert np.array_equal(first[2], out[2]), "proofs differ between runs"
+    leaf_ms, leaf_n = dev.profile_read("keccak_leaf")
+    dev.profile(False)
+    trace.free()
+    dev.trim()
+    n, w = 1 << W.C4_LOG_ROWS, W.C4_WIDTH
+    perms = leaf_perms([(n, w)], 1, 17)
+    per_launch = leaf_ms / max(1, leaf_n)
+    return {
+        "workload": "2^16 x 2600 trace (90% 0/1 columns, 16-bit limbs), log_blowup 1, Keccak LMCS, arity 2 to "
+                    "degree 8, 100 queries",
+        "proofs_per_s": round(1000.0 / ms, 3),
+        "ms_per_proof": round(ms, 3),
+        "keccak_leaf_ms_per_proof": round(leaf_ms / steps, 3),
+        "keccak_f_per_s": round(perms / (per_launch * 1e-3), 1) if per_launch else None,
+    }
+
+
+# ------------------------------------------------------------------ serving
+def bench_c5(G, W, dev, n_requests, workers):
+    budget = 250 * 10**9
+    rng = np.random.default_rng(SEED)
+    pool_h, pool_w = 1 << 22, 256  # 8.6 GB of device-resident synthetic trace data
+    log("[c5] filling the device trace pool")
+    pool = dev.alloc_matrix(pool_h, pool_w)
+    pool.fill_random(SEED)
+
+    def make_request():
+        while True:
+            shapes, lb = W.c5_request(rng)
+            traces = []
+            for h, w in shapes:
+                off = int(rng.integers(0, pool_h * pool_w - h * w + 1)) & ~31
+                traces.append(pool.view(off, h, w))
+            params = (lb, 2, 4, 0, 0, 100, 0)
+            if G.request_footprint(G.HASH_POSEIDON2, G.PcsParams(*params), traces) <= budget:
+                return traces, params, shapes
+
+    reqs = [make_request() for _ in range(n_requests)]
+    srv = G.Server(0, workers=workers, hbm_budget_bytes=budget)
+    # warm every worker context (tables, allocation caches)
+    for r in [srv.submit(tr, G.PcsParams(*p)) for tr, p, _ in reqs[:workers]]:
+        srv.result(r)
+
+    log(f"[c5] saturation: {n_requests} requests submitted at once")
+    t0 = time.perf_counter()
+    tickets = [srv.submit(tr, G.PcsParams(*p)) for tr, p, _ in reqs]
+    served = [srv.result(t) for t in tickets]
+    sat_s = time.perf_counter() - t0
+    sat_rate = n_requests / sat_s
+
+    log("[c5] plain: the same requests one after another on one context")
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    t0 = time.perf_counter()
+    plain = [G.prove(cfg, G.PcsParams(*p), tr) for tr, p, _ in reqs]
+    plain_s = time.perf_counter() - t0
+    dev.trim()
+    same = all(np.array_equal(a[2], s.fields) and np.array_equal(a[3], s.commitments)
+               and a[0].tolist() == s.root.tolist() for a, s in zip(plain, served))
+
+    rates = []
+    for frac in (0.5, 0.8, 1.0):
+        rate = frac * sat_rate
+        gaps = rng.exponential(1.0 / rate, size=n_requests)
+        log(f"[c5] Poisson arrivals at {rate:.2f} requests/s")
+        tickets = []
+        start = time.perf_counter()
+        due = start
+        for (tr, p, _), g in zip(reqs, gaps):
+            due += g
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append(srv.submit(tr, G.PcsParams(*p)))
+        res = [srv.result(t) for t in tickets]
+        lat = np.array([r.latency_ms for r in res])
+        span = (max(r.done_ms for r in res) - min(r.submit_ms for r in res)) / 1e3
+        same = same and all(np.array_equal(a[2], r.fields) for a, r in zip(plain, res))
+        rates.append({"offered_per_s": round(rate, 3), "proofs_per_s": round(n_requests / span, 3),
+                      "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 2)})
+    st = srv.stats()
+    srv.close()
+    pool.free()
+    dev.trim()
+    return {
+        "workload": f"{n_requests} requests: 1-8 traces, heights 2^14..2^22 log-uniform, widths 16..128, "
+                    "log_blowup 1..3, Poseidon2 LMCS, arity 4 to degree 16, 100 queries; traces resident in HBM",
+        "workers": workers,
+        "hbm_budget_gb": budget / 1e9,
+        "saturation_proofs_per_s": round(sat_rate, 3),
+        "plain_sequential_proofs_per_s": round(n_requests / plain_s, 3),
+        "speedup_vs_plain": round(plain_s / sat_s, 3),
+        "bit_exact_served_vs_plain": bool(same),
+        "poisson": rates,
+        "peak_inflight_gb": round(st["peak_inflight_bytes"] / 1e9, 2),
+        "peak_running": st["peak_running"],
+    }
+
+
+def cpu_baseline(W, log_rows):
+    import oracle as O
+
+    w = W.C2_WIDTH
+    log(f"cpu baseline: oracle on 2^{log_rows} x {w} (same parameters), {O.get_threads()} threads")
+    tr = [O.rand_matrix(SEED, 1 << log_rows, w)]
+    c0 = time.perf_counter()
+    O.pcs_trace_prove(O.PcsParams(*W.C2_PARAMS), O.HASH_P2, tr)
+    c1 = time.perf_counter()
+    scale = (1 << W.C2_LOG_ROWS) // (1 << log_rows)
+    return {
+        "value": round(1.0 / ((c1 - c0) * scale), 6),
+        "unit": "proofs/s",
+        "cores": O.get_threads(),
+        "kind": "port",
+        "sample": f"oracle proof of a 2^{log_rows} x {w} trace with the same parameters took {c1 - c0:.2f} s; "
+                  f"scaled x{scale} (linear in rows) to the 2^{W.C2_LOG_ROWS}-row workload",
     }
 
 
@@ -59,127 +369,29 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--sections", default="c2,c3,c4,c5", help="comma list of c2 (headline), c3, c4, c5")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log-rows", type=int, default=17)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(independent proofs shard one per device; see DESIGN.md)", file=sys.stderr)
+              "(independent proofs shard one per device)", file=sys.stderr)
         sys.exit(2)
 
     import p3gpu as G
+    from p3gpu import workloads as W
 
-    n, w, b = 1 << LOG_N, WIDTH, LOG_BLOWUP
-    log(f"generating trace 2^{LOG_N} x {w} from SmallRng({SEED})")
-    host = G.fill_random(SEED, n * w).reshape(n, w)
+    sections = set(args.sections.split(","))
     dev = G.Device(0)
-    trace = dev.upload(host)
-    del host
-    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
-    params = G.PcsParams(*PARAMS)
+    c2 = bench_c2(G, W, dev, args.steps, args.warmup)
+    c3 = bench_c3(G, W, dev, max(2, args.steps // 3), args.warmup) if "c3" in sections else None
+    c4 = bench_c4(G, W, dev, max(2, args.steps // 2), args.warmup) if "c4" in sections else None
+    c5 = bench_c5(G, W, dev, args.serve_requests, args.serve_workers) if "c5" in sections else None
+    cpu = None if args.no_cpu_baseline else cpu_baseline(W, args.cpu_sample_log_rows)
 
-    def step():
-        return G.prove(cfg, params, [trace])
-
-    log(f"warmup x{args.warmup}")
-    first = None
-    for _ in range(args.warmup):
-        first = step()
-    dev.synchronize()
-    dev.profile(True)
-    dev.profile_reset()
-    log(f"timing x{args.steps}")
-    t0 = time.perf_counter()
-    out = None
-    for _ in range(args.steps):
-        out = step()
-    dev.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    if first is not None:
-        assert np.array_equal(first[2], out[2]) and np.array_equal(first[3], out[3]), "proofs differ between runs"
-
-    fams = ["lde_inverse_pass", "lde_forward_pass", "p2_leaf", "compress_layer", "compress_top", "deep_weights",
-            "deep_colsum", "deep_quotient", "fri_fold"]
-    per = {}
-    for f in fams:
-        tot, cnt = dev.profile_read(f)
-        per[f] = (tot / args.steps, cnt // max(1, args.steps))
-    dev.profile(False)
-    dom = max(per, key=lambda k: per[k][0])
-    alg = algorithmic(n, w, b)
-
-    # the leaf-hash launch that hashes the 2^24-row LDE: one per proof for the trace
-    # tree plus one per FRI round; its per-launch duration is taken from the trace
-    # tree launch, measured separately below
-    log("timing commit phase (LDE + LMCS) alone")
-    dev.profile(True)
-    dev.profile_reset()
-    tc0 = time.perf_counter()
-    reps = max(3, args.steps // 2)
-    for _ in range(reps):
-        t = G.commit_traces(cfg, [trace], b)
-        t.free()
-    dev.synchronize()
-    tc1 = time.perf_counter()
-    commit_ms = (tc1 - tc0) * 1e3 / reps
-    leaf_ms, leaf_n = dev.profile_read("p2_leaf")
-    leaf_ms /= max(1, leaf_n)
-    lde_ms = (dev.profile_read("lde_inverse_pass")[0] + dev.profile_read("lde_forward_pass")[0]) / reps
-    dev.profile(False)
-
-    perms = alg["p2_leaf_perms"]
-    leaf_bytes = alg["p2_leaf_bytes"]
-    # VALU work of the leaf kernel: 32-bit lane-ops per Poseidon2 permutation, from the
-    # instruction count of the compiled kernel (SQ_INSTS_VALU per wave / 64 leaves per wave
-    # / permutations per leaf) recorded in profiles/p2_leaf_c2.md
-    valu_ops_per_perm = None
-    prof_file = os.path.join(ROOT, "profiles", "p2_leaf_c2.md")
-    traffic = None
-    if os.path.exists(prof_file):
-        for line in open(prof_file):
-            if line.startswith("valu_lane_ops_per_perm:"):
-                valu_ops_per_perm = float(line.split(":")[1])
-            if line.startswith("hbm_bytes_per_launch:"):
-                traffic = float(line.split(":")[1])
-    roof = {
-        "kernel": "p2_leaf (Poseidon2-GL12 LMCS leaf hashing, 2^24 leaves x 8 permutations)",
-        "bound": "valu",
-        "achieved": None,
-        "peak": round(VALU_PEAK_TOPS, 2),
-        "unit": "TOP/s",
-        "frac": None,
-        "traffic": traffic,
-        "launch_ms": round(leaf_ms, 3),
-        "permutations_per_s": round(perms / (leaf_ms * 1e-3), 1) if leaf_ms else None,
-        "hbm_view": {"achieved": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9, 1) if leaf_ms else None,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s"},
-    }
-    if valu_ops_per_perm and leaf_ms:
-        ach = perms * valu_ops_per_perm / (leaf_ms * 1e-3) / 1e12
-        roof["achieved"] = round(ach, 2)
-        roof["frac"] = round(ach / VALU_PEAK_TOPS, 3)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        import oracle as O
-
-        ls = args.cpu_sample_log_rows
-        log(f"cpu baseline: oracle on 2^{ls} x {w} (same parameters), {O.get_threads()} threads")
-        tr = [O.rand_matrix(SEED, 1 << ls, w)]
-        c0 = time.perf_counter()
-        O.pcs_trace_prove(O.PcsParams(*PARAMS), O.HASH_P2, tr)
-        c1 = time.perf_counter()
-        scale = (1 << LOG_N) // (1 << ls)
-        cpu = {
-            "value": round(1.0 / ((c1 - c0) * scale), 6),
-            "unit": "proofs/s",
-            "cores": O.get_threads(),
-            "kind": "port",
-            "sample": f"oracle proof of a 2^{ls} x {w} trace with the same parameters took {c1 - c0:.2f} s; "
-                      f"scaled x{scale} (linear in rows) to the 2^{LOG_N}-row workload",
-        }
-
+    ms = c2["ms"]
     line = {
         "metric": "proofs/s batched vs plain on 1 MI355X, bit-exact; LDE+commit GB/s; p99 latency",
         "value": round(1000.0 / ms, 4),
@@ -192,16 +404,19 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64 (Goldilocks)",
-        "data": "synthetic: uniform Goldilocks trace from SmallRng(2025)",
+        "data": "synthetic: uniform Goldilocks traces from SmallRng(2025) and the device counter generator",
         "config": {"workload": "plain single-trace prove: 2^22 x 64 trace, log_blowup 2 (LDE 2^24 rows), "
                                "Poseidon2 LMCS, DEEP at 2 points, FRI arity 4 to degree 16, 100 queries",
                    "mode": "plain", "parallelism": "1 GPU"},
-        "lde_commit_gbps": round(alg["lde_commit"] / (commit_ms * 1e-3) / 1e9, 1),
-        "commit_ms": round(commit_ms, 3),
-        "lde_ms": round(lde_ms, 3),
-        "kernel_ms_per_proof": {k: round(v[0], 3) for k, v in per.items()},
-        "dominant_kernel": dom,
-        "roofline": roof,
+        "lde_commit_gbps": c2["lde_commit_gbps"],
+        "commit_ms": c2["commit_ms"],
+        "kernel_ms_per_proof": c2["kernel_ms_per_proof"],
+        "dominant_kernel": c2["dominant_kernel"],
+        "roofline": c2["roofline"],
+        "other_kernels_hbm": c2["other_kernels_hbm"],
+        "batched": c3,
+        "keccak": c4,
+        "serving": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

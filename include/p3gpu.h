@@ -74,6 +74,9 @@ int p3g_ctx_create(int device, p3g_ctx** out);
 void p3g_ctx_destroy(p3g_ctx* ctx);
 int p3g_ctx_synchronize(p3g_ctx* ctx);
 int p3g_ctx_trim(p3g_ctx* ctx); /* release cached device blocks */
+/* device bytes held by live allocations, by the block cache, and the peak of live bytes
+ * (optionally reset to the current value) */
+int p3g_ctx_memory(p3g_ctx* ctx, uint64_t* live, uint64_t* cached, uint64_t* peak_live, int reset_peak);
 
 /* ---- per-kernel timing (instrumentation; no reference counterpart) ----
  * When enabled, HIP events are recorded on the context's stream around each
@@ -212,6 +215,51 @@ int p3g_prove_planned(p3g_ctx* ctx, p3g_commit_plan* plan, const p3g_pcs_params*
                       uint64_t root[4], uint64_t eval_points_out[4]);
 /* synthetic canonical words generated on the device (counter-based SplitMix64; not the SmallRng stream) */
 int p3g_fill_random_device(p3g_ctx* ctx, uint64_t seed, uint64_t n, uint64_t* dptr);
+
+/* ---- prove-request server (no reference counterpart: continuous batching of
+ * independent proofs on one GPU) ----
+ * W worker contexts (own stream and allocation cache each) pull requests in
+ * arrival order; a request is admitted when a worker is free and its device
+ * footprint (p3g_request_footprint) fits in the HBM budget beside the requests
+ * in flight.  A request that does not fit lets later ones that do go first
+ * until it has waited backfill_ms.  Each request runs the p3g_prove flow on a
+ * fresh transcript of its hash's challenger kind; its result is independent of
+ * the other requests.  Trace buffers belong to the caller and must stay valid
+ * until the request is done (pinned host memory from p3g_host_alloc makes the
+ * copies asynchronous-speed; device pointers with P3G_DEVICE_PTRS skip them). */
+typedef struct p3g_server p3g_server;
+typedef struct {
+  uint32_t hash_kind;
+  p3g_pcs_params params;
+  const p3g_matrix* traces; /* ascending power-of-two heights */
+  uint64_t k;
+  uint32_t flags; /* P3G_DEVICE_PTRS */
+} p3g_request;
+typedef struct {
+  int status;
+  uint64_t root[4];
+  uint64_t eval_points[4];
+  uint64_t num_fields;
+  uint64_t num_commitments;
+  double submit_ms, admit_ms, done_ms; /* since server creation */
+  uint32_t worker;
+  uint64_t footprint_bytes;
+} p3g_result_info;
+typedef struct {
+  uint64_t admitted, completed, queued, running;
+  uint64_t inflight_bytes, peak_inflight_bytes, peak_running, budget_bytes;
+} p3g_server_counters;
+uint64_t p3g_request_footprint(const p3g_request* req); /* device bytes reserved at admission */
+int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms, p3g_server** out);
+int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket); /* P3G_E_RANGE: above the budget */
+int p3g_server_wait(p3g_server* s, uint64_t ticket, p3g_result_info* info);     /* blocks; returns the proof status */
+int p3g_server_poll(p3g_server* s, uint64_t ticket, int* done);
+int p3g_server_proof(p3g_server* s, uint64_t ticket, uint64_t* fields, uint64_t* commitments);
+int p3g_server_release(p3g_server* s, uint64_t ticket);
+int p3g_server_stats(p3g_server* s, p3g_server_counters* out);
+void p3g_server_destroy(p3g_server* s); /* finishes running requests, drops queued ones */
+int p3g_host_alloc(uint64_t bytes, void** out); /* pinned host memory */
+int p3g_host_free(void* p);
 
 /* ---- one-shot proof (the pcs_trace harness flow, p3-miden-lifted-fri/benches/pcs_trace.rs:60-100,
  * with commit_traces semantics): commit the traces, observe the root, sample two

@@ -5,23 +5,9 @@
 #include <string>
 
 #include "../../include/p3gpu.h"
-#include "internal.hpp"
-#include "batch.hpp"
+#include "handles.hpp"
 
 using namespace p3g;
-
-struct p3g_ctx {
-  Ctx c;
-};
-struct p3g_tree {
-  Tree* t;
-};
-struct p3g_transcript {
-  Transcript tx;
-};
-struct p3g_commit_plan {
-  CommitPlan* p;
-};
 
 namespace p3g {
 void fill_random_device(Ctx& ctx, uint64_t seed, uint64_t n, uint64_t* out, hipStream_t s);
@@ -98,6 +84,16 @@ void p3g_ctx_destroy(p3g_ctx* ctx) {
   if (ctx->c.dev_p2) (void)hipFree(ctx->c.dev_p2);
   if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
+}
+
+int p3g_ctx_memory(p3g_ctx* ctx, uint64_t* live, uint64_t* cached, uint64_t* peak_live, int reset_peak) {
+  if (!ctx) return P3G_E_INVAL;
+  std::lock_guard<std::mutex> g(ctx->c.mu);
+  if (live) *live = ctx->c.live_bytes;
+  if (cached) *cached = ctx->c.cached_bytes;
+  if (peak_live) *peak_live = ctx->c.peak_live_bytes;
+  if (reset_peak) ctx->c.peak_live_bytes = ctx->c.live_bytes;
+  return P3G_OK;
 }
 
 int p3g_ctx_synchronize(p3g_ctx* ctx) {

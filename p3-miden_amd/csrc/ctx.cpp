@@ -17,6 +17,8 @@ void* Ctx::alloc(size_t bytes) {
     free_blocks.erase(it);
     cached_bytes -= sz;
     live_blocks[p] = sz;
+    live_bytes += sz;
+    if (live_bytes > peak_live_bytes) peak_live_bytes = live_bytes;
     return p;
   }
   void* p = nullptr;
@@ -34,6 +36,8 @@ void* Ctx::alloc(size_t bytes) {
     }
   }
   live_blocks[p] = bytes;
+  live_bytes += bytes;
+  if (live_bytes > peak_live_bytes) peak_live_bytes = live_bytes;
   return p;
 }
 
@@ -44,6 +48,7 @@ void Ctx::release(void* p) {
   if (it == live_blocks.end()) return;
   free_blocks.emplace(it->second, p);
   cached_bytes += it->second;
+  live_bytes -= it->second;
   live_blocks.erase(it);
 }
 

@@ -66,6 +66,7 @@ struct Ctx {
   std::multimap<size_t, void*> free_blocks;
   std::map<void*, size_t> live_blocks;
   size_t cached_bytes = 0;
+  size_t live_bytes = 0, peak_live_bytes = 0;
 
   // per-kernel timing with HIP events on this context's stream (off by default)
   bool profiling = false;

@@ -1,0 +1,348 @@
+// Prove-request server: continuous batching of independent PCS proofs on one GPU.
+//
+// Each request is the pcs_trace flow of p3g_prove (commit_traces, observe the
+// root, sample two extension points, open_with_channel) over its own traces.
+// The server owns W worker contexts, each with its own stream and allocation
+// cache, so up to W requests are on the device at once and their kernels,
+// copies and host-side transcript work overlap.  Requests are admitted in
+// arrival order as soon as a worker is free AND the request's device footprint
+// fits in the HBM budget next to the requests already in flight; a request
+// that does not fit lets later, smaller ones go first (backfill) until it has
+// waited longer than the backfill window, after which nothing overtakes it.
+// A proof depends only on its request, never on what runs beside it, so each
+// result is byte-identical to running the request alone.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/p3gpu.h"
+#include "handles.hpp"
+
+using namespace p3g;
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Request {
+  uint64_t ticket = 0;
+  uint32_t hash = 0;
+  p3g_pcs_params params{};
+  std::vector<p3g_matrix> traces;
+  uint32_t flags = 0;
+  uint64_t footprint = 0;
+  clk::time_point t_submit, t_admit, t_done;
+  bool admitted = false, done = false;
+  int status = P3G_OK;
+  uint32_t worker = 0;
+  uint64_t root[4] = {0, 0, 0, 0}, z[4] = {0, 0, 0, 0};
+  std::vector<uint64_t> fields, comms;
+  std::string error;
+};
+
+}  // namespace
+
+struct p3g_server {
+  int device = 0;
+  uint64_t budget = 0;
+  double backfill_ms = 0;
+  std::vector<p3g_ctx*> ctxs;
+  std::vector<std::thread> threads;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;
+  std::map<uint64_t, Request*> reqs;
+  uint64_t next_ticket = 1;
+  uint64_t inflight_bytes = 0, peak_inflight_bytes = 0;
+  uint32_t running = 0, peak_running = 0;
+  uint64_t admitted = 0, completed = 0;
+  bool stop = false;
+  clk::time_point t0;
+
+  void worker_loop(uint32_t w);
+  Request* pick_locked();
+  uint64_t cached_total();
+  void trim_if_over_budget(uint32_t w);
+};
+
+uint64_t p3g_server::cached_total() {
+  uint64_t t = 0;
+  for (auto* c : ctxs) {
+    std::lock_guard<std::mutex> g(c->c.mu);
+    t += c->c.cached_bytes;
+  }
+  return t;
+}
+
+// First queued request that fits; the queue head may be overtaken only while
+// it is younger than the backfill window.
+Request* p3g_server::pick_locked() {
+  const auto now = clk::now();
+  for (auto it = queue.begin(); it != queue.end(); ++it) {
+    Request* r = *it;
+    if (inflight_bytes + r->footprint <= budget) {
+      queue.erase(it);
+      return r;
+    }
+    const double waited = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+    if (waited > backfill_ms) return nullptr;
+  }
+  return nullptr;
+}
+
+void p3g_server::worker_loop(uint32_t w) {
+  (void)hipSetDevice(device);
+  p3g_ctx* ctx = ctxs[w];
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      for (;;) {
+        if (stop) return;
+        r = pick_locked();
+        if (r) break;
+        // re-check on every completion and submission; a timed wait lets an
+        // aged queue head stop the backfill
+        cv_work.wait_for(lk, std::chrono::milliseconds(5));
+      }
+      r->admitted = true;
+      r->t_admit = clk::now();
+      r->worker = w;
+      inflight_bytes += r->footprint;
+      running++;
+      admitted++;
+      if (inflight_bytes > peak_inflight_bytes) peak_inflight_bytes = inflight_bytes;
+      if (running > peak_running) peak_running = running;
+    }
+    // blocks cached by the workers' allocators count against the budget too:
+    // a worker frees only its own cache, and only while its stream is idle
+    // (between requests), so no block is freed under a kernel still using it
+    trim_if_over_budget(w);
+
+    p3g_transcript* tx = nullptr;
+    int rc = p3g_transcript_create(r->hash == P3G_HASH_POSEIDON2_GL12 ? P3G_CHALLENGER_DUPLEX_P2 : P3G_CHALLENGER_KECCAK,
+                                   &tx);
+    if (rc == P3G_OK)
+      rc = p3g_prove(ctx, r->hash, &r->params, r->traces.data(), r->traces.size(), r->flags, tx, r->root, r->z);
+    if (rc == P3G_OK) {
+      r->fields = tx->tx.fields;
+      r->comms = tx->tx.comms;
+    } else {
+      r->error = p3g_last_error();
+    }
+    p3g_transcript_free(tx);
+    {
+      std::lock_guard<std::mutex> g(mu);
+      r->status = rc;
+      r->done = true;
+      r->t_done = clk::now();
+      inflight_bytes -= r->footprint;
+      running--;
+      completed++;
+    }
+    trim_if_over_budget(w);
+    cv_done.notify_all();
+    cv_work.notify_all();
+  }
+}
+
+void p3g_server::trim_if_over_budget(uint32_t w) {
+  uint64_t inflight_now;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    inflight_now = inflight_bytes;
+  }
+  if (inflight_now + cached_total() <= budget) return;
+  p3g_ctx* c = ctxs[w];
+  (void)hipStreamSynchronize(c->c.stream);
+  c->c.trim();
+}
+
+extern "C" {
+
+uint64_t p3g_request_footprint(const p3g_request* req) {
+  if (!req || !req->traces || req->k == 0) return 0;
+  const uint64_t B = 1ull << req->params.log_blowup;
+  uint64_t trace = 0, lde = 0, L = 0, hmin = ~0ull, hmax = 0;
+  for (uint64_t i = 0; i < req->k; i++) {
+    const uint64_t n = req->traces[i].height, w = req->traces[i].width;
+    trace += 8 * n * w;
+    lde += 8 * n * B * w;
+    if (n * B > L) L = n * B;
+    if (n < hmin) hmin = n;
+    if (n > hmax) hmax = n;
+  }
+  const uint64_t W = req->hash_kind == P3G_HASH_POSEIDON2_GL12 ? 12 : 25;
+  uint64_t bytes = lde + 64 * L;                        // LDEs and digest layers
+  if (!(req->flags & P3G_DEVICE_PTRS)) bytes += trace;  // device copies of host traces
+  if (hmin != hmax) bytes += 2 * L * W * 8;             // lifted sponge states
+  bytes += 112 * L + 32 * (L / B);                      // DEEP evaluations, weights, FRI layers and trees
+  return bytes + bytes / 8 + (64ull << 20);             // allocator rounding and tables
+}
+
+int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms,
+                      p3g_server** out) {
+  if (!out || workers == 0 || workers > 16 || hbm_budget_bytes == 0) return P3G_E_INVAL;
+  *out = nullptr;
+  p3g_server* s = new p3g_server();
+  s->device = device;
+  s->budget = hbm_budget_bytes;
+  s->backfill_ms = backfill_ms;
+  s->t0 = clk::now();
+  for (uint32_t w = 0; w < workers; w++) {
+    p3g_ctx* c = nullptr;
+    int rc = p3g_ctx_create(device, &c);
+    if (rc != P3G_OK) {
+      for (auto* x : s->ctxs) p3g_ctx_destroy(x);
+      delete s;
+      return rc;
+    }
+    s->ctxs.push_back(c);
+  }
+  for (uint32_t w = 0; w < workers; w++) s->threads.emplace_back([s, w] { s->worker_loop(w); });
+  *out = s;
+  return P3G_OK;
+}
+
+int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket) {
+  if (!s || !req || !ticket || !req->traces || req->k == 0) return P3G_E_INVAL;
+  if (req->hash_kind > P3G_HASH_KECCAK_GL) return P3G_E_INVAL;
+  const uint64_t fp = p3g_request_footprint(req);
+  if (fp > s->budget) return P3G_E_RANGE;
+  Request* r = new Request();
+  r->hash = req->hash_kind;
+  r->params = req->params;
+  r->traces.assign(req->traces, req->traces + req->k);
+  r->flags = req->flags;
+  r->footprint = fp;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    r->ticket = s->next_ticket++;
+    r->t_submit = clk::now();
+    s->reqs[r->ticket] = r;
+    s->queue.push_back(r);
+    *ticket = r->ticket;
+  }
+  s->cv_work.notify_all();
+  return P3G_OK;
+}
+
+static double ms_since(const p3g_server* s, clk::time_point t) {
+  return std::chrono::duration<double, std::milli>(t - s->t0).count();
+}
+
+int p3g_server_wait(p3g_server* s, uint64_t ticket, p3g_result_info* info) {
+  if (!s) return P3G_E_INVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return P3G_E_RANGE;
+  Request* r = it->second;
+  s->cv_done.wait(lk, [&] { return r->done || s->stop; });
+  if (!r->done) return P3G_E_INVAL;
+  if (info) {
+    info->status = r->status;
+    memcpy(info->root, r->root, 32);
+    memcpy(info->eval_points, r->z, 32);
+    info->num_fields = r->fields.size();
+    info->num_commitments = r->comms.size() / 4;
+    info->submit_ms = ms_since(s, r->t_submit);
+    info->admit_ms = ms_since(s, r->t_admit);
+    info->done_ms = ms_since(s, r->t_done);
+    info->worker = r->worker;
+    info->footprint_bytes = r->footprint;
+  }
+  return r->status;
+}
+
+int p3g_server_poll(p3g_server* s, uint64_t ticket, int* done) {
+  if (!s || !done) return P3G_E_INVAL;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return P3G_E_RANGE;
+  *done = it->second->done ? 1 : 0;
+  return P3G_OK;
+}
+
+int p3g_server_proof(p3g_server* s, uint64_t ticket, uint64_t* fields, uint64_t* commitments) {
+  if (!s) return P3G_E_INVAL;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return P3G_E_RANGE;
+  Request* r = it->second;
+  if (!r->done) return P3G_E_INVAL;
+  if (r->status != P3G_OK) return r->status;
+  if (fields && !r->fields.empty()) memcpy(fields, r->fields.data(), r->fields.size() * 8);
+  if (commitments && !r->comms.empty()) memcpy(commitments, r->comms.data(), r->comms.size() * 8);
+  return P3G_OK;
+}
+
+int p3g_server_release(p3g_server* s, uint64_t ticket) {
+  if (!s) return P3G_E_INVAL;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return P3G_E_RANGE;
+  if (!it->second->done) return P3G_E_INVAL;
+  delete it->second;
+  s->reqs.erase(it);
+  return P3G_OK;
+}
+
+int p3g_server_stats(p3g_server* s, p3g_server_counters* out) {
+  if (!s || !out) return P3G_E_INVAL;
+  std::lock_guard<std::mutex> g(s->mu);
+  out->admitted = s->admitted;
+  out->completed = s->completed;
+  out->queued = s->queue.size();
+  out->running = s->running;
+  out->inflight_bytes = s->inflight_bytes;
+  out->peak_inflight_bytes = s->peak_inflight_bytes;
+  out->peak_running = s->peak_running;
+  out->budget_bytes = s->budget;
+  return P3G_OK;
+}
+
+void p3g_server_destroy(p3g_server* s) {
+  if (!s) return;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    // queued requests are dropped; running ones finish first
+    for (auto* r : s->queue) {
+      r->status = P3G_E_INVAL;
+      r->done = true;
+    }
+    s->queue.clear();
+    s->cv_done.wait(lk, [&] { return s->running == 0; });
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  s->cv_done.notify_all();
+  for (auto& t : s->threads) t.join();
+  for (auto& kv : s->reqs) delete kv.second;
+  for (auto* c : s->ctxs) p3g_ctx_destroy(c);
+  delete s;
+}
+
+int p3g_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return P3G_E_INVAL;
+  *out = nullptr;
+  if (hipHostMalloc(out, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return P3G_E_NOMEM;
+  }
+  return P3G_OK;
+}
+
+int p3g_host_free(void* p) {
+  if (p && hipHostFree(p) != hipSuccess) {
+    (void)hipGetLastError();
+    return P3G_E_DEVICE;
+  }
+  return P3G_OK;
+}
+
+}  // extern "C"

@@ -130,6 +130,23 @@ class CallbackChannel:
         self.struct = _Channel(None, *self._fns)
 
 
+class _Request(C.Structure):
+    _fields_ = [("hash_kind", C.c_uint32), ("params", PcsParams), ("traces", C.POINTER(_Matrix)),
+                ("k", C.c_uint64), ("flags", C.c_uint32)]
+
+
+class _ResultInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("root", C.c_uint64 * 4), ("eval_points", C.c_uint64 * 4),
+                ("num_fields", C.c_uint64), ("num_commitments", C.c_uint64), ("submit_ms", C.c_double),
+                ("admit_ms", C.c_double), ("done_ms", C.c_double), ("worker", C.c_uint32),
+                ("footprint_bytes", C.c_uint64)]
+
+
+class _ServerCounters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("admitted", "completed", "queued", "running", "inflight_bytes",
+                                          "peak_inflight_bytes", "peak_running", "budget_bytes")]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         return None
@@ -141,6 +158,7 @@ def _load():
         "p3g_ctx_destroy": (None, [_vp]),
         "p3g_ctx_synchronize": (C.c_int, [_vp]),
         "p3g_ctx_trim": (C.c_int, [_vp]),
+        "p3g_ctx_memory": (C.c_int, [_vp, _u64p, _u64p, _u64p, C.c_int]),
         "p3g_profile_enable": (C.c_int, [_vp, C.c_int]),
         "p3g_profile_reset": (C.c_int, [_vp]),
         "p3g_profile_read": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), _u64p]),
@@ -191,6 +209,17 @@ def _load():
         "p3g_commit_plan_destroy": (None, [_vp]),
         "p3g_prove_planned": (C.c_int, [_vp, _vp, C.POINTER(PcsParams), _vp, _u64p, _u64p]),
         "p3g_fill_random_device": (C.c_int, [_vp, C.c_uint64, C.c_uint64, _vp]),
+        "p3g_request_footprint": (C.c_uint64, [C.POINTER(_Request)]),
+        "p3g_server_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.c_double, C.POINTER(_vp)]),
+        "p3g_server_submit": (C.c_int, [_vp, C.POINTER(_Request), _u64p]),
+        "p3g_server_wait": (C.c_int, [_vp, C.c_uint64, C.POINTER(_ResultInfo)]),
+        "p3g_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(C.c_int)]),
+        "p3g_server_proof": (C.c_int, [_vp, C.c_uint64, _u64p, _u64p]),
+        "p3g_server_release": (C.c_int, [_vp, C.c_uint64]),
+        "p3g_server_stats": (C.c_int, [_vp, C.POINTER(_ServerCounters)]),
+        "p3g_server_destroy": (None, [_vp]),
+        "p3g_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
+        "p3g_host_free": (C.c_int, [_vp]),
         "p3g_prove": (C.c_int, [_vp, C.c_uint32, C.POINTER(PcsParams), C.POINTER(_Matrix), C.c_uint64, C.c_uint32, _vp, _u64p, _u64p]),
     }
     for name, (res, args) in sig.items():
@@ -266,6 +295,12 @@ class Device:
     def trim(self):
         _check(lib().p3g_ctx_trim(self.h), "trim")
 
+    def memory(self, reset_peak: bool = False) -> dict:
+        """Device bytes: live allocations, block cache, peak live since the last reset."""
+        v = np.zeros(3, np.uint64)
+        _check(lib().p3g_ctx_memory(self.h, _p(v[0:1]), _p(v[1:2]), _p(v[2:3]), int(reset_peak)), "ctx_memory")
+        return {"live": int(v[0]), "cached": int(v[1]), "peak_live": int(v[2])}
+
     def profile(self, on: bool = True):
         """Record HIP events around every launch of the named kernel families."""
         _check(lib().p3g_profile_enable(self.h, 1 if on else 0), "profile_enable")
@@ -279,6 +314,12 @@ class Device:
         n = np.zeros(1, np.uint64)
         _check(lib().p3g_profile_read(self.h, name.encode(), C.byref(ms), _p(n)), "profile_read")
         return float(ms.value), int(n[0])
+
+    def alloc_matrix(self, height: int, width: int) -> "DeviceMatrix":
+        """Uninitialised height x width matrix in HBM."""
+        ptr = _vp()
+        _check(lib().p3g_alloc(self.h, max(8, 8 * height * width), C.byref(ptr)), "alloc")
+        return self._adopt(DeviceMatrix(self, ptr, height, width))
 
     def upload(self, arr: np.ndarray) -> "DeviceMatrix":
         a = u64(arr)
@@ -294,6 +335,17 @@ class DeviceMatrix:
 
     def __init__(self, dev: Device, ptr, height: int, width: int, owned: bool = True):
         self.dev, self.ptr, self.height, self.width, self.owned = dev, ptr, height, width, owned
+
+    def view(self, offset_words: int, height: int, width: int) -> "DeviceMatrix":
+        """A non-owning height x width matrix starting offset_words into this buffer."""
+        if offset_words + height * width > self.height * self.width:
+            raise ValueError("view out of range")
+        base = self.ptr.value if isinstance(self.ptr, _vp) else int(self.ptr)
+        return DeviceMatrix(self.dev, _vp(base + 8 * offset_words), height, width, owned=False)
+
+    def fill_random(self, seed: int) -> None:
+        """Canonical synthetic values from the device counter generator."""
+        fill_random_device(self.dev, seed, self.height * self.width, self.ptr)
 
     def download(self) -> np.ndarray:
         out = np.empty(self.height * self.width, np.uint64)
@@ -639,3 +691,103 @@ class CommitPlan:
 
 def fill_random_device(device: Device, seed: int, n: int, ptr) -> None:
     _check(lib().p3g_fill_random_device(device.h, seed, n, ptr), "fill_random_device")
+
+
+def _request(hash_kind, params: PcsParams, traces):
+    arr, keep, dev = _matrices(traces)
+    req = _Request(hash_kind, params, C.cast(arr, C.POINTER(_Matrix)), len(traces), DEVICE_PTRS if dev else 0)
+    return req, (arr, keep, traces)
+
+
+def request_footprint(hash_kind: int, params: PcsParams, traces) -> int:
+    """Device bytes the server reserves for this request at admission."""
+    req, _keep = _request(hash_kind, params, traces)
+    return int(lib().p3g_request_footprint(C.byref(req)))
+
+
+class ServedProof:
+    def __init__(self, info, fields, comms):
+        self.root = np.array(list(info.root), np.uint64)
+        z = list(info.eval_points)
+        self.eval_points = ((int(z[0]), int(z[1])), (int(z[2]), int(z[3])))
+        self.fields, self.commitments = fields, comms
+        self.submit_ms, self.admit_ms, self.done_ms = info.submit_ms, info.admit_ms, info.done_ms
+        self.latency_ms = info.done_ms - info.submit_ms
+        self.worker = int(info.worker)
+        self.footprint_bytes = int(info.footprint_bytes)
+
+
+class Server:
+    """Prove-request server: W worker contexts on one GPU, admission by device
+    footprint within an HBM budget, continuous batching of arriving requests."""
+
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget_bytes: int = 250 * 10**9,
+                 backfill_ms: float = 50.0):
+        h = _vp()
+        _check(lib().p3g_server_create(device, workers, int(hbm_budget_bytes), float(backfill_ms), C.byref(h)),
+               "server_create")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, traces, params: PcsParams, hash_kind: int = HASH_POSEIDON2) -> int:
+        """Queue a proof of `traces` (host arrays or DeviceMatrix); returns a ticket.
+        The arrays are kept alive here until the result is collected."""
+        req, keep = _request(hash_kind, params, traces)
+        t = np.zeros(1, np.uint64)
+        _check(lib().p3g_server_submit(self.h, C.byref(req), _p(t)), "server_submit")
+        self._keep[int(t[0])] = keep
+        return int(t[0])
+
+    def done(self, ticket: int) -> bool:
+        d = C.c_int(0)
+        _check(lib().p3g_server_poll(self.h, ticket, C.byref(d)), "server_poll")
+        return bool(d.value)
+
+    def result(self, ticket: int) -> ServedProof:
+        """Wait for the request, return its proof and timings, and release it."""
+        info = _ResultInfo()
+        rc = lib().p3g_server_wait(self.h, ticket, C.byref(info))
+        try:
+            _check(rc, "server_wait")
+            f = np.zeros(max(1, info.num_fields), np.uint64)
+            c = np.zeros(max(1, 4 * info.num_commitments), np.uint64)
+            _check(lib().p3g_server_proof(self.h, ticket, _p(f), _p(c)), "server_proof")
+            return ServedProof(info, f[: info.num_fields], c[: 4 * info.num_commitments].reshape(-1, 4))
+        finally:
+            lib().p3g_server_release(self.h, ticket)
+            self._keep.pop(ticket, None)
+
+    def stats(self) -> dict:
+        s = _ServerCounters()
+        _check(lib().p3g_server_stats(self.h, C.byref(s)), "server_stats")
+        return {n: int(getattr(s, n)) for n, _ in _ServerCounters._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().p3g_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        self.close()
+
+
+class PinnedArray:
+    """A numpy view of pinned host memory (p3g_host_alloc), for fast host->device copies."""
+
+    def __init__(self, shape):
+        n = int(np.prod(shape))
+        p = _vp()
+        _check(lib().p3g_host_alloc(max(8, 8 * n), C.byref(p)), "host_alloc")
+        self.ptr = p
+        buf = (C.c_uint64 * max(1, n)).from_address(p.value)
+        self.array = np.ctypeslib.as_array(buf)[:n].reshape(shape)
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            lib().p3g_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
