@@ -221,7 +221,8 @@ int p3g_fill_random_device(p3g_ctx* ctx, uint64_t seed, uint64_t n, uint64_t* dp
  * W worker contexts (own stream and allocation cache each) pull requests in
  * arrival order; a request is admitted when a worker is free and its device
  * footprint (p3g_request_footprint) fits in the HBM budget beside the requests
- * in flight.  A request that does not fit lets later ones that do go first
+ * in flight.  The workers share one device block pool; the budget is clamped to
+ * what the device has free at creation, less 4 GiB.  A request that does not fit lets later ones that do go first
  * until it has waited backfill_ms.  Each request runs the p3g_prove flow on a
  * fresh transcript of its hash's challenger kind; its result is independent of
  * the other requests.  Trace buffers belong to the caller and must stay valid
@@ -248,6 +249,8 @@ typedef struct {
 typedef struct {
   uint64_t admitted, completed, queued, running;
   uint64_t inflight_bytes, peak_inflight_bytes, peak_running, budget_bytes;
+  uint64_t n_malloc, n_free; /* device allocator calls of the shared pool */
+  double malloc_ms, free_ms; /* host time spent in them */
 } p3g_server_counters;
 uint64_t p3g_request_footprint(const p3g_request* req); /* device bytes reserved at admission */
 int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms, p3g_server** out);

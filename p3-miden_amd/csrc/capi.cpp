@@ -79,7 +79,13 @@ void p3g_ctx_destroy(p3g_ctx* ctx) {
   (void)hipSetDevice(ctx->c.device);
   if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
   ctx->c.trim();
-  for (auto& kv : ctx->c.live_blocks) (void)hipFree(kv.first);
+  if (ctx->c.pool) {
+    std::vector<void*> held;
+    for (auto& kv : ctx->c.live_blocks) held.push_back(kv.first);
+    for (void* p : held) ctx->c.release(p);  // back to the shared pool, which frees them
+  } else {
+    for (auto& kv : ctx->c.live_blocks) (void)hipFree(kv.first);
+  }
   for (auto& kv : ctx->c.pow_tables) (void)hipFree(kv.second);
   if (ctx->c.dev_p2) (void)hipFree(ctx->c.dev_p2);
   if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);

@@ -1,13 +1,115 @@
 // Device context: stream, Poseidon2 constants, cached power tables and a
 // size-bucketed cache of device allocations (the prover allocates the same
 // shapes proof after proof, so blocks are recycled instead of hipMalloc'd).
+#include <chrono>
+
 #include "internal.hpp"
 
 namespace p3g {
 
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void* SharedPool::take(std::multimap<size_t, Block>::iterator it, hipStream_t s) {
+  Block b = it->second;
+  free_blocks.erase(it);
+  cached_bytes -= b.sz;
+  if (b.ev) {
+    if (b.s != s) {
+      P3G_CHECK_HIP(hipStreamWaitEvent(s, b.ev, 0));
+      n_cross_stream++;
+    }
+    (void)hipEventDestroy(b.ev);
+  }
+  live[b.p] = b.sz;
+  live_bytes += b.sz;
+  return b.p;
+}
+
+void SharedPool::free_one(std::multimap<size_t, Block>::iterator it) {
+  Block b = it->second;
+  free_blocks.erase(it);
+  cached_bytes -= b.sz;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (b.ev) {
+    (void)hipEventSynchronize(b.ev);
+    (void)hipEventDestroy(b.ev);
+  }
+  (void)hipFree(b.p);
+  n_free++;
+  free_ms += ms_since(t0);
+}
+
+void* SharedPool::alloc(size_t bytes, hipStream_t s) {
+  if (bytes == 0) bytes = 8;
+  bytes = (bytes + 255) & ~size_t(255);
+  std::lock_guard<std::mutex> g(mu);
+  auto it = free_blocks.lower_bound(bytes);
+  if (it != free_blocks.end() && it->first <= bytes + bytes / 4) return take(it, s);
+  if (live_bytes + cached_bytes + bytes > limit) {
+    if (it != free_blocks.end()) return take(it, s);  // an oversized cached block rather than growing
+    while (!free_blocks.empty() && live_bytes + cached_bytes + bytes > limit) free_one(std::prev(free_blocks.end()));
+  }
+  void* p = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    while (!free_blocks.empty()) free_one(std::prev(free_blocks.end()));
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw ApiError(E_NOMEM, "device allocation failed");
+    }
+  }
+  n_malloc++;
+  malloc_ms += ms_since(t0);
+  live[p] = bytes;
+  live_bytes += bytes;
+  return p;
+}
+
+void SharedPool::release(void* p, hipStream_t s) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = live.find(p);
+  if (it == live.end()) return;
+  hipEvent_t ev = nullptr;
+  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, s) != hipSuccess) {
+    // without an event the block is only safe after the stream drains
+    (void)hipGetLastError();
+    if (ev) (void)hipEventDestroy(ev);
+    ev = nullptr;
+    (void)hipStreamSynchronize(s);
+  }
+  free_blocks.emplace(it->second, Block{p, it->second, s, ev});
+  cached_bytes += it->second;
+  live_bytes -= it->second;
+  live.erase(it);
+}
+
+void SharedPool::trim() {
+  std::lock_guard<std::mutex> g(mu);
+  while (!free_blocks.empty()) free_one(std::prev(free_blocks.end()));
+}
+
+SharedPool::~SharedPool() {
+  trim();
+  for (auto& kv : live) (void)hipFree(kv.first);
+}
+
 void* Ctx::alloc(size_t bytes) {
   if (bytes == 0) bytes = 8;
   bytes = (bytes + 255) & ~size_t(255);
+  if (pool) {
+    void* p = pool->alloc(bytes, stream);
+    std::lock_guard<std::mutex> g(mu);
+    live_blocks[p] = bytes;
+    live_bytes += bytes;
+    if (live_bytes > peak_live_bytes) peak_live_bytes = live_bytes;
+    return p;
+  }
   std::lock_guard<std::mutex> g(mu);
   auto it = free_blocks.lower_bound(bytes);
   // reuse a cached block when it is not more than 25% larger than requested
@@ -22,6 +124,7 @@ void* Ctx::alloc(size_t bytes) {
     return p;
   }
   void* p = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
     // drop the cache and retry once
@@ -35,6 +138,8 @@ void* Ctx::alloc(size_t bytes) {
       throw ApiError(E_NOMEM, "device allocation failed");
     }
   }
+  n_malloc++;
+  malloc_ms += ms_since(t0);
   live_blocks[p] = bytes;
   live_bytes += bytes;
   if (live_bytes > peak_live_bytes) peak_live_bytes = live_bytes;
@@ -46,6 +151,12 @@ void Ctx::release(void* p) {
   std::lock_guard<std::mutex> g(mu);
   auto it = live_blocks.find(p);
   if (it == live_blocks.end()) return;
+  if (pool) {
+    live_bytes -= it->second;
+    live_blocks.erase(it);
+    pool->release(p, stream);
+    return;
+  }
   free_blocks.emplace(it->second, p);
   cached_bytes += it->second;
   live_bytes -= it->second;
@@ -54,7 +165,10 @@ void Ctx::release(void* p) {
 
 void Ctx::trim() {
   std::lock_guard<std::mutex> g(mu);
+  const auto t0 = std::chrono::steady_clock::now();
+  n_free += free_blocks.size();
   for (auto& kv : free_blocks) (void)hipFree(kv.second);
+  free_ms += ms_since(t0);
   free_blocks.clear();
   cached_bytes = 0;
 }

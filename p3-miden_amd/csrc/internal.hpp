@@ -53,6 +53,38 @@ inline uint32_t hash_width(uint32_t h) { return h == HASH_P2 ? 12 : 25; }
 inline uint32_t hash_alignment(uint32_t h) { return h == HASH_P2 ? 8 : 17; }
 inline uint64_t aligned_len(uint64_t n, uint64_t a) { return a <= 1 ? n : (n + a - 1) / a * a; }
 
+// ------------------------------------------------------------------ shared block pool
+// Device blocks shared by several contexts (one stream each).  A released
+// block remembers its stream and an event recorded there; a context on another
+// stream that takes it makes its own stream wait for that event first, so a
+// block never changes hands while kernels of its previous owner still use it.
+// live + cached bytes stay within `limit`: beyond it cached blocks are reused
+// even when oversized, and only then freed.
+struct SharedPool {
+  struct Block {
+    void* p;
+    size_t sz;
+    hipStream_t s;
+    hipEvent_t ev;
+  };
+  std::mutex mu;
+  size_t limit = 0;
+  std::multimap<size_t, Block> free_blocks;
+  std::map<void*, size_t> live;
+  size_t live_bytes = 0, cached_bytes = 0;
+  uint64_t n_malloc = 0, n_free = 0, n_cross_stream = 0;
+  double malloc_ms = 0, free_ms = 0;
+
+  void* alloc(size_t bytes, hipStream_t s);
+  void release(void* p, hipStream_t s);
+  void trim();
+  ~SharedPool();
+
+ private:
+  void* take(std::multimap<size_t, Block>::iterator it, hipStream_t s);
+  void free_one(std::multimap<size_t, Block>::iterator it);
+};
+
 // ------------------------------------------------------------------ context
 struct Ctx {
   int device = 0;
@@ -67,10 +99,16 @@ struct Ctx {
   std::map<void*, size_t> live_blocks;
   size_t cached_bytes = 0;
   size_t live_bytes = 0, peak_live_bytes = 0;
+  // device allocator calls made for this context (hipMalloc / hipFree) and their host time
+  uint64_t n_malloc = 0, n_free = 0;
+  double malloc_ms = 0, free_ms = 0;
 
   // per-kernel timing with HIP events on this context's stream (off by default)
   bool profiling = false;
   std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof;
+
+  // when set, device blocks come from this pool (shared with other contexts)
+  SharedPool* pool = nullptr;
 
   void* alloc(size_t bytes);
   void release(void* p);

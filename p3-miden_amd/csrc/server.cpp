@@ -9,8 +9,12 @@
 // fits in the HBM budget next to the requests already in flight; a request
 // that does not fit lets later, smaller ones go first (backfill) until it has
 // waited longer than the backfill window, after which nothing overtakes it.
-// A proof depends only on its request, never on what runs beside it, so each
-// result is byte-identical to running the request alone.
+// The workers draw device memory from one shared block pool (blocks change
+// hands between worker streams through events), so a block freed by one
+// request is reused by the next whichever worker runs it, and the pool never
+// holds more than the device can give.  A proof depends only on its request,
+// never on what runs beside it, so each result is byte-identical to running
+// the request alone.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -52,6 +56,7 @@ struct p3g_server {
   uint64_t budget = 0;
   double backfill_ms = 0;
   std::vector<p3g_ctx*> ctxs;
+  SharedPool pool;
   std::vector<std::thread> threads;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -66,18 +71,7 @@ struct p3g_server {
 
   void worker_loop(uint32_t w);
   Request* pick_locked();
-  uint64_t cached_total();
-  void trim_if_over_budget(uint32_t w);
 };
-
-uint64_t p3g_server::cached_total() {
-  uint64_t t = 0;
-  for (auto* c : ctxs) {
-    std::lock_guard<std::mutex> g(c->c.mu);
-    t += c->c.cached_bytes;
-  }
-  return t;
-}
 
 // First queued request that fits; the queue head may be overtaken only while
 // it is younger than the backfill window.
@@ -119,11 +113,6 @@ void p3g_server::worker_loop(uint32_t w) {
       if (inflight_bytes > peak_inflight_bytes) peak_inflight_bytes = inflight_bytes;
       if (running > peak_running) peak_running = running;
     }
-    // blocks cached by the workers' allocators count against the budget too:
-    // a worker frees only its own cache, and only while its stream is idle
-    // (between requests), so no block is freed under a kernel still using it
-    trim_if_over_budget(w);
-
     p3g_transcript* tx = nullptr;
     int rc = p3g_transcript_create(r->hash == P3G_HASH_POSEIDON2_GL12 ? P3G_CHALLENGER_DUPLEX_P2 : P3G_CHALLENGER_KECCAK,
                                    &tx);
@@ -145,22 +134,9 @@ void p3g_server::worker_loop(uint32_t w) {
       running--;
       completed++;
     }
-    trim_if_over_budget(w);
     cv_done.notify_all();
     cv_work.notify_all();
   }
-}
-
-void p3g_server::trim_if_over_budget(uint32_t w) {
-  uint64_t inflight_now;
-  {
-    std::lock_guard<std::mutex> g(mu);
-    inflight_now = inflight_bytes;
-  }
-  if (inflight_now + cached_total() <= budget) return;
-  p3g_ctx* c = ctxs[w];
-  (void)hipStreamSynchronize(c->c.stream);
-  c->c.trim();
 }
 
 extern "C" {
@@ -202,8 +178,19 @@ int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, d
       delete s;
       return rc;
     }
+    c->c.pool = &s->pool;
     s->ctxs.push_back(c);
   }
+  // the pool may hold what the device has free now, less a margin for tables
+  // and the callers' own buffers; the admission budget never exceeds it
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    free_b = (size_t)hbm_budget_bytes;
+  }
+  const size_t margin = (size_t)4 << 30;
+  s->pool.limit = free_b > 2 * margin ? free_b - margin : free_b / 2;
+  if (s->budget > s->pool.limit) s->budget = s->pool.limit;
   for (uint32_t w = 0; w < workers; w++) s->threads.emplace_back([s, w] { s->worker_loop(w); });
   *out = s;
   return P3G_OK;
@@ -303,6 +290,11 @@ int p3g_server_stats(p3g_server* s, p3g_server_counters* out) {
   out->peak_inflight_bytes = s->peak_inflight_bytes;
   out->peak_running = s->peak_running;
   out->budget_bytes = s->budget;
+  std::lock_guard<std::mutex> g2(s->pool.mu);
+  out->n_malloc = s->pool.n_malloc;
+  out->n_free = s->pool.n_free;
+  out->malloc_ms = s->pool.malloc_ms;
+  out->free_ms = s->pool.free_ms;
   return P3G_OK;
 }
 

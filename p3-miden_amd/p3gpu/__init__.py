@@ -144,7 +144,8 @@ class _ResultInfo(C.Structure):
 
 class _ServerCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("admitted", "completed", "queued", "running", "inflight_bytes",
-                                          "peak_inflight_bytes", "peak_running", "budget_bytes")]
+                                          "peak_inflight_bytes", "peak_running", "budget_bytes", "n_malloc",
+                                          "n_free")] + [("malloc_ms", C.c_double), ("free_ms", C.c_double)]
 
 
 def _load():
@@ -760,7 +761,7 @@ class Server:
     def stats(self) -> dict:
         s = _ServerCounters()
         _check(lib().p3g_server_stats(self.h, C.byref(s)), "server_stats")
-        return {n: int(getattr(s, n)) for n, _ in _ServerCounters._fields_}
+        return {n: getattr(s, n) for n, _ in _ServerCounters._fields_}
 
     def close(self):
         if getattr(self, "h", None):
