@@ -92,6 +92,7 @@ int p3g_alloc(p3g_ctx* ctx, uint64_t bytes, void** dptr);
 int p3g_free(p3g_ctx* ctx, void* dptr);
 int p3g_upload(p3g_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int p3g_download(p3g_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int p3g_copy_device(p3g_ctx* ctx, void* dst, const void* src, uint64_t bytes); /* device to device */
 /* RowMajorMatrix::rand(SmallRng::seed_from_u64(seed), n/w, w) as n canonical words, on the host. */
 int p3g_fill_random(uint64_t seed, uint64_t n, uint64_t* host_out);
 

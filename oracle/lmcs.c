@@ -32,9 +32,18 @@ void orc_mock_config(size_t width, size_t rate) { mock_width = width; mock_rate 
 
 size_t orc_hash_width(int kind) { return kind == ORC_HASH_P2 ? 12 : kind == ORC_HASH_KECCAK ? 25 : mock_width; }
 size_t orc_hash_rate(int kind) { return kind == ORC_HASH_P2 ? 8 : kind == ORC_HASH_KECCAK ? 17 : mock_rate; }
+/* compute_field_alignment (p3-miden-stateful-hasher/src/serializing_sponge.rs:166-203):
+ * the fewest field elements that fill whole sponge chunks of rate words of
+ * word_bytes each: inner = word_bytes * rate, alignment = inner / gcd(field_bytes, inner). */
+size_t orc_field_alignment(size_t field_bytes, size_t word_bytes, size_t rate) {
+    size_t inner = word_bytes * rate, a = field_bytes, b = inner;
+    while (b) { size_t t = a % b; a = b; b = t; }
+    return inner / a;
+}
+
 size_t orc_hash_alignment(int kind) {
-    /* P2: RATE.  Keccak serializing: lcm(8, 8*17)/8 = 17.  Mock: RATE. */
-    return kind == ORC_HASH_P2 ? 8 : kind == ORC_HASH_KECCAK ? 17 : mock_rate;
+    /* P2: RATE (field sponge).  Keccak: 8-byte Goldilocks serialised to u64 lanes, rate 17.  Mock: RATE. */
+    return kind == ORC_HASH_P2 ? 8 : kind == ORC_HASH_KECCAK ? orc_field_alignment(8, 8, 17) : mock_rate;
 }
 
 static void permute(int kind, uint64_t* s) {

@@ -4,6 +4,29 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Only the tests, the smoke check and the CPU
  * baseline of the benchmark load liboracle.so, and only as the checker.
+ *
+ * Pins (tests/test_oracle_pins.py, tests/test_golden.py): field arithmetic and
+ * two-adic generators by closed form; the xoshiro256++ generator by its
+ * published reference vector; NTT / coset LDE / bit reversal by naive DFT and
+ * interpolation; the stateful sponge by the reference's mock-permutation KAT
+ * and padding-equivalence tests; alignment by the reference's table; Keccak-f
+ * by SHA3 (hashlib); LMCS structure by lifted == upsampled == concatenated;
+ * DEEP by polynomial evaluation and the reference's negative-coefficient /
+ * Horner identities; FRI folding by evaluation of low-degree polynomials and
+ * the reference's final-polynomial KAT; prover/verifier round trips.
+ *
+ * Parity unpinned (the algorithm lives in un-vendored Plonky3 0.5.0 / rand
+ * 0.10, which cannot be built or imported here, and no reference test holds
+ * an output vector for it):
+ *   - Poseidon2Goldilocks<12> round constants drawn from SmallRng(2025), the
+ *     MATRIX_DIAG_12_GOLDILOCKS diagonal and the exact round layout
+ *     (orc_p2_init_constants, orc_p2_permute);
+ *   - SmallRng seeding (SplitMix64) and StandardUniform<Goldilocks> rejection
+ *     sampling order (orc_rng_seed, orc_rng_gl, orc_rand_fill);
+ *   - TruncatedPermutation / PaddingFreeSponge compression layouts (orc_lmcs_*);
+ *   - DuplexChallenger observe/sample/grind semantics and the Keccak
+ *     transcript, which the reference leaves build-defined (orc_tx_*);
+ *   - Goldilocks u64 serialisation into the Keccak sponge.
  */
 #ifndef ORACLE_H
 #define ORACLE_H
@@ -58,6 +81,7 @@ void orc_mock_config(size_t width, size_t rate);
 size_t orc_hash_width(int kind);
 size_t orc_hash_rate(int kind);
 size_t orc_hash_alignment(int kind);
+size_t orc_field_alignment(size_t field_bytes, size_t word_bytes, size_t rate);
 void orc_sponge_absorb(int kind, uint64_t* state, const uint64_t* in, size_t len);
 void orc_compress(int kind, const uint64_t* left, const uint64_t* right, uint64_t* out);
 void orc_hash_rows(int kind, const uint64_t* elems, size_t n, uint64_t* out4);
@@ -141,6 +165,9 @@ void orc_fri_fold_row(unsigned log_arity, const uint64_t* row /* arity EF */, ui
                       const uint64_t* beta, uint64_t* out2);
 
 /* open_with_channel (p3-miden-lifted-fri/src/prover.rs:35-116); deep_evals_out optional (L*2) */
+/* FriPolys::new on caller evals (L EF, bit-reversed): rounds; final poly (descending) to final_out */
+size_t orc_fri_prove(const orc_pcs_params* p, int hash_kind, const uint64_t* evals, unsigned log_lde_height,
+                     orc_tx* tx, uint64_t* final_out);
 void orc_open_with_channel(const orc_pcs_params* p, int hash_kind, orc_tree* const* trees, size_t ntrees,
                            unsigned log_lde_height, const uint64_t* z, size_t npts, orc_tx* tx,
                            uint64_t* deep_evals_out);

@@ -64,6 +64,8 @@ def _load():
         "orc_bit_reverse_rows": (None, [_u64p, C.c_size_t, C.c_size_t]),
         "orc_naive_coset_dft": (None, [_u64p, C.c_size_t, C.c_size_t, C.c_uint64, _u64p]),
         "orc_hash_alignment": (C.c_size_t, [C.c_int]),
+        "orc_field_alignment": (C.c_size_t, [C.c_size_t, C.c_size_t, C.c_size_t]),
+        "orc_fri_prove": (C.c_size_t, [C.c_void_p, C.c_int, _u64p, C.c_uint, C.c_void_p, _u64p]),
         "orc_mock_config": (None, [C.c_size_t, C.c_size_t]),
         "orc_sponge_absorb": (None, [C.c_int, _u64p, _u64p, C.c_size_t]),
         "orc_compress": (None, [C.c_int, _u64p, _u64p, _u64p]),
@@ -232,6 +234,11 @@ def mock_config(width: int, rate: int) -> None:
 
 def hash_alignment(kind: int) -> int:
     return int(lib.orc_hash_alignment(kind))
+
+
+def field_alignment(field_bytes: int, word_bytes: int, rate: int) -> int:
+    """compute_field_alignment (serializing_sponge.rs:166-203)."""
+    return int(lib.orc_field_alignment(field_bytes, word_bytes, rate))
 
 
 def sponge_absorb(kind: int, state: np.ndarray, values) -> np.ndarray:
@@ -501,3 +508,14 @@ def set_threads(n: int) -> None:
 
 def get_threads() -> int:
     return int(lib.orc_get_threads())
+
+
+def fri_prove(params: PcsParams, kind: int, evals, log_lde_height: int, tx: Transcript):
+    """FriPolys::new over caller evaluations (L extension elements, bit-reversed order).
+    Returns (rounds, final polynomial as (deg, 2) words in descending degree order)."""
+    ev = u64(evals).reshape(-1, 2)
+    assert ev.shape[0] == 1 << log_lde_height
+    fdeg = params.final_degree(log_lde_height)
+    out = np.zeros((max(1, fdeg), 2), np.uint64)
+    n = lib.orc_fri_prove(C.byref(params), kind, _p(ev), log_lde_height, tx.h, _p(out))
+    return int(n), out[:fdeg]

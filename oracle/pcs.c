@@ -216,6 +216,84 @@ static void send_ef_slice(orc_tx* tx, const ef2* v, size_t n) {
     orc_tx_send_fields(tx, (const uint64_t*)v, 2 * n);
 }
 
+/* FriPolys::new (p3-miden-lifted-fri/src/fri/prover.rs:90-223): per round
+ * commit the (m/a) x a EF matrix (flattened c0,c1) as an unaligned tree, send
+ * its root, grind, sample beta, fold; then the final polynomial: truncate to the
+ * final degree, bit-reverse, inverse DFT, reverse to descending order, send.
+ * Consumes `cur`; returns the number of rounds; trees go to ftrees[r]. */
+static size_t fri_commit(const orc_pcs_params* p, int hash_kind, ef2* cur, unsigned log_lde_height, orc_tx* tx,
+                         orc_tree** ftrees, uint64_t* final_out) {
+    size_t L = (size_t)1 << log_lde_height;
+    unsigned la = p->log_arity;
+    size_t arity = (size_t)1 << la;
+    size_t domain = L;
+    size_t fdeg = orc_fri_final_degree(p, log_lde_height);
+    size_t fdomain = fdeg << p->log_blowup;
+    size_t nrounds = 0;
+    gl ginv = gl_inv(gl_two_adic_generator(log_lde_height));
+    size_t ns = domain >> la;
+    gl* sinv = (gl*)malloc((ns ? ns : 1) * sizeof(gl));
+    {
+        gl x = 1;
+        unsigned lns = log2_exact(ns);
+        for (size_t k = 0; k < ns; k++) { sinv[bitrev_idx(k, lns)] = x; x = gl_mul(x, ginv); }
+    }
+    while (domain > fdomain) {
+        size_t fd = domain >> la;
+        const uint64_t* mp = (const uint64_t*)cur;
+        size_t hh = fd, ww = 2 * arity;
+        orc_tree* t = orc_lmcs_build(hash_kind, &mp, &hh, &ww, 1, 1);
+        uint64_t root[4];
+        orc_tree_root(t, root);
+        orc_tx_send_commitment(tx, root);
+        orc_tx_grind(tx, p->folding_pow_bits);
+        uint64_t bb[2];
+        orc_tx_sample_ef(tx, bb);
+        ef2* nxt = (ef2*)malloc(fd * sizeof(ef2));
+        #pragma omp parallel for schedule(static)
+        for (size_t k = 0; k < fd; k++) {
+            uint64_t o[2];
+            orc_fri_fold_row(la, (const uint64_t*)(cur + k * arity), sinv[k], bb, o);
+            nxt[k] = ef_make(o[0], o[1]);
+        }
+        ftrees[nrounds++] = t;
+        free(cur);
+        cur = nxt;
+        domain = fd;
+        size_t nn = domain >> la;
+        gl* s2 = (gl*)malloc((nn ? nn : 1) * sizeof(gl));
+        for (size_t k = 0; k < nn; k++) s2[k] = gl_pow(sinv[k * arity], arity);
+        free(sinv);
+        sinv = s2;
+    }
+    free(sinv);
+    {
+        ef2* fp = (ef2*)malloc(fdeg * sizeof(ef2));
+        unsigned lf = log2_exact(fdeg);
+        for (size_t i = 0; i < fdeg; i++) fp[bitrev_idx(i, lf)] = cur[i];
+        orc_idft_ext(fp, fdeg);
+        for (size_t i = 0; i < fdeg / 2; i++) { ef2 t = fp[i]; fp[i] = fp[fdeg - 1 - i]; fp[fdeg - 1 - i] = t; }
+        send_ef_slice(tx, fp, fdeg);
+        if (final_out) memcpy(final_out, fp, fdeg * sizeof(ef2));
+        free(fp);
+    }
+    free(cur);
+    return nrounds;
+}
+
+size_t orc_fri_prove(const orc_pcs_params* p, int hash_kind, const uint64_t* evals, unsigned log_lde_height,
+                     orc_tx* tx, uint64_t* final_out) {
+    size_t L = (size_t)1 << log_lde_height;
+    ef2* cur = (ef2*)malloc(L * sizeof(ef2));
+    memcpy(cur, evals, L * sizeof(ef2));
+    size_t maxr = orc_fri_num_rounds(p, log_lde_height) + 1;
+    orc_tree** ftrees = (orc_tree**)calloc(maxr + 1, sizeof(orc_tree*));
+    size_t n = fri_commit(p, hash_kind, cur, log_lde_height, tx, ftrees, final_out);
+    for (size_t r = 0; r < n; r++) orc_tree_free(ftrees[r]);
+    free(ftrees);
+    return n;
+}
+
 void orc_open_with_channel(const orc_pcs_params* p, int hash_kind, orc_tree* const* trees, size_t ntrees,
                            unsigned log_lde_height, const uint64_t* z, size_t npts, orc_tx* tx,
                            uint64_t* deep_evals_out) {
@@ -286,61 +364,9 @@ void orc_open_with_channel(const orc_pcs_params* p, int hash_kind, orc_tree* con
 
     /* ---- FRI commit phase (FriPolys::new) ---- */
     unsigned la = p->log_arity;
-    size_t arity = (size_t)1 << la;
-    size_t domain = L;
-    size_t fdeg = orc_fri_final_degree(p, log_lde_height);
-    size_t fdomain = fdeg << p->log_blowup;
     size_t maxr = orc_fri_num_rounds(p, log_lde_height) + 1;
     orc_tree** ftrees = (orc_tree**)calloc(maxr + 1, sizeof(orc_tree*));
-    size_t nrounds = 0;
-    gl ginv = gl_inv(gl_two_adic_generator(log_lde_height));
-    size_t ns = domain >> la;
-    gl* sinv = (gl*)malloc((ns ? ns : 1) * sizeof(gl));
-    {
-        gl x = 1;
-        unsigned lns = log2_exact(ns);
-        for (size_t k = 0; k < ns; k++) { sinv[bitrev_idx(k, lns)] = x; x = gl_mul(x, ginv); }
-    }
-    ef2* cur = deep;
-    while (domain > fdomain) {
-        size_t fd = domain >> la;
-        const uint64_t* mp = (const uint64_t*)cur;
-        size_t hh = fd, ww = 2 * arity;
-        orc_tree* t = orc_lmcs_build(hash_kind, &mp, &hh, &ww, 1, 1);
-        uint64_t root[4];
-        orc_tree_root(t, root);
-        orc_tx_send_commitment(tx, root);
-        orc_tx_grind(tx, p->folding_pow_bits);
-        uint64_t bb[2];
-        orc_tx_sample_ef(tx, bb);
-        ef2* nxt = (ef2*)malloc(fd * sizeof(ef2));
-        #pragma omp parallel for schedule(static)
-        for (size_t k = 0; k < fd; k++) {
-            uint64_t o[2];
-            orc_fri_fold_row(la, (const uint64_t*)(cur + k * arity), sinv[k], bb, o);
-            nxt[k] = ef_make(o[0], o[1]);
-        }
-        ftrees[nrounds++] = t;
-        free(cur);
-        cur = nxt;
-        domain = fd;
-        size_t nn = domain >> la;
-        gl* s2 = (gl*)malloc((nn ? nn : 1) * sizeof(gl));
-        for (size_t k = 0; k < nn; k++) s2[k] = gl_pow(sinv[k * arity], arity);
-        free(sinv);
-        sinv = s2;
-    }
-    free(sinv);
-    {
-        ef2* fp = (ef2*)malloc(fdeg * sizeof(ef2));
-        unsigned lf = log2_exact(fdeg);
-        for (size_t i = 0; i < fdeg; i++) fp[bitrev_idx(i, lf)] = cur[i];
-        orc_idft_ext(fp, fdeg);
-        for (size_t i = 0; i < fdeg / 2; i++) { ef2 t = fp[i]; fp[i] = fp[fdeg - 1 - i]; fp[fdeg - 1 - i] = t; }
-        send_ef_slice(tx, fp, fdeg);
-        free(fp);
-    }
-    free(cur);
+    size_t nrounds = fri_commit(p, hash_kind, deep, log_lde_height, tx, ftrees, NULL);
 
     /* ---- query phase ---- */
     orc_tx_grind(tx, p->query_pow_bits);

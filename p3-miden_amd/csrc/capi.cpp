@@ -590,6 +590,16 @@ int p3g_open_with_callbacks(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t
   });
 }
 
+int p3g_copy_device(p3g_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || ((!dst || !src) && bytes)) return P3G_E_INVAL;
+  return guard([&] {
+    if (!bytes) return;
+    P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
+    P3G_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
+    P3G_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
 int p3g_fill_random_device(p3g_ctx* ctx, uint64_t seed, uint64_t n, uint64_t* dptr) {
   if (!ctx || (!dptr && n)) return P3G_E_INVAL;
   return guard([&] {

@@ -167,6 +167,7 @@ def _load():
         "p3g_free": (C.c_int, [_vp, _vp]),
         "p3g_upload": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
         "p3g_download": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
+        "p3g_copy_device": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
         "p3g_fill_random": (C.c_int, [C.c_uint64, C.c_uint64, _u64p]),
         "p3g_coset_lde_batch": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, _u64p]),
         "p3g_lmcs_alignment": (C.c_uint32, [C.c_uint32]),
@@ -347,6 +348,12 @@ class DeviceMatrix:
     def fill_random(self, seed: int) -> None:
         """Canonical synthetic values from the device counter generator."""
         fill_random_device(self.dev, seed, self.height * self.width, self.ptr)
+
+    def copy_from(self, other: "DeviceMatrix") -> None:
+        """Device-to-device copy of a matrix of the same size."""
+        if other.height * other.width != self.height * self.width:
+            raise ValueError("size mismatch")
+        _check(lib().p3g_copy_device(self.dev.h, self.ptr, other.ptr, 8 * self.height * self.width), "copy_device")
 
     def download(self) -> np.ndarray:
         out = np.empty(self.height * self.width, np.uint64)

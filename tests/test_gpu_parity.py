@@ -4,6 +4,9 @@ oracle on the same seeded inputs.  Bit-exact (integer field arithmetic).
 Sizes stay where the oracle finishes in seconds; the full-size configuration
 is covered in test_gpu_full_size.py through size-independent properties.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -290,3 +293,20 @@ def test_fill_random_device(device):
     G.fill_random_device(device, 99, 1000, d.ptr)
     got = d.download().ravel().tolist()
     assert got == _splitmix_fill(99, 1000)
+
+
+# ---------------------------------------------------------------- committed regression digests
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "regression.json")) as _fh:
+    _REG = json.load(_fh)
+
+
+@pytest.mark.parametrize("name", sorted(_REG))
+def test_gpu_proofs_match_golden_digests(device, name):
+    import hashlib
+    r = _REG[name]
+    tr = [O.rand_matrix(r["seed"] + i, h, w) for i, (h, w) in enumerate(r["shapes"])]
+    root, z, f, c = G.prove(G.LmcsConfig(device, r["kind"]), G.PcsParams(*r["params"]), tr)
+    assert [int(x) for x in root] == r["root"]
+    assert [list(p) for p in z] == r["eval_points"]
+    b = f.astype("<u8").tobytes() + c.astype("<u8").ravel().tobytes()
+    assert hashlib.sha256(b).hexdigest() == r["proof_sha256"]
