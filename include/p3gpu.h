@@ -204,11 +204,15 @@ int p3g_open_with_callbacks(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t
  * returned by run is a view valid until the next run or destroy. */
 #define P3G_PLAN_GRAPH 1u
 #define P3G_PLAN_AUTOTUNE 2u
+#define P3G_PLAN_PER_TRACE 4u /* without autotuning: one NTT launch per trace and pass */
 typedef struct p3g_commit_plan p3g_commit_plan;
 int p3g_commit_plan_create(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* heights, const uint64_t* widths,
                            uint64_t k, uint32_t log_blowup, uint32_t flags, p3g_commit_plan** out);
 int p3g_commit_plan_input(p3g_commit_plan* plan, uint64_t i, uint64_t** dptr);
-int p3g_commit_plan_info(const p3g_commit_plan* plan, uint32_t* kmax, uint32_t* leaf_threads, uint32_t* launches);
+/* the launch shapes in use: NTT stages per pass, leaf workgroup size, launch count, and whether NTT passes
+ * are grouped over traces (1) or launched per trace (0) */
+int p3g_commit_plan_info(const p3g_commit_plan* plan, uint32_t* kmax, uint32_t* leaf_threads, uint32_t* launches,
+                         uint32_t* grouped);
 int p3g_commit_plan_run(p3g_commit_plan* plan, p3g_tree** view);
 void p3g_commit_plan_destroy(p3g_commit_plan* plan);
 /* run the plan, observe the root, sample two points, open (the pcs_trace flow of p3g_prove) */

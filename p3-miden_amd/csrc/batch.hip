@@ -29,8 +29,11 @@
 namespace p3g {
 
 // ---- kernels and helpers defined in lde.hip / lmcs.hip
-extern void launch_ntt_grouped(uint32_t mode, bool ntt8, const void* descs, const uint64_t* starts, uint32_t nd,
-                               uint64_t blocks, hipStream_t s);
+extern void launch_ntt_grouped(uint32_t mode, bool reg, uint32_t K, const void* descs, const uint64_t* starts,
+                               uint32_t nd, uint64_t blocks, hipStream_t s);
+extern void launch_ntt_single(uint32_t mode, bool reg, const void* args, uint64_t blocks, hipStream_t s);
+extern std::vector<uint32_t> ntt_pass_split(uint32_t logn, uint32_t kmax);
+extern bool ntt_reg_ok(uint32_t K, uint32_t logC);
 extern void launch_leaf_raw(uint32_t hash, int threads, const void* mats, uint32_t nmats, const uint64_t* prev,
                             uint32_t prev_shift, uint64_t* out_states, uint64_t* out_digests, uint64_t h,
                             const P2Consts* p2, hipStream_t s);
@@ -48,12 +51,17 @@ constexpr uint32_t NTT_INV_FIRST = 0, NTT_INV = 1, NTT_FWD_FIRST = 2, NTT_FWD = 
 
 struct PassDesc {
   uint32_t trace, mode, s0, K;
-  bool ntt8;
+  bool reg;  // register-blocked kernel (K stages on 2^K x 16 tiles)
   uint64_t blocks;
 };
 
+struct TuneChoice {
+  uint32_t kmax;
+  int leaf_threads;
+  bool grouped;
+};
 std::mutex g_tune_mu;
-std::map<std::string, std::pair<uint32_t, int>> g_tune_cache;
+std::map<std::string, TuneChoice> g_tune_cache;
 }  // namespace
 
 CommitPlan::~CommitPlan() {
@@ -76,10 +84,11 @@ std::string CommitPlan::shape_key() const {
   return k;
 }
 
-void CommitPlan::build_launches(uint32_t kmax, int leaf_threads) {
+void CommitPlan::build_launches(uint32_t kmax, int leaf_threads, bool grouped) {
   launches.clear();
   this->kmax = kmax;
   this->leaf_threads = leaf_threads;
+  this->grouped = grouped;
   const uint32_t B = 1u << log_blowup;
   // ---- NTT passes per trace
   std::vector<std::vector<PassDesc>> inv(traces.size()), fwd(traces.size());
@@ -91,14 +100,15 @@ void CommitPlan::build_launches(uint32_t kmax, int leaf_threads) {
     while ((2u << logC) <= 16 && (2ull << logC) <= w) logC++;
     const uint64_t ncol = (w + (1ull << logC) - 1) >> logC;
     auto add = [&](std::vector<PassDesc>& v, uint32_t mode, uint32_t s0, uint32_t K, uint64_t nblk) {
-      const bool n8 = (K == 8 && logC == 4 && kmax == 8);
-      v.push_back(PassDesc{(uint32_t)j, mode, s0, K, n8, (n >> K) * ncol * nblk});
+      v.push_back(PassDesc{(uint32_t)j, mode, s0, K, ntt_reg_ok(K, logC), (n >> K) * ncol * nblk});
     };
-    uint32_t K0 = logn < kmax ? logn : kmax;
-    add(inv[j], NTT_INV_FIRST, 0, K0, 1);
-    for (uint32_t s0 = K0; s0 < logn; s0 += kmax) add(inv[j], NTT_INV, s0, std::min<uint32_t>(kmax, logn - s0), 1);
-    add(fwd[j], NTT_FWD_FIRST, 0, K0, 1);
-    for (uint32_t s0 = K0; s0 < logn; s0 += kmax) add(fwd[j], NTT_FWD, s0, std::min<uint32_t>(kmax, logn - s0), B);
+    const std::vector<uint32_t> split = ntt_pass_split(logn, kmax);
+    uint32_t s0 = 0;
+    for (size_t i = 0; i < split.size(); i++) {
+      add(inv[j], i == 0 ? NTT_INV_FIRST : NTT_INV, s0, split[i], 1);
+      add(fwd[j], i == 0 ? NTT_FWD_FIRST : NTT_FWD, s0, split[i], i == 0 ? 1 : B);
+      s0 += split[i];
+    }
     max_inv = std::max(max_inv, inv[j].size());
     max_fwd = std::max(max_fwd, fwd[j].size());
   }
@@ -116,15 +126,39 @@ void CommitPlan::build_launches(uint32_t kmax, int leaf_threads) {
   std::vector<Pending> pend;
   auto group = [&](std::vector<std::vector<PassDesc>>& per, size_t depth) {
     for (size_t i = 0; i < depth; i++)
-      for (int variant = 0; variant < 2; variant++) {
+      // variants: the generic LDS kernel (0) and register-blocked kernels by K (5..8)
+      for (int variant : {0, 5, 6, 7, 8}) {
         std::vector<PassDesc> g;
         for (auto& v : per)
-          if (i < v.size() && (int)v[i].ntt8 == variant) g.push_back(v[i]);
+          if (i < v.size() && (v[i].reg ? (int)v[i].K : 0) == variant) g.push_back(v[i]);
         if (g.empty()) continue;
+        if (!grouped) {
+          for (const PassDesc& d : g) {
+            Pending ps{};
+            ps.l.kind = Launch::NTT_SINGLE;
+            ps.l.mode = d.mode;
+            ps.l.reg = d.reg;
+            ps.l.K = d.K;
+            ps.l.nd = 1;
+            ps.l.blocks = d.blocks;
+            ps.l.args.resize(ntt_args_size());
+            const DevMat& tr = traces[d.trace];
+            uint32_t logC = 0;
+            while ((2u << logC) <= 16 && (2ull << logC) <= tr.width) logC++;
+            const uint32_t ncol = (uint32_t)((tr.width + (1ull << logC) - 1) >> logC);
+            const bool inverse = d.mode <= NTT_INV;
+            make_ntt_args(ps.l.args.data(), tr.ptr, ldes[d.trace].ptr, inverse ? tw_inv[d.trace] : tw_fwd[d.trace],
+                          scale_dev[d.trace], scale_stride[d.trace], tr.height, log2_exact(tr.height), d.s0, d.K,
+                          (uint32_t)tr.width, logC, ncol, log_blowup, d.mode);
+            pend.push_back(std::move(ps));
+          }
+          continue;
+        }
         Pending pd{};
         pd.l.kind = Launch::NTT;
         pd.l.mode = g[0].mode;
-        pd.l.ntt8 = variant == 1;
+        pd.l.reg = variant != 0;
+        pd.l.K = (uint32_t)variant;
         pd.l.nd = (uint32_t)g.size();
         pd.desc_off = reserve_bytes(ntt_args_size() * g.size());
         pd.start_off = reserve_bytes(8 * (g.size() + 1));
@@ -184,8 +218,10 @@ void CommitPlan::build_launches(uint32_t kmax, int leaf_threads) {
   P3G_CHECK_HIP(hipMemcpyAsync(blob, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
   P3G_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   for (auto& pd : pend) {
-    Launch l = pd.l;
-    if (l.kind == Launch::NTT) {
+    Launch l = std::move(pd.l);
+    if (l.kind == Launch::NTT_SINGLE) {
+      // descriptor travels by value
+    } else if (l.kind == Launch::NTT) {
       l.descs = blob + pd.desc_off;
       l.starts = reinterpret_cast<const uint64_t*>(blob + pd.start_off);
     } else {
@@ -206,7 +242,9 @@ void CommitPlan::build_launches(uint32_t kmax, int leaf_threads) {
 void CommitPlan::enqueue(hipStream_t s) {
   for (auto& l : launches) {
     if (l.kind == Launch::NTT) {
-      launch_ntt_grouped(l.mode, l.ntt8, l.descs, l.starts, l.nd, l.blocks, s);
+      launch_ntt_grouped(l.mode, l.reg, l.K, l.descs, l.starts, l.nd, l.blocks, s);
+    } else if (l.kind == Launch::NTT_SINGLE) {
+      launch_ntt_single(l.mode, l.reg, l.args.data(), l.blocks, s);
     } else {
       launch_leaf_raw(hash, leaf_threads, l.descs, l.nd, l.prev >= 0 ? states[l.prev] : nullptr, l.prev_shift,
                       l.out_state >= 0 ? states[l.out_state] : nullptr, l.final ? digests : nullptr, l.h,
@@ -252,41 +290,36 @@ void CommitPlan::autotune(hipStream_t s) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      build_launches(it->second.first, it->second.second);
+      build_launches(it->second.kmax, it->second.leaf_threads, it->second.grouped);
       return;
     }
   }
-  uint32_t best_k = 8;
-  int best_t = 128;
-  double best = 1e30;
-  for (uint32_t k : {8u, 6u, 4u}) {
-    build_launches(k, 128);
+  TuneChoice best{8, 128, true};
+  double best_ms = 1e30;
+  auto trial = [&](TuneChoice c) {
+    build_launches(c.kmax, c.leaf_threads, c.grouped);
     time_once(s);  // warm
     const double ms = time_once(s);
-    tune_log += "kmax=" + std::to_string(k) + " leaf=128: " + std::to_string(ms) + " ms\n";
-    if (ms < best) {
-      best = ms;
-      best_k = k;
+    tune_log += "kmax=" + std::to_string(c.kmax) + " leaf=" + std::to_string(c.leaf_threads) +
+                (c.grouped ? " grouped" : " per-trace") + ": " + std::to_string(ms) + " ms\n";
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
     }
-  }
-  for (int t : {64, 256}) {
-    build_launches(best_k, t);
-    time_once(s);
-    const double ms = time_once(s);
-    tune_log += "kmax=" + std::to_string(best_k) + " leaf=" + std::to_string(t) + ": " + std::to_string(ms) + " ms\n";
-    if (ms < best) {
-      best = ms;
-      best_t = t;
-    }
-  }
-  build_launches(best_k, best_t);
+  };
+  // NTT launch structure and stage grouping first, then the leaf workgroup size
+  for (bool grouped : {true, false})
+    for (uint32_t k : {8u, 6u}) trial(TuneChoice{k, 128, grouped});
+  const TuneChoice ntt_best = best;
+  for (int t : {64, 256}) trial(TuneChoice{ntt_best.kmax, t, ntt_best.grouped});
+  build_launches(best.kmax, best.leaf_threads, best.grouped);
   std::lock_guard<std::mutex> g(g_tune_mu);
-  g_tune_cache[key] = {best_k, best_t};
+  g_tune_cache[key] = best;
 }
 
 CommitPlan* make_commit_plan(Ctx& ctx, uint32_t hash, const std::vector<uint64_t>& heights,
                              const std::vector<uint64_t>& widths, unsigned log_blowup, bool use_graph,
-                             bool autotune) {
+                             bool autotune, bool grouped) {
   if (heights.empty() || heights.size() != widths.size()) throw ApiError(E_EMPTY, "at least one trace required");
   for (size_t i = 0; i < heights.size(); i++) {
     if (heights[i] == 0 || (heights[i] & (heights[i] - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
@@ -362,7 +395,7 @@ CommitPlan* make_commit_plan(Ctx& ctx, uint32_t hash, const std::vector<uint64_t
       p->states[1] = (uint64_t*)ctx.alloc(p->height * W * 8);
     }
     if (autotune) p->autotune(ctx.stream);
-    else p->build_launches(8, 128);
+    else p->build_launches(8, 128, grouped);
   } catch (...) {
     delete p;
     throw;

@@ -9,13 +9,15 @@ namespace p3g {
 
 struct CommitPlan {
   struct Launch {
-    enum Kind { NTT, LEAF } kind = NTT;
+    enum Kind { NTT, NTT_SINGLE, LEAF } kind = NTT;
     uint32_t mode = 0;
-    bool ntt8 = false;
+    bool reg = false;  // register-blocked NTT kernel
+    uint32_t K = 0;     // its stage count
     uint32_t nd = 0;
     uint64_t blocks = 0;
     const void* descs = nullptr;
     const uint64_t* starts = nullptr;
+    std::vector<uint8_t> args;  // NTT_SINGLE: the pass descriptor, passed by value
     // leaf hashing
     uint64_t h = 0;
     int prev = -1, out_state = -1;
@@ -41,6 +43,7 @@ struct CommitPlan {
   std::vector<Launch> launches;
   uint32_t kmax = 8;
   int leaf_threads = 128;
+  bool grouped = true;  // one launch per pass index over all traces, or one per trace
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t root[4] = {0, 0, 0, 0};
@@ -48,7 +51,7 @@ struct CommitPlan {
 
   ~CommitPlan();
   std::string shape_key() const;
-  void build_launches(uint32_t kmax, int leaf_threads);
+  void build_launches(uint32_t kmax, int leaf_threads, bool grouped);
   void enqueue(hipStream_t s);
   void capture(hipStream_t s);
   double time_once(hipStream_t s);
@@ -58,6 +61,7 @@ struct CommitPlan {
 };
 
 CommitPlan* make_commit_plan(Ctx& ctx, uint32_t hash, const std::vector<uint64_t>& heights,
-                             const std::vector<uint64_t>& widths, unsigned log_blowup, bool use_graph, bool autotune);
+                             const std::vector<uint64_t>& widths, unsigned log_blowup, bool use_graph, bool autotune,
+                             bool grouped = true);
 
 }  // namespace p3g

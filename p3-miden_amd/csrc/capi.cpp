@@ -618,7 +618,7 @@ int p3g_commit_plan_create(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* hei
     P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
     std::vector<uint64_t> h(heights, heights + k), w(widths, widths + k);
     CommitPlan* p = make_commit_plan(ctx->c, hash_kind, h, w, log_blowup, (flags & P3G_PLAN_GRAPH) != 0,
-                                     (flags & P3G_PLAN_AUTOTUNE) != 0);
+                                     (flags & P3G_PLAN_AUTOTUNE) != 0, (flags & P3G_PLAN_PER_TRACE) == 0);
     *out = new p3g_commit_plan{p};
   });
 }
@@ -629,8 +629,10 @@ int p3g_commit_plan_input(p3g_commit_plan* plan, uint64_t i, uint64_t** dptr) {
   return P3G_OK;
 }
 
-int p3g_commit_plan_info(const p3g_commit_plan* plan, uint32_t* kmax, uint32_t* leaf_threads, uint32_t* launches) {
+int p3g_commit_plan_info(const p3g_commit_plan* plan, uint32_t* kmax, uint32_t* leaf_threads, uint32_t* launches,
+                         uint32_t* grouped) {
   if (!plan) return P3G_E_INVAL;
+  if (grouped) *grouped = plan->p->grouped ? 1u : 0u;
   if (kmax) *kmax = plan->p->kmax;
   if (leaf_threads) *leaf_threads = (uint32_t)plan->p->leaf_threads;
   if (launches) *launches = (uint32_t)plan->p->launches.size();

@@ -168,72 +168,78 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs& a, uint64_t bid) {
 }
 
 
-// Full 8-stage pass on a 256-row x 16-column tile, register blocked: each
-// thread owns one column and 16 rows, performs four butterfly stages in
-// registers, exchanges through an XOR-swizzled LDS tile once, and performs the
-// other four stages in registers.  Layout A: thread (g, c) holds rows
-// x = 16 g + k; layout B: rows x = g + 16 k (k = 0..15).
+// Register-blocked pass of LR stages (5 <= LR <= 8) on a 2^LR-row x 16-column
+// tile with 2^LR threads: each thread owns one column and 16 rows, performs the
+// stages on x bits 0..3 in registers (layout A: x = 16 g + k), exchanges through
+// an XOR-swizzled LDS tile once, and performs the stages on x bits 4..LR-1 in
+// registers (layout B: x = g + G k with G = 2^(LR-4)).  Inverse passes run
+// A then B, forward passes B then A.
 __device__ __forceinline__ uint32_t swz(uint32_t x, uint32_t c) { return x * 16 + (c ^ ((x >> 4) & 15)); }
 
-template <uint32_t MODE>
-__device__ __forceinline__ void ntt8_body(const NttArgs& a, uint64_t bid) {
-  __shared__ uint64_t tile[256 * 16];
-  __shared__ uint64_t tws[256];
+template <uint32_t MODE, int LR>
+__device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
+  constexpr uint32_t T = 1u << LR;      // threads = rows per tile
+  constexpr uint32_t G = 1u << (LR - 4);  // row groups in layout A / row stride in layout B
+  __shared__ uint64_t tile[T * 16];
+  __shared__ uint64_t tws[T];
   const uint32_t tid = threadIdx.x;
   const uint32_t c = tid & 15, g = tid >> 4;
   constexpr bool inverse = MODE <= INV;
   const uint32_t ct = (uint32_t)(bid % a.ncoltiles);
   bid /= a.ncoltiles;
-  const uint32_t lb = inverse ? a.s0 : a.logn - a.s0 - 8;
-  const uint64_t nunits = a.n >> 8;
+  const uint32_t lb = inverse ? a.s0 : a.logn - a.s0 - LR;
+  const uint64_t nunits = a.n >> LR;
   const uint64_t u = bid % nunits;
   const uint64_t blk = bid / nunits;
   const uint64_t lo = u & ((1ull << lb) - 1);
   const uint64_t hi = u >> lb;
-  const uint64_t base = hi << (lb + 8) | lo;
+  const uint64_t base = hi << (lb + LR) | lo;
   const uint64_t xstride = 1ull << lb;
   const uint32_t col = ct * 16 + c;
   const bool live = col < a.w;
 
-  for (uint32_t e = tid; e < 255; e += NTT_T) {
+  for (uint32_t e = tid; e < T - 1; e += T) {
     uint32_t t = 31 - __clz(e + 1);
     uint32_t m = e + 1 - (1u << t);
     uint64_t ex;
     if (inverse) ex = ((uint64_t)m << a.s0 | lo) << (a.logn - a.s0 - t - 1);
-    else ex = ((uint64_t)m << lb | lo) << (a.s0 + 8 - 1 - t);
+    else ex = ((uint64_t)m << lb | lo) << (a.s0 + LR - 1 - t);
     tws[e] = pw(a.tw, ex);
   }
 
+  // inverse passes start in layout A, forward passes in layout B.  The first
+  // forward pass reads its coefficient tile once and emits every coset from it.
+  uint64_t in[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t x = inverse ? 16 * g + k : g + G * k;
+    const uint64_t idx = base + (uint64_t)x * xstride;
+    uint64_t val = 0;
+    if (live) {
+      if (MODE == INV_FIRST) val = a.src[(uint64_t)bitrev32((uint32_t)idx, a.logn) * a.w + col];
+      else val = a.dst[((MODE == FWD ? blk * a.n : 0) + idx) * a.w + col];
+    }
+    in[k] = val;
+  }
   __syncthreads();  // twiddles ready
   const int nrep = MODE == FWD_FIRST ? (1 << a.logB) : 1;
-  // cosets in descending order: coset 0 writes output block 0, which holds the
-  // coefficients, so it must come last
+  // cosets in descending order (block 0 last, as in every pass order)
   for (int rep = nrep - 1; rep >= 0; rep--) {
     uint64_t r[16];
-    // inverse passes start in layout A, forward passes in layout B; the first
-    // forward pass re-reads its coefficient tile for every coset (L2 hits)
-    // FWD_FIRST: coefficient k_glob = base + (g + 16 k) xstride is scaled by
-    // c_t^k_glob / n, built as f0 * step^k (the table's lo half carries 1/n)
-    uint64_t f = 0, step = 0;
     if (MODE == FWD_FIRST) {
+      // coefficient k_glob = base + (g + G k) xstride is scaled by c_t^k_glob / n,
+      // built as f0 * step^k (the table's lo half carries 1/n)
       const uint64_t* sc = a.scale + rep * a.scale_stride;
-      f = pw(sc, base + (uint64_t)g * xstride);
-      step = gl_mul(pw(sc, 16 * xstride), a.n % P);
-    }
+      uint64_t f = pw(sc, base + (uint64_t)g * xstride);
+      const uint64_t step = gl_mul(pw(sc, (uint64_t)G * xstride), a.n % P);
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const uint32_t x = inverse ? 16 * g + k : g + 16 * k;
-      const uint64_t idx = base + (uint64_t)x * xstride;
-      uint64_t val = 0;
-      if (live) {
-        if (MODE == INV_FIRST) val = a.src[(uint64_t)bitrev32((uint32_t)idx, a.logn) * a.w + col];
-        else val = a.dst[((MODE == FWD ? blk * a.n : 0) + idx) * a.w + col];
-      }
-      if (MODE == FWD_FIRST) {
-        val = gl_mul(val, f);
+      for (int k = 0; k < 16; k++) {
+        r[k] = gl_mul(in[k], f);
         f = gl_mul(f, step);
       }
-      r[k] = val;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; k++) r[k] = in[k];
     }
     if (inverse) {
       // stages on x bits 0..3 (k bits), layout A
@@ -253,15 +259,16 @@ __device__ __forceinline__ void ntt8_body(const NttArgs& a, uint64_t bid) {
       for (int k = 0; k < 16; k++) tile[swz(16 * g + k, c)] = r[k];
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < 16; k++) r[k] = tile[swz(g + 16 * k, c)];
-      // stages on x bits 4..7, layout B
+      for (int k = 0; k < 16; k++) r[k] = tile[swz(g + G * k, c)];
+      // stages on x bits 4..LR-1, layout B (x bit t is k bit t - (LR - 4))
 #pragma unroll
-      for (int t = 4; t < 8; t++) {
+      for (int t = 4; t < LR; t++) {
+        const int kb = t - (LR - 4);
 #pragma unroll
         for (int k0 = 0; k0 < 16; k0++) {
-          if (k0 & (1 << (t - 4))) continue;
-          const int k1 = k0 | (1 << (t - 4));
-          const uint64_t tw = tws[(1 << t) - 1 + g + 16 * (k0 & ((1 << (t - 4)) - 1))];
+          if (k0 & (1 << kb)) continue;
+          const int k1 = k0 | (1 << kb);
+          const uint64_t tw = tws[(1 << t) - 1 + g + G * (k0 & ((1 << kb) - 1))];
           const uint64_t x = r[k0], y = gl_mul(r[k1], tw);
           r[k0] = gl_add(x, y);
           r[k1] = gl_sub(x, y);
@@ -269,18 +276,19 @@ __device__ __forceinline__ void ntt8_body(const NttArgs& a, uint64_t bid) {
       }
       if (live) {
 #pragma unroll
-        for (int k = 0; k < 16; k++) a.dst[(base + (uint64_t)(g + 16 * k) * xstride) * a.w + col] = r[k];
+        for (int k = 0; k < 16; k++) a.dst[(base + (uint64_t)(g + G * k) * xstride) * a.w + col] = r[k];
       }
     } else {
-      // DIF: x bits 7..4 first (layout B, k bits 3..0)
+      // DIF: x bits LR-1..4 first (layout B), then bits 3..0 (layout A)
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const int q = 7 - t;
+      for (int t = 0; t < LR - 4; t++) {
+        const int q = LR - 1 - t;
+        const int kb = q - (LR - 4);
 #pragma unroll
         for (int k0 = 0; k0 < 16; k0++) {
-          if (k0 & (1 << (q - 4))) continue;
-          const int k1 = k0 | (1 << (q - 4));
-          const uint64_t tw = tws[(1 << q) - 1 + g + 16 * (k0 & ((1 << (q - 4)) - 1))];
+          if (k0 & (1 << kb)) continue;
+          const int k1 = k0 | (1 << kb);
+          const uint64_t tw = tws[(1 << q) - 1 + g + G * (k0 & ((1 << kb) - 1))];
           const uint64_t x = r[k0], y = r[k1];
           r[k0] = gl_add(x, y);
           r[k1] = gl_mul(gl_sub(x, y), tw);
@@ -288,14 +296,12 @@ __device__ __forceinline__ void ntt8_body(const NttArgs& a, uint64_t bid) {
       }
       __syncthreads();  // previous repetition's readers are done with the tile
 #pragma unroll
-      for (int k = 0; k < 16; k++) tile[swz(g + 16 * k, c)] = r[k];
+      for (int k = 0; k < 16; k++) tile[swz(g + G * k, c)] = r[k];
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < 16; k++) r[k] = tile[swz(16 * g + k, c)];
-      // x bits 3..0, layout A
 #pragma unroll
-      for (int t = 4; t < 8; t++) {
-        const int q = 7 - t;
+      for (int q = 3; q >= 0; q--) {
 #pragma unroll
         for (int k0 = 0; k0 < 16; k0++) {
           if (k0 & (1 << q)) continue;
@@ -307,9 +313,15 @@ __device__ __forceinline__ void ntt8_body(const NttArgs& a, uint64_t bid) {
         }
       }
       const uint64_t ob = MODE == FWD_FIRST ? bitrev32((uint32_t)rep, a.logB) : (MODE == FWD ? blk : 0);
+      // the row stride is made opaque inside the coset loop so that the 16 store
+      // addresses are formed per coset instead of being held across it
+      uint64_t xs = xstride;
+      if (MODE == FWD_FIRST) asm volatile("" : "+v"(xs));
       if (live) {
+        uint64_t* out = a.dst + ((ob * a.n + base + (uint64_t)(16 * g) * xs) * a.w + col);
+        const uint64_t rs = xs * a.w;
 #pragma unroll
-        for (int k = 0; k < 16; k++) a.dst[(ob * a.n + base + (uint64_t)(16 * g + k) * xstride) * a.w + col] = r[k];
+        for (int k = 0; k < 16; k++) out[(uint64_t)k * rs] = r[k];
       }
     }
   }
@@ -319,9 +331,13 @@ __global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4)))
   ntt_pass_body(a, blockIdx.x);
 }
 
-template <uint32_t MODE>
-__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4))) ntt8_kernel(NttArgs a) {
-  ntt8_body<MODE>(a, blockIdx.x);
+// Register-blocked kernels: 2^LR threads per workgroup.  The 8-stage forward
+// passes are held to 4 waves per EU; FWD_FIRST gets more registers (its coset
+// loop keeps the scale state live) at 3 waves.
+template <uint32_t MODE, int LR>
+__global__ void __launch_bounds__(1 << LR) __attribute__((amdgpu_waves_per_eu(MODE == FWD_FIRST ? 3 : 4)))
+ntt_reg_kernel(NttArgs a) {
+  ntt_reg_body<MODE, LR>(a, blockIdx.x);
 }
 
 // One launch for the same pass of several transforms (the traces of a batch):
@@ -339,13 +355,52 @@ ntt_pass_grouped_kernel(const NttArgs* __restrict__ d, const uint64_t* __restric
   ntt_pass_body(a, blockIdx.x - starts[k]);
 }
 
-template <uint32_t MODE>
-__global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4)))
-ntt8_grouped_kernel(const NttArgs* __restrict__ d, const uint64_t* __restrict__ starts, uint32_t nd) {
+template <uint32_t MODE, int LR>
+__global__ void __launch_bounds__(1 << LR) __attribute__((amdgpu_waves_per_eu(MODE == FWD_FIRST ? 3 : 4)))
+ntt_reg_grouped_kernel(const NttArgs* __restrict__ d, const uint64_t* __restrict__ starts, uint32_t nd) {
   const uint32_t k = find_desc(starts, nd, blockIdx.x);
   const NttArgs a = d[k];
-  ntt8_body<MODE>(a, blockIdx.x - starts[k]);
+  ntt_reg_body<MODE, LR>(a, blockIdx.x - starts[k]);
 }
+
+// host-side dispatch over (mode, LR) for the register-blocked kernels
+template <int LR>
+static void launch_reg_lr(uint32_t mode, const NttArgs* a, const NttArgs* d, const uint64_t* starts, uint32_t nd,
+                          uint64_t blocks, hipStream_t s) {
+  const dim3 grid((uint32_t)blocks), block(1u << LR);
+  if (a) {
+    switch (mode) {
+      case INV_FIRST: hipLaunchKernelGGL((ntt_reg_kernel<INV_FIRST, LR>), grid, block, 0, s, *a); break;
+      case INV: hipLaunchKernelGGL((ntt_reg_kernel<INV, LR>), grid, block, 0, s, *a); break;
+      case FWD_FIRST: hipLaunchKernelGGL((ntt_reg_kernel<FWD_FIRST, LR>), grid, block, 0, s, *a); break;
+      default: hipLaunchKernelGGL((ntt_reg_kernel<FWD, LR>), grid, block, 0, s, *a); break;
+    }
+  } else {
+    switch (mode) {
+      case INV_FIRST: hipLaunchKernelGGL((ntt_reg_grouped_kernel<INV_FIRST, LR>), grid, block, 0, s, d, starts, nd); break;
+      case INV: hipLaunchKernelGGL((ntt_reg_grouped_kernel<INV, LR>), grid, block, 0, s, d, starts, nd); break;
+      case FWD_FIRST: hipLaunchKernelGGL((ntt_reg_grouped_kernel<FWD_FIRST, LR>), grid, block, 0, s, d, starts, nd); break;
+      default: hipLaunchKernelGGL((ntt_reg_grouped_kernel<FWD, LR>), grid, block, 0, s, d, starts, nd); break;
+    }
+  }
+}
+
+// Launches a register-blocked pass of K stages (5 <= K <= 8): either one
+// transform (a != nullptr) or a grouped launch over descriptors d/starts.
+static void launch_reg(uint32_t K, uint32_t mode, const NttArgs* a, const NttArgs* d, const uint64_t* starts,
+                       uint32_t nd, uint64_t blocks, hipStream_t s) {
+  switch (K) {
+    case 5: launch_reg_lr<5>(mode, a, d, starts, nd, blocks, s); break;
+    case 6: launch_reg_lr<6>(mode, a, d, starts, nd, blocks, s); break;
+    case 7: launch_reg_lr<7>(mode, a, d, starts, nd, blocks, s); break;
+    default: launch_reg_lr<8>(mode, a, d, starts, nd, blocks, s); break;
+  }
+  P3G_CHECK_HIP(hipGetLastError());
+}
+
+bool ntt_reg_ok(uint32_t K, uint32_t logC) { return logC == 4 && K >= 5 && K <= 8; }
+
+std::vector<uint32_t> ntt_pass_split(uint32_t logn, uint32_t kmax);
 
 void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
                      uint64_t shift, uint64_t* d_out, hipStream_t s) {
@@ -408,35 +463,29 @@ void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, 
     a.mode = mode;
     uint64_t grid = (n >> K) * ncol * nblk;
     ProfScope ps(ctx, mode <= INV ? "lde_inverse_pass" : "lde_forward_pass", s);
-    if (K == 8 && C == 16) {
-      switch (mode) {
-        case INV_FIRST: hipLaunchKernelGGL(ntt8_kernel<INV_FIRST>, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a); break;
-        case INV: hipLaunchKernelGGL(ntt8_kernel<INV>, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a); break;
-        case FWD_FIRST: hipLaunchKernelGGL(ntt8_kernel<FWD_FIRST>, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a); break;
-        default: hipLaunchKernelGGL(ntt8_kernel<FWD>, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a); break;
-      }
-    }
-    else
+    if (ntt_reg_ok(K, logC)) {
+      launch_reg(K, mode, &a, nullptr, nullptr, 0, grid, s);
+    } else {
       hipLaunchKernelGGL(ntt_pass_kernel, dim3((uint32_t)grid), dim3(NTT_T), 0, s, a);
-    P3G_CHECK_HIP(hipGetLastError());
+      P3G_CHECK_HIP(hipGetLastError());
+    }
   };
 
-  // inverse (DIT), first pass gathers bit-reversed rows from the trace
+  // inverse (DIT), first pass gathers bit-reversed rows from the trace;
+  // then forward (DIF) per coset, the first pass emitting all B cosets
+  const std::vector<uint32_t> split = ntt_pass_split(logn, NTT_K);
   {
-    uint32_t K0 = logn < (unsigned)NTT_K ? logn : NTT_K;
-    launch(INV_FIRST, 0, K0, 1);
-    for (uint32_t s0 = K0; s0 < logn; s0 += NTT_K) {
-      uint32_t K = (logn - s0) < (unsigned)NTT_K ? logn - s0 : NTT_K;
-      launch(INV, s0, K, 1);
+    uint32_t s0 = 0;
+    for (size_t i = 0; i < split.size(); i++) {
+      launch(i == 0 ? INV_FIRST : INV, s0, split[i], 1);
+      s0 += split[i];
     }
   }
-  // forward (DIF) per coset
   {
-    uint32_t K0 = logn < (unsigned)NTT_K ? logn : NTT_K;
-    launch(FWD_FIRST, 0, K0, 1);
-    for (uint32_t s0 = K0; s0 < logn; s0 += NTT_K) {
-      uint32_t K = (logn - s0) < (unsigned)NTT_K ? logn - s0 : NTT_K;
-      launch(FWD, s0, K, B);
+    uint32_t s0 = 0;
+    for (size_t i = 0; i < split.size(); i++) {
+      launch(i == 0 ? FWD_FIRST : FWD, s0, split[i], i == 0 ? 1 : B);
+      s0 += split[i];
     }
   }
   ctx.release(scale);
@@ -490,23 +539,43 @@ void make_ntt_args(void* out, const uint64_t* src, uint64_t* dst, const uint64_t
   memcpy(out, &a, sizeof(a));
 }
 
+// One pass of one transform with its descriptor passed by value (no host
+// interaction: safe inside stream capture).
+void launch_ntt_single(uint32_t mode, bool reg, const void* args, uint64_t blocks, hipStream_t s) {
+  NttArgs a;
+  memcpy(&a, args, sizeof(a));
+  if (reg) {
+    launch_reg(a.K, mode, &a, nullptr, nullptr, 0, blocks, s);
+  } else {
+    hipLaunchKernelGGL(ntt_pass_kernel, dim3((uint32_t)blocks), dim3(NTT_T), 0, s, a);
+    P3G_CHECK_HIP(hipGetLastError());
+  }
+}
+
 // Grouped launch of one pass over several transforms (no host interaction:
-// safe inside stream capture).
-void launch_ntt_grouped(uint32_t mode, bool ntt8, const void* descs, const uint64_t* starts, uint32_t nd,
+// safe inside stream capture); register-blocked groups share one K.
+void launch_ntt_grouped(uint32_t mode, bool reg, uint32_t K, const void* descs, const uint64_t* starts, uint32_t nd,
                         uint64_t blocks, hipStream_t s) {
   const NttArgs* d = reinterpret_cast<const NttArgs*>(descs);
-  const dim3 grid((uint32_t)blocks), block(NTT_T);
-  if (ntt8) {
-    switch (mode) {
-      case INV_FIRST: hipLaunchKernelGGL(ntt8_grouped_kernel<INV_FIRST>, grid, block, 0, s, d, starts, nd); break;
-      case INV: hipLaunchKernelGGL(ntt8_grouped_kernel<INV>, grid, block, 0, s, d, starts, nd); break;
-      case FWD_FIRST: hipLaunchKernelGGL(ntt8_grouped_kernel<FWD_FIRST>, grid, block, 0, s, d, starts, nd); break;
-      default: hipLaunchKernelGGL(ntt8_grouped_kernel<FWD>, grid, block, 0, s, d, starts, nd); break;
-    }
+  if (reg) {
+    launch_reg(K, mode, nullptr, d, starts, nd, blocks, s);
   } else {
-    hipLaunchKernelGGL(ntt_pass_grouped_kernel, grid, block, 0, s, d, starts, nd);
+    hipLaunchKernelGGL(ntt_pass_grouped_kernel, dim3((uint32_t)blocks), dim3(NTT_T), 0, s, d, starts, nd);
+    P3G_CHECK_HIP(hipGetLastError());
   }
-  P3G_CHECK_HIP(hipGetLastError());
+}
+
+// Stages per pass for a size-2^logn transform: as few passes as kmax allows,
+// stage counts balanced so that no pass falls far below the others.
+std::vector<uint32_t> ntt_pass_split(uint32_t logn, uint32_t kmax) {
+  std::vector<uint32_t> out;
+  if (logn == 0) {
+    out.push_back(0);
+    return out;
+  }
+  const uint32_t np = (logn + kmax - 1) / kmax;
+  for (uint32_t i = 0; i < np; i++) out.push_back(logn / np + (i < logn % np ? 1 : 0));
+  return out;
 }
 
 }  // namespace p3g

@@ -206,7 +206,8 @@ def _load():
         "p3g_open_with_callbacks": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, C.POINTER(_Channel)]),
         "p3g_commit_plan_create": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
         "p3g_commit_plan_input": (C.c_int, [_vp, C.c_uint64, C.POINTER(_vp)]),
-        "p3g_commit_plan_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "p3g_commit_plan_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                           C.POINTER(C.c_uint32)]),
         "p3g_commit_plan_run": (C.c_int, [_vp, C.POINTER(_vp)]),
         "p3g_commit_plan_destroy": (None, [_vp]),
         "p3g_prove_planned": (C.c_int, [_vp, _vp, C.POINTER(PcsParams), _vp, _u64p, _u64p]),
@@ -631,6 +632,7 @@ def prove(lmcs: LmcsConfig, params: PcsParams, traces, channel: ProverTranscript
 
 PLAN_GRAPH = 1
 PLAN_AUTOTUNE = 2
+PLAN_PER_TRACE = 4
 
 
 class CommitPlan:
@@ -638,14 +640,15 @@ class CommitPlan:
     all traces, optional hipGraph replay and autotuned launch shapes.  Results
     are identical to commit_traces on the same traces."""
 
-    def __init__(self, lmcs: LmcsConfig, shapes, log_blowup: int, graph: bool = True, autotune: bool = False):
+    def __init__(self, lmcs: LmcsConfig, shapes, log_blowup: int, graph: bool = True, autotune: bool = False,
+                 per_trace: bool = False):
         self.lmcs = lmcs
         self.shapes = [(int(h), int(w)) for h, w in shapes]
         self.log_blowup = log_blowup
         hs = u64([h for h, _ in self.shapes]) if shapes else np.zeros(1, np.uint64)
         ws = u64([w for _, w in self.shapes]) if shapes else np.zeros(1, np.uint64)
         h = _vp()
-        flags = (PLAN_GRAPH if graph else 0) | (PLAN_AUTOTUNE if autotune else 0)
+        flags = (PLAN_GRAPH if graph else 0) | (PLAN_AUTOTUNE if autotune else 0) | (PLAN_PER_TRACE if per_trace else 0)
         _check(lib().p3g_commit_plan_create(lmcs.dev.h, lmcs.hash_kind, _p(hs), _p(ws), len(self.shapes), log_blowup,
                                             flags, C.byref(h)), "commit_plan_create")
         self.h = h
@@ -676,9 +679,9 @@ class CommitPlan:
                    "fill_random_device")
 
     def info(self):
-        k, t, n = C.c_uint32(), C.c_uint32(), C.c_uint32()
-        _check(lib().p3g_commit_plan_info(self.h, C.byref(k), C.byref(t), C.byref(n)), "commit_plan_info")
-        return {"kmax": k.value, "leaf_threads": t.value, "launches": n.value}
+        k, t, n, g = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(lib().p3g_commit_plan_info(self.h, C.byref(k), C.byref(t), C.byref(n), C.byref(g)), "commit_plan_info")
+        return {"kmax": k.value, "leaf_threads": t.value, "launches": n.value, "grouped": bool(g.value)}
 
     def run(self) -> LiftedMerkleTree:
         """Recompute LDEs and tree; the returned view is valid until the next run."""

@@ -216,6 +216,7 @@ def test_open_through_caller_channel_callbacks(device, kind):
 # ---------------------------------------------------------------- batched commit plans
 PLAN_CASES = [
     (G.HASH_POSEIDON2, [(1 << 10, 8)], 1, True, False),
+    (G.HASH_POSEIDON2, [(16, 3), (256, 10), (1024, 50), (1 << 13, 17)], 2, True, "per_trace"),
     (G.HASH_POSEIDON2, [(16, 3), (256, 10), (1024, 50), (1024, 16)], 2, True, True),
     (G.HASH_POSEIDON2, [(2, 1), (8, 20), (4096, 33)], 1, False, False),
     (G.HASH_POSEIDON2, [(1 << 12, 8), (1 << 13, 100), (1 << 14, 17)], 1, True, True),
@@ -226,7 +227,9 @@ PLAN_CASES = [
 @pytest.mark.parametrize("kind,shapes,lb,graph,autotune", PLAN_CASES)
 def test_commit_plan_matches_oracle(device, kind, shapes, lb, graph, autotune):
     cfg = G.LmcsConfig(device, kind)
-    plan = G.CommitPlan(cfg, shapes, lb, graph=graph, autotune=autotune)
+    plan = G.CommitPlan(cfg, shapes, lb, graph=graph, autotune=autotune is True, per_trace=autotune == "per_trace")
+    if autotune == "per_trace":
+        assert plan.info()["grouped"] is False
     for rnd in range(2):  # the second run replays the same plan on new inputs
         tr = _mats(500 + 17 * rnd, shapes)
         for i, m in enumerate(tr):
