@@ -352,6 +352,45 @@ def bench_c5(G, W, dev, n_requests, workers):
     }
 
 
+# ------------------------------------------------------------------ many small proofs
+def bench_small(G, W, dev, n_proofs, workers):
+    """n_proofs independent proofs of 2^14 x 32 traces (the CPU-sized configuration's shape):
+    one after another on one context (plain) against the prove-request server."""
+    n, w = 1 << W.C1_LOG_ROWS, W.C1_WIDTH
+    params = G.PcsParams(*W.C1_PARAMS)
+    pool = dev.alloc_matrix(n * n_proofs, w)
+    pool.fill_random(SEED)
+    traces = [pool.view(i * n * w, n, w) for i in range(n_proofs)]
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    for t in traces[:2]:
+        G.prove(cfg, params, [t])
+    t0 = time.perf_counter()
+    plain = [G.prove(cfg, params, [t]) for t in traces]
+    plain_s = time.perf_counter() - t0
+    srv = G.Server(0, workers=workers, hbm_budget_bytes=64 * 10**9)
+    for r in [srv.submit([t], params) for t in traces[:workers]]:
+        srv.result(r)
+    t0 = time.perf_counter()
+    res = [srv.result(tk) for tk in [srv.submit([t], params) for t in traces]]
+    served_s = time.perf_counter() - t0
+    same = all(np.array_equal(a[2], r.fields) and np.array_equal(a[3], r.commitments) for a, r in zip(plain, res))
+    lat = np.array([r.latency_ms for r in res])
+    srv.close()
+    pool.free()
+    dev.trim()
+    return {
+        "workload": f"{n_proofs} independent proofs of 2^14 x 32 traces (log_blowup 1, arity 2 to degree 8, 64 queries)",
+        "plain_proofs_per_s": round(n_proofs / plain_s, 2),
+        "plain_ms_per_proof": round(plain_s * 1e3 / n_proofs, 3),
+        "served_proofs_per_s": round(n_proofs / served_s, 2),
+        "speedup": round(plain_s / served_s, 3),
+        "workers": workers,
+        "bit_exact_served_vs_plain": bool(same),
+        "p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat, 99)), 2),
+    }
+
+
 def cpu_baseline(W, log_rows):
     import oracle as O
 
@@ -377,7 +416,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--sections", default="c2,c3,c4,c5", help="comma list of c2 (headline), c3, c4, c5")
+    ap.add_argument("--sections", default="c2,c3,c4,c5,small",
+                    help="comma list of c2 (headline), c3, c4, c5, small")
+    ap.add_argument("--small-proofs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -405,6 +446,7 @@ def main():
     c3 = bench_c3(G, W, dev, max(2, args.steps // 3), args.warmup) if "c3" in sections else None
     c4 = bench_c4(G, W, dev, max(2, args.steps // 2), args.warmup) if "c4" in sections else None
     c5 = bench_c5(G, W, dev, args.serve_requests, args.serve_workers) if "c5" in sections else None
+    small = bench_small(G, W, dev, args.small_proofs, args.serve_workers) if "small" in sections else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(W, args.cpu_sample_log_rows)
 
     ms = c2["ms"]
@@ -433,6 +475,7 @@ def main():
         "batched": c3,
         "keccak": c4,
         "serving": c5,
+        "small_proofs": small,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

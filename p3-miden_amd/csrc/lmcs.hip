@@ -26,6 +26,7 @@
 
 #include "internal.hpp"
 #include "p2_fast.hpp"
+#include "p2_coop.hpp"
 
 namespace p3g {
 
@@ -94,6 +95,31 @@ __global__ void __launch_bounds__(T) p2_leaf_kernel(LeafArgs a) {
   } else {
 #pragma unroll
     for (int k = 0; k < 12; k++) a.out_states[i * 12 + k] = st[k];
+  }
+}
+
+// Poseidon2 leaf hashing of a short matrix group: one 16-lane row per leaf
+// (the cooperative permutation); same sponge as p2_leaf_kernel.
+constexpr uint64_t COOP_LEAF_MAX = 1u << 14;
+__global__ void __launch_bounds__(256) p2_leaf_coop_kernel(LeafArgs a) {
+  const uint64_t i = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+  const uint32_t l = threadIdx.x & 15;
+  if (i >= a.h) return;  // uniform over the 16-lane row
+  uint64_t x = 0;
+  if (a.prev_states && l < 12) x = a.prev_states[(i >> a.prev_shift) * 12 + l];
+  for (uint32_t m = 0; m < a.nmats; m++) {
+    const uint64_t w = a.mats[m].width;
+    const uint64_t* row = a.mats[m].ptr + i * w;
+    for (uint64_t j = 0; j < w; j += 8) {
+      // overwrite the rate, zero-filling past the row's end
+      if (l < 8) x = j + l < w ? row[j + l] : 0;
+      x = p2_permute_coop(x, l, a.p2);
+    }
+  }
+  if (a.out_digests) {
+    if (l < 4) a.out_digests[i * 4 + l] = x;
+  } else if (l < 12) {
+    a.out_states[i * 12 + l] = x;
   }
 }
 
@@ -183,6 +209,20 @@ __global__ void __launch_bounds__(CMP_T) compress_layer_kernel(const uint64_t* _
   for (int k = 0; k < 4; k++) next[i * 4 + k] = o[k];
 }
 
+// Poseidon2 layer with few nodes: one 16-lane row per node (the cooperative
+// permutation), for levels too small to fill the device one node per lane.
+constexpr uint64_t COOP_LAYER_MAX = 1u << 14;
+__global__ void __launch_bounds__(256) compress_layer_coop_kernel(const uint64_t* __restrict__ prev,
+                                                                  uint64_t* __restrict__ next, uint64_t len,
+                                                                  const P2Consts* p2) {
+  const uint64_t node = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+  const uint32_t l = threadIdx.x & 15;
+  if (node >= len) return;  // uniform over the 16-lane row
+  uint64_t x = l < 8 ? prev[node * 8 + l] : 0;
+  x = p2_permute_coop(x, l, p2);
+  if (l < 4) next[node * 4 + l] = x;
+}
+
 // Remaining levels once a layer has at most 2*TOP_T digests: one workgroup,
 // levels staged through LDS, every level written out.
 constexpr int TOP_T = 256;
@@ -204,6 +244,51 @@ __global__ void __launch_bounds__(TOP_T) compress_top_kernel(uint64_t* digests, 
       for (int k = 0; k < 4; k++) {
         buf[cur ^ 1][tid * 4 + k] = o[k];
         digests[(noff + tid) * 4 + k] = o[k];
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+    off = noff;
+    len = nl;
+  }
+}
+
+// Poseidon2 variant of the tree top: the wide first levels one node per lane,
+// the narrow ones (<= 128 nodes) with the 16-lane cooperative permutation,
+// whose dependent chain is about a fifth as long.
+constexpr int TOPC_T = 1024;
+__global__ void __launch_bounds__(TOPC_T) compress_top_p2_kernel(uint64_t* digests, uint64_t off, uint64_t len,
+                                                                 const P2Consts* p2) {
+  __shared__ uint64_t buf[2][TOP_T * 2 * 4];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t l = tid & 15, row = tid >> 4;
+  for (uint64_t e = tid; e < len * 4; e += TOPC_T) buf[0][e] = digests[off * 4 + e];
+  __syncthreads();
+  int cur = 0;
+  while (len > 1) {
+    const uint64_t nl = len / 2;
+    const uint64_t noff = off + len;
+    if (nl > 128) {
+      if (tid < nl) {
+        uint64_t o[4];
+        compress_one(HASH_P2, &buf[cur][tid * 8], &buf[cur][tid * 8 + 4], o, p2);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          buf[cur ^ 1][tid * 4 + k] = o[k];
+          digests[(noff + tid) * 4 + k] = o[k];
+        }
+      }
+    } else {
+      for (uint64_t base = 0; base < nl; base += TOPC_T / 16) {
+        const uint64_t node = base + row;
+        if (node < nl) {  // uniform over the 16-lane row
+          uint64_t x = l < 8 ? buf[cur][node * 8 + l] : 0;
+          x = p2_permute_coop(x, l, p2);
+          if (l < 4) {
+            buf[cur ^ 1][node * 4 + l] = x;
+            digests[(noff + node) * 4 + l] = x;
+          }
+        }
       }
     }
     __syncthreads();
@@ -242,7 +327,9 @@ static void launch_leaf(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& grou
   uint32_t grid = (uint32_t)((h + LEAF_T - 1) / LEAF_T);
   {
   ProfScope ps(ctx, hash == HASH_P2 ? "p2_leaf" : "keccak_leaf", s);
-  if (hash == HASH_P2) hipLaunchKernelGGL(p2_leaf_kernel<LEAF_T>, dim3(grid), dim3(LEAF_T), 0, s, a);
+  if (hash == HASH_P2 && h <= COOP_LEAF_MAX)
+    hipLaunchKernelGGL(p2_leaf_coop_kernel, dim3((uint32_t)((h * 16 + 255) / 256)), dim3(256), 0, s, a);
+  else if (hash == HASH_P2) hipLaunchKernelGGL(p2_leaf_kernel<LEAF_T>, dim3(grid), dim3(LEAF_T), 0, s, a);
   else hipLaunchKernelGGL(keccak_leaf_kernel<LEAF_T>, dim3(grid), dim3(LEAF_T), 0, s, a);
   P3G_CHECK_HIP(hipGetLastError());
   }
@@ -262,13 +349,20 @@ static void compress_layers(Ctx* ctx, uint32_t hash, uint64_t* digests, const st
     uint32_t grid = (uint32_t)((nl + CMP_T - 1) / CMP_T);
     const uint64_t* prev = digests + layer_off[l] * 4;
     uint64_t* next = digests + layer_off[l + 1] * 4;
+    auto layer = [&] {
+      if (hash == HASH_P2 && nl <= COOP_LAYER_MAX)
+        hipLaunchKernelGGL(compress_layer_coop_kernel, dim3((uint32_t)((nl * 16 + 255) / 256)), dim3(256), 0, s, prev,
+                           next, nl, p2);
+      else if (hash == HASH_P2)
+        hipLaunchKernelGGL(compress_layer_kernel<HASH_P2>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
+      else
+        hipLaunchKernelGGL(compress_layer_kernel<HASH_KECCAK>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
+    };
     if (ctx) {
       ProfScope ps(*ctx, "compress_layer", s);
-      if (hash == HASH_P2) hipLaunchKernelGGL(compress_layer_kernel<HASH_P2>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
-      else hipLaunchKernelGGL(compress_layer_kernel<HASH_KECCAK>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
+      layer();
     } else {
-      if (hash == HASH_P2) hipLaunchKernelGGL(compress_layer_kernel<HASH_P2>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
-      else hipLaunchKernelGGL(compress_layer_kernel<HASH_KECCAK>, dim3(grid), dim3(CMP_T), 0, s, prev, next, nl, p2);
+      layer();
     }
     P3G_CHECK_HIP(hipGetLastError());
     len = nl;
@@ -277,7 +371,7 @@ static void compress_layers(Ctx* ctx, uint32_t hash, uint64_t* digests, const st
   if (len > 1) {
     auto top = [&] {
       if (hash == HASH_P2)
-        hipLaunchKernelGGL(compress_top_kernel<HASH_P2>, dim3(1), dim3(TOP_T), 0, s, digests, layer_off[l], len, p2);
+        hipLaunchKernelGGL(compress_top_p2_kernel, dim3(1), dim3(TOPC_T), 0, s, digests, layer_off[l], len, p2);
       else
         hipLaunchKernelGGL(compress_top_kernel<HASH_KECCAK>, dim3(1), dim3(TOP_T), 0, s, digests, layer_off[l], len, p2);
     };
@@ -321,7 +415,9 @@ void launch_leaf_raw(uint32_t hash, int threads, const void* mats, uint32_t nmat
   a.h = h;
   a.p2 = p2;
   const uint32_t grid = (uint32_t)((h + threads - 1) / threads);
-  if (hash == HASH_P2) {
+  if (hash == HASH_P2 && h <= COOP_LEAF_MAX) {
+    hipLaunchKernelGGL(p2_leaf_coop_kernel, dim3((uint32_t)((h * 16 + 255) / 256)), dim3(256), 0, s, a);
+  } else if (hash == HASH_P2) {
     if (threads == 64) hipLaunchKernelGGL(p2_leaf_kernel<64>, dim3(grid), dim3(64), 0, s, a);
     else if (threads == 256) hipLaunchKernelGGL(p2_leaf_kernel<256>, dim3(grid), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(p2_leaf_kernel<128>, dim3(grid), dim3(128), 0, s, a);
