@@ -367,15 +367,30 @@ def bench_small(G, W, dev, n_proofs, workers):
     t0 = time.perf_counter()
     plain = [G.prove(cfg, params, [t]) for t in traces]
     plain_s = time.perf_counter() - t0
-    srv = G.Server(0, workers=workers, hbm_budget_bytes=64 * 10**9)
+    srv = G.Server(0, workers=workers, hbm_budget_bytes=64 * 10**9, max_batch=64)
     for r in [srv.submit([t], params) for t in traces[:workers]]:
         srv.result(r)
     t0 = time.perf_counter()
     res = [srv.result(tk) for tk in [srv.submit([t], params) for t in traces]]
     served_s = time.perf_counter() - t0
+    sst = srv.stats()
     same = all(np.array_equal(a[2], r.fields) and np.array_equal(a[3], r.commitments) for a, r in zip(plain, res))
     lat = np.array([r.latency_ms for r in res])
     srv.close()
+    # lock-step batches: every device phase one launch over the whole batch
+    batched = {}
+    for bsz in (16, 64, 256):
+        if bsz > n_proofs:
+            continue
+        G.prove_many(cfg, params, [[t] for t in traces[:bsz]])  # warm (tables, allocation cache)
+        t0 = time.perf_counter()
+        outs = []
+        for b0 in range(0, n_proofs, bsz):
+            outs += G.prove_many(cfg, params, [[t] for t in traces[b0:b0 + bsz]])
+        el = time.perf_counter() - t0
+        same = same and all(np.array_equal(a[2], o[2]) and np.array_equal(a[3], o[3]) for a, o in zip(plain, outs))
+        batched[str(bsz)] = {"proofs_per_s": round(n_proofs / el, 2), "ms_per_batch": round(el * 1e3 * bsz / n_proofs, 2),
+                             "speedup_vs_plain": round(plain_s / el, 2)}
     pool.free()
     dev.trim()
     return {
@@ -385,9 +400,11 @@ def bench_small(G, W, dev, n_proofs, workers):
         "served_proofs_per_s": round(n_proofs / served_s, 2),
         "speedup": round(plain_s / served_s, 3),
         "workers": workers,
-        "bit_exact_served_vs_plain": bool(same),
-        "p50_ms": round(float(np.percentile(lat, 50)), 2),
-        "p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "served_batches": {"batches": sst["batches"], "batched_requests": sst["batched_requests"]},
+        "bit_exact_served_and_batched_vs_plain": bool(same),
+        "served_p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "served_p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "batched_lockstep": batched,
     }
 
 

@@ -227,7 +227,10 @@ int p3g_fill_random_device(p3g_ctx* ctx, uint64_t seed, uint64_t n, uint64_t* dp
  * arrival order; a request is admitted when a worker is free and its device
  * footprint (p3g_request_footprint) fits in the HBM budget beside the requests
  * in flight.  The workers share one device block pool; the budget is clamped to
- * what the device has free at creation, less 4 GiB.  A request that does not fit lets later ones that do go first
+ * what the device has free at creation, less 4 GiB.  A worker that admits a
+ * request also takes the later queued requests with the same hash, parameters
+ * and trace shapes (up to max_batch, within the budget) and proves them
+ * together with p3g_prove_many.  A request that does not fit lets later ones that do go first
  * until it has waited backfill_ms.  Each request runs the p3g_prove flow on a
  * fresh transcript of its hash's challenger kind; its result is independent of
  * the other requests.  Trace buffers belong to the caller and must stay valid
@@ -253,12 +256,15 @@ typedef struct {
 } p3g_result_info;
 typedef struct {
   uint64_t admitted, completed, queued, running;
+  uint64_t batches, batched_requests; /* lock-step batches run, and the requests they held */
   uint64_t inflight_bytes, peak_inflight_bytes, peak_running, budget_bytes;
   uint64_t n_malloc, n_free; /* device allocator calls of the shared pool */
   double malloc_ms, free_ms; /* host time spent in them */
 } p3g_server_counters;
 uint64_t p3g_request_footprint(const p3g_request* req); /* device bytes reserved at admission */
-int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms, p3g_server** out);
+/* max_batch: requests of one shape signature proven together in lock-step (1 disables batching) */
+int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms, uint32_t max_batch,
+                      p3g_server** out);
 int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket); /* P3G_E_RANGE: above the budget */
 int p3g_server_wait(p3g_server* s, uint64_t ticket, p3g_result_info* info);     /* blocks; returns the proof status */
 int p3g_server_poll(p3g_server* s, uint64_t ticket, int* done);
@@ -268,6 +274,16 @@ int p3g_server_stats(p3g_server* s, p3g_server_counters* out);
 void p3g_server_destroy(p3g_server* s); /* finishes running requests, drops queued ones */
 int p3g_host_alloc(uint64_t bytes, void** out); /* pinned host memory */
 int p3g_host_free(void* p);
+
+/* ---- batched proofs (no reference counterpart: many independent proofs in lock-step) ----
+ * k jobs of t traces each; traces[j*t + i] is trace i of job j, and every job
+ * has the same heights and widths.  Each job is the p3g_prove flow on its own
+ * transcript chs[j]; every device phase (LDE passes, leaf/node hashing, DEEP,
+ * FRI rounds, query gathers) is one launch over all k jobs.  Outputs: roots
+ * (k*4 words) and eval_points (k*4: two extension points per job). */
+int p3g_prove_many(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* params, const p3g_matrix* traces,
+                   uint64_t k, uint64_t t, uint32_t flags, p3g_transcript* const* chs, uint64_t* roots,
+                   uint64_t* eval_points);
 
 /* ---- one-shot proof (the pcs_trace harness flow, p3-miden-lifted-fri/benches/pcs_trace.rs:60-100,
  * with commit_traces semantics): commit the traces, observe the root, sample two

@@ -1,6 +1,7 @@
 // extern "C" entry points declared in include/p3gpu.h.  Every call catches
 // library and HIP failures and turns them into status codes; nothing throws
 // across the boundary.
+#include <array>
 #include <cstring>
 #include <string>
 
@@ -683,6 +684,60 @@ int p3g_prove_planned(p3g_ctx* ctx, p3g_commit_plan* plan, const p3g_pcs_params*
   if (root) memcpy(root, r, 32);
   if (eval_points_out) memcpy(eval_points_out, z, 32);
   return rc;
+}
+
+int p3g_prove_many(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* params, const p3g_matrix* traces,
+                   uint64_t k, uint64_t t, uint32_t flags, p3g_transcript* const* chs, uint64_t* roots,
+                   uint64_t* eval_points) {
+  if (!ctx || !params || !chs || (k && (!traces || t == 0)) || k > 65535) return P3G_E_INVAL;
+  if (k == 0) return P3G_OK;
+  for (uint64_t j = 0; j < k; j++)
+    if (!chs[j]) return P3G_E_INVAL;
+  return guard([&] {
+    check_hash(hash_kind);
+    P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
+    Ctx& c = ctx->c;
+    PcsParams p{params->log_blowup,   params->log_folding_arity, params->log_final_degree, params->folding_pow_bits,
+                params->deep_pow_bits, params->num_queries,      params->query_pow_bits};
+    std::vector<std::vector<DevMat>> tr(k);
+    std::vector<uint64_t*> uploaded;
+    try {
+      for (uint64_t j = 0; j < k; j++)
+        for (uint64_t i = 0; i < t; i++) {
+          const p3g_matrix& m = traces[j * t + i];
+          if (!m.data && m.height * m.width) throw ApiError(E_INVAL, "null matrix");
+          DevMat d;
+          d.height = m.height;
+          d.width = m.width;
+          d.owned = false;
+          if (flags & P3G_DEVICE_PTRS) {
+            d.ptr = const_cast<uint64_t*>(m.data);
+          } else {
+            d.ptr = to_device(c, m.data, m.height * m.width);
+            uploaded.push_back(d.ptr);
+          }
+          tr[j].push_back(d);
+        }
+      std::vector<Transcript*> txs(k);
+      for (uint64_t j = 0; j < k; j++) txs[j] = &chs[j]->tx;
+      std::vector<std::array<uint64_t, 4>> r;
+      std::vector<std::array<ef2, 2>> z;
+      prove_many(c, hash_kind, p, tr, txs, r, z, c.stream);
+      for (uint64_t j = 0; j < k; j++) {
+        if (roots) memcpy(roots + 4 * j, r[j].data(), 32);
+        if (eval_points) {
+          eval_points[4 * j] = z[j][0].c0;
+          eval_points[4 * j + 1] = z[j][0].c1;
+          eval_points[4 * j + 2] = z[j][1].c0;
+          eval_points[4 * j + 3] = z[j][1].c1;
+        }
+      }
+    } catch (...) {
+      for (auto* u : uploaded) c.release(u);
+      throw;
+    }
+    for (auto* u : uploaded) c.release(u);
+  });
 }
 
 int p3g_prove(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* params, const p3g_matrix* traces, uint64_t k,

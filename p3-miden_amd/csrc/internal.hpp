@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <mutex>
 #include <string>
@@ -228,7 +229,22 @@ size_t fri_final_degree(const PcsParams& p, unsigned log_domain);
 void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
                      uint64_t shift, uint64_t* d_out, hipStream_t s);
 void bit_reverse_rows(Ctx& ctx, const uint64_t* src, uint64_t* dst, uint64_t h, uint64_t w, hipStream_t s);
+// one transform of a batch of same-shaped coset LDEs (bit-reversed rows out)
+struct LdeJob {
+  const uint64_t* src;
+  uint64_t* dst;
+  uint64_t n, w, shift;
+};
+void coset_lde_many(Ctx& ctx, const std::vector<LdeJob>& jobs, unsigned log_blowup, hipStream_t s);
 
+// trees of k jobs with identical matrix shapes, built with shared launches; the
+// trees take ownership of the matrices whose DevMat::owned is set
+std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<std::vector<DevMat>>& mats,
+                                   uint32_t alignment, hipStream_t s);
+size_t prove_batch_pieces(const Tree& t, const std::vector<uint64_t>& indices,
+                          std::vector<std::pair<const uint64_t*, uint64_t>>& pieces);
+size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces, size_t first,
+                        size_t last, size_t nrows, const uint64_t* raw, Channel& tx);
 Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s);
 void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out);
 void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s);
@@ -242,5 +258,10 @@ Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, 
 
 void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned log_lde_height, const ef2* z, int npts,
                        const std::vector<Tree*>& trees, Channel& tx, hipStream_t s);
+
+// k same-shaped proofs in lock-step (the p3g_prove flow per job)
+void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<std::vector<DevMat>>& traces,
+                const std::vector<Transcript*>& txs, std::vector<std::array<uint64_t, 4>>& roots,
+                std::vector<std::array<ef2, 2>>& zs, hipStream_t s);
 
 }  // namespace p3g

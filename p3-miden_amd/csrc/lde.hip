@@ -401,6 +401,8 @@ static void launch_reg(uint32_t K, uint32_t mode, const NttArgs* a, const NttArg
 bool ntt_reg_ok(uint32_t K, uint32_t logC) { return logC == 4 && K >= 5 && K <= 8; }
 
 std::vector<uint32_t> ntt_pass_split(uint32_t logn, uint32_t kmax);
+void launch_ntt_grouped(uint32_t mode, bool reg, uint32_t K, const void* descs, const uint64_t* starts, uint32_t nd,
+                        uint64_t blocks, hipStream_t s);
 
 void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
                      uint64_t shift, uint64_t* d_out, hipStream_t s) {
@@ -563,6 +565,106 @@ void launch_ntt_grouped(uint32_t mode, bool reg, uint32_t K, const void* descs, 
     hipLaunchKernelGGL(ntt_pass_grouped_kernel, dim3((uint32_t)blocks), dim3(NTT_T), 0, s, d, starts, nd);
     P3G_CHECK_HIP(hipGetLastError());
   }
+}
+
+// Coset LDEs of many traces of one shape (n x w) at once: every pass is one
+// grouped launch over all transforms, scale tables shared per distinct shift.
+void coset_lde_many(Ctx& ctx, const std::vector<LdeJob>& jobs, unsigned log_blowup, hipStream_t s) {
+  if (jobs.empty()) return;
+  const uint64_t n = jobs[0].n, w = jobs[0].w;
+  for (auto& j : jobs)
+    if (j.n != n || j.w != w) throw ApiError(E_INVAL, "batched LDE needs one trace shape");
+  if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
+  if (w == 0) return;
+  const unsigned logn = log2_exact(n);
+  if (logn + log_blowup > 32) throw ApiError(E_RANGE, "LDE height exceeds the field two-adicity");
+  uint32_t logC = 0;
+  while ((2u << logC) <= NTT_C && (2ull << logC) <= w) logC++;
+  const uint32_t ncol = (uint32_t)((w + (1u << logC) - 1) >> logC);
+  const uint32_t B = 1u << log_blowup;
+  const uint32_t twbits = logn ? logn - 1 : 0;
+  const uint64_t* tw_inv = ctx.pow_table(gl_inv(gl_two_adic_generator(logn)), twbits);
+  const uint64_t* tw_fwd = ctx.pow_table(gl_two_adic_generator(logn), twbits);
+  const uint64_t stride = POW_LO + pow_table_hi_len(logn);
+  std::map<uint64_t, uint64_t*> scales;
+  std::vector<uint64_t*> owned;
+  std::vector<NttArgs> descs;     // host images of the uploads (outlive them)
+  std::vector<uint64_t> starts;
+  auto cleanup = [&] {
+    (void)hipStreamSynchronize(s);
+    for (auto* p : owned) ctx.release(p);
+  };
+  try {
+    for (auto& j : jobs) {
+      if (scales.count(j.shift)) continue;
+      std::vector<uint64_t> host(stride * B);
+      const uint64_t ninv = gl_inv_pow2(logn);
+      const uint64_t om = gl_two_adic_generator(logn + log_blowup);
+      uint64_t ct = j.shift;
+      for (uint32_t t = 0; t < B; t++) {
+        uint64_t* tab = host.data() + t * stride;
+        uint64_t v = ninv;
+        for (uint32_t i = 0; i < POW_LO; i++) {
+          tab[i] = v;
+          v = gl_mul(v, ct);
+        }
+        uint64_t step = gl_pow(ct, POW_LO), h = 1;
+        for (uint64_t i = 0; i < pow_table_hi_len(logn); i++) {
+          tab[POW_LO + i] = h;
+          h = gl_mul(h, step);
+        }
+        ct = gl_mul(ct, om);
+      }
+      uint64_t* sc = (uint64_t*)ctx.alloc(host.size() * 8);
+      owned.push_back(sc);
+      P3G_CHECK_HIP(hipMemcpyAsync(sc, host.data(), host.size() * 8, hipMemcpyHostToDevice, s));
+      P3G_CHECK_HIP(hipStreamSynchronize(s));
+      scales[j.shift] = sc;
+    }
+    // descriptors of every pass for every job, uploaded once: [pass][job]
+    const std::vector<uint32_t> split = ntt_pass_split(logn, NTT_K);
+    const size_t np = 2 * split.size(), k = jobs.size();
+    descs.assign(np * k, NttArgs{});
+    starts.assign(np * (k + 1), 0);
+    std::vector<uint32_t> modes(np), Ks(np);
+    std::vector<uint64_t> nblks(np);
+    uint32_t s0 = 0;
+    for (size_t i = 0; i < split.size(); i++) {
+      modes[i] = i == 0 ? INV_FIRST : INV;
+      modes[split.size() + i] = i == 0 ? FWD_FIRST : FWD;
+      Ks[i] = Ks[split.size() + i] = split[i];
+      nblks[i] = 1;
+      nblks[split.size() + i] = i == 0 ? 1 : B;
+      for (size_t q = 0; q < 2; q++) {
+        const size_t pi = q * split.size() + i;
+        for (size_t jj = 0; jj < k; jj++) {
+          const LdeJob& jb = jobs[jj];
+          make_ntt_args(&descs[pi * k + jj], jb.src, jb.dst, q == 0 ? tw_inv : tw_fwd, scales[jb.shift], stride, n,
+                        logn, s0, split[i], (uint32_t)w, logC, ncol, log_blowup, modes[pi]);
+          starts[pi * (k + 1) + jj] = jj * ((n >> split[i]) * ncol * nblks[pi]);
+        }
+        starts[pi * (k + 1) + k] = k * ((n >> split[i]) * ncol * nblks[pi]);
+      }
+      s0 += split[i];
+    }
+    NttArgs* ddesc = (NttArgs*)ctx.alloc(descs.size() * sizeof(NttArgs));
+    owned.push_back((uint64_t*)ddesc);
+    uint64_t* dstarts = (uint64_t*)ctx.alloc(starts.size() * 8);
+    owned.push_back(dstarts);
+    P3G_CHECK_HIP(hipMemcpyAsync(ddesc, descs.data(), descs.size() * sizeof(NttArgs), hipMemcpyHostToDevice, s));
+    P3G_CHECK_HIP(hipMemcpyAsync(dstarts, starts.data(), starts.size() * 8, hipMemcpyHostToDevice, s));
+    for (size_t pi = 0; pi < np; pi++) {
+      ProfScope ps(ctx, modes[pi] <= INV ? "lde_inverse_pass" : "lde_forward_pass", s);
+      launch_ntt_grouped(modes[pi], ntt_reg_ok(Ks[pi], logC), Ks[pi], ddesc + pi * k, dstarts + pi * (k + 1),
+                         (uint32_t)k, starts[pi * (k + 1) + k], s);
+    }
+    // the descriptor uploads came from host vectors that die here
+    P3G_CHECK_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
 }
 
 // Stages per pass for a size-2^logn transform: as few passes as kmax allows,

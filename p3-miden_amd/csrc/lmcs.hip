@@ -48,9 +48,7 @@ struct LeafArgs {
   const P2Consts* p2;
 };
 
-template <int T>
-__global__ void __launch_bounds__(T) p2_leaf_kernel(LeafArgs a) {
-  const uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x;
+__device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i) {
   if (i >= a.h) return;
   uint64_t st[12];
   if (a.prev_states) {
@@ -98,12 +96,15 @@ __global__ void __launch_bounds__(T) p2_leaf_kernel(LeafArgs a) {
   }
 }
 
+template <int T>
+__global__ void __launch_bounds__(T) p2_leaf_kernel(LeafArgs a) {
+  p2_leaf_body(a, (uint64_t)blockIdx.x * T + threadIdx.x);
+}
+
 // Poseidon2 leaf hashing of a short matrix group: one 16-lane row per leaf
 // (the cooperative permutation); same sponge as p2_leaf_kernel.
 constexpr uint64_t COOP_LEAF_MAX = 1u << 14;
-__global__ void __launch_bounds__(256) p2_leaf_coop_kernel(LeafArgs a) {
-  const uint64_t i = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
-  const uint32_t l = threadIdx.x & 15;
+__device__ __forceinline__ void p2_leaf_coop_body(const LeafArgs& a, uint64_t i, uint32_t l) {
   if (i >= a.h) return;  // uniform over the 16-lane row
   uint64_t x = 0;
   if (a.prev_states && l < 12) x = a.prev_states[(i >> a.prev_shift) * 12 + l];
@@ -123,9 +124,11 @@ __global__ void __launch_bounds__(256) p2_leaf_coop_kernel(LeafArgs a) {
   }
 }
 
-template <int T>
-__global__ void __launch_bounds__(T) keccak_leaf_kernel(LeafArgs a) {
-  const uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x;
+__global__ void __launch_bounds__(256) p2_leaf_coop_kernel(LeafArgs a) {
+  p2_leaf_coop_body(a, ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4, threadIdx.x & 15);
+}
+
+__device__ __forceinline__ void keccak_leaf_body(const LeafArgs& a, uint64_t i) {
   if (i >= a.h) return;
   uint64_t st[25];
   if (a.prev_states) {
@@ -161,6 +164,22 @@ __global__ void __launch_bounds__(T) keccak_leaf_kernel(LeafArgs a) {
   }
 }
 
+template <int T>
+__global__ void __launch_bounds__(T) keccak_leaf_kernel(LeafArgs a) {
+  keccak_leaf_body(a, (uint64_t)blockIdx.x * T + threadIdx.x);
+}
+
+// ---- job-indexed forms for batches of same-shaped trees (blockIdx.y = job)
+__global__ void __launch_bounds__(128) p2_leaf_many_kernel(const LeafArgs* __restrict__ args) {
+  p2_leaf_body(args[blockIdx.y], (uint64_t)blockIdx.x * 128 + threadIdx.x);
+}
+__global__ void __launch_bounds__(256) p2_leaf_coop_many_kernel(const LeafArgs* __restrict__ args) {
+  p2_leaf_coop_body(args[blockIdx.y], ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4, threadIdx.x & 15);
+}
+__global__ void __launch_bounds__(128) keccak_leaf_many_kernel(const LeafArgs* __restrict__ args) {
+  keccak_leaf_body(args[blockIdx.y], (uint64_t)blockIdx.x * 128 + threadIdx.x);
+}
+
 constexpr int CMP_T = 128;
 
 __device__ __forceinline__ void compress_one(uint32_t hash, const uint64_t* l, const uint64_t* r, uint64_t* out,
@@ -192,10 +211,8 @@ __device__ __forceinline__ void compress_one(uint32_t hash, const uint64_t* l, c
 }
 
 template <uint32_t HASH>
-__global__ void __launch_bounds__(CMP_T) compress_layer_kernel(const uint64_t* __restrict__ prev,
-                                                               uint64_t* __restrict__ next, uint64_t len,
-                                                               const P2Consts* p2) {
-  const uint64_t i = (uint64_t)blockIdx.x * CMP_T + threadIdx.x;
+__device__ __forceinline__ void compress_layer_body(const uint64_t* __restrict__ prev, uint64_t* __restrict__ next,
+                                                    uint64_t len, const P2Consts* p2, uint64_t i) {
   if (i >= len) return;
   uint64_t l[4], r[4], o[4];
   const uint64_t* src = prev + i * 8;
@@ -209,26 +226,52 @@ __global__ void __launch_bounds__(CMP_T) compress_layer_kernel(const uint64_t* _
   for (int k = 0; k < 4; k++) next[i * 4 + k] = o[k];
 }
 
+template <uint32_t HASH>
+__global__ void __launch_bounds__(CMP_T) compress_layer_kernel(const uint64_t* __restrict__ prev,
+                                                               uint64_t* __restrict__ next, uint64_t len,
+                                                               const P2Consts* p2) {
+  compress_layer_body<HASH>(prev, next, len, p2, (uint64_t)blockIdx.x * CMP_T + threadIdx.x);
+}
+
+// one layer of a batch of same-shaped trees: job j reads layer[j].prev, writes layer[j].next
+struct LayerJob {
+  const uint64_t* prev;
+  uint64_t* next;
+};
+template <uint32_t HASH>
+__global__ void __launch_bounds__(CMP_T) compress_layer_many_kernel(const LayerJob* __restrict__ jobs, uint64_t len,
+                                                                    const P2Consts* p2) {
+  const LayerJob j = jobs[blockIdx.y];
+  compress_layer_body<HASH>(j.prev, j.next, len, p2, (uint64_t)blockIdx.x * CMP_T + threadIdx.x);
+}
+
 // Poseidon2 layer with few nodes: one 16-lane row per node (the cooperative
 // permutation), for levels too small to fill the device one node per lane.
 constexpr uint64_t COOP_LAYER_MAX = 1u << 14;
-__global__ void __launch_bounds__(256) compress_layer_coop_kernel(const uint64_t* __restrict__ prev,
-                                                                  uint64_t* __restrict__ next, uint64_t len,
-                                                                  const P2Consts* p2) {
-  const uint64_t node = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
-  const uint32_t l = threadIdx.x & 15;
+__device__ __forceinline__ void compress_layer_coop_body(const uint64_t* __restrict__ prev, uint64_t* __restrict__ next,
+                                                         uint64_t len, const P2Consts* p2, uint64_t node, uint32_t l) {
   if (node >= len) return;  // uniform over the 16-lane row
   uint64_t x = l < 8 ? prev[node * 8 + l] : 0;
   x = p2_permute_coop(x, l, p2);
   if (l < 4) next[node * 4 + l] = x;
+}
+__global__ void __launch_bounds__(256) compress_layer_coop_kernel(const uint64_t* __restrict__ prev,
+                                                                  uint64_t* __restrict__ next, uint64_t len,
+                                                                  const P2Consts* p2) {
+  compress_layer_coop_body(prev, next, len, p2, ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4, threadIdx.x & 15);
+}
+__global__ void __launch_bounds__(256) compress_layer_coop_many_kernel(const LayerJob* __restrict__ jobs, uint64_t len,
+                                                                       const P2Consts* p2) {
+  const LayerJob j = jobs[blockIdx.y];
+  compress_layer_coop_body(j.prev, j.next, len, p2, ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4,
+                           threadIdx.x & 15);
 }
 
 // Remaining levels once a layer has at most 2*TOP_T digests: one workgroup,
 // levels staged through LDS, every level written out.
 constexpr int TOP_T = 256;
 template <uint32_t HASH>
-__global__ void __launch_bounds__(TOP_T) compress_top_kernel(uint64_t* digests, uint64_t off, uint64_t len,
-                                                             const P2Consts* p2) {
+__device__ __forceinline__ void compress_top_body(uint64_t* digests, uint64_t off, uint64_t len, const P2Consts* p2) {
   __shared__ uint64_t buf[2][TOP_T * 2 * 4];
   const uint32_t tid = threadIdx.x;
   for (uint64_t e = tid; e < len * 4; e += TOP_T) buf[0][e] = digests[off * 4 + e];
@@ -253,12 +296,24 @@ __global__ void __launch_bounds__(TOP_T) compress_top_kernel(uint64_t* digests, 
   }
 }
 
+template <uint32_t HASH>
+__global__ void __launch_bounds__(TOP_T) compress_top_kernel(uint64_t* digests, uint64_t off, uint64_t len,
+                                                             const P2Consts* p2) {
+  compress_top_body<HASH>(digests, off, len, p2);
+}
+// batch of same-shaped trees: one workgroup per tree (digest arrays per job)
+template <uint32_t HASH>
+__global__ void __launch_bounds__(TOP_T) compress_top_many_kernel(uint64_t* const* __restrict__ digests, uint64_t off,
+                                                                  uint64_t len, const P2Consts* p2) {
+  compress_top_body<HASH>(digests[blockIdx.x], off, len, p2);
+}
+
 // Poseidon2 variant of the tree top: the wide first levels one node per lane,
 // the narrow ones (<= 128 nodes) with the 16-lane cooperative permutation,
 // whose dependent chain is about a fifth as long.
 constexpr int TOPC_T = 1024;
-__global__ void __launch_bounds__(TOPC_T) compress_top_p2_kernel(uint64_t* digests, uint64_t off, uint64_t len,
-                                                                 const P2Consts* p2) {
+__device__ __forceinline__ void compress_top_p2_body(uint64_t* digests, uint64_t off, uint64_t len,
+                                                     const P2Consts* p2) {
   __shared__ uint64_t buf[2][TOP_T * 2 * 4];
   const uint32_t tid = threadIdx.x;
   const uint32_t l = tid & 15, row = tid >> 4;
@@ -296,6 +351,15 @@ __global__ void __launch_bounds__(TOPC_T) compress_top_p2_kernel(uint64_t* diges
     off = noff;
     len = nl;
   }
+}
+
+__global__ void __launch_bounds__(TOPC_T) compress_top_p2_kernel(uint64_t* digests, uint64_t off, uint64_t len,
+                                                                 const P2Consts* p2) {
+  compress_top_p2_body(digests, off, len, p2);
+}
+__global__ void __launch_bounds__(TOPC_T) compress_top_p2_many_kernel(uint64_t* const* __restrict__ digests,
+                                                                      uint64_t off, uint64_t len, const P2Consts* p2) {
+  compress_top_p2_body(digests[blockIdx.x], off, len, p2);
 }
 
 __global__ void gather_kernel(const uint64_t* const* __restrict__ srcs, const uint64_t* __restrict__ lens,
@@ -484,6 +548,181 @@ Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint3
   return t;
 }
 
+std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<std::vector<DevMat>>& mats,
+                                   uint32_t alignment, hipStream_t s) {
+  const size_t k = mats.size();
+  std::vector<Tree*> trees;
+  if (k == 0) return trees;
+  const std::vector<DevMat>& m0 = mats[0];
+  if (m0.empty()) throw ApiError(E_EMPTY, "cannot commit empty batch");
+  uint64_t prevh = 0;
+  for (auto& m : m0) {
+    if (m.height == 0 || (m.height & (m.height - 1))) throw ApiError(E_NOT_POW2, "matrix height must be a power of two");
+    if (m.height < prevh) throw ApiError(E_UNSORTED, "matrices must be sorted by height");
+    prevh = m.height;
+  }
+  for (auto& mj : mats) {
+    if (mj.size() != m0.size()) throw ApiError(E_INVAL, "batched trees need one shape");
+    for (size_t i = 0; i < mj.size(); i++)
+      if (mj[i].height != m0[i].height || mj[i].width != m0[i].width) throw ApiError(E_INVAL, "batched trees need one shape");
+  }
+  const uint64_t H = m0.back().height;
+  const unsigned depth = log2_exact(H);
+  const uint32_t W = hash_width(hash);
+  std::vector<uint64_t*> scratch;
+  uint8_t* blob = nullptr;
+  std::vector<uint8_t> host;  // host image of the launch descriptors (outlives its upload)
+  try {
+    std::vector<uint64_t> layer_off(depth + 1);
+    uint64_t off = 0;
+    for (unsigned l = 0; l <= depth; l++) {
+      layer_off[l] = off;
+      off += H >> l;
+    }
+    for (size_t j = 0; j < k; j++) {
+      Tree* t = new Tree();
+      t->ctx = &ctx;
+      t->hash = hash;
+      t->alignment = alignment < 1 ? 1 : alignment;
+      t->mats = mats[j];
+      t->height = H;
+      t->depth = depth;
+      t->layer_off = layer_off;
+      trees.push_back(t);
+      t->digests = (uint64_t*)ctx.alloc(off * 4 * sizeof(uint64_t));
+    }
+    // height groups
+    std::vector<std::pair<size_t, size_t>> groups;
+    for (size_t i = 0; i < m0.size();) {
+      size_t e = i;
+      while (e < m0.size() && m0[e].height == m0[i].height) e++;
+      groups.emplace_back(i, e);
+      i = e;
+    }
+    std::vector<uint64_t*> states[2];
+    if (groups.size() > 1)
+      for (int b = 0; b < 2; b++)
+        for (size_t j = 0; j < k; j++) {
+          uint64_t* p = (uint64_t*)ctx.alloc(H * W * sizeof(uint64_t));
+          scratch.push_back(p);
+          states[b].push_back(p);
+        }
+    // descriptor layout: [LeafMat arrays | LeafArgs per group | LayerJob per level | digest ptrs]
+    auto reserve = [&](size_t bytes) {
+      size_t o = (host.size() + 255) & ~size_t(255);
+      host.resize(o + bytes);
+      return o;
+    };
+    std::vector<size_t> args_off(groups.size());
+    for (size_t g = 0; g < groups.size(); g++) {
+      const size_t nm = groups[g].second - groups[g].first;
+      std::vector<size_t> mats_off(k);
+      for (size_t j = 0; j < k; j++) mats_off[j] = reserve(nm * sizeof(LeafMat));
+      args_off[g] = reserve(k * sizeof(LeafArgs));
+      const uint64_t h = m0[groups[g].first].height;
+      for (size_t j = 0; j < k; j++) {
+        for (size_t q = 0; q < nm; q++) {
+          LeafMat lm{mats[j][groups[g].first + q].ptr, mats[j][groups[g].first + q].width};
+          memcpy(host.data() + mats_off[j] + q * sizeof(LeafMat), &lm, sizeof(lm));
+        }
+        LeafArgs a{};
+        a.mats = reinterpret_cast<const LeafMat*>(mats_off[j]);  // relocated after upload
+        a.nmats = (uint32_t)nm;
+        a.prev_states = g ? states[(g - 1) & 1][j] : nullptr;
+        a.prev_shift = g ? log2_exact(h / m0[groups[g - 1].first].height) : 0;
+        a.out_states = g + 1 < groups.size() ? states[g & 1][j] : nullptr;
+        a.out_digests = g + 1 < groups.size() ? nullptr : trees[j]->digests;
+        a.h = h;
+        a.p2 = ctx.dev_p2;
+        memcpy(host.data() + args_off[g] + j * sizeof(LeafArgs), &a, sizeof(a));
+      }
+    }
+    std::vector<size_t> level_off;
+    uint64_t len = H;
+    unsigned lvl = 0;
+    while (len > 2 * TOP_T) {
+      const size_t o = reserve(k * sizeof(LayerJob));
+      for (size_t j = 0; j < k; j++) {
+        LayerJob lj{trees[j]->digests + layer_off[lvl] * 4, trees[j]->digests + layer_off[lvl + 1] * 4};
+        memcpy(host.data() + o + j * sizeof(LayerJob), &lj, sizeof(lj));
+      }
+      level_off.push_back(o);
+      len /= 2;
+      lvl++;
+    }
+    const size_t dptr_off = reserve(k * sizeof(uint64_t*));
+    for (size_t j = 0; j < k; j++) memcpy(host.data() + dptr_off + j * sizeof(uint64_t*), &trees[j]->digests, 8);
+    blob = (uint8_t*)ctx.alloc(host.size());
+    // relocate the LeafMat array pointers to device addresses
+    for (size_t g = 0; g < groups.size(); g++)
+      for (size_t j = 0; j < k; j++) {
+        LeafArgs* a = reinterpret_cast<LeafArgs*>(host.data() + args_off[g] + j * sizeof(LeafArgs));
+        a->mats = reinterpret_cast<const LeafMat*>(blob + reinterpret_cast<uintptr_t>(a->mats));
+      }
+    P3G_CHECK_HIP(hipMemcpyAsync(blob, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    {
+      ProfScope ps(ctx, hash == HASH_P2 ? "p2_leaf" : "keccak_leaf", s);
+      for (size_t g = 0; g < groups.size(); g++) {
+        const uint64_t h = m0[groups[g].first].height;
+        const LeafArgs* da = reinterpret_cast<const LeafArgs*>(blob + args_off[g]);
+        if (hash == HASH_P2 && h <= COOP_LEAF_MAX)
+          hipLaunchKernelGGL(p2_leaf_coop_many_kernel, dim3((uint32_t)((h * 16 + 255) / 256), (uint32_t)k), dim3(256),
+                             0, s, da);
+        else if (hash == HASH_P2)
+          hipLaunchKernelGGL(p2_leaf_many_kernel, dim3((uint32_t)((h + 127) / 128), (uint32_t)k), dim3(128), 0, s, da);
+        else
+          hipLaunchKernelGGL(keccak_leaf_many_kernel, dim3((uint32_t)((h + 127) / 128), (uint32_t)k), dim3(128), 0, s,
+                             da);
+        P3G_CHECK_HIP(hipGetLastError());
+      }
+    }
+    len = H;
+    for (size_t li = 0; li < level_off.size(); li++) {
+      const uint64_t nl = len / 2;
+      const LayerJob* dj = reinterpret_cast<const LayerJob*>(blob + level_off[li]);
+      ProfScope ps(ctx, "compress_layer", s);
+      if (hash == HASH_P2 && nl <= COOP_LAYER_MAX)
+        hipLaunchKernelGGL(compress_layer_coop_many_kernel, dim3((uint32_t)((nl * 16 + 255) / 256), (uint32_t)k),
+                           dim3(256), 0, s, dj, nl, ctx.dev_p2);
+      else if (hash == HASH_P2)
+        hipLaunchKernelGGL(compress_layer_many_kernel<HASH_P2>, dim3((uint32_t)((nl + CMP_T - 1) / CMP_T), (uint32_t)k),
+                           dim3(CMP_T), 0, s, dj, nl, ctx.dev_p2);
+      else
+        hipLaunchKernelGGL(compress_layer_many_kernel<HASH_KECCAK>,
+                           dim3((uint32_t)((nl + CMP_T - 1) / CMP_T), (uint32_t)k), dim3(CMP_T), 0, s, dj, nl,
+                           ctx.dev_p2);
+      P3G_CHECK_HIP(hipGetLastError());
+      len = nl;
+    }
+    if (len > 1) {
+      uint64_t* const* dd = reinterpret_cast<uint64_t* const*>(blob + dptr_off);
+      ProfScope ps(ctx, "compress_top", s);
+      if (hash == HASH_P2)
+        hipLaunchKernelGGL(compress_top_p2_many_kernel, dim3((uint32_t)k), dim3(TOPC_T), 0, s, dd,
+                           layer_off[level_off.size()], len, ctx.dev_p2);
+      else
+        hipLaunchKernelGGL(compress_top_many_kernel<HASH_KECCAK>, dim3((uint32_t)k), dim3(TOP_T), 0, s, dd,
+                           layer_off[level_off.size()], len, ctx.dev_p2);
+      P3G_CHECK_HIP(hipGetLastError());
+    }
+    // roots
+    std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+    for (size_t j = 0; j < k; j++) pieces.emplace_back(trees[j]->digests + layer_off[depth] * 4, 4);
+    std::vector<uint64_t> raw;
+    gather_to_host(ctx, pieces, raw, s);  // synchronises the stream
+    for (size_t j = 0; j < k; j++) memcpy(trees[j]->root, raw.data() + 4 * j, 32);
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    for (auto* p : scratch) ctx.release(p);
+    ctx.release(blob);
+    for (auto* t : trees) delete t;
+    throw;
+  }
+  for (auto* p : scratch) ctx.release(p);
+  ctx.release(blob);
+  return trees;
+}
+
 void gather_to_host(Ctx& ctx, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces,
                     std::vector<uint64_t>& out, hipStream_t s) {
   out.clear();
@@ -533,20 +772,23 @@ void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out) {
   }
 }
 
-void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s) {
+// prove_batch in two halves so that many trees can share one device gather:
+// the pieces (rows, then missing siblings) in hint order, then the hints.
+size_t prove_batch_pieces(const Tree& t, const std::vector<uint64_t>& indices,
+                          std::vector<std::pair<const uint64_t*, uint64_t>>& pieces) {
   std::set<uint64_t> uniq(indices.begin(), indices.end());
   for (uint64_t idx : uniq)
     if (idx >= t.height) throw ApiError(E_RANGE, "query index out of range");
   // 1. rows of every matrix at every unique index, in leaf order
-  std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+  const size_t first = pieces.size();
   for (uint64_t idx : uniq)
     for (auto& m : t.mats) {
       uint64_t row = idx >> log2_exact(t.height / m.height);
       pieces.emplace_back(m.ptr + row * m.width, m.width);
     }
+  const size_t nrows = pieces.size() - first;
   // 2. missing siblings, level by level, left to right
   std::vector<uint64_t> known(uniq.begin(), uniq.end());
-  size_t nrows = pieces.size();
   for (unsigned layer = 0; layer < t.depth; layer++) {
     std::vector<uint64_t> parents;
     for (size_t q = 0; q < known.size(); q++) {
@@ -561,22 +803,36 @@ void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Chann
     }
     known.swap(parents);
   }
-  std::vector<uint64_t> raw;
-  gather_to_host(*t.ctx, pieces, raw, s);
+  return nrows;
+}
+
+// Emits the hints of pieces [first, last) (the first nrows are matrix rows)
+// from their gathered words starting at raw; returns the words consumed.
+size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces, size_t first,
+                        size_t last, size_t nrows, const uint64_t* raw, Channel& tx) {
   size_t off = 0;
   std::vector<uint64_t> padded;
-  for (size_t p = 0; p < nrows; p++) {
+  for (size_t p = first; p < first + nrows; p++) {
     uint64_t w = pieces[p].second;
     uint64_t aw = aligned_len(w, t.alignment);
-    padded.assign(raw.begin() + off, raw.begin() + off + w);
+    padded.assign(raw + off, raw + off + w);
     padded.resize(aw, 0);
     tx.hint_fields(padded.data(), aw);
     off += w;
   }
-  for (size_t p = nrows; p < pieces.size(); p++) {
-    tx.hint_commitment(raw.data() + off);
+  for (size_t p = first + nrows; p < last; p++) {
+    tx.hint_commitment(raw + off);
     off += 4;
   }
+  return off;
+}
+
+void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s) {
+  std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+  const size_t nrows = prove_batch_pieces(t, indices, pieces);
+  std::vector<uint64_t> raw;
+  gather_to_host(*t.ctx, pieces, raw, s);
+  prove_batch_emit(t, pieces, 0, pieces.size(), nrows, raw.data(), tx);
 }
 
 void host_sponge_absorb(uint32_t hash, uint64_t* state, const uint64_t* in, size_t len) {

@@ -18,6 +18,7 @@
 //   and evaluates at beta / s (s^-1 = omega_L^{-bitrev(k) * L/m} from a
 //   two-level table).  The final polynomial is recovered on the host.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <map>
 
@@ -39,9 +40,8 @@ constexpr int DW_T = 256;
 constexpr int DW_CH = 8;
 
 template <int N>
-__global__ void __launch_bounds__(DW_T) deep_weights_kernel(const uint64_t* __restrict__ powL, unsigned logL,
-                                                            Points zp, uint64_t d, ef2* __restrict__ W) {
-  const uint64_t i0 = ((uint64_t)blockIdx.x * DW_T + threadIdx.x) * DW_CH;
+__device__ __forceinline__ void deep_weights_body(const uint64_t* __restrict__ powL, unsigned logL, const Points& zp,
+                                                  uint64_t d, ef2* __restrict__ W, uint64_t i0) {
   if (i0 >= d) return;
   const int cnt = (int)((d - i0) < (uint64_t)DW_CH ? (d - i0) : DW_CH);
   uint64_t xs[DW_CH], d0[DW_CH][N], nrm[DW_CH][N], pre[DW_CH][N];
@@ -81,8 +81,23 @@ __global__ void __launch_bounds__(DW_T) deep_weights_kernel(const uint64_t* __re
 }
 
 template <int N>
-__global__ void fold_weights_kernel(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t len, uint64_t cs) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(DW_T) deep_weights_kernel(const uint64_t* __restrict__ powL, unsigned logL,
+                                                            Points zp, uint64_t d, ef2* __restrict__ W) {
+  deep_weights_body<N>(powL, logL, zp, d, W, ((uint64_t)blockIdx.x * DW_T + threadIdx.x) * DW_CH);
+}
+
+// ---- job-indexed forms for batches of same-shaped proofs (job = blockIdx.y, or .z)
+template <int N>
+__global__ void __launch_bounds__(DW_T) deep_weights_many_kernel(const uint64_t* __restrict__ powL, unsigned logL,
+                                                                 const Points* __restrict__ zps, uint64_t d,
+                                                                 ef2* const* __restrict__ Ws) {
+  const Points zp = zps[blockIdx.y];
+  deep_weights_body<N>(powL, logL, zp, d, Ws[blockIdx.y], ((uint64_t)blockIdx.x * DW_T + threadIdx.x) * DW_CH);
+}
+
+template <int N>
+__device__ __forceinline__ void fold_weights_body(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t len,
+                                                  uint64_t cs, uint64_t i) {
   if (i >= len) return;
 #pragma unroll
   for (int j = 0; j < N; j++) {
@@ -91,19 +106,28 @@ __global__ void fold_weights_kernel(const ef2* __restrict__ in, ef2* __restrict_
     out[i * N + j] = s;
   }
 }
+template <int N>
+__global__ void fold_weights_kernel(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t len, uint64_t cs) {
+  fold_weights_body<N>(in, out, len, cs, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+template <int N>
+__global__ void fold_weights_many_kernel(const ef2* const* __restrict__ ins, ef2* const* __restrict__ outs,
+                                         uint64_t len, uint64_t cs) {
+  fold_weights_body<N>(ins[blockIdx.y], outs[blockIdx.y], len, cs, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 constexpr int CS_COLS = 64;
 constexpr int CS_SLICES = 4;
 constexpr int CS_T = CS_COLS * CS_SLICES;
 
 template <int N>
-__global__ void __launch_bounds__(CS_T) deep_colsum_kernel(const uint64_t* __restrict__ M, uint64_t w, uint64_t rows,
-                                                           const ef2* __restrict__ W, uint64_t rows_per_block,
-                                                           ef2* __restrict__ partial) {
+__device__ __forceinline__ void deep_colsum_body(const uint64_t* __restrict__ M, uint64_t w, uint64_t rows,
+                                                 const ef2* __restrict__ W, uint64_t rows_per_block,
+                                                 ef2* __restrict__ partial, uint32_t bx, uint32_t by) {
   __shared__ ef2 red[CS_SLICES][CS_COLS][N];
   const uint32_t cl = threadIdx.x % CS_COLS, rs = threadIdx.x / CS_COLS;
-  const uint64_t col = (uint64_t)blockIdx.x * CS_COLS + cl;
-  const uint64_t r0 = (uint64_t)blockIdx.y * rows_per_block;
+  const uint64_t col = (uint64_t)bx * CS_COLS + cl;
+  const uint64_t r0 = (uint64_t)by * rows_per_block;
   const uint64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
   ef2 acc[N];
 #pragma unroll
@@ -123,19 +147,42 @@ __global__ void __launch_bounds__(CS_T) deep_colsum_kernel(const uint64_t* __res
     for (int j = 0; j < N; j++) {
       ef2 s = red[0][cl][j];
       for (int k = 1; k < CS_SLICES; k++) s = ef_add(s, red[k][cl][j]);
-      partial[((uint64_t)blockIdx.y * w + col) * N + j] = s;
+      partial[((uint64_t)by * w + col) * N + j] = s;
     }
   }
 }
+template <int N>
+__global__ void __launch_bounds__(CS_T) deep_colsum_kernel(const uint64_t* __restrict__ M, uint64_t w, uint64_t rows,
+                                                           const ef2* __restrict__ W, uint64_t rows_per_block,
+                                                           ef2* __restrict__ partial) {
+  deep_colsum_body<N>(M, w, rows, W, rows_per_block, partial, blockIdx.x, blockIdx.y);
+}
+template <int N>
+__global__ void __launch_bounds__(CS_T) deep_colsum_many_kernel(const uint64_t* const* __restrict__ Ms, uint64_t w,
+                                                                uint64_t rows, const ef2* const* __restrict__ Ws,
+                                                                uint64_t rows_per_block, ef2* const* __restrict__ partials) {
+  deep_colsum_body<N>(Ms[blockIdx.z], w, rows, Ws[blockIdx.z], rows_per_block, partials[blockIdx.z], blockIdx.x,
+                      blockIdx.y);
+}
 
 template <int N>
-__global__ void colsum_reduce_kernel(const ef2* __restrict__ partial, uint64_t nblocks, uint64_t w,
-                                     ef2* __restrict__ out) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void colsum_reduce_body(const ef2* __restrict__ partial, uint64_t nblocks, uint64_t w,
+                                                   ef2* __restrict__ out, uint64_t t) {
   if (t >= w * N) return;
   ef2 s = ef_from(0);
   for (uint64_t b = 0; b < nblocks; b++) s = ef_add(s, partial[b * w * N + t]);
   out[t] = s;
+}
+template <int N>
+__global__ void colsum_reduce_kernel(const ef2* __restrict__ partial, uint64_t nblocks, uint64_t w,
+                                     ef2* __restrict__ out) {
+  colsum_reduce_body<N>(partial, nblocks, w, out, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+template <int N>
+__global__ void colsum_reduce_many_kernel(const ef2* const* __restrict__ partials, uint64_t nblocks, uint64_t w,
+                                          ef2* const* __restrict__ outs) {
+  colsum_reduce_body<N>(partials[blockIdx.y], nblocks, w, outs[blockIdx.y],
+                        (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 struct QMat {
@@ -167,10 +214,10 @@ constexpr int DQ_ROWS = DQ_T * DQ_CH;
 // 128-byte row segment); each thread then reduces its own row from LDS with
 // the (uniform, scalar-loaded) -alpha^(W-1-c) coefficients.
 template <int N>
-__global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4))) deep_quotient_kernel(QuotArgs a) {
+__device__ __forceinline__ void deep_quotient_body(const QuotArgs& a, uint32_t bx) {
   __shared__ uint64_t tile[256 * 16];
   const uint32_t tid = threadIdx.x;
-  const uint64_t wbase = (uint64_t)blockIdx.x * DQ_ROWS;
+  const uint64_t wbase = (uint64_t)bx * DQ_ROWS;
   ef2 acc[DQ_CH];
 #pragma unroll
   for (int r = 0; r < DQ_CH; r++) {
@@ -238,17 +285,27 @@ __global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4))) 
     if (i < a.L) a.out[i] = res[r];
   }
 }
+template <int N>
+__global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4))) deep_quotient_kernel(QuotArgs a) {
+  deep_quotient_body<N>(a, blockIdx.x);
+}
+template <int N>
+__global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4)))
+deep_quotient_many_kernel(const QuotArgs* __restrict__ args) {
+  const QuotArgs a = args[blockIdx.y];
+  deep_quotient_body<N>(a, blockIdx.x);
+}
 
 struct FoldTw {
   uint64_t w[8];  // inverse twiddles for the in-row inverse DFT (stage-major)
 };
 
 template <int LA>
-__global__ void fri_fold_kernel(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t nrows,
-                                unsigned log_rows, const uint64_t* __restrict__ powLinv, unsigned exp_shift,
-                                ef2 beta, FoldTw tw, uint64_t ainv) {
+__device__ __forceinline__ void fri_fold_body(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t nrows,
+                                              unsigned log_rows, const uint64_t* __restrict__ powLinv,
+                                              unsigned exp_shift, ef2 beta, const FoldTw& tw, uint64_t ainv,
+                                              uint64_t k) {
   constexpr int A = 1 << LA;
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nrows) return;
   ef2 v[A];
 #pragma unroll
@@ -278,6 +335,26 @@ __global__ void fri_fold_kernel(const ef2* __restrict__ in, ef2* __restrict__ ou
 #pragma unroll
   for (int j = A - 2; j >= 0; j--) r = ef_add(ef_mul(r, x), v[j]);
   out[k] = ef_mul_base(r, ainv);
+}
+template <int LA>
+__global__ void fri_fold_kernel(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t nrows,
+                                unsigned log_rows, const uint64_t* __restrict__ powLinv, unsigned exp_shift,
+                                ef2 beta, FoldTw tw, uint64_t ainv) {
+  fri_fold_body<LA>(in, out, nrows, log_rows, powLinv, exp_shift, beta, tw, ainv,
+                    (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+struct FoldJob {
+  const ef2* in;
+  ef2* out;
+  ef2 beta;
+};
+template <int LA>
+__global__ void fri_fold_many_kernel(const FoldJob* __restrict__ jobs, uint64_t nrows, unsigned log_rows,
+                                     const uint64_t* __restrict__ powLinv, unsigned exp_shift, FoldTw tw,
+                                     uint64_t ainv) {
+  const FoldJob j = jobs[blockIdx.y];
+  fri_fold_body<LA>(j.in, j.out, nrows, log_rows, powLinv, exp_shift, j.beta, tw, ainv,
+                    (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------------ host driver
@@ -589,6 +666,404 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
   }
   for (auto* t : ftrees) delete t;
   for (auto* b : bufs) ctx.release(b);
+}
+
+
+// ------------------------------------------------------------------ batched proofs
+// k independent proofs in lock-step: the p3g_prove flow (commit_traces, observe
+// the root, sample two extension points, open_with_channel) for jobs whose
+// trace lists have one shape.  Every device phase is a single launch over all
+// jobs (job-indexed kernels), followed by one synchronisation; only the
+// transcripts run per job on the host, each in exactly the order of the
+// single-proof flow, so every proof is the one p3g_prove would produce.
+void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<std::vector<DevMat>>& traces,
+                const std::vector<Transcript*>& txs, std::vector<std::array<uint64_t, 4>>& roots,
+                std::vector<std::array<ef2, 2>>& zs, hipStream_t s) {
+  constexpr int N = 2;
+  const size_t k = traces.size();
+  if (k == 0 || txs.size() != k) throw ApiError(E_INVAL, "one transcript per job required");
+  const std::vector<DevMat>& t0 = traces[0];
+  if (t0.empty()) throw ApiError(E_EMPTY, "at least one trace required");
+  for (size_t i = 0; i < t0.size(); i++) {
+    if (t0[i].height == 0 || (t0[i].height & (t0[i].height - 1)))
+      throw ApiError(E_NOT_POW2, "trace height must be a power of two");
+    if (i && t0[i].height < t0[i - 1].height) throw ApiError(E_UNSORTED, "traces must be sorted by height");
+  }
+  for (auto& tj : traces) {
+    if (tj.size() != t0.size()) throw ApiError(E_INVAL, "batched proofs need one trace shape");
+    for (size_t i = 0; i < tj.size(); i++)
+      if (tj[i].height != t0[i].height || tj[i].width != t0[i].width)
+        throw ApiError(E_INVAL, "batched proofs need one trace shape");
+  }
+  if (p.log_arity < 1 || p.log_arity > 3) throw ApiError(E_INVAL, "invalid folding arity");
+  if (p.log_blowup == 0 || p.num_queries == 0) throw ApiError(E_INVAL, "invalid PCS parameters");
+  const unsigned lmax = log2_exact(t0.back().height);
+  const unsigned logL = lmax + p.log_blowup;
+  if (logL > 32) throw ApiError(E_RANGE, "LDE height exceeds the field two-adicity");
+  const uint64_t L = 1ull << logL;
+  const uint32_t B = 1u << p.log_blowup;
+
+  std::vector<Tree*> trees;
+  std::vector<std::vector<Tree*>> ftrees;  // [round][job]
+  std::vector<ef2*> bufs;                  // every FRI input buffer (Q and folds)
+  std::vector<void*> tmp;                  // released at the end
+  std::vector<std::vector<uint8_t>> staged;  // host images of uploaded argument arrays
+  auto cleanup = [&] {
+    (void)hipStreamSynchronize(s);
+    for (auto& r : ftrees)
+      for (auto* t : r) delete t;
+    for (auto* t : trees) delete t;
+    for (auto* b : bufs) ctx.release(b);
+    for (auto* q : tmp) ctx.release(q);
+  };
+  try {
+    // ---- commit: batched LDEs, then the k trees (which take the LDEs)
+    std::vector<std::vector<DevMat>> ldes(k);
+    try {
+      for (size_t j = 0; j < k; j++)
+        for (auto& tr : traces[j]) {
+          DevMat m;
+          m.height = tr.height << p.log_blowup;
+          m.width = tr.width;
+          m.owned = true;
+          m.ptr = (uint64_t*)ctx.alloc(m.height * m.width * 8);
+          ldes[j].push_back(m);
+        }
+      for (size_t i = 0; i < t0.size(); i++) {
+        uint64_t shift = GENERATOR;
+        for (unsigned q = log2_exact(t0[i].height); q < lmax; q++) shift = gl_mul(shift, shift);
+        std::vector<LdeJob> jobs;
+        for (size_t j = 0; j < k; j++)
+          jobs.push_back(LdeJob{traces[j][i].ptr, ldes[j][i].ptr, t0[i].height, t0[i].width, shift});
+        coset_lde_many(ctx, jobs, p.log_blowup, s);
+      }
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      for (auto& lj : ldes)
+        for (auto& m : lj) ctx.release(m.ptr);
+      throw;
+    }
+    trees = lmcs_build_many(ctx, hash, ldes, hash_alignment(hash), s);
+    roots.resize(k);
+    zs.resize(k);
+    std::vector<Points> zp(k);
+    for (size_t j = 0; j < k; j++) {
+      memcpy(roots[j].data(), trees[j]->root, 32);
+      txs[j]->observe_commitment(trees[j]->root);
+      zs[j][0] = txs[j]->sample_ef();
+      zs[j][1] = txs[j]->sample_ef();
+      zp[j] = Points{};
+      zp[j].z[0] = zs[j][0];
+      zp[j].z[1] = zs[j][1];
+    }
+
+    // ---- DEEP evaluations at the two points, all jobs per launch
+    const std::vector<DevMat>& m0 = trees[0]->mats;
+    const size_t nm = m0.size();
+    const uint32_t alignment = trees[0]->alignment;
+    std::vector<uint64_t> awidths;
+    uint64_t totw = 0, tota = 0;
+    for (auto& m : m0) {
+      awidths.push_back(aligned_len(m.width, alignment));
+      totw += m.width;
+      tota += awidths.back();
+    }
+    const unsigned logd = logL - p.log_blowup;
+    const uint64_t d = 1ull << logd;
+    const uint64_t* powL = ctx.pow_table(gl_two_adic_generator(logL), logL);
+    // device scratch image: [points | weight ptrs ...] built per phase
+    // argument arrays go up through host copies (staged) that outlive every
+    // asynchronous copy, the error path included
+    auto upload = [&](const void* src, size_t bytes) {
+      void* dst = ctx.alloc(bytes ? bytes : 8);
+      tmp.push_back(dst);
+      if (bytes) {
+        staged.emplace_back((const uint8_t*)src, (const uint8_t*)src + bytes);
+        P3G_CHECK_HIP(hipMemcpyAsync(dst, staged.back().data(), bytes, hipMemcpyHostToDevice, s));
+      }
+      return dst;
+    };
+    std::map<uint64_t, std::vector<ef2*>> wts;  // weight vectors per trace height, per job
+    {
+      std::vector<ef2*> top(k);
+      for (size_t j = 0; j < k; j++) {
+        top[j] = (ef2*)ctx.alloc(d * N * sizeof(ef2));
+        tmp.push_back(top[j]);
+      }
+      const Points* dz = (const Points*)upload(zp.data(), k * sizeof(Points));
+      ef2* const* dW = (ef2* const*)upload(top.data(), k * sizeof(ef2*));
+      const uint64_t threads = (d + DW_CH - 1) / DW_CH;
+      ProfScope ps(ctx, "deep_weights", s);
+      hipLaunchKernelGGL(deep_weights_many_kernel<N>, dim3((uint32_t)((threads + DW_T - 1) / DW_T), (uint32_t)k),
+                         dim3(DW_T), 0, s, powL, logL, dz, d, dW);
+      P3G_CHECK_HIP(hipGetLastError());
+      wts[d] = top;
+    }
+    {
+      std::vector<uint64_t> degs;
+      for (auto& m : m0) degs.push_back(m.height >> p.log_blowup);
+      std::sort(degs.begin(), degs.end());
+      degs.erase(std::unique(degs.begin(), degs.end()), degs.end());
+      uint64_t prev_len = d;
+      for (auto it = degs.rbegin(); it != degs.rend(); ++it) {
+        const uint64_t len = *it;
+        if (len == d) continue;
+        std::vector<ef2*> nw(k);
+        for (size_t j = 0; j < k; j++) {
+          nw[j] = (ef2*)ctx.alloc(len * N * sizeof(ef2));
+          tmp.push_back(nw[j]);
+        }
+        const ef2* const* din = (const ef2* const*)upload(wts[prev_len].data(), k * sizeof(ef2*));
+        ef2* const* dout = (ef2* const*)upload(nw.data(), k * sizeof(ef2*));
+        hipLaunchKernelGGL(fold_weights_many_kernel<N>, dim3((uint32_t)((len + 127) / 128), (uint32_t)k), dim3(128), 0,
+                           s, din, dout, len, prev_len / len);
+        P3G_CHECK_HIP(hipGetLastError());
+        wts[len] = nw;
+        prev_len = len;
+      }
+    }
+    ef2* evals_dev = (ef2*)ctx.alloc(k * (totw ? totw : 1) * N * sizeof(ef2));
+    tmp.push_back(evals_dev);
+    {
+      uint64_t off = 0;
+      for (size_t mi = 0; mi < nm; mi++) {
+        const uint64_t w = m0[mi].width, rows = m0[mi].height >> p.log_blowup;
+        const uint64_t rpb = rows > 1024ull * 1024 ? (rows + 1023) / 1024 : 1024;
+        const uint64_t nblk = (rows + rpb - 1) / rpb;
+        const uint64_t ncb = (w + CS_COLS - 1) / CS_COLS;
+        std::vector<const uint64_t*> Ms(k);
+        std::vector<ef2*> parts(k), outs(k);
+        for (size_t j = 0; j < k; j++) {
+          Ms[j] = trees[j]->mats[mi].ptr;
+          parts[j] = (ef2*)ctx.alloc(nblk * w * N * sizeof(ef2));
+          tmp.push_back(parts[j]);
+          outs[j] = evals_dev + (j * totw + off) * N;
+        }
+        const uint64_t* const* dM = (const uint64_t* const*)upload(Ms.data(), k * sizeof(uint64_t*));
+        const ef2* const* dW = (const ef2* const*)upload(wts[rows].data(), k * sizeof(ef2*));
+        ef2* const* dP = (ef2* const*)upload(parts.data(), k * sizeof(ef2*));
+        ef2* const* dO = (ef2* const*)upload(outs.data(), k * sizeof(ef2*));
+        {
+          ProfScope ps(ctx, "deep_colsum", s);
+          hipLaunchKernelGGL(deep_colsum_many_kernel<N>, dim3((uint32_t)ncb, (uint32_t)nblk, (uint32_t)k), dim3(CS_T),
+                             0, s, dM, w, rows, dW, rpb, dP);
+          P3G_CHECK_HIP(hipGetLastError());
+        }
+        const uint64_t nt = w * N;
+        hipLaunchKernelGGL(colsum_reduce_many_kernel<N>, dim3((uint32_t)((nt + 127) / 128), (uint32_t)k), dim3(128), 0,
+                           s, (const ef2* const*)dP, nblk, w, dO);
+        P3G_CHECK_HIP(hipGetLastError());
+        off += w;
+      }
+    }
+    std::vector<ef2> evals_host(k * totw * N);
+    if (totw)
+      P3G_CHECK_HIP(hipMemcpyAsync(evals_host.data(), evals_dev, evals_host.size() * sizeof(ef2), hipMemcpyDeviceToHost, s));
+    P3G_CHECK_HIP(hipStreamSynchronize(s));
+
+    // ---- per job: scale, send, grind, alpha and beta; quotient arguments
+    const uint64_t ginv = gl_inv(GENERATOR), dinv = gl_inv_pow2(logd);
+    std::vector<QMat> qm_all(k * nm);
+    std::vector<ef2> coeffs_all(k * (tota ? tota : 1));
+    std::vector<QuotArgs> qa(k);
+    std::vector<ef2*> Q(k);
+    for (size_t j = 0; j < k; j++) {
+      Q[j] = (ef2*)ctx.alloc(L * sizeof(ef2));
+      bufs.push_back(Q[j]);
+    }
+    QMat* dqm = (QMat*)ctx.alloc(qm_all.size() * sizeof(QMat));
+    tmp.push_back(dqm);
+    ef2* dcoef = (ef2*)ctx.alloc(coeffs_all.size() * sizeof(ef2));
+    tmp.push_back(dcoef);
+    const uint64_t* powQ = ctx.pow_table(gl_two_adic_generator(logL), logL);
+    for (size_t j = 0; j < k; j++) {
+      ef2* ev = evals_host.data() + j * totw * N;
+      ef2 sc[N];
+      for (int q = 0; q < N; q++) {
+        ef2 t = ef_mul_base(zs[j][q], ginv);
+        for (unsigned b = 0; b < logd; b++) t = ef_sqr(t);
+        sc[q] = ef_mul_base(ef_sub(t, ef_from(1)), dinv);
+      }
+      for (uint64_t c = 0; c < totw; c++)
+        for (int q = 0; q < N; q++) ev[c * N + q] = ef_mul(ev[c * N + q], sc[q]);
+      std::vector<ef2> aligned((size_t)tota * N, ef_from(0));
+      {
+        uint64_t off = 0, aoff = 0;
+        for (size_t m = 0; m < nm; m++) {
+          for (uint64_t c = 0; c < m0[m].width; c++)
+            for (int q = 0; q < N; q++) aligned[q * tota + aoff + c] = ev[(off + c) * N + q];
+          off += m0[m].width;
+          aoff += awidths[m];
+        }
+      }
+      Transcript& tx = *txs[j];
+      for (int q = 0; q < N; q++) tx.send_fields(reinterpret_cast<const uint64_t*>(aligned.data() + q * tota), 2 * tota);
+      tx.grind(p.deep_pow_bits);
+      const ef2 alpha = tx.sample_ef();
+      const ef2 beta = tx.sample_ef();
+      QuotArgs a{};
+      for (int q = 0; q < N; q++) {
+        ef2 acc = ef_from(0);
+        for (uint64_t c = 0; c < tota; c++) acc = ef_add(ef_mul(acc, alpha), aligned[q * tota + c]);
+        a.fred[q] = acc;
+        a.z.z[q] = zs[j][q];
+      }
+      {
+        ef2 pw = ef_neg(ef_from(1));
+        ef2* cf = coeffs_all.data() + j * tota;
+        for (uint64_t i = tota; i-- > 0;) {
+          cf[i] = pw;
+          pw = ef_mul(pw, alpha);
+        }
+      }
+      uint64_t aoff = 0;
+      for (size_t m = 0; m < nm; m++) {
+        qm_all[j * nm + m] = QMat{trees[j]->mats[m].ptr, m0[m].width, log2_exact(L / m0[m].height), (uint32_t)aoff};
+        aoff += awidths[m];
+      }
+      a.mats = dqm + j * nm;
+      a.nmats = (uint32_t)nm;
+      a.coeffs = dcoef + j * tota;
+      a.L = L;
+      a.logL = logL;
+      a.powL = powQ;
+      a.bpow[0] = ef_from(1);
+      a.bpow[1] = beta;
+      a.out = Q[j];
+      qa[j] = a;
+    }
+    P3G_CHECK_HIP(hipMemcpyAsync(dqm, qm_all.data(), qm_all.size() * sizeof(QMat), hipMemcpyHostToDevice, s));
+    P3G_CHECK_HIP(hipMemcpyAsync(dcoef, coeffs_all.data(), coeffs_all.size() * sizeof(ef2), hipMemcpyHostToDevice, s));
+    const QuotArgs* dqa = (const QuotArgs*)upload(qa.data(), k * sizeof(QuotArgs));
+    {
+      ProfScope ps(ctx, "deep_quotient", s);
+      hipLaunchKernelGGL(deep_quotient_many_kernel<N>, dim3((uint32_t)((L + DQ_ROWS - 1) / DQ_ROWS), (uint32_t)k),
+                         dim3(DQ_T), 0, s, dqa);
+      P3G_CHECK_HIP(hipGetLastError());
+    }
+    P3G_CHECK_HIP(hipStreamSynchronize(s));  // host argument images above die with this scope
+
+    // ---- FRI commit phase, all jobs per round
+    const unsigned la = p.log_arity;
+    const uint64_t arity = 1ull << la;
+    const size_t fdeg = fri_final_degree(p, logL);
+    const uint64_t fdomain = (uint64_t)fdeg << p.log_blowup;
+    const unsigned tbits = logL > la ? logL - la : 0;
+    const uint64_t* powLinv = ctx.pow_table(gl_inv(gl_two_adic_generator(logL)), tbits);
+    FoldTw tw{};
+    {
+      int twi = 0;
+      for (unsigned st = 1; st <= la; st++) {
+        uint64_t wm = gl_inv(gl_two_adic_generator(st));
+        uint64_t h = 1ull << (st - 1), x = 1;
+        for (uint64_t q = 0; q < h; q++) {
+          tw.w[twi + q] = x;
+          x = gl_mul(x, wm);
+        }
+        twi += (int)h;
+      }
+    }
+    const uint64_t ainv = gl_inv_pow2(la);
+    std::vector<ef2*> cur = Q;
+    uint64_t domain = L;
+    while (domain > fdomain) {
+      const uint64_t fd = domain >> la;
+      std::vector<std::vector<DevMat>> fm(k);
+      for (size_t j = 0; j < k; j++) {
+        DevMat m;
+        m.ptr = reinterpret_cast<uint64_t*>(cur[j]);
+        m.height = fd;
+        m.width = 2 * arity;
+        m.owned = false;
+        fm[j].push_back(m);
+      }
+      ftrees.push_back(lmcs_build_many(ctx, hash, fm, 1, s));
+      std::vector<FoldJob> fj(k);
+      std::vector<ef2*> nxt(k);
+      for (size_t j = 0; j < k; j++) {
+        Transcript& tx = *txs[j];
+        tx.send_commitment(ftrees.back()[j]->root);
+        tx.grind(p.folding_pow_bits);
+        nxt[j] = (ef2*)ctx.alloc(fd * sizeof(ef2));
+        bufs.push_back(nxt[j]);
+        fj[j] = FoldJob{cur[j], nxt[j], tx.sample_ef()};
+      }
+      const FoldJob* dfj = (const FoldJob*)upload(fj.data(), k * sizeof(FoldJob));
+      const unsigned log_rows = log2_exact(fd);
+      const unsigned exp_shift = logL - log2_exact(domain);
+      const dim3 grid((uint32_t)((fd + 255) / 256), (uint32_t)k);
+      {
+        ProfScope ps(ctx, "fri_fold", s);
+        switch (la) {
+          case 1:
+            hipLaunchKernelGGL(fri_fold_many_kernel<1>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
+            break;
+          case 2:
+            hipLaunchKernelGGL(fri_fold_many_kernel<2>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
+            break;
+          default:
+            hipLaunchKernelGGL(fri_fold_many_kernel<3>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
+            break;
+        }
+        P3G_CHECK_HIP(hipGetLastError());
+      }
+      P3G_CHECK_HIP(hipStreamSynchronize(s));  // fj dies with this iteration
+      cur = nxt;
+      domain = fd;
+    }
+    // ---- final polynomials
+    {
+      std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+      for (size_t j = 0; j < k; j++) pieces.emplace_back(reinterpret_cast<const uint64_t*>(cur[j]), 2 * fdeg);
+      std::vector<uint64_t> raw;
+      gather_to_host(ctx, pieces, raw, s);
+      const unsigned lf = log2_exact(fdeg);
+      for (size_t j = 0; j < k; j++) {
+        const ef2* fin = reinterpret_cast<const ef2*>(raw.data() + j * 2 * fdeg);
+        std::vector<ef2> nat(fdeg);
+        for (size_t i = 0; i < fdeg; i++) nat[bitrev32((uint32_t)i, lf)] = fin[i];
+        host_idft_ext(nat);
+        std::reverse(nat.begin(), nat.end());
+        txs[j]->send_fields(reinterpret_cast<const uint64_t*>(nat.data()), 2 * fdeg);
+      }
+    }
+    // ---- query phase: indices per job, then one gather for every opening of every job
+    struct Seg {
+      const Tree* t;
+      size_t first, last, nrows;
+    };
+    std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+    std::vector<std::vector<Seg>> segs(k);
+    for (size_t j = 0; j < k; j++) {
+      Transcript& tx = *txs[j];
+      tx.grind(p.query_pow_bits);
+      std::vector<uint64_t> idx(p.num_queries);
+      for (uint32_t q = 0; q < p.num_queries; q++) idx[q] = bitrev32((uint32_t)tx.sample_bits(logL), logL);
+      size_t first = pieces.size();
+      size_t nr = prove_batch_pieces(*trees[j], idx, pieces);
+      segs[j].push_back(Seg{trees[j], first, pieces.size(), nr});
+      std::vector<uint64_t> ri(idx.size());
+      for (size_t r = 0; r < ftrees.size(); r++) {
+        const unsigned sh = la * (unsigned)(r + 1);
+        for (size_t q = 0; q < idx.size(); q++) ri[q] = idx[q] >> sh;
+        first = pieces.size();
+        nr = prove_batch_pieces(*ftrees[r][j], ri, pieces);
+        segs[j].push_back(Seg{ftrees[r][j], first, pieces.size(), nr});
+      }
+    }
+    std::vector<uint64_t> raw;
+    gather_to_host(ctx, pieces, raw, s);
+    std::vector<size_t> woff(pieces.size() + 1, 0);
+    for (size_t i = 0; i < pieces.size(); i++) woff[i + 1] = woff[i] + pieces[i].second;
+    for (size_t j = 0; j < k; j++)
+      for (auto& sg : segs[j])
+        prove_batch_emit(*sg.t, pieces, sg.first, sg.last, sg.nrows, raw.data() + woff[sg.first], *txs[j]);
+    (void)B;
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
 }
 
 }  // namespace p3g

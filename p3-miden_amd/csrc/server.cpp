@@ -16,6 +16,7 @@
 // never on what runs beside it, so each result is byte-identical to running
 // the request alone.
 #include <chrono>
+#include <cstdio>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -35,6 +36,7 @@ using clk = std::chrono::steady_clock;
 
 struct Request {
   uint64_t ticket = 0;
+  std::string sig;  // hash, parameters, input kind and trace shapes: requests with one sig batch together
   uint32_t hash = 0;
   p3g_pcs_params params{};
   std::vector<p3g_matrix> traces;
@@ -65,12 +67,16 @@ struct p3g_server {
   uint64_t next_ticket = 1;
   uint64_t inflight_bytes = 0, peak_inflight_bytes = 0;
   uint32_t running = 0, peak_running = 0;
-  uint64_t admitted = 0, completed = 0;
+  uint64_t admitted = 0, completed = 0, batches = 0, batched_requests = 0;
+  uint32_t max_batch = 1;
   bool stop = false;
   clk::time_point t0;
 
   void worker_loop(uint32_t w);
   Request* pick_locked();
+  void take_batch_locked(Request* first, std::vector<Request*>& batch);
+  void run_one(p3g_ctx* ctx, Request* r);
+  void run_batch(p3g_ctx* ctx, std::vector<Request*>& batch);
 };
 
 // First queued request that fits; the queue head may be overtaken only while
@@ -89,13 +95,79 @@ Request* p3g_server::pick_locked() {
   return nullptr;
 }
 
+// Later queued requests with the first one's signature join its batch while
+// the batch stays under max_batch and the budget holds.
+void p3g_server::take_batch_locked(Request* first, std::vector<Request*>& batch) {
+  batch.assign(1, first);
+  if (max_batch <= 1) return;
+  uint64_t extra = first->footprint;  // not yet counted in inflight_bytes
+  for (auto it = queue.begin(); it != queue.end() && batch.size() < max_batch;) {
+    Request* r = *it;
+    if (r->sig == first->sig && inflight_bytes + extra + r->footprint <= budget) {
+      extra += r->footprint;
+      batch.push_back(r);
+      it = queue.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+void p3g_server::run_one(p3g_ctx* ctx, Request* r) {
+  p3g_transcript* tx = nullptr;
+  int rc = p3g_transcript_create(r->hash == P3G_HASH_POSEIDON2_GL12 ? P3G_CHALLENGER_DUPLEX_P2 : P3G_CHALLENGER_KECCAK,
+                                 &tx);
+  if (rc == P3G_OK)
+    rc = p3g_prove(ctx, r->hash, &r->params, r->traces.data(), r->traces.size(), r->flags, tx, r->root, r->z);
+  if (rc == P3G_OK) {
+    r->fields = tx->tx.fields;
+    r->comms = tx->tx.comms;
+  } else {
+    r->error = p3g_last_error();
+  }
+  p3g_transcript_free(tx);
+  r->status = rc;
+}
+
+void p3g_server::run_batch(p3g_ctx* ctx, std::vector<Request*>& batch) {
+  const size_t k = batch.size(), t = batch[0]->traces.size();
+  std::vector<p3g_transcript*> txs(k, nullptr);
+  std::vector<p3g_matrix> flat;
+  int rc = P3G_OK;
+  for (size_t j = 0; j < k && rc == P3G_OK; j++) {
+    rc = p3g_transcript_create(batch[j]->hash == P3G_HASH_POSEIDON2_GL12 ? P3G_CHALLENGER_DUPLEX_P2
+                                                                          : P3G_CHALLENGER_KECCAK,
+                               &txs[j]);
+    flat.insert(flat.end(), batch[j]->traces.begin(), batch[j]->traces.end());
+  }
+  std::vector<uint64_t> roots(4 * k), zs(4 * k);
+  if (rc == P3G_OK)
+    rc = p3g_prove_many(ctx, batch[0]->hash, &batch[0]->params, flat.data(), k, t, batch[0]->flags, txs.data(),
+                        roots.data(), zs.data());
+  const std::string err = rc == P3G_OK ? std::string() : std::string(p3g_last_error());
+  for (size_t j = 0; j < k; j++) {
+    Request* r = batch[j];
+    r->status = rc;
+    if (rc == P3G_OK) {
+      memcpy(r->root, roots.data() + 4 * j, 32);
+      memcpy(r->z, zs.data() + 4 * j, 32);
+      r->fields = txs[j]->tx.fields;
+      r->comms = txs[j]->tx.comms;
+    } else {
+      r->error = err;
+    }
+    p3g_transcript_free(txs[j]);
+  }
+}
+
 void p3g_server::worker_loop(uint32_t w) {
   (void)hipSetDevice(device);
   p3g_ctx* ctx = ctxs[w];
+  std::vector<Request*> batch;
   for (;;) {
-    Request* r = nullptr;
     {
       std::unique_lock<std::mutex> lk(mu);
+      Request* r = nullptr;
       for (;;) {
         if (stop) return;
         r = pick_locked();
@@ -104,35 +176,35 @@ void p3g_server::worker_loop(uint32_t w) {
         // aged queue head stop the backfill
         cv_work.wait_for(lk, std::chrono::milliseconds(5));
       }
-      r->admitted = true;
-      r->t_admit = clk::now();
-      r->worker = w;
-      inflight_bytes += r->footprint;
-      running++;
-      admitted++;
+      take_batch_locked(r, batch);
+      const auto now = clk::now();
+      for (auto* b : batch) {
+        b->admitted = true;
+        b->t_admit = now;
+        b->worker = w;
+        inflight_bytes += b->footprint;
+      }
+      running += (uint32_t)batch.size();
+      admitted += batch.size();
+      if (batch.size() > 1) {
+        batches++;
+        batched_requests += batch.size();
+      }
       if (inflight_bytes > peak_inflight_bytes) peak_inflight_bytes = inflight_bytes;
       if (running > peak_running) peak_running = running;
     }
-    p3g_transcript* tx = nullptr;
-    int rc = p3g_transcript_create(r->hash == P3G_HASH_POSEIDON2_GL12 ? P3G_CHALLENGER_DUPLEX_P2 : P3G_CHALLENGER_KECCAK,
-                                   &tx);
-    if (rc == P3G_OK)
-      rc = p3g_prove(ctx, r->hash, &r->params, r->traces.data(), r->traces.size(), r->flags, tx, r->root, r->z);
-    if (rc == P3G_OK) {
-      r->fields = tx->tx.fields;
-      r->comms = tx->tx.comms;
-    } else {
-      r->error = p3g_last_error();
-    }
-    p3g_transcript_free(tx);
+    if (batch.size() == 1) run_one(ctx, batch[0]);
+    else run_batch(ctx, batch);
     {
       std::lock_guard<std::mutex> g(mu);
-      r->status = rc;
-      r->done = true;
-      r->t_done = clk::now();
-      inflight_bytes -= r->footprint;
-      running--;
-      completed++;
+      const auto now = clk::now();
+      for (auto* b : batch) {
+        b->done = true;
+        b->t_done = now;
+        inflight_bytes -= b->footprint;
+      }
+      running -= (uint32_t)batch.size();
+      completed += batch.size();
     }
     cv_done.notify_all();
     cv_work.notify_all();
@@ -161,14 +233,16 @@ uint64_t p3g_request_footprint(const p3g_request* req) {
   return bytes + bytes / 8 + (64ull << 20);             // allocator rounding and tables
 }
 
-int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms,
+int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms, uint32_t max_batch,
                       p3g_server** out) {
-  if (!out || workers == 0 || workers > 16 || hbm_budget_bytes == 0) return P3G_E_INVAL;
+  if (!out || workers == 0 || workers > 16 || hbm_budget_bytes == 0 || max_batch == 0 || max_batch > 4096)
+    return P3G_E_INVAL;
   *out = nullptr;
   p3g_server* s = new p3g_server();
   s->device = device;
   s->budget = hbm_budget_bytes;
   s->backfill_ms = backfill_ms;
+  s->max_batch = max_batch;
   s->t0 = clk::now();
   for (uint32_t w = 0; w < workers; w++) {
     p3g_ctx* c = nullptr;
@@ -207,6 +281,16 @@ int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket) {
   r->traces.assign(req->traces, req->traces + req->k);
   r->flags = req->flags;
   r->footprint = fp;
+  {
+    char buf[96];
+    const p3g_pcs_params& q = req->params;
+    snprintf(buf, sizeof(buf), "%u/%u/%u.%u.%u.%u.%u.%u.%u", req->hash_kind, req->flags & P3G_DEVICE_PTRS, q.log_blowup,
+             q.log_folding_arity, q.log_final_degree, q.folding_pow_bits, q.deep_pow_bits, q.num_queries,
+             q.query_pow_bits);
+    r->sig = buf;
+    for (uint64_t i = 0; i < req->k; i++)
+      r->sig += ":" + std::to_string(req->traces[i].height) + "x" + std::to_string(req->traces[i].width);
+  }
   {
     std::lock_guard<std::mutex> g(s->mu);
     r->ticket = s->next_ticket++;
@@ -286,6 +370,8 @@ int p3g_server_stats(p3g_server* s, p3g_server_counters* out) {
   out->completed = s->completed;
   out->queued = s->queue.size();
   out->running = s->running;
+  out->batches = s->batches;
+  out->batched_requests = s->batched_requests;
   out->inflight_bytes = s->inflight_bytes;
   out->peak_inflight_bytes = s->peak_inflight_bytes;
   out->peak_running = s->peak_running;

@@ -143,7 +143,8 @@ class _ResultInfo(C.Structure):
 
 
 class _ServerCounters(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("admitted", "completed", "queued", "running", "inflight_bytes",
+    _fields_ = [(n, C.c_uint64) for n in ("admitted", "completed", "queued", "running", "batches", "batched_requests",
+                                          "inflight_bytes",
                                           "peak_inflight_bytes", "peak_running", "budget_bytes", "n_malloc",
                                           "n_free")] + [("malloc_ms", C.c_double), ("free_ms", C.c_double)]
 
@@ -213,7 +214,7 @@ def _load():
         "p3g_prove_planned": (C.c_int, [_vp, _vp, C.POINTER(PcsParams), _vp, _u64p, _u64p]),
         "p3g_fill_random_device": (C.c_int, [_vp, C.c_uint64, C.c_uint64, _vp]),
         "p3g_request_footprint": (C.c_uint64, [C.POINTER(_Request)]),
-        "p3g_server_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.c_double, C.POINTER(_vp)]),
+        "p3g_server_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.c_double, C.c_uint32, C.POINTER(_vp)]),
         "p3g_server_submit": (C.c_int, [_vp, C.POINTER(_Request), _u64p]),
         "p3g_server_wait": (C.c_int, [_vp, C.c_uint64, C.POINTER(_ResultInfo)]),
         "p3g_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(C.c_int)]),
@@ -223,6 +224,8 @@ def _load():
         "p3g_server_destroy": (None, [_vp]),
         "p3g_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
         "p3g_host_free": (C.c_int, [_vp]),
+        "p3g_prove_many": (C.c_int, [_vp, C.c_uint32, C.POINTER(PcsParams), C.POINTER(_Matrix), C.c_uint64, C.c_uint64,
+                                     C.c_uint32, C.POINTER(_vp), _u64p, _u64p]),
         "p3g_prove": (C.c_int, [_vp, C.c_uint32, C.POINTER(PcsParams), C.POINTER(_Matrix), C.c_uint64, C.c_uint32, _vp, _u64p, _u64p]),
     }
     for name, (res, args) in sig.items():
@@ -700,6 +703,33 @@ class CommitPlan:
         return root, ((int(z[0]), int(z[1])), (int(z[2]), int(z[3]))), fields, comms
 
 
+def prove_many(lmcs: LmcsConfig, params: PcsParams, jobs, channels=None):
+    """Independent proofs of many same-shaped jobs (each a list of traces) in lock-step.
+    Returns one (root, eval_points, fields, commitments) per job, each identical to prove()."""
+    k = len(jobs)
+    if k == 0:
+        return []
+    t = len(jobs[0])
+    if any(len(j) != t for j in jobs):
+        raise ValueError("every job needs the same number of traces")
+    if channels is None:
+        kind = CHALLENGER_DUPLEX_P2 if lmcs.hash_kind == HASH_POSEIDON2 else CHALLENGER_KECCAK
+        channels = [ProverTranscript(kind) for _ in range(k)]
+    flat = [m for j in jobs for m in j]
+    arr, keep, dev = _matrices(flat)
+    hs = (_vp * k)(*[c.h for c in channels])
+    roots = np.zeros(4 * k, np.uint64)
+    zs = np.zeros(4 * k, np.uint64)
+    _check(lib().p3g_prove_many(lmcs.dev.h, lmcs.hash_kind, C.byref(params), arr, k, t, DEVICE_PTRS if dev else 0,
+                                hs, _p(roots), _p(zs)), "prove_many")
+    out = []
+    for j, c in enumerate(channels):
+        z = zs[4 * j: 4 * j + 4]
+        f, cm = c.data()
+        out.append((roots[4 * j: 4 * j + 4].copy(), ((int(z[0]), int(z[1])), (int(z[2]), int(z[3]))), f, cm))
+    return out
+
+
 def fill_random_device(device: Device, seed: int, n: int, ptr) -> None:
     _check(lib().p3g_fill_random_device(device.h, seed, n, ptr), "fill_random_device")
 
@@ -733,10 +763,10 @@ class Server:
     footprint within an HBM budget, continuous batching of arriving requests."""
 
     def __init__(self, device: int = 0, workers: int = 4, hbm_budget_bytes: int = 250 * 10**9,
-                 backfill_ms: float = 50.0):
+                 backfill_ms: float = 50.0, max_batch: int = 64):
         h = _vp()
-        _check(lib().p3g_server_create(device, workers, int(hbm_budget_bytes), float(backfill_ms), C.byref(h)),
-               "server_create")
+        _check(lib().p3g_server_create(device, workers, int(hbm_budget_bytes), float(backfill_ms), int(max_batch),
+                                       C.byref(h)), "server_create")
         self.h = h
         self._keep = {}
 

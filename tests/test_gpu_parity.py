@@ -313,3 +313,40 @@ def test_gpu_proofs_match_golden_digests(device, name):
     assert [list(p) for p in z] == r["eval_points"]
     b = f.astype("<u8").tobytes() + c.astype("<u8").ravel().tobytes()
     assert hashlib.sha256(b).hexdigest() == r["proof_sha256"]
+
+
+# ---------------------------------------------------------------- batched lock-step proofs
+MANY_CASES = [
+    (G.HASH_POSEIDON2, [(1 << 10, 16)], (2, 2, 4, 0, 0, 100, 0), 5),
+    (G.HASH_POSEIDON2, [(8, 2), (64, 11), (256, 7), (256, 30)], (1, 1, 1, 0, 0, 40, 0), 3),
+    (G.HASH_POSEIDON2, [(1 << 14, 32)], (1, 1, 3, 0, 0, 64, 0), 4),
+    (G.HASH_POSEIDON2, [(1 << 8, 5)], (3, 2, 0, 1, 2, 20, 3), 6),
+    (G.HASH_POSEIDON2, [(1 << 9, 9)], (1, 3, 2, 0, 0, 30, 0), 1),
+    (G.HASH_KECCAK, [(32, 18), (128, 17)], (2, 2, 1, 0, 1, 25, 2), 3),
+    (G.HASH_KECCAK, [(1 << 9, 40)], (1, 1, 2, 0, 0, 50, 0), 2),
+]
+
+
+@pytest.mark.parametrize("kind,shapes,params,k", MANY_CASES)
+def test_prove_many_matches_oracle(device, kind, shapes, params, k):
+    jobs = [_mats(3000 + 10 * j, shapes) for j in range(k)]
+    got = G.prove_many(G.LmcsConfig(device, kind), G.PcsParams(*params), jobs)
+    assert len(got) == k
+    for tr, (root, z, f, c) in zip(jobs, got):
+        root_o, z_o, f_o, c_o, _ = O.pcs_trace_prove(O.PcsParams(*params), kind, tr)
+        assert root.tolist() == root_o.tolist() and z == tuple(z_o)
+        assert np.array_equal(f, f_o) and np.array_equal(c, c_o)
+
+
+def test_prove_many_device_inputs_and_errors(device):
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    params = G.PcsParams(1, 2, 1, 0, 0, 16, 0)
+    host = [_mats(70 + j, [(256, 12)]) for j in range(4)]
+    jobs = [[device.upload(m) for m in h] for h in host]
+    got = G.prove_many(cfg, params, jobs)
+    for h, (root, z, f, c) in zip(host, got):
+        r1, z1, f1, c1 = G.prove(cfg, params, h)
+        assert root.tolist() == r1.tolist() and np.array_equal(f, f1) and np.array_equal(c, c1)
+    with pytest.raises(G.P3GError) as e:
+        G.prove_many(cfg, params, [_mats(1, [(256, 12)]), _mats(2, [(256, 13)])])
+    assert e.value.code == 1  # shapes differ between jobs

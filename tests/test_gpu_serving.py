@@ -102,3 +102,23 @@ def test_footprint_covers_peak_device_memory(device, shapes, params):
         used = device.memory()["peak_live"] - base["live"]
         fp = G.request_footprint(kind, G.PcsParams(*params), tr)
         assert used <= fp, (kind, used, fp)
+
+
+def test_server_batches_same_shape_requests():
+    # same-shaped requests that queue up while the workers are busy are proven
+    # together in lock-step; every result is still the single-proof one
+    reqs = [O.rand_matrix(900 + i, 1 << 10, 16) for i in range(32)]
+    params = (1, 2, 2, 0, 0, 30, 0)
+    srv = G.Server(0, workers=2, hbm_budget_bytes=8 << 30, max_batch=8)
+    tickets = [srv.submit([m], G.PcsParams(*params)) for m in reqs]
+    res = [srv.result(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 32 and st["batched_requests"] > 0, st
+    dev = G.Device(0)
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    for m, r in zip(reqs, res):
+        root, z, f, c = G.prove(cfg, G.PcsParams(*params), [m])
+        assert root.tolist() == r.root.tolist() and z == r.eval_points
+        assert np.array_equal(f, r.fields) and np.array_equal(c, r.commitments)
+    dev.close()
