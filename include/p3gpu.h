@@ -271,6 +271,7 @@ int p3g_server_poll(p3g_server* s, uint64_t ticket, int* done);
 int p3g_server_proof(p3g_server* s, uint64_t ticket, uint64_t* fields, uint64_t* commitments);
 int p3g_server_release(p3g_server* s, uint64_t ticket);
 int p3g_server_stats(p3g_server* s, p3g_server_counters* out);
+int p3g_server_trim(p3g_server* s);     /* frees the pool's cached (unused) device blocks */
 void p3g_server_destroy(p3g_server* s); /* finishes running requests, drops queued ones */
 int p3g_host_alloc(uint64_t bytes, void** out); /* pinned host memory */
 int p3g_host_free(void* p);

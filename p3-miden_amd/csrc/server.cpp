@@ -384,6 +384,12 @@ int p3g_server_stats(p3g_server* s, p3g_server_counters* out) {
   return P3G_OK;
 }
 
+int p3g_server_trim(p3g_server* s) {
+  if (!s) return P3G_E_INVAL;
+  s->pool.trim();
+  return P3G_OK;
+}
+
 void p3g_server_destroy(p3g_server* s) {
   if (!s) return;
   {

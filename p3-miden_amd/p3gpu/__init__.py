@@ -222,6 +222,7 @@ def _load():
         "p3g_server_release": (C.c_int, [_vp, C.c_uint64]),
         "p3g_server_stats": (C.c_int, [_vp, C.POINTER(_ServerCounters)]),
         "p3g_server_destroy": (None, [_vp]),
+        "p3g_server_trim": (C.c_int, [_vp]),
         "p3g_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
         "p3g_host_free": (C.c_int, [_vp]),
         "p3g_prove_many": (C.c_int, [_vp, C.c_uint32, C.POINTER(PcsParams), C.POINTER(_Matrix), C.c_uint64, C.c_uint64,
@@ -802,6 +803,10 @@ class Server:
         s = _ServerCounters()
         _check(lib().p3g_server_stats(self.h, C.byref(s)), "server_stats")
         return {n: getattr(s, n) for n, _ in _ServerCounters._fields_}
+
+    def trim(self):
+        """Free the device blocks the server's pool holds but does not use."""
+        _check(lib().p3g_server_trim(self.h), "server_trim")
 
     def close(self):
         if getattr(self, "h", None):
