@@ -1,11 +1,107 @@
 // Device context: stream, Poseidon2 constants, cached power tables and a
 // size-bucketed cache of device allocations (the prover allocates the same
 // shapes proof after proof, so blocks are recycled instead of hipMalloc'd).
+#include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <exception>
+#include <functional>
+#include <thread>
 
 #include "internal.hpp"
 
 namespace p3g {
+
+// ------------------------------------------------------------------ host thread pool
+// Persistent workers for per-job host loops (transcripts, hint assembly).
+// One parallel_for at a time; a caller that finds the pool busy runs its loop
+// itself, so concurrent callers never wait on each other.
+namespace {
+struct HostPool {
+  std::mutex run_mu, mu;
+  std::condition_variable cv, cv_done;
+  std::vector<std::thread> threads;
+  const std::function<void(size_t)>* fn = nullptr;
+  size_t n = 0;
+  std::atomic<size_t> next{0};
+  size_t active = 0;
+  uint64_t gen = 0;
+  bool stop = false;
+  std::exception_ptr err;
+
+  HostPool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nt = std::min(16u, hw > 1 ? hw : 1u);
+    for (unsigned i = 1; i < nt; i++) threads.emplace_back([this] { loop(); });
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto& t : threads) t.join();
+  }
+  void work() {
+    for (;;) {
+      const size_t i = next.fetch_add(1);
+      if (i >= n) break;
+      try {
+        (*fn)(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> g(mu);
+        if (!err) err = std::current_exception();
+      }
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return stop || gen != seen; });
+      if (stop) return;
+      seen = gen;
+      lk.unlock();
+      work();
+      lk.lock();
+      if (--active == 0) cv_done.notify_all();
+    }
+  }
+  void run(size_t count, const std::function<void(size_t)>& f) {
+    std::unique_lock<std::mutex> rl(run_mu, std::try_to_lock);
+    if (!rl.owns_lock() || threads.empty() || count < 2) {
+      for (size_t i = 0; i < count; i++) f(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(mu);
+      fn = &f;
+      n = count;
+      next = 0;
+      active = threads.size();
+      err = nullptr;
+      gen++;
+    }
+    cv.notify_all();
+    work();
+    std::exception_ptr e;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_done.wait(lk, [&] { return active == 0; });
+      e = err;
+      fn = nullptr;
+    }
+    if (e) std::rethrow_exception(e);
+  }
+};
+HostPool& host_pool() {
+  static HostPool p;
+  return p;
+}
+}  // namespace
+
+void parallel_for(size_t n, const std::function<void(size_t)>& f) { host_pool().run(n, f); }
 
 static double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -97,6 +193,21 @@ void SharedPool::trim() {
 SharedPool::~SharedPool() {
   trim();
   for (auto& kv : live) (void)hipFree(kv.first);
+}
+
+uint64_t* Ctx::host_stage(size_t words) {
+  if (words <= pinned_words) return pinned;
+  if (pinned) (void)hipHostFree(pinned);
+  pinned = nullptr;
+  pinned_words = 0;
+  size_t want = words + words / 2;
+  if (hipHostMalloc((void**)&pinned, want * 8, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    pinned = nullptr;
+    throw ApiError(E_NOMEM, "pinned host allocation failed");
+  }
+  pinned_words = want;
+  return pinned;
 }
 
 void* Ctx::alloc(size_t bytes) {

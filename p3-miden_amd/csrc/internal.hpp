@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -110,6 +111,11 @@ struct Ctx {
 
   // when set, device blocks come from this pool (shared with other contexts)
   SharedPool* pool = nullptr;
+  // pinned host staging for gathers, grown on demand; contents valid until the
+  // next host_stage call on this context
+  uint64_t* pinned = nullptr;
+  size_t pinned_words = 0;
+  uint64_t* host_stage(size_t words);
 
   void* alloc(size_t bytes);
   void release(void* p);
@@ -248,6 +254,10 @@ size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_
 Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s);
 void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out);
 void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s);
+// gathers the pieces into the context's pinned staging buffer; the returned
+// words are valid until the next gather on this context
+const uint64_t* gather_pinned(Ctx& ctx, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces,
+                              hipStream_t s);
 void gather_to_host(Ctx& ctx, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces,
                     std::vector<uint64_t>& out, hipStream_t s);
 
@@ -258,6 +268,9 @@ Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, 
 
 void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned log_lde_height, const ef2* z, int npts,
                        const std::vector<Tree*>& trees, Channel& tx, hipStream_t s);
+
+// f(0..n-1) on the library's host worker threads (serially when the pool is busy)
+void parallel_for(size_t n, const std::function<void(size_t)>& f);
 
 // k same-shaped proofs in lock-step (the p3g_prove flow per job)
 void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<std::vector<DevMat>>& traces,

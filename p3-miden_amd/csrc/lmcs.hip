@@ -362,12 +362,17 @@ __global__ void __launch_bounds__(TOPC_T) compress_top_p2_many_kernel(uint64_t* 
   compress_top_p2_body(digests[blockIdx.x], off, len, p2);
 }
 
-__global__ void gather_kernel(const uint64_t* const* __restrict__ srcs, const uint64_t* __restrict__ lens,
-                              const uint64_t* __restrict__ offs, uint64_t* __restrict__ out) {
-  const uint32_t b = blockIdx.x;
-  const uint64_t* s = srcs[b];
-  uint64_t n = lens[b], o = offs[b];
-  for (uint64_t e = threadIdx.x; e < n; e += blockDim.x) out[o + e] = s[e];
+// 16 lanes per piece (most pieces are a digest or a short row): piece
+// b = global lane / 16 copies lens[b] words from srcs[b] to out[offs[b]..].
+__global__ void __launch_bounds__(256) gather_kernel(const uint64_t* const* __restrict__ srcs,
+                                                     const uint64_t* __restrict__ lens,
+                                                     const uint64_t* __restrict__ offs, uint64_t np,
+                                                     uint64_t* __restrict__ out) {
+  const uint64_t b = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+  if (b >= np) return;
+  const uint64_t* src = srcs[b];
+  const uint64_t n = lens[b], o = offs[b];
+  for (uint64_t e = threadIdx.x & 15; e < n; e += 16) out[o + e] = src[e];
 }
 
 // ------------------------------------------------------------------ host side
@@ -665,7 +670,9 @@ std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<st
       for (size_t g = 0; g < groups.size(); g++) {
         const uint64_t h = m0[groups[g].first].height;
         const LeafArgs* da = reinterpret_cast<const LeafArgs*>(blob + args_off[g]);
-        if (hash == HASH_P2 && h <= COOP_LEAF_MAX)
+        // the cooperative form costs more lanes per permutation: worth it only
+        // while the whole launch (all jobs) is too small to fill the device
+        if (hash == HASH_P2 && h * k <= COOP_LEAF_MAX)
           hipLaunchKernelGGL(p2_leaf_coop_many_kernel, dim3((uint32_t)((h * 16 + 255) / 256), (uint32_t)k), dim3(256),
                              0, s, da);
         else if (hash == HASH_P2)
@@ -681,7 +688,7 @@ std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<st
       const uint64_t nl = len / 2;
       const LayerJob* dj = reinterpret_cast<const LayerJob*>(blob + level_off[li]);
       ProfScope ps(ctx, "compress_layer", s);
-      if (hash == HASH_P2 && nl <= COOP_LAYER_MAX)
+      if (hash == HASH_P2 && nl * k <= COOP_LAYER_MAX)
         hipLaunchKernelGGL(compress_layer_coop_many_kernel, dim3((uint32_t)((nl * 16 + 255) / 256), (uint32_t)k),
                            dim3(256), 0, s, dj, nl, ctx.dev_p2);
       else if (hash == HASH_P2)
@@ -723,35 +730,44 @@ std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<st
   return trees;
 }
 
+const uint64_t* gather_pinned(Ctx& ctx, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces,
+                              hipStream_t s) {
+  const size_t np = pieces.size();
+  uint64_t total = 0;
+  for (auto& pc : pieces) total += pc.second;
+  // pinned image: [srcs | lens | offs | gathered words]
+  uint64_t* hs = ctx.host_stage(3 * np + total + 1);
+  if (np == 0 || total == 0) return hs + 3 * np;
+  uint64_t acc = 0;
+  for (size_t i = 0; i < np; i++) {
+    hs[i] = reinterpret_cast<uint64_t>(pieces[i].first);
+    hs[np + i] = pieces[i].second;
+    hs[2 * np + i] = acc;
+    acc += pieces[i].second;
+  }
+  uint64_t* stage = (uint64_t*)ctx.alloc((3 * np + total) * sizeof(uint64_t));
+  try {
+    P3G_CHECK_HIP(hipMemcpyAsync(stage, hs, 3 * np * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(gather_kernel, dim3((uint32_t)((np * 16 + 255) / 256)), dim3(256), 0, s,
+                       (const uint64_t* const*)stage, stage + np, stage + 2 * np, (uint64_t)np, stage + 3 * np);
+    P3G_CHECK_HIP(hipGetLastError());
+    P3G_CHECK_HIP(hipMemcpyAsync(hs + 3 * np, stage + 3 * np, total * 8, hipMemcpyDeviceToHost, s));
+    P3G_CHECK_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    ctx.release(stage);
+    throw;
+  }
+  ctx.release(stage);
+  return hs + 3 * np;
+}
+
 void gather_to_host(Ctx& ctx, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces,
                     std::vector<uint64_t>& out, hipStream_t s) {
-  out.clear();
-  if (pieces.empty()) return;
-  std::vector<const uint64_t*> srcs(pieces.size());
-  std::vector<uint64_t> lens(pieces.size()), offs(pieces.size());
   uint64_t total = 0;
-  for (size_t i = 0; i < pieces.size(); i++) {
-    srcs[i] = pieces[i].first;
-    lens[i] = pieces[i].second;
-    offs[i] = total;
-    total += pieces[i].second;
-  }
-  out.resize(total);
-  if (total == 0) return;
-  size_t np = pieces.size();
-  // one staging block: [srcs | lens | offs | data]
-  uint64_t* stage = (uint64_t*)ctx.alloc((3 * np + total) * sizeof(uint64_t));
-  std::vector<uint64_t> hdr(3 * np);
-  memcpy(hdr.data(), srcs.data(), np * 8);
-  memcpy(hdr.data() + np, lens.data(), np * 8);
-  memcpy(hdr.data() + 2 * np, offs.data(), np * 8);
-  P3G_CHECK_HIP(hipMemcpyAsync(stage, hdr.data(), 3 * np * 8, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(gather_kernel, dim3((uint32_t)np), dim3(64), 0, s, (const uint64_t* const*)stage,
-                     stage + np, stage + 2 * np, stage + 3 * np);
-  P3G_CHECK_HIP(hipGetLastError());
-  P3G_CHECK_HIP(hipMemcpyAsync(out.data(), stage + 3 * np, total * 8, hipMemcpyDeviceToHost, s));
-  P3G_CHECK_HIP(hipStreamSynchronize(s));
-  ctx.release(stage);
+  for (auto& pc : pieces) total += pc.second;
+  const uint64_t* p = gather_pinned(ctx, pieces, s);
+  out.assign(p, p + total);
 }
 
 void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out) {
@@ -771,6 +787,7 @@ void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out) {
     off += m.width;
   }
 }
+
 
 // prove_batch in two halves so that many trees can share one device gather:
 // the pieces (rows, then missing siblings) in hint order, then the hints.
@@ -830,9 +847,8 @@ size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_
 void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s) {
   std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
   const size_t nrows = prove_batch_pieces(t, indices, pieces);
-  std::vector<uint64_t> raw;
-  gather_to_host(*t.ctx, pieces, raw, s);
-  prove_batch_emit(t, pieces, 0, pieces.size(), nrows, raw.data(), tx);
+  const uint64_t* raw = gather_pinned(*t.ctx, pieces, s);
+  prove_batch_emit(t, pieces, 0, pieces.size(), nrows, raw, tx);
 }
 
 void host_sponge_absorb(uint32_t hash, uint64_t* state, const uint64_t* in, size_t len) {

@@ -876,7 +876,7 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
     ef2* dcoef = (ef2*)ctx.alloc(coeffs_all.size() * sizeof(ef2));
     tmp.push_back(dcoef);
     const uint64_t* powQ = ctx.pow_table(gl_two_adic_generator(logL), logL);
-    for (size_t j = 0; j < k; j++) {
+    parallel_for(k, [&](size_t j) {
       ef2* ev = evals_host.data() + j * totw * N;
       ef2 sc[N];
       for (int q = 0; q < N; q++) {
@@ -931,7 +931,7 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
       a.bpow[1] = beta;
       a.out = Q[j];
       qa[j] = a;
-    }
+    });
     P3G_CHECK_HIP(hipMemcpyAsync(dqm, qm_all.data(), qm_all.size() * sizeof(QMat), hipMemcpyHostToDevice, s));
     P3G_CHECK_HIP(hipMemcpyAsync(dcoef, coeffs_all.data(), coeffs_all.size() * sizeof(ef2), hipMemcpyHostToDevice, s));
     const QuotArgs* dqa = (const QuotArgs*)upload(qa.data(), k * sizeof(QuotArgs));
@@ -981,13 +981,16 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
       std::vector<FoldJob> fj(k);
       std::vector<ef2*> nxt(k);
       for (size_t j = 0; j < k; j++) {
-        Transcript& tx = *txs[j];
-        tx.send_commitment(ftrees.back()[j]->root);
-        tx.grind(p.folding_pow_bits);
         nxt[j] = (ef2*)ctx.alloc(fd * sizeof(ef2));
         bufs.push_back(nxt[j]);
-        fj[j] = FoldJob{cur[j], nxt[j], tx.sample_ef()};
       }
+      const std::vector<Tree*>& ft = ftrees.back();
+      parallel_for(k, [&](size_t j) {
+        Transcript& tx = *txs[j];
+        tx.send_commitment(ft[j]->root);
+        tx.grind(p.folding_pow_bits);
+        fj[j] = FoldJob{cur[j], nxt[j], tx.sample_ef()};
+      });
       const FoldJob* dfj = (const FoldJob*)upload(fj.data(), k * sizeof(FoldJob));
       const unsigned log_rows = log2_exact(fd);
       const unsigned exp_shift = logL - log2_exact(domain);
@@ -1018,46 +1021,55 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
       std::vector<uint64_t> raw;
       gather_to_host(ctx, pieces, raw, s);
       const unsigned lf = log2_exact(fdeg);
-      for (size_t j = 0; j < k; j++) {
+      parallel_for(k, [&](size_t j) {
         const ef2* fin = reinterpret_cast<const ef2*>(raw.data() + j * 2 * fdeg);
         std::vector<ef2> nat(fdeg);
         for (size_t i = 0; i < fdeg; i++) nat[bitrev32((uint32_t)i, lf)] = fin[i];
         host_idft_ext(nat);
         std::reverse(nat.begin(), nat.end());
         txs[j]->send_fields(reinterpret_cast<const uint64_t*>(nat.data()), 2 * fdeg);
-      }
+      });
     }
     // ---- query phase: indices per job, then one gather for every opening of every job
     struct Seg {
       const Tree* t;
       size_t first, last, nrows;
     };
-    std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+    // per job (in parallel): indices and the pieces of every opening, then one
+    // gather for all jobs, then the hints per job (in parallel)
+    std::vector<std::vector<std::pair<const uint64_t*, uint64_t>>> jp(k);
     std::vector<std::vector<Seg>> segs(k);
-    for (size_t j = 0; j < k; j++) {
+    parallel_for(k, [&](size_t j) {
       Transcript& tx = *txs[j];
       tx.grind(p.query_pow_bits);
       std::vector<uint64_t> idx(p.num_queries);
       for (uint32_t q = 0; q < p.num_queries; q++) idx[q] = bitrev32((uint32_t)tx.sample_bits(logL), logL);
-      size_t first = pieces.size();
-      size_t nr = prove_batch_pieces(*trees[j], idx, pieces);
-      segs[j].push_back(Seg{trees[j], first, pieces.size(), nr});
+      size_t first = 0;
+      size_t nr = prove_batch_pieces(*trees[j], idx, jp[j]);
+      segs[j].push_back(Seg{trees[j], first, jp[j].size(), nr});
       std::vector<uint64_t> ri(idx.size());
       for (size_t r = 0; r < ftrees.size(); r++) {
         const unsigned sh = la * (unsigned)(r + 1);
         for (size_t q = 0; q < idx.size(); q++) ri[q] = idx[q] >> sh;
-        first = pieces.size();
-        nr = prove_batch_pieces(*ftrees[r][j], ri, pieces);
-        segs[j].push_back(Seg{ftrees[r][j], first, pieces.size(), nr});
+        first = jp[j].size();
+        nr = prove_batch_pieces(*ftrees[r][j], ri, jp[j]);
+        segs[j].push_back(Seg{ftrees[r][j], first, jp[j].size(), nr});
       }
+    });
+    std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+    std::vector<size_t> job_piece0(k), job_word0(k);
+    size_t words = 0;
+    for (size_t j = 0; j < k; j++) {
+      job_piece0[j] = pieces.size();
+      job_word0[j] = words;
+      for (auto& pc : jp[j]) words += pc.second;
+      pieces.insert(pieces.end(), jp[j].begin(), jp[j].end());
     }
-    std::vector<uint64_t> raw;
-    gather_to_host(ctx, pieces, raw, s);
-    std::vector<size_t> woff(pieces.size() + 1, 0);
-    for (size_t i = 0; i < pieces.size(); i++) woff[i + 1] = woff[i] + pieces[i].second;
-    for (size_t j = 0; j < k; j++)
-      for (auto& sg : segs[j])
-        prove_batch_emit(*sg.t, pieces, sg.first, sg.last, sg.nrows, raw.data() + woff[sg.first], *txs[j]);
+    const uint64_t* raw = gather_pinned(ctx, pieces, s);
+    parallel_for(k, [&](size_t j) {
+      const uint64_t* base = raw + job_word0[j];
+      for (auto& sg : segs[j]) base += prove_batch_emit(*sg.t, jp[j], sg.first, sg.last, sg.nrows, base, *txs[j]);
+    });
     (void)B;
   } catch (...) {
     cleanup();
