@@ -203,6 +203,8 @@ struct QuotArgs {
   ef2 fred[MAXPTS];
   ef2 bpow[MAXPTS];
   ef2* out;
+  uint32_t nsplit;  // column tiles dealt round-robin over nsplit workgroups per row block
+  ef2* partial;     // nsplit > 1: per row and split, the partial column sum (L x nsplit)
 };
 
 constexpr int DQ_T = 256;
@@ -213,21 +215,23 @@ constexpr int DQ_ROWS = DQ_T * DQ_CH;
 // through LDS in 256-row x 16-column tiles with coalesced loads (16 lanes per
 // 128-byte row segment); each thread then reduces its own row from LDS with
 // the (uniform, scalar-loaded) -alpha^(W-1-c) coefficients.
-template <int N>
-__device__ __forceinline__ void deep_quotient_body(const QuotArgs& a, uint32_t bx) {
+// acc[r] = sum over this split's column tiles of -alpha^(W-1-c) f_c(x_i) for
+// rows i = wbase + tid + 256 r (tiles g with g % nsplit == split)
+__device__ __forceinline__ void deep_quotient_acc(const QuotArgs& a, uint64_t wbase, uint32_t split, uint32_t nsplit,
+                                                  ef2* acc) {
   __shared__ uint64_t tile[256 * 16];
   const uint32_t tid = threadIdx.x;
-  const uint64_t wbase = (uint64_t)bx * DQ_ROWS;
-  ef2 acc[DQ_CH];
 #pragma unroll
   for (int r = 0; r < DQ_CH; r++) {
     ef2 s = ef_from(0);
+    uint32_t g = 0;
 #pragma unroll 1
     for (uint32_t m = 0; m < a.nmats; m++) {
       const QMat mm = a.mats[m];
       const ef2* cf = a.coeffs + mm.coeff_off;
 #pragma unroll 1
-      for (uint64_t c0 = 0; c0 < mm.width; c0 += 16) {
+      for (uint64_t c0 = 0; c0 < mm.width; c0 += 16, g++) {
+        if (g % nsplit != split) continue;
         const uint32_t nc = (uint32_t)((mm.width - c0) < 16 ? (mm.width - c0) : 16);
         __syncthreads();
 #pragma unroll 4
@@ -247,6 +251,11 @@ __device__ __forceinline__ void deep_quotient_body(const QuotArgs& a, uint32_t b
     }
     acc[r] = s;
   }
+}
+
+template <int N>
+__device__ __forceinline__ void deep_quotient_finish(const QuotArgs& a, uint64_t wbase, const ef2* acc) {
+  const uint32_t tid = threadIdx.x;
   uint64_t d0[DQ_CH][N], nrm[DQ_CH][N], pre[DQ_CH][N];
   uint64_t run = 1;
 #pragma unroll
@@ -286,8 +295,40 @@ __device__ __forceinline__ void deep_quotient_body(const QuotArgs& a, uint32_t b
   }
 }
 template <int N>
+__device__ __forceinline__ void deep_quotient_body(const QuotArgs& a, uint32_t bx) {
+  const uint64_t wbase = (uint64_t)bx * DQ_ROWS;
+  ef2 acc[DQ_CH];
+  deep_quotient_acc(a, wbase, 0, 1, acc);
+  deep_quotient_finish<N>(a, wbase, acc);
+}
+template <int N>
 __global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4))) deep_quotient_kernel(QuotArgs a) {
   deep_quotient_body<N>(a, blockIdx.x);
+}
+// column-split form for short, wide LDEs: partial sums, then a finishing pass
+__global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4))) deep_quotient_partial_kernel(QuotArgs a) {
+  const uint64_t wbase = (uint64_t)blockIdx.x * DQ_ROWS;
+  ef2 acc[DQ_CH];
+  deep_quotient_acc(a, wbase, blockIdx.y, a.nsplit, acc);
+#pragma unroll
+  for (int r = 0; r < DQ_CH; r++) {
+    const uint64_t i = wbase + threadIdx.x + 256 * r;
+    if (i < a.L) a.partial[i * a.nsplit + blockIdx.y] = acc[r];
+  }
+}
+template <int N>
+__global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4))) deep_quotient_finish_kernel(QuotArgs a) {
+  const uint64_t wbase = (uint64_t)blockIdx.x * DQ_ROWS;
+  ef2 acc[DQ_CH];
+#pragma unroll
+  for (int r = 0; r < DQ_CH; r++) {
+    const uint64_t i = wbase + threadIdx.x + 256 * r;
+    ef2 sum = ef_from(0);
+    if (i < a.L)
+      for (uint32_t y = 0; y < a.nsplit; y++) sum = ef_add(sum, a.partial[i * a.nsplit + y]);
+    acc[r] = sum;
+  }
+  deep_quotient_finish<N>(a, wbase, acc);
 }
 template <int N>
 __global__ void __launch_bounds__(DQ_T) __attribute__((amdgpu_waves_per_eu(4)))
@@ -472,12 +513,31 @@ static void run_quotient(Ctx& ctx, unsigned logL, const ef2* z, const std::vecto
   a.bpow[0] = ef_from(1);
   for (int j = 1; j < N; j++) a.bpow[j] = ef_mul(a.bpow[j - 1], beta);
   a.out = out;
+  // short, wide LDEs: deal the column tiles over more workgroups than there are
+  // row blocks, so the launch fills the device
+  const uint64_t nwg = (L + DQ_ROWS - 1) / DQ_ROWS;
+  uint64_t tiles = 0;
+  for (auto& q : qm) tiles += (q.width + 15) / 16;
+  a.nsplit = 1;
+  a.partial = nullptr;
+  if (nwg < 1024 && tiles > 1) {
+    uint64_t want = 2048 / nwg;
+    a.nsplit = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, want));
+  }
   {
-  ProfScope ps(ctx, "deep_quotient", s);
-  hipLaunchKernelGGL(deep_quotient_kernel<N>, dim3((uint32_t)((L + DQ_ROWS - 1) / DQ_ROWS)), dim3(DQ_T), 0, s, a);
-  P3G_CHECK_HIP(hipGetLastError());
+    ProfScope ps(ctx, "deep_quotient", s);
+    if (a.nsplit == 1) {
+      hipLaunchKernelGGL(deep_quotient_kernel<N>, dim3((uint32_t)nwg), dim3(DQ_T), 0, s, a);
+    } else {
+      a.partial = (ef2*)ctx.alloc(L * a.nsplit * sizeof(ef2));
+      hipLaunchKernelGGL(deep_quotient_partial_kernel, dim3((uint32_t)nwg, a.nsplit), dim3(DQ_T), 0, s, a);
+      P3G_CHECK_HIP(hipGetLastError());
+      hipLaunchKernelGGL(deep_quotient_finish_kernel<N>, dim3((uint32_t)nwg), dim3(DQ_T), 0, s, a);
+    }
+    P3G_CHECK_HIP(hipGetLastError());
   }
   P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx.release(a.partial);
   ctx.release(dq);
   ctx.release(dc);
 }
@@ -930,6 +990,8 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
       a.bpow[0] = ef_from(1);
       a.bpow[1] = beta;
       a.out = Q[j];
+      a.nsplit = 1;
+      a.partial = nullptr;
       qa[j] = a;
     });
     P3G_CHECK_HIP(hipMemcpyAsync(dqm, qm_all.data(), qm_all.size() * sizeof(QMat), hipMemcpyHostToDevice, s));
