@@ -90,6 +90,7 @@ void p3g_ctx_destroy(p3g_ctx* ctx) {
   for (auto& kv : ctx->c.pow_tables) (void)hipFree(kv.second);
   if (ctx->c.dev_p2) (void)hipFree(ctx->c.dev_p2);
   if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
+  ctx->c.free_upload_ring();
   if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }

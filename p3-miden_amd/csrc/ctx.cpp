@@ -1,6 +1,7 @@
 // Device context: stream, Poseidon2 constants, cached power tables and a
 // size-bucketed cache of device allocations (the prover allocates the same
 // shapes proof after proof, so blocks are recycled instead of hipMalloc'd).
+#include <cstring>
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -208,6 +209,90 @@ uint64_t* Ctx::host_stage(size_t words) {
   }
   pinned_words = want;
   return pinned;
+}
+
+constexpr size_t UPLOAD_RING_BYTES = size_t(8) << 20;
+
+void Ctx::upload_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (need > UPLOAD_RING_BYTES) {  // rare: a large image goes straight from the caller's memory
+    P3G_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    P3G_CHECK_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  std::lock_guard<std::mutex> g(up_mu);
+  if (!up_ring) {
+    if (hipHostMalloc((void**)&up_ring, UPLOAD_RING_BYTES, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      up_ring = nullptr;
+      throw ApiError(E_NOMEM, "pinned host allocation failed");
+    }
+    up_cap = UPLOAD_RING_BYTES;
+    up_head = 0;
+  }
+  // retire finished segments from the front without waiting
+  while (!up_inflight.empty() && hipEventQuery(up_inflight.front().ev) == hipSuccess) {
+    up_spare.push_back(up_inflight.front().ev);
+    up_inflight.pop_front();
+  }
+  const size_t pos = up_head + need <= up_cap ? up_head : 0;
+  auto overlaps = [&] {
+    for (const UpSeg& f : up_inflight)
+      if (f.start < pos + need && pos < f.end) return true;
+    return false;
+  };
+  // oldest first: every earlier copy completes no later than the ones after it
+  while (overlaps()) {
+    UpSeg f = up_inflight.front();
+    up_inflight.pop_front();
+    const hipError_t e = hipEventSynchronize(f.ev);
+    up_spare.push_back(f.ev);
+    P3G_CHECK_HIP(e);
+  }
+  memcpy(up_ring + pos, src, bytes);
+  hipEvent_t ev;
+  if (!up_spare.empty()) {
+    ev = up_spare.back();
+    up_spare.pop_back();
+  } else {
+    P3G_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  const hipError_t e1 = hipMemcpyAsync(dst, up_ring + pos, bytes, hipMemcpyHostToDevice, s);
+  const hipError_t e2 = e1 == hipSuccess ? hipEventRecord(ev, s) : e1;
+  if (e2 != hipSuccess) {
+    // the copy may still be queued: keep the region fenced by the stream itself
+    (void)hipStreamSynchronize(s);
+    up_spare.push_back(ev);
+    P3G_CHECK_HIP(e2);
+  }
+  up_inflight.push_back(UpSeg{pos, pos + need, ev});
+  up_head = pos + need;
+}
+
+void* Ctx::upload(const void* src, size_t bytes, hipStream_t s) {
+  void* d = alloc(bytes ? bytes : 8);
+  try {
+    upload_async(d, src, bytes, s);
+  } catch (...) {
+    release(d);
+    throw;
+  }
+  return d;
+}
+
+void Ctx::free_upload_ring() {
+  std::lock_guard<std::mutex> g(up_mu);
+  for (const UpSeg& f : up_inflight) {
+    (void)hipEventSynchronize(f.ev);
+    (void)hipEventDestroy(f.ev);
+  }
+  up_inflight.clear();
+  for (hipEvent_t ev : up_spare) (void)hipEventDestroy(ev);
+  up_spare.clear();
+  if (up_ring) (void)hipHostFree(up_ring);
+  up_ring = nullptr;
+  up_cap = up_head = 0;
 }
 
 void* Ctx::alloc(size_t bytes) {

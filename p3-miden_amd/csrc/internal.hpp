@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <deque>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -116,6 +117,24 @@ struct Ctx {
   uint64_t* pinned = nullptr;
   size_t pinned_words = 0;
   uint64_t* host_stage(size_t words);
+
+  // Pinned upload ring.  upload_async copies the caller's bytes into pinned
+  // memory and enqueues the device copy from there, so the caller's buffer may
+  // die at once and nothing waits for the stream.  A ring region is reused
+  // only after the event recorded behind its copy has completed.
+  struct UpSeg {
+    size_t start, end;
+    hipEvent_t ev;
+  };
+  uint8_t* up_ring = nullptr;
+  size_t up_cap = 0, up_head = 0;
+  std::deque<UpSeg> up_inflight;
+  std::vector<hipEvent_t> up_spare;
+  std::mutex up_mu;
+  void upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
+  // alloc + upload_async; the block is released by the caller as usual
+  void* upload(const void* src, size_t bytes, hipStream_t s);
+  void free_upload_ring();
 
   void* alloc(size_t bytes);
   void release(void* p);

@@ -441,9 +441,7 @@ void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, 
       }
       ct = gl_mul(ct, om);
     }
-    P3G_CHECK_HIP(hipMemcpyAsync(scale, host.data(), host.size() * 8, hipMemcpyHostToDevice, s));
-    // the host vector dies at scope end: make the copy complete first
-    P3G_CHECK_HIP(hipStreamSynchronize(s));
+    ctx.upload_async(scale, host.data(), host.size() * 8, s);
   }
 
   auto launch = [&](uint32_t mode, uint32_t s0, uint32_t K, uint64_t nblk) {
@@ -588,7 +586,7 @@ void coset_lde_many(Ctx& ctx, const std::vector<LdeJob>& jobs, unsigned log_blow
   const uint64_t stride = POW_LO + pow_table_hi_len(logn);
   std::map<uint64_t, uint64_t*> scales;
   std::vector<uint64_t*> owned;
-  std::vector<NttArgs> descs;     // host images of the uploads (outlive them)
+  std::vector<NttArgs> descs;
   std::vector<uint64_t> starts;
   auto cleanup = [&] {
     (void)hipStreamSynchronize(s);
@@ -615,10 +613,8 @@ void coset_lde_many(Ctx& ctx, const std::vector<LdeJob>& jobs, unsigned log_blow
         }
         ct = gl_mul(ct, om);
       }
-      uint64_t* sc = (uint64_t*)ctx.alloc(host.size() * 8);
+      uint64_t* sc = (uint64_t*)ctx.upload(host.data(), host.size() * 8, s);
       owned.push_back(sc);
-      P3G_CHECK_HIP(hipMemcpyAsync(sc, host.data(), host.size() * 8, hipMemcpyHostToDevice, s));
-      P3G_CHECK_HIP(hipStreamSynchronize(s));
       scales[j.shift] = sc;
     }
     // descriptors of every pass for every job, uploaded once: [pass][job]
@@ -647,24 +643,21 @@ void coset_lde_many(Ctx& ctx, const std::vector<LdeJob>& jobs, unsigned log_blow
       }
       s0 += split[i];
     }
-    NttArgs* ddesc = (NttArgs*)ctx.alloc(descs.size() * sizeof(NttArgs));
+    NttArgs* ddesc = (NttArgs*)ctx.upload(descs.data(), descs.size() * sizeof(NttArgs), s);
     owned.push_back((uint64_t*)ddesc);
-    uint64_t* dstarts = (uint64_t*)ctx.alloc(starts.size() * 8);
+    uint64_t* dstarts = (uint64_t*)ctx.upload(starts.data(), starts.size() * 8, s);
     owned.push_back(dstarts);
-    P3G_CHECK_HIP(hipMemcpyAsync(ddesc, descs.data(), descs.size() * sizeof(NttArgs), hipMemcpyHostToDevice, s));
-    P3G_CHECK_HIP(hipMemcpyAsync(dstarts, starts.data(), starts.size() * 8, hipMemcpyHostToDevice, s));
     for (size_t pi = 0; pi < np; pi++) {
       ProfScope ps(ctx, modes[pi] <= INV ? "lde_inverse_pass" : "lde_forward_pass", s);
       launch_ntt_grouped(modes[pi], ntt_reg_ok(Ks[pi], logC), Ks[pi], ddesc + pi * k, dstarts + pi * (k + 1),
                          (uint32_t)k, starts[pi * (k + 1) + k], s);
     }
-    // the descriptor uploads came from host vectors that die here
-    P3G_CHECK_HIP(hipStreamSynchronize(s));
   } catch (...) {
     cleanup();
     throw;
   }
-  cleanup();
+  // stream-ordered release: nothing here waits for the transforms
+  for (auto* p : owned) ctx.release(p);
 }
 
 // Stages per pass for a size-2^logn transform: as few passes as kmax allows,

@@ -308,42 +308,37 @@ __global__ void __launch_bounds__(TOP_T) compress_top_many_kernel(uint64_t* cons
   compress_top_body<HASH>(digests[blockIdx.x], off, len, p2);
 }
 
-// Poseidon2 variant of the tree top: the wide first levels one node per lane,
-// the narrow ones (<= 128 nodes) with the 16-lane cooperative permutation,
-// whose dependent chain is about a fifth as long.
+// Poseidon2 variant of the tree top: every level with the 16-lane cooperative
+// permutation (whose dependent chain is about a fifth of the one-lane form's),
+// one node per 16-lane row.
 constexpr int TOPC_T = 1024;
+// The top starts once a layer holds at most this many digests, so that every
+// level in it is a single pass of the 64 rows.  Measured: a level inside the
+// top costs about as much as a separate cooperative layer launch (~17 us),
+// while a wider level inside it (two passes, or one node per lane) costs
+// ~50 us.
+constexpr uint64_t TOPC_MAX_LEN = 2 * (TOPC_T / 16);
+__host__ __device__ constexpr uint64_t top_max_len(uint32_t hash) {
+  return hash == HASH_P2 ? TOPC_MAX_LEN : 2 * TOP_T;
+}
+// len <= TOPC_MAX_LEN digests at layer offset off
 __device__ __forceinline__ void compress_top_p2_body(uint64_t* digests, uint64_t off, uint64_t len,
                                                      const P2Consts* p2) {
-  __shared__ uint64_t buf[2][TOP_T * 2 * 4];
+  __shared__ uint64_t buf[2][TOPC_MAX_LEN * 4];
   const uint32_t tid = threadIdx.x;
-  const uint32_t l = tid & 15, row = tid >> 4;
+  const uint32_t l = tid & 15, node = tid >> 4;
   for (uint64_t e = tid; e < len * 4; e += TOPC_T) buf[0][e] = digests[off * 4 + e];
   __syncthreads();
   int cur = 0;
   while (len > 1) {
     const uint64_t nl = len / 2;
     const uint64_t noff = off + len;
-    if (nl > 128) {
-      if (tid < nl) {
-        uint64_t o[4];
-        compress_one(HASH_P2, &buf[cur][tid * 8], &buf[cur][tid * 8 + 4], o, p2);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          buf[cur ^ 1][tid * 4 + k] = o[k];
-          digests[(noff + tid) * 4 + k] = o[k];
-        }
-      }
-    } else {
-      for (uint64_t base = 0; base < nl; base += TOPC_T / 16) {
-        const uint64_t node = base + row;
-        if (node < nl) {  // uniform over the 16-lane row
-          uint64_t x = l < 8 ? buf[cur][node * 8 + l] : 0;
-          x = p2_permute_coop(x, l, p2);
-          if (l < 4) {
-            buf[cur ^ 1][node * 4 + l] = x;
-            digests[(noff + node) * 4 + l] = x;
-          }
-        }
+    if (node < nl) {  // uniform over the 16-lane row
+      uint64_t x = l < 8 ? buf[cur][node * 8 + l] : 0;
+      x = p2_permute_coop(x, l, p2);
+      if (l < 4) {
+        buf[cur ^ 1][node * 4 + l] = x;
+        digests[(noff + node) * 4 + l] = x;
       }
     }
     __syncthreads();
@@ -382,8 +377,7 @@ static void launch_leaf(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& grou
                         hipStream_t s) {
   std::vector<LeafMat> hm(group.size());
   for (size_t i = 0; i < group.size(); i++) hm[i] = LeafMat{group[i].ptr, group[i].width};
-  LeafMat* dm = (LeafMat*)ctx.alloc(hm.size() * sizeof(LeafMat));
-  P3G_CHECK_HIP(hipMemcpyAsync(dm, hm.data(), hm.size() * sizeof(LeafMat), hipMemcpyHostToDevice, s));
+  LeafMat* dm = (LeafMat*)ctx.upload(hm.data(), hm.size() * sizeof(LeafMat), s);
   LeafArgs a{};
   a.mats = dm;
   a.nmats = (uint32_t)hm.size();
@@ -402,9 +396,7 @@ static void launch_leaf(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& grou
   else hipLaunchKernelGGL(keccak_leaf_kernel<LEAF_T>, dim3(grid), dim3(LEAF_T), 0, s, a);
   P3G_CHECK_HIP(hipGetLastError());
   }
-  // the descriptor copy is from pageable host memory: complete it before hm dies
-  P3G_CHECK_HIP(hipStreamSynchronize(s));
-  ctx.release(dm);
+  ctx.release(dm);  // stream-ordered: a later owner's work queues behind this launch
 }
 
 // All compression layers of a tree whose leaf digests are in place.  Only
@@ -413,7 +405,7 @@ static void compress_layers(Ctx* ctx, uint32_t hash, uint64_t* digests, const st
                             uint64_t height, const P2Consts* p2, hipStream_t s) {
   uint64_t len = height;
   unsigned l = 0;
-  while (len > 2 * TOP_T) {
+  while (len > top_max_len(hash)) {
     uint64_t nl = len / 2;
     uint32_t grid = (uint32_t)((nl + CMP_T - 1) / CMP_T);
     const uint64_t* prev = digests + layer_off[l] * 4;
@@ -438,6 +430,7 @@ static void compress_layers(Ctx* ctx, uint32_t hash, uint64_t* digests, const st
     l++;
   }
   if (len > 1) {
+    if (len > top_max_len(hash)) throw ApiError(E_RANGE, "tree top wider than its kernel");
     auto top = [&] {
       if (hash == HASH_P2)
         hipLaunchKernelGGL(compress_top_p2_kernel, dim3(1), dim3(TOPC_T), 0, s, digests, layer_off[l], len, p2);
@@ -645,7 +638,7 @@ std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<st
     std::vector<size_t> level_off;
     uint64_t len = H;
     unsigned lvl = 0;
-    while (len > 2 * TOP_T) {
+    while (len > top_max_len(hash)) {
       const size_t o = reserve(k * sizeof(LayerJob));
       for (size_t j = 0; j < k; j++) {
         LayerJob lj{trees[j]->digests + layer_off[lvl] * 4, trees[j]->digests + layer_off[lvl + 1] * 4};
@@ -664,7 +657,7 @@ std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<st
         LeafArgs* a = reinterpret_cast<LeafArgs*>(host.data() + args_off[g] + j * sizeof(LeafArgs));
         a->mats = reinterpret_cast<const LeafMat*>(blob + reinterpret_cast<uintptr_t>(a->mats));
       }
-    P3G_CHECK_HIP(hipMemcpyAsync(blob, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    ctx.upload_async(blob, host.data(), host.size(), s);
     {
       ProfScope ps(ctx, hash == HASH_P2 ? "p2_leaf" : "keccak_leaf", s);
       for (size_t g = 0; g < groups.size(); g++) {
@@ -702,6 +695,7 @@ std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<st
       len = nl;
     }
     if (len > 1) {
+      if (len > top_max_len(hash)) throw ApiError(E_RANGE, "tree top wider than its kernel");
       uint64_t* const* dd = reinterpret_cast<uint64_t* const*>(blob + dptr_off);
       ProfScope ps(ctx, "compress_top", s);
       if (hash == HASH_P2)

@@ -494,11 +494,8 @@ static void run_quotient(Ctx& ctx, unsigned logL, const ef2* z, const std::vecto
     qm.push_back(QMat{mats[i].ptr, mats[i].width, log2_exact(L / mats[i].height), (uint32_t)aoff});
     aoff += awidths[i];
   }
-  QMat* dq = (QMat*)ctx.alloc(qm.size() * sizeof(QMat));
-  ef2* dc = (ef2*)ctx.alloc((coeffs.size() ? coeffs.size() : 1) * sizeof(ef2));
-  P3G_CHECK_HIP(hipMemcpyAsync(dq, qm.data(), qm.size() * sizeof(QMat), hipMemcpyHostToDevice, s));
-  if (!coeffs.empty())
-    P3G_CHECK_HIP(hipMemcpyAsync(dc, coeffs.data(), coeffs.size() * sizeof(ef2), hipMemcpyHostToDevice, s));
+  QMat* dq = (QMat*)ctx.upload(qm.data(), qm.size() * sizeof(QMat), s);
+  ef2* dc = (ef2*)ctx.upload(coeffs.data(), coeffs.size() * sizeof(ef2), s);
   QuotArgs a{};
   a.mats = dq;
   a.nmats = (uint32_t)qm.size();
@@ -536,7 +533,7 @@ static void run_quotient(Ctx& ctx, unsigned logL, const ef2* z, const std::vecto
     }
     P3G_CHECK_HIP(hipGetLastError());
   }
-  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  // stream-ordered releases: the quotient's consumers queue behind it
   ctx.release(a.partial);
   ctx.release(dq);
   ctx.release(dc);
@@ -711,13 +708,27 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
     tx.grind(p.query_pow_bits);
     std::vector<uint64_t> idx(p.num_queries);
     for (uint32_t q = 0; q < p.num_queries; q++) idx[q] = bitrev32((uint32_t)tx.sample_bits(logL), logL);
-    for (auto* t : trees) tree_prove_batch(*t, idx, tx, s);
+    // every opening of every tree in one gather, then the hints tree by tree
+    struct Seg {
+      const Tree* t;
+      size_t first, last, nrows;
+    };
+    std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+    std::vector<Seg> segs;
+    auto collect = [&](const Tree& t, const std::vector<uint64_t>& ix) {
+      const size_t first = pieces.size();
+      const size_t nr = prove_batch_pieces(t, ix, pieces);
+      segs.push_back(Seg{&t, first, pieces.size(), nr});
+    };
+    for (auto* t : trees) collect(*t, idx);
     std::vector<uint64_t> ri(idx.size());
     for (size_t r = 0; r < ftrees.size(); r++) {
       unsigned sh = la * (unsigned)(r + 1);
       for (size_t q = 0; q < idx.size(); q++) ri[q] = idx[q] >> sh;
-      tree_prove_batch(*ftrees[r], ri, tx, s);
+      collect(*ftrees[r], ri);
     }
+    const uint64_t* raw = gather_pinned(ctx, pieces, s);
+    for (auto& sg : segs) raw += prove_batch_emit(*sg.t, pieces, sg.first, sg.last, sg.nrows, raw, tx);
   } catch (...) {
     for (auto* t : ftrees) delete t;
     for (auto* b : bufs) ctx.release(b);
@@ -767,7 +778,6 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
   std::vector<std::vector<Tree*>> ftrees;  // [round][job]
   std::vector<ef2*> bufs;                  // every FRI input buffer (Q and folds)
   std::vector<void*> tmp;                  // released at the end
-  std::vector<std::vector<uint8_t>> staged;  // host images of uploaded argument arrays
   auto cleanup = [&] {
     (void)hipStreamSynchronize(s);
     for (auto& r : ftrees)
@@ -831,16 +841,11 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
     const unsigned logd = logL - p.log_blowup;
     const uint64_t d = 1ull << logd;
     const uint64_t* powL = ctx.pow_table(gl_two_adic_generator(logL), logL);
-    // device scratch image: [points | weight ptrs ...] built per phase
-    // argument arrays go up through host copies (staged) that outlive every
-    // asynchronous copy, the error path included
+    // argument arrays go up through the context's pinned upload ring: the host
+    // images may die at once and nothing waits for the stream
     auto upload = [&](const void* src, size_t bytes) {
-      void* dst = ctx.alloc(bytes ? bytes : 8);
+      void* dst = ctx.upload(src, bytes, s);
       tmp.push_back(dst);
-      if (bytes) {
-        staged.emplace_back((const uint8_t*)src, (const uint8_t*)src + bytes);
-        P3G_CHECK_HIP(hipMemcpyAsync(dst, staged.back().data(), bytes, hipMemcpyHostToDevice, s));
-      }
       return dst;
     };
     std::map<uint64_t, std::vector<ef2*>> wts;  // weight vectors per trace height, per job
@@ -994,8 +999,8 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
       a.partial = nullptr;
       qa[j] = a;
     });
-    P3G_CHECK_HIP(hipMemcpyAsync(dqm, qm_all.data(), qm_all.size() * sizeof(QMat), hipMemcpyHostToDevice, s));
-    P3G_CHECK_HIP(hipMemcpyAsync(dcoef, coeffs_all.data(), coeffs_all.size() * sizeof(ef2), hipMemcpyHostToDevice, s));
+    ctx.upload_async(dqm, qm_all.data(), qm_all.size() * sizeof(QMat), s);
+    ctx.upload_async(dcoef, coeffs_all.data(), coeffs_all.size() * sizeof(ef2), s);
     const QuotArgs* dqa = (const QuotArgs*)upload(qa.data(), k * sizeof(QuotArgs));
     {
       ProfScope ps(ctx, "deep_quotient", s);
@@ -1003,7 +1008,6 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
                          dim3(DQ_T), 0, s, dqa);
       P3G_CHECK_HIP(hipGetLastError());
     }
-    P3G_CHECK_HIP(hipStreamSynchronize(s));  // host argument images above die with this scope
 
     // ---- FRI commit phase, all jobs per round
     const unsigned la = p.log_arity;
@@ -1072,7 +1076,6 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
         }
         P3G_CHECK_HIP(hipGetLastError());
       }
-      P3G_CHECK_HIP(hipStreamSynchronize(s));  // fj dies with this iteration
       cur = nxt;
       domain = fd;
     }
