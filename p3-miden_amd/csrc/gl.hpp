@@ -27,15 +27,17 @@ P3G_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
 #endif
 }
 
-// Canonical add: a + b if a < p - b, else a - (p - b); 32-bit carry chains.
+// Canonical add of canonical a, b: s = a + b and t = s + eps (= s - p mod
+// 2^64); the result is t when either sum carries out of 64 bits, else s.
+// Both candidates are computed and one is selected (no divergent branch).
 P3G_HD uint64_t gl_add(uint64_t a, uint64_t b) {
-  unsigned int b0, b1, bt0, bt1;
-  const uint32_t n0 = __builtin_subc(1u, (uint32_t)b, 0u, &b0);  // p - b, low word
-  const uint32_t n1 = __builtin_subc(0xFFFFFFFFu, (uint32_t)(b >> 32), b0, &b1);
-  const uint32_t t0 = __builtin_subc((uint32_t)a, n0, 0u, &bt0);
-  const uint32_t t1 = __builtin_subc((uint32_t)(a >> 32), n1, bt0, &bt1);
-  (void)b1;
-  return bt1 ? a + b : (((uint64_t)t1 << 32) | t0);
+  unsigned int c0, c1, d0, d1;
+  const uint32_t s0 = __builtin_addc((uint32_t)a, (uint32_t)b, 0u, &c0);
+  const uint32_t s1 = __builtin_addc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c0, &c1);
+  const uint32_t t0 = __builtin_addc(s0, 0xFFFFFFFFu, 0u, &d0);
+  const uint32_t t1 = __builtin_addc(s1, 0u, d0, &d1);
+  const bool over = (c1 | d1) != 0;
+  return ((uint64_t)(over ? t1 : s1) << 32) | (over ? t0 : s0);
 }
 
 P3G_HD uint64_t gl_sub(uint64_t a, uint64_t b) {

@@ -209,17 +209,19 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
 
   // inverse passes start in layout A, forward passes in layout B.  The first
   // forward pass reads its coefficient tile once and emits every coset from it.
+  // one branch around all 16 loads (columns past the width read as zero)
   uint64_t in[16];
+  if (live) {
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    const uint32_t x = inverse ? 16 * g + k : g + G * k;
-    const uint64_t idx = base + (uint64_t)x * xstride;
-    uint64_t val = 0;
-    if (live) {
-      if (MODE == INV_FIRST) val = a.src[(uint64_t)bitrev32((uint32_t)idx, a.logn) * a.w + col];
-      else val = a.dst[((MODE == FWD ? blk * a.n : 0) + idx) * a.w + col];
+    for (int k = 0; k < 16; k++) {
+      const uint32_t x = inverse ? 16 * g + k : g + G * k;
+      const uint64_t idx = base + (uint64_t)x * xstride;
+      if (MODE == INV_FIRST) in[k] = a.src[(uint64_t)bitrev32((uint32_t)idx, a.logn) * a.w + col];
+      else in[k] = a.dst[((MODE == FWD ? blk * a.n : 0) + idx) * a.w + col];
     }
-    in[k] = val;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; k++) in[k] = 0;
   }
   __syncthreads();  // twiddles ready
   const int nrep = MODE == FWD_FIRST ? (1 << a.logB) : 1;

@@ -22,7 +22,9 @@ def _mats(seed, shapes):
 
 # ---------------------------------------------------------------- coset LDE
 @pytest.mark.parametrize("logn,w,bits", [(0, 1, 1), (0, 5, 3), (1, 1, 1), (2, 3, 2), (3, 16, 1), (5, 17, 2),
-                                         (8, 4, 1), (9, 33, 1), (10, 64, 2), (12, 8, 3), (14, 32, 1), (16, 3, 2)])
+                                         (8, 4, 1), (9, 33, 1), (10, 64, 2), (12, 8, 3), (14, 32, 1), (16, 3, 2),
+                                         # two-pass transforms with 9..11-stage passes (8- and 4-column tiles)
+                                         (17, 8, 1), (18, 12, 2), (20, 16, 1), (21, 9, 1), (22, 8, 2)])
 def test_coset_lde_matches_oracle(device, logn, w, bits):
     tr = O.rand_matrix(1000 + logn, 1 << logn, w)
     shift = O.gl_pow(7, 1 + logn % 3)
@@ -221,6 +223,8 @@ PLAN_CASES = [
     (G.HASH_POSEIDON2, [(2, 1), (8, 20), (4096, 33)], 1, False, False),
     (G.HASH_POSEIDON2, [(1 << 12, 8), (1 << 13, 100), (1 << 14, 17)], 1, True, True),
     (G.HASH_KECCAK, [(64, 40), (256, 17), (256, 2)], 1, True, False),
+    (G.HASH_POSEIDON2, [(1 << 9, 5), (1 << 17, 8), (1 << 18, 12)], 1, True, True),
+    (G.HASH_POSEIDON2, [(1 << 17, 8), (1 << 18, 12)], 1, False, "per_trace"),
 ]
 
 
@@ -324,6 +328,7 @@ MANY_CASES = [
     (G.HASH_POSEIDON2, [(1 << 9, 9)], (1, 3, 2, 0, 0, 30, 0), 1),
     (G.HASH_KECCAK, [(32, 18), (128, 17)], (2, 2, 1, 0, 1, 25, 2), 3),
     (G.HASH_KECCAK, [(1 << 9, 40)], (1, 1, 2, 0, 0, 50, 0), 2),
+    (G.HASH_POSEIDON2, [(1 << 17, 8)], (1, 1, 3, 0, 0, 20, 0), 2),
 ]
 
 
