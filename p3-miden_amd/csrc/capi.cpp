@@ -61,6 +61,9 @@ int p3g_ctx_create(int device, p3g_ctx** out) {
   p3g_ctx* c = new p3g_ctx();
   int rc = guard([&] {
     c->c.device = device;
+    // the NTT kernels multiply by 16th roots of unity as shifts
+    if (gl_pow(2, OMEGA16_LOG2) != gl_two_adic_generator(4))
+      throw ApiError(P3G_E_INVAL, "omega_16 is not 2^OMEGA16_LOG2");
     P3G_CHECK_HIP(hipSetDevice(device));
     P3G_CHECK_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
     p2_generate_constants(2025, &c->c.host_p2);

@@ -88,6 +88,60 @@ P3G_HD uint64_t gl_mul(uint64_t a, uint64_t b) {
   mul128(a, b, lo, hi);
   return gl_reduce(lo, hi);
 }
+
+// x * 2^s for canonical x, 0 <= s < 192 (2^96 == -1, 2^192 == 1): shifts and
+// one reduction, no multiplies.  Written for s known at compile time (the
+// branches fold away).  For 64 < s < 96 with u = s - 64:
+// x 2^s = (lo_u + hi_u 2^64) 2^64 == lo_u 2^64 - hi_u 2^32, since 2^128 == -2^32.
+P3G_HD uint64_t gl_mul_pow2(uint64_t x, uint32_t s) {
+  s %= 192;
+  const bool neg = s >= 96;
+  s %= 96;
+  uint64_t r;
+  if (s == 0) r = x;
+  else if (s < 64) r = gl_reduce(x << s, x >> (64 - s));
+  else if (s == 64) r = gl_reduce(0, x);
+  else {
+    const uint32_t u = s - 64;
+    const uint64_t lo_u = x << u, hi_u = x >> (64 - u);
+    const uint64_t a = gl_reduce(0, lo_u), b = hi_u << 32;  // b < 2^64 - 2^32 < p
+    r = gl_sub(a, b);
+  }
+  return neg ? gl_neg(r) : r;
+}
+
+// Lazy sum of 128-bit products: value = lo + hi 2^64 + c 2^128, where c counts
+// the carries out of 128 bits.  A dot product pays one reduction at the end
+// instead of one per term.
+struct Acc192 {
+  uint64_t lo, hi;
+  uint32_t c;
+};
+P3G_HD Acc192 acc192_zero() { return Acc192{0, 0, 0}; }
+// a += x * y
+P3G_HD void acc192_mad(Acc192& a, uint64_t x, uint64_t y) {
+  uint64_t plo, phi;
+  mul128(x, y, plo, phi);
+  unsigned int c0, c1, c2, c3;
+  const uint32_t r0 = __builtin_addc((uint32_t)a.lo, (uint32_t)plo, 0u, &c0);
+  const uint32_t r1 = __builtin_addc((uint32_t)(a.lo >> 32), (uint32_t)(plo >> 32), c0, &c1);
+  const uint32_t r2 = __builtin_addc((uint32_t)a.hi, (uint32_t)phi, c1, &c2);
+  const uint32_t r3 = __builtin_addc((uint32_t)(a.hi >> 32), (uint32_t)(phi >> 32), c2, &c3);
+  a.lo = ((uint64_t)r1 << 32) | r0;
+  a.hi = ((uint64_t)r3 << 32) | r2;
+  a.c += c3;
+}
+// canonical value mod p; 2^128 == -2^32 and c 2^32 < p
+P3G_HD uint64_t acc192_reduce(const Acc192& a) { return gl_sub(gl_reduce(a.lo, a.hi), (uint64_t)a.c << 32); }
+
+// omega_16 = gl_two_adic_generator(4) = 2^156 (checked at context creation):
+// powers of the 16th roots of unity are shifts.
+constexpr uint32_t OMEGA16_LOG2 = 156;
+// x * omega_16^j (inverse: omega_16^-j)
+P3G_HD uint64_t gl_mul_w16(uint64_t x, uint32_t j, bool inverse) {
+  const uint32_t e = (OMEGA16_LOG2 * j) % 192;
+  return gl_mul_pow2(x, inverse ? (192 - e) % 192 : e);
+}
 P3G_HD uint64_t gl_sqr(uint64_t a) { return gl_mul(a, a); }
 
 P3G_HD uint64_t gl_pow(uint64_t a, uint64_t e) {

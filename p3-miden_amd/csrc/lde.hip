@@ -180,8 +180,11 @@ template <uint32_t MODE, int LR>
 __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
   constexpr uint32_t T = 1u << LR;      // threads = rows per tile
   constexpr uint32_t G = 1u << (LR - 4);  // row groups in layout A / row stride in layout B
+  constexpr int KB0 = 8 - LR;             // layout B: k bits below KB0 belong to layout-A stages
   __shared__ uint64_t tile[T * 16];
   __shared__ uint64_t tws[T];
+  __shared__ uint64_t tfa[16];  // layout-A twist factors (the same for every thread)
+  __shared__ uint64_t tfb[T];   // layout-B twist factors, entry 16 g + k
   const uint32_t tid = threadIdx.x;
   const uint32_t c = tid & 15, g = tid >> 4;
   constexpr bool inverse = MODE <= INV;
@@ -207,8 +210,6 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
     tws[e] = pw(a.tw, ex);
   }
 
-  // inverse passes start in layout A, forward passes in layout B.  The first
-  // forward pass reads its coefficient tile once and emits every coset from it.
   // one branch around all 16 loads (columns past the width read as zero)
   uint64_t in[16];
   if (live) {
@@ -224,6 +225,34 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
     for (int k = 0; k < 16; k++) in[k] = 0;
   }
   __syncthreads();  // twiddles ready
+
+  // Each group of stages on the 4 k bits is a twist by per-element factors
+  // followed by butterflies whose twiddles are 16th roots of unity (shifts):
+  // the stage-t twiddle tws[2^t - 1 + m] factors into omega_{2^(t+1)}^(high
+  // bits of m) times an entry that does not depend on the butterfly's active k
+  // bits, and those entries multiply out to one factor per element.  Inverse
+  // (DIT) groups twist before their butterflies, forward (DIF) groups after.
+  // Layout A: stages t = 0..3 are k bits 0..3, factor entries tws[2^t - 1].
+  // Layout B: stages t = 4..LR-1 are k bits KB0..3; k bits below KB0 (s) and
+  // the row group g select the entries tws[2^t - 1 + g + G s].
+  for (uint32_t e = tid; e < 16; e += T) {
+    uint64_t f = 1;
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++)
+      if ((e >> kb) & 1) f = gl_mul(f, tws[(1u << kb) - 1]);
+    tfa[e] = f;
+  }
+  {
+    const uint32_t gq = tid >> 4, k = tid & 15;  // T = 16 G entries, one per thread
+    const uint32_t sb = k & ((1u << KB0) - 1);
+    uint64_t f = 1;
+#pragma unroll
+    for (int kb = KB0; kb < 4; kb++)
+      if ((k >> kb) & 1) f = gl_mul(f, tws[(1u << (kb + LR - 4)) - 1 + gq + G * sb]);
+    tfb[tid] = f;
+  }
+  __syncthreads();  // twist factors ready
+
   const int nrep = MODE == FWD_FIRST ? (1 << a.logB) : 1;
   // cosets in descending order (block 0 last, as in every pass order)
   for (int rep = nrep - 1; rep >= 0; rep--) {
@@ -246,13 +275,15 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
     if (inverse) {
       // stages on x bits 0..3 (k bits), layout A
 #pragma unroll
+      for (int k = 1; k < 16; k++) r[k] = gl_mul(r[k], tfa[k]);
+#pragma unroll
       for (int t = 0; t < 4; t++) {
 #pragma unroll
         for (int k0 = 0; k0 < 16; k0++) {
           if (k0 & (1 << t)) continue;
           const int k1 = k0 | (1 << t);
-          const uint64_t tw = tws[(1 << t) - 1 + (k0 & ((1 << t) - 1))];
-          const uint64_t x = r[k0], y = gl_mul(r[k1], tw);
+          const uint32_t jw = (uint32_t)(k0 & ((1 << t) - 1)) << (3 - t);
+          const uint64_t x = r[k0], y = gl_mul_w16(r[k1], jw, true);
           r[k0] = gl_add(x, y);
           r[k1] = gl_sub(x, y);
         }
@@ -264,14 +295,17 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
       for (int k = 0; k < 16; k++) r[k] = tile[swz(g + G * k, c)];
       // stages on x bits 4..LR-1, layout B (x bit t is k bit t - (LR - 4))
 #pragma unroll
+      for (int k = 0; k < 16; k++)
+        if (k >> KB0) r[k] = gl_mul(r[k], tfb[16 * g + k]);
+#pragma unroll
       for (int t = 4; t < LR; t++) {
         const int kb = t - (LR - 4);
 #pragma unroll
         for (int k0 = 0; k0 < 16; k0++) {
           if (k0 & (1 << kb)) continue;
           const int k1 = k0 | (1 << kb);
-          const uint64_t tw = tws[(1 << t) - 1 + g + G * (k0 & ((1 << kb) - 1))];
-          const uint64_t x = r[k0], y = gl_mul(r[k1], tw);
+          const uint32_t jw = (uint32_t)((k0 >> KB0) & ((1 << (kb - KB0)) - 1)) << (3 - kb + KB0);
+          const uint64_t x = r[k0], y = gl_mul_w16(r[k1], jw, true);
           r[k0] = gl_add(x, y);
           r[k1] = gl_sub(x, y);
         }
@@ -290,12 +324,15 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
         for (int k0 = 0; k0 < 16; k0++) {
           if (k0 & (1 << kb)) continue;
           const int k1 = k0 | (1 << kb);
-          const uint64_t tw = tws[(1 << q) - 1 + g + G * (k0 & ((1 << kb) - 1))];
+          const uint32_t jw = (uint32_t)((k0 >> KB0) & ((1 << (kb - KB0)) - 1)) << (3 - kb + KB0);
           const uint64_t x = r[k0], y = r[k1];
           r[k0] = gl_add(x, y);
-          r[k1] = gl_mul(gl_sub(x, y), tw);
+          r[k1] = gl_mul_w16(gl_sub(x, y), jw, false);
         }
       }
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        if (k >> KB0) r[k] = gl_mul(r[k], tfb[16 * g + k]);
       __syncthreads();  // previous repetition's readers are done with the tile
 #pragma unroll
       for (int k = 0; k < 16; k++) tile[swz(g + G * k, c)] = r[k];
@@ -308,12 +345,14 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
         for (int k0 = 0; k0 < 16; k0++) {
           if (k0 & (1 << q)) continue;
           const int k1 = k0 | (1 << q);
-          const uint64_t tw = tws[(1 << q) - 1 + (k0 & ((1 << q) - 1))];
+          const uint32_t jw = (uint32_t)(k0 & ((1 << q) - 1)) << (3 - q);
           const uint64_t x = r[k0], y = r[k1];
           r[k0] = gl_add(x, y);
-          r[k1] = gl_mul(gl_sub(x, y), tw);
+          r[k1] = gl_mul_w16(gl_sub(x, y), jw, false);
         }
       }
+#pragma unroll
+      for (int k = 1; k < 16; k++) r[k] = gl_mul(r[k], tfa[k]);
       const uint64_t ob = MODE == FWD_FIRST ? bitrev32((uint32_t)rep, a.logB) : (MODE == FWD ? blk : 0);
       // the row stride is made opaque inside the coset loop so that the 16 store
       // addresses are formed per coset instead of being held across it

@@ -129,18 +129,23 @@ __device__ __forceinline__ void deep_colsum_body(const uint64_t* __restrict__ M,
   const uint64_t col = (uint64_t)bx * CS_COLS + cl;
   const uint64_t r0 = (uint64_t)by * rows_per_block;
   const uint64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
-  ef2 acc[N];
+  // unreduced sums of the two extension coordinates per point, reduced once
+  Acc192 a0[N], a1[N];
 #pragma unroll
-  for (int j = 0; j < N; j++) acc[j] = ef_from(0);
+  for (int j = 0; j < N; j++) a0[j] = a1[j] = acc192_zero();
   if (col < w) {
     for (uint64_t r = r0 + rs; r < r1; r += CS_SLICES) {
       uint64_t v = M[r * w + col];
 #pragma unroll
-      for (int j = 0; j < N; j++) acc[j] = ef_add(acc[j], ef_mul_base(W[r * N + j], v));
+      for (int j = 0; j < N; j++) {
+        const ef2 wt = W[r * N + j];
+        acc192_mad(a0[j], wt.c0, v);
+        acc192_mad(a1[j], wt.c1, v);
+      }
     }
   }
 #pragma unroll
-  for (int j = 0; j < N; j++) red[rs][cl][j] = acc[j];
+  for (int j = 0; j < N; j++) red[rs][cl][j] = ef2{acc192_reduce(a0[j]), acc192_reduce(a1[j])};
   __syncthreads();
   if (rs == 0 && col < w) {
 #pragma unroll
@@ -223,7 +228,8 @@ __device__ __forceinline__ void deep_quotient_acc(const QuotArgs& a, uint64_t wb
   const uint32_t tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < DQ_CH; r++) {
-    ef2 s = ef_from(0);
+    // unreduced sums of the two extension coordinates, reduced once per row
+    Acc192 s0 = acc192_zero(), s1 = acc192_zero();
     uint32_t g = 0;
 #pragma unroll 1
     for (uint32_t m = 0; m < a.nmats; m++) {
@@ -245,11 +251,15 @@ __device__ __forceinline__ void deep_quotient_acc(const QuotArgs& a, uint64_t wb
         }
         __syncthreads();
 #pragma unroll 4
-        for (uint32_t cc = 0; cc < nc; cc++)
-          s = ef_add(s, ef_mul_base(cf[c0 + cc], tile[tid * 16 + (cc ^ (tid & 15))]));
+        for (uint32_t cc = 0; cc < nc; cc++) {
+          const uint64_t v = tile[tid * 16 + (cc ^ (tid & 15))];
+          const ef2 k = cf[c0 + cc];
+          acc192_mad(s0, k.c0, v);
+          acc192_mad(s1, k.c1, v);
+        }
       }
     }
-    acc[r] = s;
+    acc[r] = ef2{acc192_reduce(s0), acc192_reduce(s1)};
   }
 }
 
