@@ -129,7 +129,24 @@ P3G_HD void p2_permute(uint64_t* s, const P2Consts* __restrict__ c) {
 
 // ---------------------------------------------------------------- Keccak-f[1600]
 
-P3G_HD uint64_t rotl64(uint64_t x, int k) { return k ? (x << k) | (x >> (64 - k)) : x; }
+#if defined(__HIP_DEVICE_COMPILE__)
+// Device: a 64-bit rotation is two 32-bit funnel shifts (v_alignbit_b32)
+// instead of two 64-bit shifts and two ORs.  k is a compile-time constant at
+// every call site, so the branches fold.
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) {
+  const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  if (k == 0) return x;
+  if (k == 32) return ((uint64_t)lo << 32) | hi;
+  // rotating by k >= 32 is rotating the half-swapped word by k - 32
+  const uint32_t h = k < 32 ? hi : lo, l = k < 32 ? lo : hi;
+  const uint32_t m = (uint32_t)(k < 32 ? k : k - 32);
+  const uint32_t nh = __builtin_amdgcn_alignbit(h, l, 32 - m);
+  const uint32_t nl = __builtin_amdgcn_alignbit(l, h, 32 - m);
+  return ((uint64_t)nh << 32) | nl;
+}
+#else
+inline uint64_t rotl64(uint64_t x, int k) { return k ? (x << k) | (x >> (64 - k)) : x; }
+#endif
 
 P3G_HD void keccak_f(uint64_t* a) {
   const uint64_t RC[24] = {
