@@ -410,8 +410,7 @@ void CommitPlan::run(hipStream_t s) {
   } else {
     enqueue(s);
   }
-  P3G_CHECK_HIP(hipMemcpyAsync(root, digests + layer_off[depth] * 4, 32, hipMemcpyDeviceToHost, s));
-  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx->read_back(root, digests + layer_off[depth] * 4, 32, s);
 }
 
 Tree* CommitPlan::tree_view() {

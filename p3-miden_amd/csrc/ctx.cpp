@@ -281,6 +281,28 @@ void* Ctx::upload(const void* src, size_t bytes, hipStream_t s) {
   return d;
 }
 
+void Ctx::read_back(void* dst, const void* dsrc, size_t bytes, hipStream_t s) {
+  if (bytes == 0) {
+    P3G_CHECK_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  if (bytes > rb_bytes) {
+    if (rb_pinned) (void)hipHostFree(rb_pinned);
+    rb_pinned = nullptr;
+    rb_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 64 << 10);
+    if (hipHostMalloc((void**)&rb_pinned, want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      rb_pinned = nullptr;
+      throw ApiError(E_NOMEM, "pinned host allocation failed");
+    }
+    rb_bytes = want;
+  }
+  P3G_CHECK_HIP(hipMemcpyAsync(rb_pinned, dsrc, bytes, hipMemcpyDeviceToHost, s));
+  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  memcpy(dst, rb_pinned, bytes);
+}
+
 void Ctx::free_upload_ring() {
   std::lock_guard<std::mutex> g(up_mu);
   for (const UpSeg& f : up_inflight) {
@@ -293,6 +315,9 @@ void Ctx::free_upload_ring() {
   if (up_ring) (void)hipHostFree(up_ring);
   up_ring = nullptr;
   up_cap = up_head = 0;
+  if (rb_pinned) (void)hipHostFree(rb_pinned);
+  rb_pinned = nullptr;
+  rb_bytes = 0;
 }
 
 void* Ctx::alloc(size_t bytes) {

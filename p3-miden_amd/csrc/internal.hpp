@@ -136,6 +136,13 @@ struct Ctx {
   void* upload(const void* src, size_t bytes, hipStream_t s);
   void free_upload_ring();
 
+  // Small device-to-host reads (roots, evaluations, final polynomials) go
+  // through their own pinned buffer: the copy is a plain async DMA and the
+  // round trip is shorter than through pageable memory.  Synchronises s.
+  uint8_t* rb_pinned = nullptr;
+  size_t rb_bytes = 0;
+  void read_back(void* dst, const void* dsrc, size_t bytes, hipStream_t s);
+
   void* alloc(size_t bytes);
   void release(void* p);
   void trim();

@@ -539,8 +539,7 @@ Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint3
     i = j;
   }
   build_layers(ctx, *t, s);
-  P3G_CHECK_HIP(hipMemcpyAsync(t->root, t->digests + t->layer_off[t->depth] * 4, 32, hipMemcpyDeviceToHost, s));
-  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx.read_back(t->root, t->digests + t->layer_off[t->depth] * 4, 32, s);
   ctx.release(states[0]);
   ctx.release(states[1]);
   return t;

@@ -474,9 +474,7 @@ static void run_deep_kernels(Ctx& ctx, const PcsParams& p, unsigned logL, const 
     off += m.width;
   }
   evals_host.resize(totw * N);
-  if (totw)
-    P3G_CHECK_HIP(hipMemcpyAsync(evals_host.data(), evals_dev, totw * N * sizeof(ef2), hipMemcpyDeviceToHost, s));
-  P3G_CHECK_HIP(hipStreamSynchronize(s));
+  ctx.read_back(evals_host.data(), evals_dev, totw * N * sizeof(ef2), s);
   ctx.release(evals_dev);
   for (auto& kv : wts) ctx.release(kv.second);
   // barycentric scaling s_j = ((z_j / g)^d - 1) / d
@@ -701,8 +699,7 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
     }
     // final polynomial: first fdeg evaluations, natural order, inverse DFT, descending
     std::vector<ef2> fin(fdeg);
-    P3G_CHECK_HIP(hipMemcpyAsync(fin.data(), cur, fdeg * sizeof(ef2), hipMemcpyDeviceToHost, s));
-    P3G_CHECK_HIP(hipStreamSynchronize(s));
+    ctx.read_back(fin.data(), cur, fdeg * sizeof(ef2), s);
     {
       const unsigned lf = log2_exact(fdeg);
       std::vector<ef2> nat(fdeg);
@@ -932,9 +929,7 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
       }
     }
     std::vector<ef2> evals_host(k * totw * N);
-    if (totw)
-      P3G_CHECK_HIP(hipMemcpyAsync(evals_host.data(), evals_dev, evals_host.size() * sizeof(ef2), hipMemcpyDeviceToHost, s));
-    P3G_CHECK_HIP(hipStreamSynchronize(s));
+    ctx.read_back(evals_host.data(), evals_dev, evals_host.size() * sizeof(ef2), s);
 
     // ---- per job: scale, send, grind, alpha and beta; quotient arguments
     const uint64_t ginv = gl_inv(GENERATOR), dinv = gl_inv_pow2(logd);
