@@ -55,6 +55,16 @@ P3G_HD uint64_t gl_neg(uint64_t a) { return a ? P - a : 0; }
 // carry chains; the final carry (+2^64) and borrow (-2^64) fold as +-eps.
 // Returns a value < 2^64 that is not necessarily < p.
 P3G_HD uint64_t gl_reduce_nc(uint64_t lo, uint64_t hi) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+  // host: the same identity on 64-bit words (each fold cannot wrap twice)
+  const uint64_t hh = hi >> 32, hl = hi & EPS;
+  uint64_t t = lo - hh;
+  if (lo < hh) t -= EPS;
+  const uint64_t t2 = (hl << 32) - hl;
+  uint64_t r = t + t2;
+  if (r < t2) r += EPS;
+  return r;
+#endif
   const uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32);
   const uint32_t h0 = (uint32_t)hi, h1 = (uint32_t)(hi >> 32);
   unsigned int sc, b0, b1, c1;
@@ -73,14 +83,21 @@ P3G_HD uint64_t gl_reduce(uint64_t lo, uint64_t hi) {
   return r >= P ? r - P : r;
 }
 
-// 64 x 64 -> 128-bit product from four 32 x 32 -> 64 multiply-adds.
+// 64 x 64 -> 128-bit product: on the device from four 32 x 32 -> 64
+// multiply-adds (v_mad_u64_u32), on the host one widening multiply.
 P3G_HD void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
   const uint64_t p00 = (uint64_t)a0 * b0;
   const uint64_t p01 = (uint64_t)a0 * b1 + (p00 >> 32);
   const uint64_t p10 = (uint64_t)a1 * b0 + (uint32_t)p01;
   hi = (uint64_t)a1 * b1 + (p01 >> 32) + (p10 >> 32);
   lo = (p10 << 32) | (uint32_t)p00;
+#else
+  const unsigned __int128 p = (unsigned __int128)a * b;
+  lo = (uint64_t)p;
+  hi = (uint64_t)(p >> 64);
+#endif
 }
 
 P3G_HD uint64_t gl_mul(uint64_t a, uint64_t b) {

@@ -277,7 +277,9 @@ size_t prove_batch_pieces(const Tree& t, const std::vector<uint64_t>& indices,
                           std::vector<std::pair<const uint64_t*, uint64_t>>& pieces);
 size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces, size_t first,
                         size_t last, size_t nrows, const uint64_t* raw, Channel& tx);
-Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s);
+// read_root = false: no stream synchronisation; the root stays on the device
+Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s,
+                 bool read_root = true);
 void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out);
 void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s);
 // gathers the pieces into the context's pinned staging buffer; the returned

@@ -491,7 +491,8 @@ void launch_leaf_raw(uint32_t hash, int threads, const void* mats, uint32_t nmat
   P3G_CHECK_HIP(hipGetLastError());
 }
 
-Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s) {
+Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s,
+                 bool read_root) {
   if (mats.empty()) throw ApiError(E_EMPTY, "cannot commit empty batch");
   uint64_t prevh = 0;
   for (auto& m : mats) {
@@ -539,7 +540,9 @@ Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint3
     i = j;
   }
   build_layers(ctx, *t, s);
-  ctx.read_back(t->root, t->digests + t->layer_off[t->depth] * 4, 32, s);
+  // without the read-back nothing here waits for the stream (the releases
+  // below are stream-ordered); t->root is then filled in by the caller
+  if (read_root) ctx.read_back(t->root, t->digests + t->layer_off[t->depth] * 4, 32, s);
   ctx.release(states[0]);
   ctx.release(states[1]);
   return t;
@@ -850,7 +853,7 @@ void host_sponge_absorb(uint32_t hash, uint64_t* state, const uint64_t* in, size
   while (j < len) {
     size_t rem = len - j;
     for (size_t k = 0; k < rate; k++) state[k] = k < rem ? in[j + k] : 0;
-    if (hash == HASH_P2) p2_permute(state, &host_p2_consts());
+    if (hash == HASH_P2) p2_permute_fast(state, &host_p2_consts());
     else keccak_f(state);
     j += rate < rem ? rate : rem;
   }
@@ -860,7 +863,7 @@ void host_compress(uint32_t hash, const uint64_t* l, const uint64_t* r, uint64_t
   uint64_t s[25] = {};
   memcpy(s, l, 32);
   memcpy(s + 4, r, 32);
-  if (hash == HASH_P2) p2_permute(s, &host_p2_consts());
+  if (hash == HASH_P2) p2_permute_fast(s, &host_p2_consts());
   else keccak_f(s);
   memcpy(out, s, 32);
 }

@@ -23,6 +23,7 @@
 #include <map>
 
 #include "internal.hpp"
+#include "p2_coop.hpp"
 
 namespace p3g {
 
@@ -394,6 +395,75 @@ __global__ void fri_fold_kernel(const ef2* __restrict__ in, ef2* __restrict__ ou
   fri_fold_body<LA>(in, out, nrows, log_rows, powLinv, exp_shift, beta, tw, ainv,
                     (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
+// beta read from device memory (written there by fs_root_beta_kernel)
+template <int LA>
+__global__ void fri_fold_dev_kernel(const ef2* __restrict__ in, ef2* __restrict__ out, uint64_t nrows,
+                                    unsigned log_rows, const uint64_t* __restrict__ powLinv, unsigned exp_shift,
+                                    const ef2* __restrict__ beta, FoldTw tw, uint64_t ainv) {
+  fri_fold_body<LA>(in, out, nrows, log_rows, powLinv, exp_shift, *beta, tw, ainv,
+                    (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The Poseidon2 duplex challenger (Transcript with CH_DUPLEX_P2) as device
+// words: rate-8 input buffer, 8-word output buffer, 12-word sponge state.
+struct DevSponge {
+  uint64_t st[12];
+  uint64_t in[8];
+  uint64_t out[8];
+  uint64_t in_len, out_len;
+};
+
+// One FRI commit round of the transcript on the device (16 threads, one DPP
+// row): send_commitment(root) then beta = sample_ef(), with exactly the
+// host's Transcript::observe / sample / duplex steps (grind with 0 bits adds
+// nothing to the sponge).  Lane l holds sponge word l; the buffers and their
+// lengths are uniform.
+__global__ void __launch_bounds__(16) fs_root_beta_kernel(DevSponge* __restrict__ sp,
+                                                          const uint64_t* __restrict__ root, ef2* __restrict__ beta,
+                                                          const P2Consts* __restrict__ p2) {
+  __shared__ uint64_t IN[8], OUT[8];
+  const uint32_t l = threadIdx.x;
+  uint64_t x = l < 12 ? sp->st[l] : 0;
+  if (l < 8) {
+    IN[l] = sp->in[l];
+    OUT[l] = sp->out[l];
+  }
+  int inl = (int)sp->in_len, outl = (int)sp->out_len;
+  __syncthreads();
+  auto duplex = [&]() {
+    if (l < 8 && (int)l < inl) x = IN[l];  // overwrite the rate with the pending input
+    inl = 0;
+    x = p2_permute_coop(x, l, p2);
+    if (l < 8) OUT[l] = x;
+    outl = 8;
+    __syncthreads();
+  };
+  for (int k = 0; k < 4; k++) {
+    const uint64_t v = root[k];
+    outl = 0;
+    if (l == 0) IN[inl] = v;
+    __syncthreads();
+    inl++;
+    if (inl == 8) duplex();
+  }
+  uint64_t b[2];
+  for (int q = 0; q < 2; q++) {
+    if (inl > 0 || outl == 0) duplex();
+    b[q] = OUT[--outl];
+  }
+  __syncthreads();
+  if (l == 0) *beta = ef2{b[0], b[1]};
+  if (l < 12) sp->st[l] = x;
+  if (l < 8) {
+    sp->in[l] = IN[l];
+    sp->out[l] = OUT[l];
+  }
+  if (l == 0) {
+    sp->in_len = (uint64_t)inl;
+    sp->out_len = (uint64_t)outl;
+  }
+}
+
 struct FoldJob {
   const ef2* in;
   ef2* out;
@@ -660,7 +730,28 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
   std::vector<ef2*> bufs;
   ef2* cur = Q;
   uint64_t domain = L;
+  // Fiat-Shamir of the commit rounds on the device, for this library's own
+  // Poseidon2 duplex transcript without folding grinding: no host round trip
+  // per round.  The host replays the rounds afterwards from the read-back
+  // roots and checks every beta against the device's.
+  Transcript* dtx = dynamic_cast<Transcript*>(&tx);
+  const bool dev_fs = dtx && dtx->kind == CH_DUPLEX_P2 && p.folding_pow_bits == 0;
+  DevSponge* dsp = nullptr;
+  ef2* dbeta = nullptr;
   try {
+    if (dev_fs) {
+      size_t nr = 0;
+      for (uint64_t d = L; d > fdomain; d >>= la) nr++;
+      DevSponge hs{};
+      memcpy(hs.st, dtx->st, sizeof(hs.st));
+      memcpy(hs.in, dtx->in, sizeof(hs.in));
+      memcpy(hs.out, dtx->out, sizeof(hs.out));
+      hs.in_len = (uint64_t)dtx->in_len;
+      hs.out_len = (uint64_t)dtx->out_len;
+      dsp = (DevSponge*)ctx.upload(&hs, sizeof(hs), s);
+      dbeta = (ef2*)ctx.alloc((nr ? nr : 1) * sizeof(ef2));
+    }
+    size_t r = 0;
     while (domain > fdomain) {
       const uint64_t fd = domain >> la;
       DevMat m;
@@ -668,38 +759,79 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
       m.height = fd;
       m.width = 2 * arity;
       m.owned = false;
-      Tree* t = lmcs_build(ctx, hash, {m}, 1, s);
+      Tree* t = lmcs_build(ctx, hash, {m}, 1, s, !dev_fs);
       ftrees.push_back(t);
       bufs.push_back(cur);
-      tx.send_commitment(t->root);
-      tx.grind(p.folding_pow_bits);
-      const ef2 b = tx.sample_ef();
+      ef2 b{0, 0};
+      if (dev_fs) {
+        hipLaunchKernelGGL(fs_root_beta_kernel, dim3(1), dim3(16), 0, s, dsp, t->digests + t->layer_off[t->depth] * 4,
+                           dbeta + r, ctx.dev_p2);
+        P3G_CHECK_HIP(hipGetLastError());
+      } else {
+        tx.send_commitment(t->root);
+        tx.grind(p.folding_pow_bits);
+        b = tx.sample_ef();
+      }
       ef2* nxt = (ef2*)ctx.alloc(fd * sizeof(ef2));
       const unsigned log_rows = log2_exact(fd);
       const unsigned exp_shift = logL - log2_exact(domain);
       const uint32_t grid = (uint32_t)((fd + 255) / 256);
       ProfScope ps(ctx, "fri_fold", s);
+      const ef2* db = dbeta + r;
       switch (la) {
         case 1:
-          hipLaunchKernelGGL(fri_fold_kernel<1>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
-                             exp_shift, b, tw, ainv);
+          if (dev_fs)
+            hipLaunchKernelGGL(fri_fold_dev_kernel<1>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                               exp_shift, db, tw, ainv);
+          else
+            hipLaunchKernelGGL(fri_fold_kernel<1>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                               exp_shift, b, tw, ainv);
           break;
         case 2:
-          hipLaunchKernelGGL(fri_fold_kernel<2>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
-                             exp_shift, b, tw, ainv);
+          if (dev_fs)
+            hipLaunchKernelGGL(fri_fold_dev_kernel<2>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                               exp_shift, db, tw, ainv);
+          else
+            hipLaunchKernelGGL(fri_fold_kernel<2>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                               exp_shift, b, tw, ainv);
           break;
         default:
-          hipLaunchKernelGGL(fri_fold_kernel<3>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
-                             exp_shift, b, tw, ainv);
+          if (dev_fs)
+            hipLaunchKernelGGL(fri_fold_dev_kernel<3>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                               exp_shift, db, tw, ainv);
+          else
+            hipLaunchKernelGGL(fri_fold_kernel<3>, dim3(grid), dim3(256), 0, s, cur, nxt, fd, log_rows, powLinv,
+                               exp_shift, b, tw, ainv);
           break;
       }
       P3G_CHECK_HIP(hipGetLastError());
       cur = nxt;
       domain = fd;
+      r++;
     }
     // final polynomial: first fdeg evaluations, natural order, inverse DFT, descending
     std::vector<ef2> fin(fdeg);
-    ctx.read_back(fin.data(), cur, fdeg * sizeof(ef2), s);
+    if (dev_fs) {
+      // one read-back for every round's root and beta and the final evaluations
+      const size_t nr = ftrees.size();
+      std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+      for (auto* t : ftrees) pieces.emplace_back(t->digests + t->layer_off[t->depth] * 4, 4);
+      pieces.emplace_back(reinterpret_cast<const uint64_t*>(dbeta), 2 * nr);
+      pieces.emplace_back(reinterpret_cast<const uint64_t*>(cur), 2 * fdeg);
+      const uint64_t* raw = gather_pinned(ctx, pieces, s);
+      for (size_t q = 0; q < nr; q++) {
+        memcpy(ftrees[q]->root, raw + 4 * q, 32);
+        tx.send_commitment(ftrees[q]->root);
+        tx.grind(p.folding_pow_bits);
+        const ef2 hb = tx.sample_ef();
+        const uint64_t* dbq = raw + 4 * nr + 2 * q;
+        if (hb.c0 != dbq[0] || hb.c1 != dbq[1])
+          throw ApiError(E_TRANSCRIPT, "device transcript diverged from the host replay");
+      }
+      memcpy(fin.data(), raw + 6 * nr, fdeg * sizeof(ef2));
+    } else {
+      ctx.read_back(fin.data(), cur, fdeg * sizeof(ef2), s);
+    }
     {
       const unsigned lf = log2_exact(fdeg);
       std::vector<ef2> nat(fdeg);
@@ -740,10 +872,14 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
     for (auto* t : ftrees) delete t;
     for (auto* b : bufs) ctx.release(b);
     if (cur) ctx.release(cur);
+    ctx.release(dsp);
+    ctx.release(dbeta);
     throw;
   }
   for (auto* t : ftrees) delete t;
   for (auto* b : bufs) ctx.release(b);
+  ctx.release(dsp);
+  ctx.release(dbeta);
 }
 
 
