@@ -272,7 +272,7 @@ void coset_lde_many(Ctx& ctx, const std::vector<LdeJob>& jobs, unsigned log_blow
 // trees of k jobs with identical matrix shapes, built with shared launches; the
 // trees take ownership of the matrices whose DevMat::owned is set
 std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<std::vector<DevMat>>& mats,
-                                   uint32_t alignment, hipStream_t s);
+                                   uint32_t alignment, hipStream_t s, bool read_roots = true);
 size_t prove_batch_pieces(const Tree& t, const std::vector<uint64_t>& indices,
                           std::vector<std::pair<const uint64_t*, uint64_t>>& pieces);
 size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces, size_t first,

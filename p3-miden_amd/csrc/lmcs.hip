@@ -549,7 +549,7 @@ Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint3
 }
 
 std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<std::vector<DevMat>>& mats,
-                                   uint32_t alignment, hipStream_t s) {
+                                   uint32_t alignment, hipStream_t s, bool read_roots) {
   const size_t k = mats.size();
   std::vector<Tree*> trees;
   if (k == 0) return trees;
@@ -708,12 +708,15 @@ std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<st
                            layer_off[level_off.size()], len, ctx.dev_p2);
       P3G_CHECK_HIP(hipGetLastError());
     }
-    // roots
-    std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
-    for (size_t j = 0; j < k; j++) pieces.emplace_back(trees[j]->digests + layer_off[depth] * 4, 4);
-    std::vector<uint64_t> raw;
-    gather_to_host(ctx, pieces, raw, s);  // synchronises the stream
-    for (size_t j = 0; j < k; j++) memcpy(trees[j]->root, raw.data() + 4 * j, 32);
+    // roots (without the read-back the roots stay on the device and nothing
+    // here waits for the stream)
+    if (read_roots) {
+      std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
+      for (size_t j = 0; j < k; j++) pieces.emplace_back(trees[j]->digests + layer_off[depth] * 4, 4);
+      std::vector<uint64_t> raw;
+      gather_to_host(ctx, pieces, raw, s);  // synchronises the stream
+      for (size_t j = 0; j < k; j++) memcpy(trees[j]->root, raw.data() + 4 * j, 32);
+    }
   } catch (...) {
     (void)hipStreamSynchronize(s);
     for (auto* p : scratch) ctx.release(p);

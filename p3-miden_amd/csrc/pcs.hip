@@ -418,9 +418,8 @@ struct DevSponge {
 // host's Transcript::observe / sample / duplex steps (grind with 0 bits adds
 // nothing to the sponge).  Lane l holds sponge word l; the buffers and their
 // lengths are uniform.
-__global__ void __launch_bounds__(16) fs_root_beta_kernel(DevSponge* __restrict__ sp,
-                                                          const uint64_t* __restrict__ root, ef2* __restrict__ beta,
-                                                          const P2Consts* __restrict__ p2) {
+__device__ __forceinline__ void fs_root_beta(DevSponge* __restrict__ sp, const uint64_t* __restrict__ root,
+                                             ef2* __restrict__ beta, const P2Consts* __restrict__ p2) {
   __shared__ uint64_t IN[8], OUT[8];
   const uint32_t l = threadIdx.x;
   uint64_t x = l < 12 ? sp->st[l] : 0;
@@ -463,12 +462,33 @@ __global__ void __launch_bounds__(16) fs_root_beta_kernel(DevSponge* __restrict_
     sp->out_len = (uint64_t)outl;
   }
 }
+__global__ void __launch_bounds__(16) fs_root_beta_kernel(DevSponge* __restrict__ sp,
+                                                          const uint64_t* __restrict__ root, ef2* __restrict__ beta,
+                                                          const P2Consts* __restrict__ p2) {
+  fs_root_beta(sp, root, beta, p2);
+}
+// one 16-lane workgroup per job: sponge j, root roots[j], beta betas[j]
+__global__ void __launch_bounds__(16) fs_root_beta_many_kernel(DevSponge* __restrict__ sps,
+                                                               const uint64_t* const* __restrict__ roots,
+                                                               ef2* __restrict__ betas,
+                                                               const P2Consts* __restrict__ p2) {
+  fs_root_beta(sps + blockIdx.x, roots[blockIdx.x], betas + blockIdx.x, p2);
+}
 
 struct FoldJob {
   const ef2* in;
   ef2* out;
   ef2 beta;
 };
+// as fri_fold_many_kernel, with job j's beta read from betas[j]
+template <int LA>
+__global__ void fri_fold_many_dev_kernel(const FoldJob* __restrict__ jobs, const ef2* __restrict__ betas, uint64_t nrows,
+                                         unsigned log_rows, const uint64_t* __restrict__ powLinv, unsigned exp_shift,
+                                         FoldTw tw, uint64_t ainv) {
+  const FoldJob j = jobs[blockIdx.y];
+  fri_fold_body<LA>(j.in, j.out, nrows, log_rows, powLinv, exp_shift, betas[blockIdx.y], tw, ainv,
+                    (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
 template <int LA>
 __global__ void fri_fold_many_kernel(const FoldJob* __restrict__ jobs, uint64_t nrows, unsigned log_rows,
                                      const uint64_t* __restrict__ powLinv, unsigned exp_shift, FoldTw tw,
@@ -1173,6 +1193,30 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
     const uint64_t ainv = gl_inv_pow2(la);
     std::vector<ef2*> cur = Q;
     uint64_t domain = L;
+    // Fiat-Shamir of the commit rounds on the device (one sponge and one
+    // 16-lane row per job), as in the single-proof path; the hosts replay
+    // every job's rounds afterwards and check every beta.
+    bool dev_fs = p.folding_pow_bits == 0;
+    for (size_t j = 0; j < k; j++) dev_fs = dev_fs && txs[j]->kind == CH_DUPLEX_P2;
+    size_t nr = 0;
+    for (uint64_t d = L; d > fdomain; d >>= la) nr++;
+    DevSponge* dsp = nullptr;
+    ef2* dbeta = nullptr;
+    if (dev_fs) {
+      std::vector<DevSponge> hs(k);
+      for (size_t j = 0; j < k; j++) {
+        const Transcript& tx = *txs[j];
+        memcpy(hs[j].st, tx.st, sizeof(hs[j].st));
+        memcpy(hs[j].in, tx.in, sizeof(hs[j].in));
+        memcpy(hs[j].out, tx.out, sizeof(hs[j].out));
+        hs[j].in_len = (uint64_t)tx.in_len;
+        hs[j].out_len = (uint64_t)tx.out_len;
+      }
+      dsp = (DevSponge*)upload(hs.data(), k * sizeof(DevSponge));
+      dbeta = (ef2*)ctx.alloc((nr ? nr : 1) * k * sizeof(ef2));
+      tmp.push_back(dbeta);
+    }
+    size_t r = 0;
     while (domain > fdomain) {
       const uint64_t fd = domain >> la;
       std::vector<std::vector<DevMat>> fm(k);
@@ -1184,7 +1228,7 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
         m.owned = false;
         fm[j].push_back(m);
       }
-      ftrees.push_back(lmcs_build_many(ctx, hash, fm, 1, s));
+      ftrees.push_back(lmcs_build_many(ctx, hash, fm, 1, s, !dev_fs));
       std::vector<FoldJob> fj(k);
       std::vector<ef2*> nxt(k);
       for (size_t j = 0; j < k; j++) {
@@ -1192,12 +1236,24 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
         bufs.push_back(nxt[j]);
       }
       const std::vector<Tree*>& ft = ftrees.back();
-      parallel_for(k, [&](size_t j) {
-        Transcript& tx = *txs[j];
-        tx.send_commitment(ft[j]->root);
-        tx.grind(p.folding_pow_bits);
-        fj[j] = FoldJob{cur[j], nxt[j], tx.sample_ef()};
-      });
+      ef2* db = dbeta + r * k;
+      if (dev_fs) {
+        std::vector<const uint64_t*> rp(k);
+        for (size_t j = 0; j < k; j++) {
+          rp[j] = ft[j]->digests + ft[j]->layer_off[ft[j]->depth] * 4;
+          fj[j] = FoldJob{cur[j], nxt[j], ef2{0, 0}};
+        }
+        const uint64_t* const* drp = (const uint64_t* const*)upload(rp.data(), k * sizeof(uint64_t*));
+        hipLaunchKernelGGL(fs_root_beta_many_kernel, dim3((uint32_t)k), dim3(16), 0, s, dsp, drp, db, ctx.dev_p2);
+        P3G_CHECK_HIP(hipGetLastError());
+      } else {
+        parallel_for(k, [&](size_t j) {
+          Transcript& tx = *txs[j];
+          tx.send_commitment(ft[j]->root);
+          tx.grind(p.folding_pow_bits);
+          fj[j] = FoldJob{cur[j], nxt[j], tx.sample_ef()};
+        });
+      }
       const FoldJob* dfj = (const FoldJob*)upload(fj.data(), k * sizeof(FoldJob));
       const unsigned log_rows = log2_exact(fd);
       const unsigned exp_shift = logL - log2_exact(domain);
@@ -1206,26 +1262,67 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
         ProfScope ps(ctx, "fri_fold", s);
         switch (la) {
           case 1:
-            hipLaunchKernelGGL(fri_fold_many_kernel<1>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
+            if (dev_fs)
+              hipLaunchKernelGGL(fri_fold_many_dev_kernel<1>, grid, dim3(256), 0, s, dfj, db, fd, log_rows, powLinv,
+                                 exp_shift, tw, ainv);
+            else
+              hipLaunchKernelGGL(fri_fold_many_kernel<1>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
             break;
           case 2:
-            hipLaunchKernelGGL(fri_fold_many_kernel<2>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
+            if (dev_fs)
+              hipLaunchKernelGGL(fri_fold_many_dev_kernel<2>, grid, dim3(256), 0, s, dfj, db, fd, log_rows, powLinv,
+                                 exp_shift, tw, ainv);
+            else
+              hipLaunchKernelGGL(fri_fold_many_kernel<2>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
             break;
           default:
-            hipLaunchKernelGGL(fri_fold_many_kernel<3>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
+            if (dev_fs)
+              hipLaunchKernelGGL(fri_fold_many_dev_kernel<3>, grid, dim3(256), 0, s, dfj, db, fd, log_rows, powLinv,
+                                 exp_shift, tw, ainv);
+            else
+              hipLaunchKernelGGL(fri_fold_many_kernel<3>, grid, dim3(256), 0, s, dfj, fd, log_rows, powLinv, exp_shift, tw, ainv);
             break;
         }
         P3G_CHECK_HIP(hipGetLastError());
       }
       cur = nxt;
       domain = fd;
+      r++;
     }
-    // ---- final polynomials
+    // ---- final polynomials (with the device path: also every round's roots and betas)
     {
       std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
       for (size_t j = 0; j < k; j++) pieces.emplace_back(reinterpret_cast<const uint64_t*>(cur[j]), 2 * fdeg);
+      const size_t rounds = ftrees.size();  // ftrees[q][j]: round q, job j
+      if (dev_fs) {
+        for (size_t q = 0; q < rounds; q++)
+          for (size_t j = 0; j < k; j++) {
+            const Tree* t = ftrees[q][j];
+            pieces.emplace_back(t->digests + t->layer_off[t->depth] * 4, 4);
+          }
+        pieces.emplace_back(reinterpret_cast<const uint64_t*>(dbeta), 2 * rounds * k);
+      }
       std::vector<uint64_t> raw;
       gather_to_host(ctx, pieces, raw, s);
+      if (dev_fs) {
+        const uint64_t* rr = raw.data() + 2 * fdeg * k;  // roots [round][job], then betas [round][job]
+        const uint64_t* bb = rr + 4 * rounds * k;
+        std::vector<int> ok(k, 1);
+        parallel_for(k, [&](size_t j) {
+          Transcript& tx = *txs[j];
+          for (size_t q = 0; q < rounds; q++) {
+            Tree* t = ftrees[q][j];
+            memcpy(t->root, rr + 4 * (q * k + j), 32);
+            tx.send_commitment(t->root);
+            tx.grind(p.folding_pow_bits);
+            const ef2 hb = tx.sample_ef();
+            const uint64_t* dbq = bb + 2 * (q * k + j);
+            if (hb.c0 != dbq[0] || hb.c1 != dbq[1]) ok[j] = 0;
+          }
+        });
+        for (size_t j = 0; j < k; j++)
+          if (!ok[j]) throw ApiError(E_TRANSCRIPT, "device transcript diverged from the host replay");
+      }
       const unsigned lf = log2_exact(fdeg);
       parallel_for(k, [&](size_t j) {
         const ef2* fin = reinterpret_cast<const ef2*>(raw.data() + j * 2 * fdeg);
