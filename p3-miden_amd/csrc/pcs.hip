@@ -227,37 +227,66 @@ __device__ __forceinline__ void deep_quotient_acc(const QuotArgs& a, uint64_t wb
                                                   ef2* acc) {
   __shared__ uint64_t tile[256 * 16];
   const uint32_t tid = threadIdx.x;
+  // this split's column tiles, in matrix order: tile g = (matrix m, first column c0)
+  auto next_tile = [&](uint32_t& m, uint64_t& c0, uint32_t& g) {
+    for (;;) {
+      if (m >= a.nmats) return false;
+      if (c0 >= a.mats[m].width) {
+        m++;
+        c0 = 0;
+        continue;
+      }
+      if (g % nsplit == split) return true;
+      c0 += 16;
+      g++;
+    }
+  };
+  // the 16 words this thread stages for tile (m, c0) of row chunk r
+  auto load_tile = [&](uint32_t m, uint64_t c0, int r, uint64_t* v) {
+    const QMat mm = a.mats[m];
+    const uint32_t nc = (uint32_t)((mm.width - c0) < 16 ? (mm.width - c0) : 16);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const uint32_t e = tid + 256 * k;
+      const uint32_t rl = e >> 4, cc = e & 15;
+      const uint64_t row = wbase + 256 * r + rl;
+      v[k] = (cc < nc && row < a.L) ? mm.ptr[(row >> mm.shift) * mm.width + c0 + cc] : 0;
+    }
+  };
 #pragma unroll
   for (int r = 0; r < DQ_CH; r++) {
     // unreduced sums of the two extension coordinates, reduced once per row
     Acc192 s0 = acc192_zero(), s1 = acc192_zero();
-    uint32_t g = 0;
-#pragma unroll 1
-    for (uint32_t m = 0; m < a.nmats; m++) {
-      const QMat mm = a.mats[m];
+    uint32_t m = 0, g = 0;
+    uint64_t c0 = 0;
+    bool have = next_tile(m, c0, g);
+    uint64_t v[16];
+    if (have) load_tile(m, c0, r, v);
+    while (have) {
+      const uint32_t cm = m;
+      const uint64_t cc0 = c0;
+      __syncthreads();  // the previous tile's readers are done
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        const uint32_t e = tid + 256 * k;
+        const uint32_t rl = e >> 4, cc = e & 15;
+        tile[rl * 16 + (cc ^ (rl & 15))] = v[k];
+      }
+      __syncthreads();
+      // issue the next tile's loads before reducing this one
+      c0 += 16;
+      g++;
+      have = next_tile(m, c0, g);
+      if (have) load_tile(m, c0, r, v);
+      const QMat mm = a.mats[cm];
       const ef2* cf = a.coeffs + mm.coeff_off;
-#pragma unroll 1
-      for (uint64_t c0 = 0; c0 < mm.width; c0 += 16, g++) {
-        if (g % nsplit != split) continue;
-        const uint32_t nc = (uint32_t)((mm.width - c0) < 16 ? (mm.width - c0) : 16);
-        __syncthreads();
+      const uint32_t nc = (uint32_t)((mm.width - cc0) < 16 ? (mm.width - cc0) : 16);
 #pragma unroll 4
-        for (int k = 0; k < 16; k++) {
-          const uint32_t e = tid + 256 * k;
-          const uint32_t rl = e >> 4, cc = e & 15;
-          const uint64_t row = wbase + 256 * r + rl;
-          uint64_t v = 0;
-          if (cc < nc && row < a.L) v = mm.ptr[(row >> mm.shift) * mm.width + c0 + cc];
-          tile[rl * 16 + (cc ^ (rl & 15))] = v;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (uint32_t cc = 0; cc < nc; cc++) {
-          const uint64_t v = tile[tid * 16 + (cc ^ (tid & 15))];
-          const ef2 k = cf[c0 + cc];
-          acc192_mad(s0, k.c0, v);
-          acc192_mad(s1, k.c1, v);
-        }
+      for (uint32_t cc = 0; cc < nc; cc++) {
+        const uint64_t x = tile[tid * 16 + (cc ^ (tid & 15))];
+        const ef2 k = cf[cc0 + cc];
+        acc192_mad(s0, k.c0, x);
+        acc192_mad(s1, k.c1, x);
       }
     }
     acc[r] = ef2{acc192_reduce(s0), acc192_reduce(s1)};
