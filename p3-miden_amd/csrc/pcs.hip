@@ -135,8 +135,26 @@ __device__ __forceinline__ void deep_colsum_body(const uint64_t* __restrict__ M,
 #pragma unroll
   for (int j = 0; j < N; j++) a0[j] = a1[j] = acc192_zero();
   if (col < w) {
-    for (uint64_t r = r0 + rs; r < r1; r += CS_SLICES) {
-      uint64_t v = M[r * w + col];
+    // 8 rows per step, their loads issued together before the reductions
+    constexpr int U = 8;
+    uint64_t r = r0 + rs;
+    for (; r + (U - 1) * CS_SLICES < r1; r += U * CS_SLICES) {
+      uint64_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] = M[(r + (uint64_t)u * CS_SLICES) * w + col];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint64_t ru = r + (uint64_t)u * CS_SLICES;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+          const ef2 wt = W[ru * N + j];
+          acc192_mad(a0[j], wt.c0, v[u]);
+          acc192_mad(a1[j], wt.c1, v[u]);
+        }
+      }
+    }
+    for (; r < r1; r += CS_SLICES) {
+      const uint64_t v = M[r * w + col];
 #pragma unroll
       for (int j = 0; j < N; j++) {
         const ef2 wt = W[r * N + j];
