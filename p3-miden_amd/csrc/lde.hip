@@ -185,6 +185,7 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
   __shared__ uint64_t tws[T];
   __shared__ uint64_t tfa[16];  // layout-A twist factors (the same for every thread)
   __shared__ uint64_t tfb[T];   // layout-B twist factors, entry 16 g + k
+  __shared__ uint64_t scl[MODE == FWD_FIRST ? G * 17 : 1];  // coset scale factors, entry 17 g + k
   const uint32_t tid = threadIdx.x;
   const uint32_t c = tid & 15, g = tid >> 4;
   constexpr bool inverse = MODE <= INV;
@@ -258,16 +259,17 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
   for (int rep = nrep - 1; rep >= 0; rep--) {
     uint64_t r[16];
     if (MODE == FWD_FIRST) {
-      // coefficient k_glob = base + (g + G k) xstride is scaled by c_t^k_glob / n,
-      // built as f0 * step^k (the table's lo half carries 1/n)
+      // coefficient k_glob = base + (g + G k) xstride is scaled by c_t^k_glob / n
+      // (the power table's lo half carries 1/n): each thread computes one
+      // factor of the tile into LDS, then multiplies its 16 elements
       const uint64_t* sc = a.scale + rep * a.scale_stride;
-      uint64_t f = pw(sc, base + (uint64_t)g * xstride);
-      const uint64_t step = gl_mul(pw(sc, (uint64_t)G * xstride), a.n % P);
-#pragma unroll
-      for (int k = 0; k < 16; k++) {
-        r[k] = gl_mul(in[k], f);
-        f = gl_mul(f, step);
+      {
+        const uint32_t gq = tid >> 4, kq = tid & 15;
+        scl[17 * gq + kq] = pw(sc, base + (uint64_t)(gq + G * kq) * xstride);
       }
+      __syncthreads();  // the previous coset's readers passed the tile barriers below
+#pragma unroll
+      for (int k = 0; k < 16; k++) r[k] = gl_mul(in[k], scl[17 * g + k]);
     } else {
 #pragma unroll
       for (int k = 0; k < 16; k++) r[k] = in[k];
