@@ -189,24 +189,34 @@ __global__ void __launch_bounds__(CS_T) deep_colsum_many_kernel(const uint64_t* 
                       blockIdx.y);
 }
 
-template <int N>
-__device__ __forceinline__ void colsum_reduce_body(const ef2* __restrict__ partial, uint64_t nblocks, uint64_t w,
+// one workgroup per output t (column x point): strided partial sums, then a
+// tree in LDS
+constexpr int CSR_T = 256;
+__device__ __forceinline__ void colsum_reduce_body(const ef2* __restrict__ partial, uint64_t nblocks, uint64_t wn,
                                                    ef2* __restrict__ out, uint64_t t) {
-  if (t >= w * N) return;
+  __shared__ ef2 red[CSR_T];
+  const uint32_t tid = threadIdx.x;
   ef2 s = ef_from(0);
-  for (uint64_t b = 0; b < nblocks; b++) s = ef_add(s, partial[b * w * N + t]);
-  out[t] = s;
+  for (uint64_t b = tid; b < nblocks; b += CSR_T) s = ef_add(s, partial[b * wn + t]);
+  red[tid] = s;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t off = CSR_T / 2; off > 0; off >>= 1) {
+    if (tid < off) red[tid] = ef_add(red[tid], red[tid + off]);
+    __syncthreads();
+  }
+  if (tid == 0) out[t] = red[0];
 }
 template <int N>
-__global__ void colsum_reduce_kernel(const ef2* __restrict__ partial, uint64_t nblocks, uint64_t w,
-                                     ef2* __restrict__ out) {
-  colsum_reduce_body<N>(partial, nblocks, w, out, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(CSR_T) colsum_reduce_kernel(const ef2* __restrict__ partial, uint64_t nblocks,
+                                                            uint64_t w, ef2* __restrict__ out) {
+  colsum_reduce_body(partial, nblocks, w * N, out, blockIdx.x);
 }
 template <int N>
-__global__ void colsum_reduce_many_kernel(const ef2* const* __restrict__ partials, uint64_t nblocks, uint64_t w,
-                                          ef2* const* __restrict__ outs) {
-  colsum_reduce_body<N>(partials[blockIdx.y], nblocks, w, outs[blockIdx.y],
-                        (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(CSR_T) colsum_reduce_many_kernel(const ef2* const* __restrict__ partials,
+                                                                 uint64_t nblocks, uint64_t w,
+                                                                 ef2* const* __restrict__ outs) {
+  colsum_reduce_body(partials[blockIdx.y], nblocks, w * N, outs[blockIdx.y], blockIdx.x);
 }
 
 struct QMat {
@@ -604,8 +614,8 @@ static void run_deep_kernels(Ctx& ctx, const PcsParams& p, unsigned logL, const 
     P3G_CHECK_HIP(hipGetLastError());
     }
     uint64_t nt = m.width * N;
-    hipLaunchKernelGGL(colsum_reduce_kernel<N>, dim3((uint32_t)((nt + 127) / 128)), dim3(128), 0, s, partial, nblk,
-                       m.width, evals_dev + off * N);
+    hipLaunchKernelGGL(colsum_reduce_kernel<N>, dim3((uint32_t)nt), dim3(CSR_T), 0, s, partial, nblk, m.width,
+                       evals_dev + off * N);
     P3G_CHECK_HIP(hipGetLastError());
     ctx.release(partial);
     off += m.width;
@@ -1125,8 +1135,8 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
           P3G_CHECK_HIP(hipGetLastError());
         }
         const uint64_t nt = w * N;
-        hipLaunchKernelGGL(colsum_reduce_many_kernel<N>, dim3((uint32_t)((nt + 127) / 128), (uint32_t)k), dim3(128), 0,
-                           s, (const ef2* const*)dP, nblk, w, dO);
+        hipLaunchKernelGGL(colsum_reduce_many_kernel<N>, dim3((uint32_t)nt, (uint32_t)k), dim3(CSR_T), 0, s,
+                           (const ef2* const*)dP, nblk, w, dO);
         P3G_CHECK_HIP(hipGetLastError());
         off += w;
       }
