@@ -10,94 +10,18 @@
 #include <functional>
 #include <thread>
 
+#include "host_pool.hpp"
 #include "internal.hpp"
 
 namespace p3g {
 
 // ------------------------------------------------------------------ host thread pool
-// Persistent workers for per-job host loops (transcripts, hint assembly).
-// One parallel_for at a time; a caller that finds the pool busy runs its loop
-// itself, so concurrent callers never wait on each other.
 namespace {
-struct HostPool {
-  std::mutex run_mu, mu;
-  std::condition_variable cv, cv_done;
-  std::vector<std::thread> threads;
-  const std::function<void(size_t)>* fn = nullptr;
-  size_t n = 0;
-  std::atomic<size_t> next{0};
-  size_t active = 0;
-  uint64_t gen = 0;
-  bool stop = false;
-  std::exception_ptr err;
-
-  HostPool() {
-    const unsigned hw = std::thread::hardware_concurrency();
-    const unsigned nt = std::min(16u, hw > 1 ? hw : 1u);
-    for (unsigned i = 1; i < nt; i++) threads.emplace_back([this] { loop(); });
-  }
-  ~HostPool() {
-    {
-      std::lock_guard<std::mutex> g(mu);
-      stop = true;
-    }
-    cv.notify_all();
-    for (auto& t : threads) t.join();
-  }
-  void work() {
-    for (;;) {
-      const size_t i = next.fetch_add(1);
-      if (i >= n) break;
-      try {
-        (*fn)(i);
-      } catch (...) {
-        std::lock_guard<std::mutex> g(mu);
-        if (!err) err = std::current_exception();
-      }
-    }
-  }
-  void loop() {
-    uint64_t seen = 0;
-    for (;;) {
-      std::unique_lock<std::mutex> lk(mu);
-      cv.wait(lk, [&] { return stop || gen != seen; });
-      if (stop) return;
-      seen = gen;
-      lk.unlock();
-      work();
-      lk.lock();
-      if (--active == 0) cv_done.notify_all();
-    }
-  }
-  void run(size_t count, const std::function<void(size_t)>& f) {
-    std::unique_lock<std::mutex> rl(run_mu, std::try_to_lock);
-    if (!rl.owns_lock() || threads.empty() || count < 2) {
-      for (size_t i = 0; i < count; i++) f(i);
-      return;
-    }
-    {
-      std::lock_guard<std::mutex> g(mu);
-      fn = &f;
-      n = count;
-      next = 0;
-      active = threads.size();
-      err = nullptr;
-      gen++;
-    }
-    cv.notify_all();
-    work();
-    std::exception_ptr e;
-    {
-      std::unique_lock<std::mutex> lk(mu);
-      cv_done.wait(lk, [&] { return active == 0; });
-      e = err;
-      fn = nullptr;
-    }
-    if (e) std::rethrow_exception(e);
-  }
-};
 HostPool& host_pool() {
-  static HostPool p;
+  static HostPool p([] {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::min(16u, hw > 1 ? hw : 1u);
+  }());
   return p;
 }
 }  // namespace
@@ -138,9 +62,19 @@ void SharedPool::free_one(std::multimap<size_t, Block>::iterator it) {
   free_ms += ms_since(t0);
 }
 
+// Block sizes are rounded up to 16 classes per octave (at most 6.25% over the
+// request; 256-byte steps below 4 KiB), so requests whose sizes differ a little,
+// such as the same phase of batches of 63 and 64 jobs, map to the same cached
+// blocks instead of each growing the pool.
+static size_t size_class(size_t bytes) {
+  if (bytes <= 4096) return bytes == 0 ? 256 : (bytes + 255) & ~size_t(255);
+  int e = 63 - __builtin_clzll((unsigned long long)bytes);
+  const size_t step = size_t(1) << (e - 4);
+  return (bytes + step - 1) & ~(step - 1);
+}
+
 void* SharedPool::alloc(size_t bytes, hipStream_t s) {
-  if (bytes == 0) bytes = 8;
-  bytes = (bytes + 255) & ~size_t(255);
+  bytes = size_class(bytes);
   std::lock_guard<std::mutex> g(mu);
   auto it = free_blocks.lower_bound(bytes);
   if (it != free_blocks.end() && it->first <= bytes + bytes / 4) return take(it, s);
