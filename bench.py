@@ -14,6 +14,8 @@ Other sections of the same JSON line:
              NTT launches, hipGraph replay, autotuned shapes) against the
              per-trace launch path, with the two proofs compared word for word;
   keccak   - 2^16 x 2600 bit-structured trace, Keccak LMCS, arity 2, 100 queries;
+  quotient_commit - commit_quotient of quotient evaluations resident in HBM
+             (N = 2^16, 2^17 with D = 2, B = 2; N = 2^22 with D = 4, B = 4);
   serving  - Poisson arrivals of prove requests (1-8 traces of 2^14..2^22 rows,
              16..128 columns, log_blowup 1..3) into the prove-request server;
              saturation throughput, then p50/p99 latency at 0.5x, 0.8x and 1.0x
@@ -417,6 +419,42 @@ def bench_small(G, W, dev, n_proofs, workers):
     }
 
 
+# ------------------------------------------------------------------ quotient commitment
+def bench_quotient(G, dev, steps, warmup):
+    """commit_quotient (quotient.rs:133-238) on quotient evaluations resident in HBM:
+    the reference bench's shapes (N = 2^16, 2^17; D = 2; B = 2) and the quotient of the
+    headline trace (N = 2^22, D = 4, B = 4)."""
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    out = []
+    for logn, d, logb in ((16, 2, 1), (17, 2, 1), (22, 4, 2)):
+        n, B = 1 << logn, 1 << logb
+        q = dev.alloc_matrix(n, 2 * d)
+        q.fill_random(SEED + logn)
+
+        def run():
+            G.commit_quotient(cfg, q, d, logb).free()
+
+        for _ in range(warmup):
+            run()
+        dev.synchronize()
+        reps = max(3, steps)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        dev.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        L = n * B
+        algo = 16 * n * d + 8 * L * 2 * d + 32 * (2 * L - 1)
+        perms = L * -(-2 * d // 8) + (L - 1)
+        out.append({"shape": f"N=2^{logn}, D={d}, log_blowup={logb} (LDE {L} x {2 * d})", "ms": round(ms, 3),
+                    "gbps": round(algo / (ms * 1e-3) / 1e9, 1),
+                    "poseidon2_perms_per_s": round(perms / (ms * 1e-3), 1)})
+        q.free()
+    dev.trim()
+    return {"workload": "quotient commitment of N*D extension elements resident in HBM, Poseidon2 LMCS (aligned)",
+            "runs": out}
+
+
 def cpu_baseline(W, log_rows):
     import oracle as O
 
@@ -442,8 +480,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--sections", default="c2,c3,c4,c5,small",
-                    help="comma list of c2 (headline), c3, c4, c5, small")
+    ap.add_argument("--sections", default="c2,c3,c4,c5,small,quotient",
+                    help="comma list of c2 (headline), c3, c4, c5, small, quotient")
     ap.add_argument("--small-proofs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=4)
@@ -475,6 +513,7 @@ def main():
     c4 = bench_c4(G, W, dev, max(2, args.steps // 2), args.warmup) if "c4" in sections else None
     c5 = bench_c5(G, W, dev, args.serve_requests, args.serve_workers) if "c5" in sections else None
     small = bench_small(G, W, dev, args.small_proofs, args.serve_workers) if "small" in sections else None
+    quot = bench_quotient(G, dev, max(3, args.steps // 2), 1) if "quotient" in sections else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(W, args.cpu_sample_log_rows)
 
     ms = c2["ms"] if c2 else None
@@ -510,6 +549,7 @@ def main():
         "keccak": c4,
         "serving": c5,
         "small_proofs": small,
+        "quotient_commit": quot,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -140,6 +140,15 @@ void p3g_tree_free(p3g_tree* t);
 int p3g_commit_traces(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* traces, uint64_t k, uint32_t log_blowup,
                       uint32_t flags, p3g_tree** out);
 
+/* ---- commit_quotient (p3-miden-lifted-stark/src/prover/quotient.rs:133-238) ----
+ * q_evals: the quotient's n*d extension elements in natural order on gJ, each as
+ * (c0, c1) -- the n x 2d base matrix, row r = [Q(g w_J^(r d + t))]_t.  Commits the
+ * bit-reversed (n*B) x 2d LDE of the chunk polynomials on gK in one aligned tree.
+ * n and d powers of two, d <= 2^log_blowup (P3G_E_INVAL otherwise).
+ * flags: P3G_DEVICE_PTRS when q_evals is a device address. */
+int p3g_commit_quotient(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* q_evals, uint64_t n, uint64_t d,
+                        uint32_t log_blowup, uint32_t flags, p3g_tree** out);
+
 /* ---- transcript (ProverChannel, p3-miden-transcript/src/prover.rs:53-145) ---- */
 int p3g_transcript_create(uint32_t challenger_kind, p3g_transcript** out);
 void p3g_transcript_free(p3g_transcript* ch);

@@ -180,6 +180,12 @@ int orc_verify(const orc_pcs_params* p, int hash_kind, const uint64_t* roots, co
 orc_tree* orc_commit_traces(int hash_kind, const uint64_t* const* traces, const size_t* heights,
                             const size_t* widths, size_t k, unsigned log_blowup);
 
+/* commit_quotient (p3-miden-lifted-stark/src/prover/quotient.rs:133-238): q_evals holds N*D
+ * extension elements (c0, c1) in natural order on gJ; out is the (N*B) x 2D bit-reversed LDE.
+ * Returns -1 (NULL) unless N and D are powers of two with D <= B. */
+int orc_quotient_lde(const uint64_t* q_evals, size_t n, size_t d, unsigned log_blowup, uint64_t* out);
+orc_tree* orc_commit_quotient(int hash_kind, const uint64_t* q_evals, size_t n, size_t d, unsigned log_blowup);
+
 void orc_set_threads(int n);
 int orc_get_threads(void);
 

@@ -106,6 +106,8 @@ def _load():
         "orc_open_with_channel": (None, [vp, C.c_int, C.POINTER(vp), C.c_size_t, C.c_uint, _u64p, C.c_size_t, vp, _u64p]),
         "orc_verify": (C.c_int, [vp, C.c_int, _u64p, _szp, _szp, C.c_size_t, C.c_uint, _u64p, C.c_size_t, vp, _u64p]),
         "orc_commit_traces": (vp, [C.c_int, C.POINTER(_u64p), _szp, _szp, C.c_size_t, C.c_uint]),
+        "orc_quotient_lde": (C.c_int, [_u64p, C.c_size_t, C.c_size_t, C.c_uint, _u64p]),
+        "orc_commit_quotient": (vp, [C.c_int, _u64p, C.c_size_t, C.c_size_t, C.c_uint]),
         "orc_set_threads": (None, [C.c_int]),
         "orc_get_threads": (C.c_int, []),
     }
@@ -392,6 +394,31 @@ def commit_traces(kind: int, traces, log_blowup: int) -> Tree:
     h = lib.orc_commit_traces(kind, ptrs, _sz([t.shape[0] for t in ts]), _sz([t.shape[1] for t in ts]), len(ts), log_blowup)
     if not h:
         raise ValueError("invalid traces")
+    return Tree(kind, None, _handle=h)
+
+
+def quotient_lde(q_evals: np.ndarray, d: int, log_blowup: int) -> np.ndarray:
+    """The bit-reversed (N*B) x 2D matrix commit_quotient commits (quotient.rs:133-235).
+
+    q_evals: N*D extension elements as an (N*D, 2) array (c0, c1), natural order on gJ."""
+    q = u64(q_evals).reshape(-1, 2)
+    if d <= 0 or q.shape[0] % d:
+        raise ValueError("quotient: q_evals length must be divisible by D")
+    n = q.shape[0] // d
+    out = np.zeros(((n << log_blowup), 2 * d), np.uint64)
+    if lib.orc_quotient_lde(_p(q), n, d, log_blowup, _p(out)) != 0:
+        raise ValueError("quotient: N and D must be powers of two with D <= blowup")
+    return out
+
+
+def commit_quotient(kind: int, q_evals: np.ndarray, d: int, log_blowup: int) -> Tree:
+    """commit_quotient (p3-miden-lifted-stark/src/prover/quotient.rs:133-238)."""
+    q = u64(q_evals).reshape(-1, 2)
+    if d <= 0 or q.shape[0] % d:
+        raise ValueError("quotient: q_evals length must be divisible by D")
+    h = lib.orc_commit_quotient(kind, _p(q), q.shape[0] // d, d, log_blowup)
+    if not h:
+        raise ValueError("quotient: N and D must be powers of two with D <= blowup")
     return Tree(kind, None, _handle=h)
 
 

@@ -413,6 +413,30 @@ int p3g_commit_traces(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* traces
   });
 }
 
+int p3g_commit_quotient(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* q_evals, uint64_t n, uint64_t d,
+                        uint32_t log_blowup, uint32_t flags, p3g_tree** out) {
+  if (!ctx || !out || !q_evals) return P3G_E_INVAL;
+  *out = nullptr;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    check_hash(hash_kind);
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    if (n == 0 || d == 0) throw ApiError(P3G_E_EMPTY, "empty quotient");
+    if (d > (1ull << 31) || n > (1ull << 32)) throw ApiError(P3G_E_RANGE, "quotient shape out of range");
+    const bool dev = flags & P3G_DEVICE_PTRS;
+    const uint64_t* dq = dev ? q_evals : to_device(c, q_evals, n * 2 * d);
+    Tree* t = nullptr;
+    try {
+      t = commit_quotient(c, hash_kind, dq, n, d, log_blowup, c.stream);
+    } catch (...) {
+      if (!dev) c.release(const_cast<uint64_t*>(dq));
+      throw;
+    }
+    if (!dev) c.release(const_cast<uint64_t*>(dq));
+    *out = new p3g_tree{t};
+  });
+}
+
 int p3g_transcript_create(uint32_t kind, p3g_transcript** out) {
   if (!out) return P3G_E_INVAL;
   if (kind != CH_DUPLEX_P2 && kind != CH_KECCAK) return P3G_E_INVAL;

@@ -258,8 +258,16 @@ size_t fri_num_rounds(const PcsParams& p, unsigned log_domain);
 size_t fri_final_degree(const PcsParams& p, unsigned log_domain);
 
 // ------------------------------------------------------------------ launchers
+// Optional twist of the coefficients between the inverse and the forward
+// transforms: coefficient k of column c is multiplied by root^(k * (c / group)),
+// with root of order 2^log_order >= n * (w / group).
+struct ColTwist {
+  uint64_t root;
+  uint32_t log_order;
+  uint32_t group;
+};
 void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
-                     uint64_t shift, uint64_t* d_out, hipStream_t s);
+                     uint64_t shift, uint64_t* d_out, hipStream_t s, const ColTwist* twist = nullptr);
 void bit_reverse_rows(Ctx& ctx, const uint64_t* src, uint64_t* dst, uint64_t h, uint64_t w, hipStream_t s);
 // one transform of a batch of same-shaped coset LDEs (bit-reversed rows out)
 struct LdeJob {
@@ -293,6 +301,10 @@ void host_sponge_absorb(uint32_t hash, uint64_t* state, const uint64_t* in, size
 void host_compress(uint32_t hash, const uint64_t* l, const uint64_t* r, uint64_t* out);
 
 Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, unsigned log_blowup, hipStream_t s);
+// commit_quotient: q holds n*d extension elements (c0, c1), natural order on gJ,
+// read as the n x 2d base matrix it already is
+Tree* commit_quotient(Ctx& ctx, uint32_t hash, const uint64_t* d_q, uint64_t n, uint64_t d, unsigned log_blowup,
+                      hipStream_t s);
 
 void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned log_lde_height, const ef2* z, int npts,
                        const std::vector<Tree*>& trees, Channel& tx, hipStream_t s);

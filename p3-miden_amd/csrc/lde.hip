@@ -447,8 +447,19 @@ std::vector<uint32_t> ntt_pass_split(uint32_t logn, uint32_t kmax);
 void launch_ntt_grouped(uint32_t mode, bool reg, uint32_t K, const void* descs, const uint64_t* starts, uint32_t nd,
                         uint64_t blocks, hipStream_t s);
 
+// coefficient k of column c times root^(k * (c / group)) (root's powers in tab)
+__global__ void coef_twist_kernel(uint64_t* __restrict__ coef, uint64_t n, uint32_t w, const uint64_t* __restrict__ tab,
+                                  uint32_t group) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * w) return;
+  const uint64_t k = i / w;
+  const uint32_t c = (uint32_t)(i - k * w);
+  const uint64_t e = k * (c / group);
+  if (e) coef[i] = gl_mul(coef[i], pw(tab, e));
+}
+
 void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
-                     uint64_t shift, uint64_t* d_out, hipStream_t s) {
+                     uint64_t shift, uint64_t* d_out, hipStream_t s, const ColTwist* twist) {
   if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
   if (w == 0) return;
   const unsigned logn = log2_exact(n);
@@ -523,6 +534,17 @@ void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, 
       launch(i == 0 ? INV_FIRST : INV, s0, split[i], 1);
       s0 += split[i];
     }
+  }
+  if (twist) {
+    // exponents k * (c / group) < n * ceil(w / group) must stay below the root's order
+    const uint64_t groups = (w + twist->group - 1) / twist->group;
+    if (twist->group == 0 || twist->log_order > 32 || n * groups > (1ull << twist->log_order))
+      throw ApiError(E_RANGE, "coefficient twist exponent exceeds the root's order");
+    const uint64_t* tab = ctx.pow_table(twist->root, twist->log_order);
+    const uint64_t total = n * w;
+    hipLaunchKernelGGL(coef_twist_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s, d_out, n,
+                       (uint32_t)w, tab, twist->group);
+    P3G_CHECK_HIP(hipGetLastError());
   }
   {
     uint32_t s0 = 0;

@@ -899,4 +899,34 @@ Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, 
   }
 }
 
+// commit_quotient (p3-miden-lifted-stark/src/prover/quotient.rs:133-238).  The
+// reference runs a plain iDFT over H on each chunk column t, scales coefficient
+// k by (omega_J^-k)^t, zero-pads and runs a plain DFT over K.  Column t of the
+// input holds q_t(g omega_J^t X) on H, so the result, q_t(g omega_K^i) for row
+// bitrev(i), is the coset LDE of that column on the shift omega_J^-t: one
+// inverse transform, a per-column twist of the coefficients (both base
+// coordinates of a chunk share t) and a shift-1 forward transform, in the
+// same passes as the trace LDE.  The single matrix is committed aligned.
+Tree* commit_quotient(Ctx& ctx, uint32_t hash, const uint64_t* d_q, uint64_t n, uint64_t d, unsigned log_blowup,
+                      hipStream_t s) {
+  if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "quotient trace height must be a power of two");
+  if (d == 0 || (d & (d - 1))) throw ApiError(E_NOT_POW2, "constraint degree must be a power of two");
+  if (d > (1ull << log_blowup)) throw ApiError(E_INVAL, "blowup must be at least the constraint degree");
+  const unsigned logn = log2_exact(n), logd = log2_exact(d);
+  if (logn + log_blowup > 32) throw ApiError(E_RANGE, "LDE height exceeds the field two-adicity");
+  DevMat m;
+  m.height = n << log_blowup;
+  m.width = 2 * d;
+  m.owned = true;
+  m.ptr = (uint64_t*)ctx.alloc(m.height * m.width * sizeof(uint64_t));
+  try {
+    const ColTwist tw{gl_inv(gl_two_adic_generator(logn + logd)), logn + logd, 2};
+    lde_coset_batch(ctx, d_q, n, 2 * d, log_blowup, 1, m.ptr, s, &tw);
+    return lmcs_build(ctx, hash, {m}, hash_alignment(hash), s);
+  } catch (...) {
+    ctx.release(m.ptr);
+    throw;
+  }
+}
+
 }  // namespace p3g

@@ -186,6 +186,8 @@ def _load():
         "p3g_tree_prove_batch": (C.c_int, [_vp, _u64p, C.c_uint64, _vp]),
         "p3g_tree_free": (None, [_vp]),
         "p3g_commit_traces": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+        "p3g_commit_quotient": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
+                                          C.POINTER(_vp)]),
         "p3g_transcript_create": (C.c_int, [C.c_uint32, C.POINTER(_vp)]),
         "p3g_transcript_free": (None, [_vp]),
         "p3g_transcript_clone": (C.c_int, [_vp, C.POINTER(_vp)]),
@@ -602,6 +604,27 @@ def commit_traces(lmcs: LmcsConfig, traces, log_blowup: int) -> LiftedMerkleTree
     h = _vp()
     _check(lib().p3g_commit_traces(lmcs.dev.h, lmcs.hash_kind, arr, len(traces), log_blowup,
                                    DEVICE_PTRS if dev else 0, C.byref(h)), "commit_traces")
+    return LiftedMerkleTree(h, lmcs.hash_kind, lmcs.dev)
+
+
+def commit_quotient(lmcs: LmcsConfig, q_evals, d: int, log_blowup: int) -> LiftedMerkleTree:
+    """commit_quotient (p3-miden-lifted-stark/src/prover/quotient.rs:133-238).
+
+    q_evals: the quotient's N*D extension elements in natural order on gJ, as an
+    (N*D, 2) host array of (c0, c1) or a DeviceMatrix of N rows x 2D words.  The tree
+    holds the bit-reversed (N*B) x 2D LDE of the D chunk polynomials on gK."""
+    if isinstance(q_evals, DeviceMatrix):
+        if q_evals.width != 2 * d:
+            raise ValueError("device quotient matrix must be N x 2D")
+        ptr, n, flags, keep = C.cast(q_evals.ptr, _u64p), q_evals.height, DEVICE_PTRS, None
+    else:
+        keep = u64(q_evals).reshape(-1)
+        if d <= 0 or keep.size % (2 * d):
+            raise ValueError("q_evals length must be a multiple of 2D words")
+        ptr, n, flags = _p(keep), keep.size // (2 * d), 0
+    h = _vp()
+    _check(lib().p3g_commit_quotient(lmcs.dev.h, lmcs.hash_kind, ptr, n, d, log_blowup, flags, C.byref(h)),
+           "commit_quotient")
     return LiftedMerkleTree(h, lmcs.hash_kind, lmcs.dev)
 
 

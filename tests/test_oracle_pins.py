@@ -469,3 +469,76 @@ def test_grind_finds_smallest_witness():
     a.observe(3); b.observe(3)
     assert a.grind(0) == 0
     assert a.sample() == b.sample()
+
+
+# ---------------------------------------------------------------- quotient commit (closed form)
+def _bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def _poly_eval(coeffs, x):
+    acc = 0
+    for c in reversed(coeffs):
+        acc = (acc * x + c) % P
+    return acc
+
+
+@pytest.mark.parametrize("log_n,d,log_b", [(3, 2, 1), (3, 2, 2), (2, 4, 2), (4, 1, 1), (3, 4, 3), (0, 2, 1)])
+def test_quotient_lde_is_chunk_polys_on_gK(log_n, d, log_b):
+    # commit_quotient (quotient.rs:107-131): column t of the N x D reshape holds the
+    # values of a degree < N polynomial q_t on g * omega_J^t * H, and the committed
+    # matrix holds q_t(g * omega_K^bitrev(i)) in row i (c0, c1 as two base columns).
+    # Built here from random polynomials by direct evaluation in big integers, so a
+    # wrong twist sign, a transposed (t, k), or a missing 1/N would fail.
+    rng = random.Random(100 + log_n * 10 + d + log_b)
+    n, B = 1 << log_n, 1 << log_b
+    log_d = d.bit_length() - 1
+    g = 7
+    w_j = pow(7, (P - 1) >> (log_n + log_d), P)
+    w_h = pow(7, (P - 1) >> log_n, P)
+    w_k = pow(7, (P - 1) >> (log_n + log_b), P)
+    polys = [[[rng.randrange(P) for _ in range(n)] for _ in range(2)] for _ in range(d)]  # [t][comp][k]
+    q = np.zeros((n * d, 2), np.uint64)
+    for r in range(n):
+        for t in range(d):
+            x = g * pow(w_j, t, P) * pow(w_h, r, P) % P
+            for comp in range(2):
+                q[r * d + t, comp] = _poly_eval(polys[t][comp], x)
+    out = O.quotient_lde(q, d, log_b)
+    assert out.shape == (n * B, 2 * d)
+    for i in range(n * B):
+        x = g * pow(w_k, _bitrev(i, log_n + log_b), P) % P
+        for t in range(d):
+            for comp in range(2):
+                assert int(out[i, 2 * t + comp]) == _poly_eval(polys[t][comp], x), (i, t, comp)
+
+
+def test_quotient_lde_degree_one_is_plain_lde():
+    # D = 1: omega_J^0 = 1, so the commit is the plain (shift 1) LDE of the two base
+    # columns read as values on H, rows bit-reversed
+    rng = np.random.default_rng(7)
+    q = rng.integers(0, P, size=(64, 2), dtype=np.uint64)
+    out = O.quotient_lde(q, 1, 2)
+    ref = O.coset_lde_batch(q, 2, 1, bit_reverse=True)
+    assert np.array_equal(out, ref)
+
+
+def test_commit_quotient_root_is_aligned_tree_of_the_lde():
+    # build_aligned_tree(vec![quotient_matrix]) (quotient.rs:233): the root of the
+    # aligned single-matrix tree over the LDE above
+    rng = np.random.default_rng(8)
+    q = rng.integers(0, P, size=(32 * 2, 2), dtype=np.uint64)
+    for kind in (O.HASH_P2, O.HASH_KECCAK):
+        t = O.commit_quotient(kind, q, 2, 1)
+        ref = O.Tree(kind, [O.quotient_lde(q, 2, 1)], alignment=O.hash_alignment(kind))
+        assert t.root().tolist() == ref.root().tolist()
+        assert t.height() == 64
+
+
+def test_quotient_rejects_blowup_below_degree():
+    # debug_assert!(b >= d) (quotient.rs:154)
+    q = np.zeros((16 * 4, 2), np.uint64)
+    with pytest.raises(ValueError):
+        O.quotient_lde(q, 4, 1)
+    with pytest.raises(ValueError):
+        O.quotient_lde(np.zeros((15, 2), np.uint64), 2, 1)
