@@ -23,6 +23,7 @@
 #include <map>
 #include <mutex>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/p3gpu.h"
@@ -120,8 +121,8 @@ void p3g_server::run_one(p3g_ctx* ctx, Request* r) {
   if (rc == P3G_OK)
     rc = p3g_prove(ctx, r->hash, &r->params, r->traces.data(), r->traces.size(), r->flags, tx, r->root, r->z);
   if (rc == P3G_OK) {
-    r->fields = tx->tx.fields;
-    r->comms = tx->tx.comms;
+    r->fields = std::move(tx->tx.fields);  // the transcript is freed below
+    r->comms = std::move(tx->tx.comms);
   } else {
     r->error = p3g_last_error();
   }
@@ -151,8 +152,8 @@ void p3g_server::run_batch(p3g_ctx* ctx, std::vector<Request*>& batch) {
     if (rc == P3G_OK) {
       memcpy(r->root, roots.data() + 4 * j, 32);
       memcpy(r->z, zs.data() + 4 * j, 32);
-      r->fields = txs[j]->tx.fields;
-      r->comms = txs[j]->tx.comms;
+      r->fields = std::move(txs[j]->tx.fields);  // the transcript is freed below
+      r->comms = std::move(txs[j]->tx.comms);
     } else {
       r->error = err;
     }

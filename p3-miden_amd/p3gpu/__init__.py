@@ -607,6 +607,23 @@ def commit_traces(lmcs: LmcsConfig, traces, log_blowup: int) -> LiftedMerkleTree
     return LiftedMerkleTree(h, lmcs.hash_kind, lmcs.dev)
 
 
+def commit_aux_traces(lmcs: LmcsConfig, aux_traces, log_blowup: int) -> LiftedMerkleTree:
+    """The auxiliary-trace commitment of prove_multi (p3-miden-lifted-stark/src/prover/mod.rs:323-337):
+    each extension-field trace is flattened to base (c0 then c1 per element, width 2w) and the
+    flattened traces go through commit_traces.  Host traces are (n, w, 2) arrays; a DeviceMatrix
+    already holds the flattened n x 2w words."""
+    flat = []
+    for t in aux_traces:
+        if isinstance(t, DeviceMatrix):
+            flat.append(t)
+        else:
+            a = u64(t)
+            if a.ndim != 3 or a.shape[2] != 2:
+                raise ValueError("host aux traces are (n, w, 2) arrays of extension elements")
+            flat.append(a.reshape(a.shape[0], 2 * a.shape[1]))
+    return commit_traces(lmcs, flat, log_blowup)
+
+
 def commit_quotient(lmcs: LmcsConfig, q_evals, d: int, log_blowup: int) -> LiftedMerkleTree:
     """commit_quotient (p3-miden-lifted-stark/src/prover/quotient.rs:133-238).
 

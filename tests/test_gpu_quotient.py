@@ -91,3 +91,22 @@ def test_commit_quotient_rejects_bad_shapes(device):
     with pytest.raises(G.P3GError) as e:
         G.commit_quotient(cfg, O.rand_matrix(3, 16 * 3, 2), 3, 2)  # D = 3
     assert e.value.code == 2  # not a power of two
+
+
+# ---------------------------------------------------------------- auxiliary traces
+def test_commit_aux_traces_is_commit_traces_of_the_flattened_traces(device):
+    # prove_multi flattens each EF aux trace to base, c0 then c1 per element
+    # (mod.rs:323-337), and commits the flattened traces with commit_traces
+    shapes = [(1 << 6, 3), (1 << 9, 1), (1 << 9, 5)]
+    aux = [O.rand_matrix(300 + i, h, 2 * w).reshape(h, w, 2) for i, (h, w) in enumerate(shapes)]
+    for gk, ok in KINDS:
+        t = G.commit_aux_traces(G.LmcsConfig(device, gk), aux, 2)
+        flat = []
+        for a in aux:
+            f = np.empty((a.shape[0], 2 * a.shape[1]), np.uint64)
+            f[:, 0::2], f[:, 1::2] = a[:, :, 0], a[:, :, 1]  # element j -> base columns 2j (c0), 2j + 1 (c1)
+            flat.append(f)
+        want = O.commit_traces(ok, flat, 2)
+        assert t.root().tolist() == want.root().tolist()
+        assert t.height() == (1 << 9) << 2
+        t.free()
