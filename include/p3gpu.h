@@ -168,6 +168,22 @@ uint64_t p3g_transcript_num_fields(const p3g_transcript* ch);
 uint64_t p3g_transcript_num_commitments(const p3g_transcript* ch);
 int p3g_transcript_data(const p3g_transcript* ch, uint64_t* fields, uint64_t* commitments);
 
+/* ---- proof wire format (TranscriptData, data.rs:8-14, has serde only; this byte
+ * encoding is the library's own) ----
+ *   bytes 0..3   "P3GT"          bytes 4..7   version 1 (u32 LE)
+ *   bytes 8..15  nf (u64 LE)     bytes 16..23 nc (u64 LE)
+ *   then nf field words and nc commitments of 4 words, each a canonical u64 LE:
+ *   24 + 8 nf + 32 nc bytes in all.
+ * encode: out == NULL only reports *len; a buffer shorter than *len gives P3G_E_RANGE;
+ * a non-canonical word gives P3G_E_INVAL.  decode: fields / commitments == NULL only
+ * report the counts; a bad header, a length that does not match, or a non-canonical
+ * word gives P3G_E_INVAL. */
+int p3g_proof_encode(const uint64_t* fields, uint64_t nf, const uint64_t* commitments, uint64_t nc, uint8_t* out,
+                     uint64_t cap, uint64_t* len);
+int p3g_proof_decode(const uint8_t* in, uint64_t len, uint64_t* fields, uint64_t* nf, uint64_t* commitments,
+                     uint64_t* nc);
+int p3g_transcript_encode(const p3g_transcript* ch, uint8_t* out, uint64_t cap, uint64_t* len);
+
 /* ---- PCS opening ----
  * open_with_channel(params, lmcs, log_lde_height, eval_points, trace_trees, channel)
  * (p3-miden-lifted-fri/src/prover.rs:35-116): DEEP quotient, FRI commit phase,

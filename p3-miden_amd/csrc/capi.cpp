@@ -532,6 +532,64 @@ int p3g_transcript_data(const p3g_transcript* ch, uint64_t* fields, uint64_t* co
   return P3G_OK;
 }
 
+static void put_le64(uint8_t* p, uint64_t v) {
+  for (int i = 0; i < 8; i++) p[i] = (uint8_t)(v >> (8 * i));
+}
+static uint64_t get_le64(const uint8_t* p) {
+  uint64_t v = 0;
+  for (int i = 0; i < 8; i++) v |= (uint64_t)p[i] << (8 * i);
+  return v;
+}
+static const uint8_t PROOF_MAGIC[4] = {'P', '3', 'G', 'T'};
+constexpr uint32_t PROOF_VERSION = 1;
+constexpr uint64_t PROOF_HEADER = 24;
+
+int p3g_proof_encode(const uint64_t* fields, uint64_t nf, const uint64_t* commitments, uint64_t nc, uint8_t* out,
+                     uint64_t cap, uint64_t* len) {
+  if (!len || (nf && !fields) || (nc && !commitments)) return P3G_E_INVAL;
+  if (nf > (1ull << 56) || nc > (1ull << 54)) return P3G_E_RANGE;
+  const uint64_t need = PROOF_HEADER + 8 * nf + 32 * nc;
+  *len = need;
+  if (!out) return P3G_OK;
+  if (cap < need) return P3G_E_RANGE;
+  if (!canonical(fields, nf) || !canonical(commitments, 4 * nc)) return P3G_E_INVAL;
+  memcpy(out, PROOF_MAGIC, 4);
+  for (int i = 0; i < 4; i++) out[4 + i] = (uint8_t)(PROOF_VERSION >> (8 * i));
+  put_le64(out + 8, nf);
+  put_le64(out + 16, nc);
+  uint8_t* p = out + PROOF_HEADER;
+  for (uint64_t i = 0; i < nf; i++, p += 8) put_le64(p, fields[i]);
+  for (uint64_t i = 0; i < 4 * nc; i++, p += 8) put_le64(p, commitments[i]);
+  return P3G_OK;
+}
+
+int p3g_proof_decode(const uint8_t* in, uint64_t len, uint64_t* fields, uint64_t* nf, uint64_t* commitments,
+                     uint64_t* nc) {
+  if (!in || !nf || !nc || len < PROOF_HEADER) return P3G_E_INVAL;
+  if (memcmp(in, PROOF_MAGIC, 4) != 0) return P3G_E_INVAL;
+  uint32_t ver = 0;
+  for (int i = 0; i < 4; i++) ver |= (uint32_t)in[4 + i] << (8 * i);
+  if (ver != PROOF_VERSION) return P3G_E_INVAL;
+  const uint64_t f = get_le64(in + 8), c = get_le64(in + 16);
+  if (f > (1ull << 56) || c > (1ull << 54) || len != PROOF_HEADER + 8 * f + 32 * c) return P3G_E_INVAL;
+  const uint8_t* p = in + PROOF_HEADER;
+  for (uint64_t i = 0; i < f + 4 * c; i++)
+    if (get_le64(p + 8 * i) >= P) return P3G_E_INVAL;
+  *nf = f;
+  *nc = c;
+  if (fields)
+    for (uint64_t i = 0; i < f; i++) fields[i] = get_le64(p + 8 * i);
+  if (commitments)
+    for (uint64_t i = 0; i < 4 * c; i++) commitments[i] = get_le64(p + 8 * (f + i));
+  return P3G_OK;
+}
+
+int p3g_transcript_encode(const p3g_transcript* ch, uint8_t* out, uint64_t cap, uint64_t* len) {
+  if (!ch) return P3G_E_INVAL;
+  return p3g_proof_encode(ch->tx.fields.data(), ch->tx.fields.size(), ch->tx.comms.data(), ch->tx.comms.size() / 4,
+                          out, cap, len);
+}
+
 int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t log_lde_height,
                           const uint64_t* eval_points, uint32_t n_points, p3g_tree* const* trees, uint64_t ntrees,
                           p3g_transcript* ch) {

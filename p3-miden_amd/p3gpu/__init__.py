@@ -204,6 +204,9 @@ def _load():
         "p3g_transcript_num_fields": (C.c_uint64, [_vp]),
         "p3g_transcript_num_commitments": (C.c_uint64, [_vp]),
         "p3g_transcript_data": (C.c_int, [_vp, _u64p, _u64p]),
+        "p3g_proof_encode": (C.c_int, [_u64p, C.c_uint64, _u64p, C.c_uint64, _vp, C.c_uint64, _u64p]),
+        "p3g_proof_decode": (C.c_int, [_vp, C.c_uint64, _u64p, _u64p, _u64p, _u64p]),
+        "p3g_transcript_encode": (C.c_int, [_vp, _vp, C.c_uint64, _u64p]),
         "p3g_open_with_channel": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, _vp]),
         "p3g_tree_prove_batch_cb": (C.c_int, [_vp, _u64p, C.c_uint64, C.POINTER(_Channel)]),
         "p3g_open_with_callbacks": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, C.POINTER(_Channel)]),
@@ -481,6 +484,14 @@ class ProverTranscript:
         _check(lib().p3g_transcript_grind(self.h, bits, _p(o)), "grind")
         return int(o[0])
 
+    def encode(self) -> bytes:
+        """The proof in the library's wire format (see encode_proof)."""
+        n = np.zeros(1, np.uint64)
+        _check(lib().p3g_transcript_encode(self.h, None, 0, _p(n)), "transcript_encode")
+        buf = C.create_string_buffer(max(1, int(n[0])))
+        _check(lib().p3g_transcript_encode(self.h, C.cast(buf, _vp), int(n[0]), _p(n)), "transcript_encode")
+        return buf.raw[: int(n[0])]
+
     def data(self):
         """TranscriptData: (fields [n], commitments [m, 4])."""
         nf = lib().p3g_transcript_num_fields(self.h)
@@ -492,7 +503,8 @@ class ProverTranscript:
 
 
 def proof_bytes(fields: np.ndarray, commitments: np.ndarray) -> bytes:
-    """Canonical proof encoding: field stream then commitment stream, each as little-endian u64."""
+    """The proof's words without a header (field stream, then commitment stream, each u64 LE): the
+    input of proof digests.  encode_proof is the self-describing wire format."""
     return u64(fields).astype("<u8").tobytes() + u64(commitments).ravel().astype("<u8").tobytes()
 
 
@@ -605,6 +617,32 @@ def commit_traces(lmcs: LmcsConfig, traces, log_blowup: int) -> LiftedMerkleTree
     _check(lib().p3g_commit_traces(lmcs.dev.h, lmcs.hash_kind, arr, len(traces), log_blowup,
                                    DEVICE_PTRS if dev else 0, C.byref(h)), "commit_traces")
     return LiftedMerkleTree(h, lmcs.hash_kind, lmcs.dev)
+
+
+def encode_proof(fields, commitments) -> bytes:
+    """TranscriptData (data.rs:8-14) as bytes: "P3GT", version 1 (u32 LE), the field count and
+    the commitment count (u64 LE each), then every canonical word as u64 LE."""
+    f = u64(np.asarray(fields, dtype=np.uint64).ravel())
+    c = u64(np.asarray(commitments, dtype=np.uint64).ravel())
+    if c.size % 4:
+        raise ValueError("commitments are 4 words each")
+    n = np.zeros(1, np.uint64)
+    fp, cp = (_p(f) if f.size else None), (_p(c) if c.size else None)
+    _check(lib().p3g_proof_encode(fp, f.size, cp, c.size // 4, None, 0, _p(n)), "proof_encode")
+    buf = C.create_string_buffer(max(1, int(n[0])))
+    _check(lib().p3g_proof_encode(fp, f.size, cp, c.size // 4, C.cast(buf, _vp), int(n[0]), _p(n)), "proof_encode")
+    return buf.raw[: int(n[0])]
+
+
+def decode_proof(data: bytes):
+    """Inverse of encode_proof: (fields [n], commitments [m, 4])."""
+    buf = C.create_string_buffer(bytes(data), max(1, len(data)))
+    nf, nc = np.zeros(1, np.uint64), np.zeros(1, np.uint64)
+    _check(lib().p3g_proof_decode(C.cast(buf, _vp), len(data), None, _p(nf), None, _p(nc)), "proof_decode")
+    f = np.zeros(max(1, int(nf[0])), np.uint64)
+    c = np.zeros(max(1, 4 * int(nc[0])), np.uint64)
+    _check(lib().p3g_proof_decode(C.cast(buf, _vp), len(data), _p(f), _p(nf), _p(c), _p(nc)), "proof_decode")
+    return f[: int(nf[0])].copy(), c[: 4 * int(nc[0])].reshape(int(nc[0]), 4).copy()
 
 
 def commit_aux_traces(lmcs: LmcsConfig, aux_traces, log_blowup: int) -> LiftedMerkleTree:

@@ -110,3 +110,56 @@ def test_proof_bytes_layout():
     c = np.array([[3, 4, 5, 6]], np.uint64)
     b = G.proof_bytes(f, c)
     assert len(b) == 48 and b[:8] == (1).to_bytes(8, "little") and b[16:24] == (3).to_bytes(8, "little")
+
+
+# ---------------------------------------------------------------- proof wire format
+def test_proof_encoding_known_bytes():
+    # header "P3GT", version 1, nf, nc; then canonical words, all little-endian
+    f = np.array([1, O.P - 1], np.uint64)
+    c = np.array([[2, 3, 4, 5]], np.uint64)
+    b = G.encode_proof(f, c)
+    want = (b"P3GT" + (1).to_bytes(4, "little") + (2).to_bytes(8, "little") + (1).to_bytes(8, "little")
+            + b"".join(int(x).to_bytes(8, "little") for x in [1, O.P - 1, 2, 3, 4, 5]))
+    assert b == want and len(b) == 24 + 8 * 2 + 32
+    df, dc = G.decode_proof(b)
+    assert df.tolist() == f.tolist() and dc.tolist() == c.tolist()
+    assert G.encode_proof([], np.zeros((0, 4), np.uint64)) == b"P3GT" + (1).to_bytes(4, "little") + bytes(16)
+
+
+def test_proof_encoding_round_trips_an_oracle_proof():
+    tr = [O.rand_matrix(5, 1 << 6, 9)]
+    _, _, f, c, _ = O.pcs_trace_prove(O.PcsParams(1, 1, 2, 0, 0, 8, 0), O.HASH_P2, tr)
+    b = G.encode_proof(f, c)
+    assert len(b) == 24 + 8 * f.size + 32 * c.shape[0]
+    assert b[24:] == G.proof_bytes(f, c)  # the header-free word stream follows the header
+    df, dc = G.decode_proof(b)
+    assert np.array_equal(df, f) and np.array_equal(dc, c)
+
+
+def test_transcript_encode_matches_encode_proof():
+    tx = G.ProverTranscript(G.CHALLENGER_DUPLEX_P2)
+    tx.send_field_slice([7, 8, 9])
+    tx.send_commitment([1, 2, 3, 4])
+    tx.hint_field_slice([10])
+    f, c = tx.data()
+    assert tx.encode() == G.encode_proof(f, c)
+
+
+def test_proof_decoding_rejects_malformed_input():
+    good = G.encode_proof([1, 2, 3], [[4, 5, 6, 7]])
+    bad = [
+        good[:-1],                                        # truncated
+        good + b"\0",                                     # trailing byte
+        b"P3GX" + good[4:],                               # magic
+        good[:4] + (2).to_bytes(4, "little") + good[8:],  # version
+        good[:8] + (4).to_bytes(8, "little") + good[16:], # field count does not match the length
+        good[:24] + O.P.to_bytes(8, "little") + good[32:],  # non-canonical word
+        b"",
+    ]
+    for b in bad:
+        with pytest.raises(G.P3GError) as e:
+            G.decode_proof(b)
+        assert e.value.code == 1
+    with pytest.raises(G.P3GError) as e:
+        G.encode_proof([O.P], np.zeros((0, 4), np.uint64))
+    assert e.value.code == 1
