@@ -171,17 +171,49 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs& a, uint64_t bid) {
 // Register-blocked pass of LR stages (5 <= LR <= 8) on a 2^LR-row x 16-column
 // tile with 2^LR threads: each thread owns one column and 16 rows, performs the
 // stages on x bits 0..3 in registers (layout A: x = 16 g + k), exchanges through
-// an XOR-swizzled LDS tile once, and performs the stages on x bits 4..LR-1 in
-// registers (layout B: x = g + G k with G = 2^(LR-4)).  Inverse passes run
-// A then B, forward passes B then A.
-__device__ __forceinline__ uint32_t swz(uint32_t x, uint32_t c) { return x * 16 + (c ^ ((x >> 4) & 15)); }
+// an LDS tile once, and performs the stages on x bits 4..LR-1 in registers
+// (layout B: x = g + G k with G = 2^(LR-4)).  Inverse passes run A then B,
+// forward passes B then A.
+//
+// The exchange moves the low 32-bit halves, then the high halves, through a
+// tile of one 32-bit word per element: half the LDS of a 64-bit tile, so more
+// workgroups fit on a CU (a 128-thread pass: 4 -> up to 7 waves per SIMD).
+// Word of element x, column c: rows of 16 words, XOR-swizzled by the row
+// group, with 16 words of spacing after every 16 rows so that the row groups
+// of one access land in different banks.
+constexpr uint32_t xchg_words(uint32_t rows) { return rows * 16 + (rows / 16) * 16; }
+__device__ __forceinline__ uint32_t swz(uint32_t x, uint32_t c) {
+  return x * 16 + (c ^ ((x >> 4) & 15)) + (x >> 4) * 16;
+}
+
+// Every thread writes its 16 values at wpos(k) and reads back the 16 at
+// rpos(k), low halves first.  The tile must be free on entry; it is in use
+// until the caller's next barrier.
+template <typename WP, typename RP>
+__device__ __forceinline__ void xchg_halves(uint32_t* tile, uint64_t (&r)[16], WP wpos, RP rpos) {
+  uint32_t hi[16], lo[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    tile[wpos(k)] = (uint32_t)r[k];
+    hi[k] = (uint32_t)(r[k] >> 32);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) lo[k] = tile[rpos(k)];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) tile[wpos(k)] = hi[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) r[k] = ((uint64_t)tile[rpos(k)] << 32) | lo[k];
+}
 
 template <uint32_t MODE, int LR>
 __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
   constexpr uint32_t T = 1u << LR;      // threads = rows per tile
   constexpr uint32_t G = 1u << (LR - 4);  // row groups in layout A / row stride in layout B
   constexpr int KB0 = 8 - LR;             // layout B: k bits below KB0 belong to layout-A stages
-  __shared__ uint64_t tile[T * 16];
+  __shared__ uint32_t tile[xchg_words(T)];
   __shared__ uint64_t tws[T];
   __shared__ uint64_t tfa[16];  // layout-A twist factors (the same for every thread)
   __shared__ uint64_t tfb[T];   // layout-B twist factors, entry 16 g + k
@@ -290,11 +322,7 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           r[k1] = gl_sub(x, y);
         }
       }
-#pragma unroll
-      for (int k = 0; k < 16; k++) tile[swz(16 * g + k, c)] = r[k];
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < 16; k++) r[k] = tile[swz(g + G * k, c)];
+      xchg_halves(tile, r, [&](int k) { return swz(16 * g + k, c); }, [&](int k) { return swz(g + G * k, c); });
       // stages on x bits 4..LR-1, layout B (x bit t is k bit t - (LR - 4))
 #pragma unroll
       for (int k = 0; k < 16; k++)
@@ -336,11 +364,7 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
       for (int k = 0; k < 16; k++)
         if (k >> KB0) r[k] = gl_mul(r[k], tfb[16 * g + k]);
       __syncthreads();  // previous repetition's readers are done with the tile
-#pragma unroll
-      for (int k = 0; k < 16; k++) tile[swz(g + G * k, c)] = r[k];
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < 16; k++) r[k] = tile[swz(16 * g + k, c)];
+      xchg_halves(tile, r, [&](int k) { return swz(g + G * k, c); }, [&](int k) { return swz(16 * g + k, c); });
 #pragma unroll
       for (int q = 3; q >= 0; q--) {
 #pragma unroll
@@ -374,9 +398,8 @@ __global__ void __launch_bounds__(NTT_T) __attribute__((amdgpu_waves_per_eu(4)))
   ntt_pass_body(a, blockIdx.x);
 }
 
-// Register-blocked kernels: 2^LR threads per workgroup.  The 8-stage forward
-// passes are held to 4 waves per EU; FWD_FIRST gets more registers (its coset
-// loop keeps the scale state live) at 3 waves.
+// Register-blocked kernels: 2^LR threads per workgroup, at least 4 waves per
+// EU (3 for FWD_FIRST, whose coset loop keeps the coefficient tile live).
 template <uint32_t MODE, int LR>
 __global__ void __launch_bounds__(1 << LR) __attribute__((amdgpu_waves_per_eu(MODE == FWD_FIRST ? 3 : 4)))
 ntt_reg_kernel(NttArgs a) {
