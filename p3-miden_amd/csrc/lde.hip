@@ -489,6 +489,15 @@ void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, 
   if (logn + log_blowup > 32) throw ApiError(E_RANGE, "LDE height exceeds the field two-adicity");
   uint32_t logC = 0;
   while ((2u << logC) <= NTT_C && (2ull << logC) <= w) logC++;
+  if (logC == 3) {
+    // widths 8..15: register tiles of 16 columns, those past the width masked,
+    // beat the per-stage LDS kernel on 8-column tiles (a 2^22 x 8 -> 2^24 LDE
+    // and its tree: 18.0 -> 17.6 ms); at widths 4..7 three quarters of the lanes
+    // would idle and the LDS kernel stays faster.  Every pass must be 5..8 stages.
+    bool reg = true;
+    for (uint32_t k : ntt_pass_split(logn, NTT_K)) reg = reg && ntt_reg_ok(k, 4);
+    if (reg) logC = 4;
+  }
   const uint32_t C = 1u << logC;
   const uint32_t ncol = (uint32_t)((w + C - 1) / C);
   const uint32_t B = 1u << log_blowup;

@@ -85,3 +85,26 @@ def test_batched_plan_equals_per_trace_path_c3(device):
     root, z, f, c = plan.prove(G.PcsParams(*params))
     logL = H.bit_length() - 1
     assert O.pcs_trace_verify(O.PcsParams(*params), O.HASH_P2, root, [w for _, w in shapes], logL, f, c) == 0
+
+
+def test_quotient_commit_contains_chunk_values_full_size(device):
+    # quotient of the 2^22-row trace at constraint degree 4, log_blowup 2.  Chunk t's
+    # input column holds q_t on g w_J^t H, and g w_J^t w_H^r = g w_K^((B/D) t + B r)
+    # lies in gK: the committed matrix (rows bit-reversed) repeats every input value
+    logn, d, logb = 22, 4, 2
+    n, B = 1 << logn, 1 << logb
+    q = device.alloc_matrix(n, 2 * d)
+    q.fill_random(31)
+    qh = q.download()
+    t = G.commit_quotient(G.LmcsConfig(device, G.HASH_POSEIDON2), q, d, logb)
+    ptr, h, w = t.matrix(0)
+    assert (h, w) == (n * B, 2 * d)
+    lde = G.DeviceMatrix(device, ptr, h, w, owned=False).download()
+    rows = _bitrev_perm(logn + logb)  # natural index -> stored row is bitrev
+    r = np.arange(n, dtype=np.int64)
+    for ti in range(d):
+        nat = (B // d) * ti + B * r
+        got = lde[rows[nat]][:, 2 * ti:2 * ti + 2]
+        assert np.array_equal(got, qh[:, 2 * ti:2 * ti + 2]), ti
+    t.free()
+    q.free()
