@@ -375,9 +375,12 @@ def bench_small(G, W, dev, n_proofs, workers):
     plain = [G.prove(cfg, params, [t]) for t in traces]
     plain_s = time.perf_counter() - t0
     srv = G.Server(0, workers=workers, hbm_budget_bytes=64 * 10**9, max_batch=64)
-    # warm every worker context and the shared block pool with the whole set
-    for r in [srv.submit([t], params) for t in traces]:
-        srv.result(r)
+    # warm every worker context and the shared block pool with the whole set,
+    # twice: the four workers interleave differently from pass to pass, and
+    # the second pass fills the size classes the first one's mix did not use
+    for _ in range(2):
+        for r in [srv.submit([t], params) for t in traces]:
+            srv.result(r)
     s0 = srv.stats()
     t0 = time.perf_counter()
     res = [srv.result(tk) for tk in [srv.submit([t], params) for t in traces]]
