@@ -422,7 +422,11 @@ int p3g_commit_quotient(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* q_eval
     check_hash(hash_kind);
     P3G_CHECK_HIP(hipSetDevice(c.device));
     if (n == 0 || d == 0) throw ApiError(P3G_E_EMPTY, "empty quotient");
-    if (d > (1ull << 31) || n > (1ull << 32)) throw ApiError(P3G_E_RANGE, "quotient shape out of range");
+    // shape checks before any upload (commit_quotient repeats them for its other callers)
+    if (n & (n - 1)) throw ApiError(P3G_E_NOT_POW2, "quotient trace height must be a power of two");
+    if (d & (d - 1)) throw ApiError(P3G_E_NOT_POW2, "constraint degree must be a power of two");
+    if (log_blowup > 32 || d > (1ull << log_blowup)) throw ApiError(P3G_E_INVAL, "blowup must be at least the constraint degree");
+    if (log2_exact(n) + log_blowup > 32) throw ApiError(P3G_E_RANGE, "LDE height exceeds the field two-adicity");
     const bool dev = flags & P3G_DEVICE_PTRS;
     const uint64_t* dq = dev ? q_evals : to_device(c, q_evals, n * 2 * d);
     Tree* t = nullptr;
