@@ -455,6 +455,22 @@ def bench_quotient(G, dev, steps, warmup):
         out.append({"shape": f"N=2^{logn}, D={d}, log_blowup={logb} (LDE {L} x {2 * d})", "ms": round(ms, 3),
                     "gbps": round(algo / (ms * 1e-3) / 1e9, 1),
                     "poseidon2_perms_per_s": round(perms / (ms * 1e-3), 1)})
+        if logn == 22:
+            # the accumulation steps ahead of the commit, on the same n*d elements:
+            # each reads and writes 16 bytes per element (HBM-bound)
+            helpers = {}
+            for name, fn in (("divide_by_vanishing", lambda: G.divide_by_vanishing(q, n, d, 7)),
+                             ("cyclic_extend_and_scale (n*d/2 -> n*d)",
+                              lambda: G.cyclic_extend_and_scale(q, n * d // 2, n * d, (3, 5)))):
+                fn()
+                dev.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    fn()
+                dev.synchronize()
+                hms = (time.perf_counter() - t0) * 1e3 / reps
+                helpers[name] = {"ms": round(hms, 3), "gbps": round(32 * n * d / (hms * 1e-3) / 1e9, 1)}
+            out[-1]["accumulation_helpers"] = helpers
         q.free()
     dev.trim()
     return {"workload": "quotient commitment of N*D extension elements resident in HBM, Poseidon2 LMCS (aligned)",

@@ -149,6 +149,17 @@ int p3g_commit_traces(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* traces
 int p3g_commit_quotient(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* q_evals, uint64_t n, uint64_t d,
                         uint32_t log_blowup, uint32_t flags, p3g_tree** out);
 
+/* ---- quotient accumulation (p3-miden-lifted-stark/src/prover/quotient.rs:28-101) ----
+ * On device buffers of extension elements (c0, c1 words), on the context's stream.
+ * extend_scale: cyclic_extend_and_scale in place on a buffer of `target` elements whose
+ *   first m hold the accumulator (m == 0 zero-fills; else scale by beta, repeat to target).
+ *   m and target powers of two, m <= target.
+ * divide_vanishing: divide_by_vanishing_in_place on the n*d quotient values in natural
+ *   order on the coset shift*J (|J| = n*d; shift = LiftedCoset::lde_shift()).
+ *   P3G_E_INVAL when the vanishing polynomial is zero there (shift^n a d-th root of unity). */
+int p3g_quotient_extend_scale(p3g_ctx* ctx, uint64_t* acc, uint64_t m, uint64_t target, const uint64_t beta[2]);
+int p3g_quotient_divide_vanishing(p3g_ctx* ctx, uint64_t* num, uint64_t n, uint64_t d, uint64_t shift);
+
 /* ---- transcript (ProverChannel, p3-miden-transcript/src/prover.rs:53-145) ---- */
 int p3g_transcript_create(uint32_t challenger_kind, p3g_transcript** out);
 void p3g_transcript_free(p3g_transcript* ch);

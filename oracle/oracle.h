@@ -185,6 +185,10 @@ orc_tree* orc_commit_traces(int hash_kind, const uint64_t* const* traces, const 
  * Returns -1 (NULL) unless N and D are powers of two with D <= B. */
 int orc_quotient_lde(const uint64_t* q_evals, size_t n, size_t d, unsigned log_blowup, uint64_t* out);
 orc_tree* orc_commit_quotient(int hash_kind, const uint64_t* q_evals, size_t n, size_t d, unsigned log_blowup);
+/* cyclic_extend_and_scale (quotient.rs:40-55), out of place (m == 0 zero-fills) */
+int orc_cyclic_extend_scale(const uint64_t* acc, size_t m, const uint64_t* beta, size_t target, uint64_t* out);
+/* divide_by_vanishing_in_place (quotient.rs:72-101): n*d EF, coset shift s; -1 if Z_H vanishes */
+int orc_divide_by_vanishing(uint64_t* num, size_t n, size_t d, uint64_t shift);
 
 void orc_set_threads(int n);
 int orc_get_threads(void);

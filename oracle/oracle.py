@@ -108,6 +108,8 @@ def _load():
         "orc_commit_traces": (vp, [C.c_int, C.POINTER(_u64p), _szp, _szp, C.c_size_t, C.c_uint]),
         "orc_quotient_lde": (C.c_int, [_u64p, C.c_size_t, C.c_size_t, C.c_uint, _u64p]),
         "orc_commit_quotient": (vp, [C.c_int, _u64p, C.c_size_t, C.c_size_t, C.c_uint]),
+        "orc_cyclic_extend_scale": (C.c_int, [_u64p, C.c_size_t, _u64p, C.c_size_t, _u64p]),
+        "orc_divide_by_vanishing": (C.c_int, [_u64p, C.c_size_t, C.c_size_t, C.c_uint64]),
         "orc_set_threads": (None, [C.c_int]),
         "orc_get_threads": (C.c_int, []),
     }
@@ -420,6 +422,26 @@ def commit_quotient(kind: int, q_evals: np.ndarray, d: int, log_blowup: int) -> 
     if not h:
         raise ValueError("quotient: N and D must be powers of two with D <= blowup")
     return Tree(kind, None, _handle=h)
+
+
+def cyclic_extend_scale(acc: np.ndarray, beta, target: int) -> np.ndarray:
+    """cyclic_extend_and_scale (quotient.rs:40-55): acc is (m, 2) EF; returns (target, 2)."""
+    a = u64(acc).reshape(-1, 2)
+    b = u64(np.asarray(beta, dtype=np.uint64).ravel())
+    out = np.zeros((target, 2), np.uint64)
+    if lib.orc_cyclic_extend_scale(_p(a) if a.size else None, a.shape[0], _p(b), target, _p(out)) != 0:
+        raise ValueError("cyclic_extend_scale: sizes must be powers of two with m <= target")
+    return out
+
+
+def divide_by_vanishing(num: np.ndarray, d: int, shift: int) -> np.ndarray:
+    """divide_by_vanishing_in_place (quotient.rs:72-101) on an (n*d, 2) EF array; returns a copy."""
+    a = u64(num).reshape(-1, 2).copy()
+    if d <= 0 or a.shape[0] % d:
+        raise ValueError("divide_by_vanishing: length must be a multiple of D")
+    if lib.orc_divide_by_vanishing(_p(a), a.shape[0] // d, d, shift) != 0:
+        raise ValueError("divide_by_vanishing: bad shape or the vanishing polynomial is zero on the coset")
+    return a
 
 
 # ---------------------------------------------------------------- PCS

@@ -16,6 +16,14 @@
  *   4. plain (unshifted) DFT of size N*B on every column               (:223-227)
  *   5. rows bit-reversed, then `build_aligned_tree` of the single matrix (:229-235)
  * B >= D is required (:154).
+ *
+ * Also the accumulation helpers ahead of it (quotient.rs:28-101):
+ *   cyclic_extend_and_scale: an empty accumulator becomes target_len zeros;
+ *     otherwise every element is multiplied by beta, then the buffer doubles
+ *     by repetition up to target_len                                   (:40-55)
+ *   divide_by_vanishing_in_place: numerator[i] / Z_H(x_i), Z_H(X) = X^N - 1,
+ *     x_i = s * omega_J^i on the quotient coset sJ (|J| = N*D); Z_H takes the
+ *     D values s^N * omega_D^t - 1, indexed by i mod D              (:72-101)
  */
 #include <stdlib.h>
 #include <string.h>
@@ -62,4 +70,52 @@ orc_tree* orc_commit_quotient(int hash_kind, const uint64_t* q_evals, size_t n, 
     orc_tree* t = orc_lmcs_build(hash_kind, mats, &h, &w, 1, orc_hash_alignment(hash_kind));
     free(lde);
     return t;
+}
+
+/* cyclic_extend_and_scale (quotient.rs:40-55), out of place: out has target
+ * elements; m == 0 zero-fills.  EF elements are (c0, c1) word pairs. */
+int orc_cyclic_extend_scale(const uint64_t* acc, size_t m, const uint64_t* beta, size_t target, uint64_t* out) {
+    if (target == 0 || (target & (target - 1)) || (m && ((m & (m - 1)) || m > target))) return -1;
+    if (m == 0) {
+        memset(out, 0, target * 2 * sizeof(uint64_t));
+        return 0;
+    }
+    const ef2 b = ef_make(beta[0], beta[1]);
+    for (size_t i = 0; i < m; i++) {
+        ef2 v = ef_mul(ef_make(acc[2 * i], acc[2 * i + 1]), b);
+        out[2 * i] = v.c0;
+        out[2 * i + 1] = v.c1;
+    }
+    for (size_t len = m; len < target; len *= 2) memcpy(out + 2 * len, out, len * 2 * sizeof(uint64_t));
+    return 0;
+}
+
+/* divide_by_vanishing_in_place (quotient.rs:72-101) on n*d EF elements; the
+ * quotient coset shift s is coset.lde_shift().  -1 when a Z_H value is zero. */
+int orc_divide_by_vanishing(uint64_t* num, size_t n, size_t d, uint64_t shift) {
+    if (n == 0 || (n & (n - 1)) || d == 0 || (d & (d - 1))) return -1;
+    const unsigned logn = log2_exact(n), logd = log2_exact(d);
+    gl s_pow_n = shift;
+    for (unsigned i = 0; i < logn; i++) s_pow_n = gl_mul(s_pow_n, s_pow_n);
+    const gl wd = gl_two_adic_generator(logd);
+    gl* inv = (gl*)malloc(d * sizeof(gl));
+    gl x = 1;
+    for (size_t t = 0; t < d; t++) {
+        const gl z = gl_sub(gl_mul(s_pow_n, x), 1);
+        if (z == 0) {
+            free(inv);
+            return -1;
+        }
+        inv[t] = gl_inv(z);
+        x = gl_mul(x, wd);
+    }
+    const size_t total = n * d;
+    #pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < total; i++) {
+        const gl f = inv[i & (d - 1)];
+        num[2 * i] = gl_mul(num[2 * i], f);
+        num[2 * i + 1] = gl_mul(num[2 * i + 1], f);
+    }
+    free(inv);
+    return 0;
 }

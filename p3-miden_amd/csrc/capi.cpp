@@ -441,6 +441,24 @@ int p3g_commit_quotient(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* q_eval
   });
 }
 
+int p3g_quotient_extend_scale(p3g_ctx* ctx, uint64_t* acc, uint64_t m, uint64_t target, const uint64_t beta[2]) {
+  if (!ctx || !acc || !beta || beta[0] >= P || beta[1] >= P) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    quotient_extend_scale(c, reinterpret_cast<ef2*>(acc), m, target, ef2{beta[0], beta[1]}, c.stream);
+  });
+}
+
+int p3g_quotient_divide_vanishing(p3g_ctx* ctx, uint64_t* num, uint64_t n, uint64_t d, uint64_t shift) {
+  if (!ctx || !num || shift >= P) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    quotient_divide_vanishing(c, reinterpret_cast<ef2*>(num), n, d, shift, c.stream);
+  });
+}
+
 int p3g_transcript_create(uint32_t kind, p3g_transcript** out) {
   if (!out) return P3G_E_INVAL;
   if (kind != CH_DUPLEX_P2 && kind != CH_KECCAK) return P3G_E_INVAL;

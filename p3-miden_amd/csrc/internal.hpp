@@ -305,6 +305,9 @@ Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, 
 // read as the n x 2d base matrix it already is
 Tree* commit_quotient(Ctx& ctx, uint32_t hash, const uint64_t* d_q, uint64_t n, uint64_t d, unsigned log_blowup,
                       hipStream_t s);
+// cyclic_extend_and_scale / divide_by_vanishing_in_place on device buffers
+void quotient_extend_scale(Ctx& ctx, ef2* acc, uint64_t m, uint64_t target, ef2 beta, hipStream_t s);
+void quotient_divide_vanishing(Ctx& ctx, ef2* num, uint64_t n, uint64_t d, uint64_t shift, hipStream_t s);
 
 void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned log_lde_height, const ef2* z, int npts,
                        const std::vector<Tree*>& trees, Channel& tx, hipStream_t s);

@@ -186,6 +186,8 @@ def _load():
         "p3g_tree_prove_batch": (C.c_int, [_vp, _u64p, C.c_uint64, _vp]),
         "p3g_tree_free": (None, [_vp]),
         "p3g_commit_traces": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+        "p3g_quotient_extend_scale": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, _u64p]),
+        "p3g_quotient_divide_vanishing": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, C.c_uint64]),
         "p3g_commit_quotient": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
                                           C.POINTER(_vp)]),
         "p3g_transcript_create": (C.c_int, [C.c_uint32, C.POINTER(_vp)]),
@@ -660,6 +662,26 @@ def commit_aux_traces(lmcs: LmcsConfig, aux_traces, log_blowup: int) -> LiftedMe
                 raise ValueError("host aux traces are (n, w, 2) arrays of extension elements")
             flat.append(a.reshape(a.shape[0], 2 * a.shape[1]))
     return commit_traces(lmcs, flat, log_blowup)
+
+
+def cyclic_extend_and_scale(acc: "DeviceMatrix", m: int, target: int, beta) -> None:
+    """cyclic_extend_and_scale (p3-miden-lifted-stark/src/prover/quotient.rs:40-55) in place on a
+    device buffer of at least `target` extension elements (rows of 2 words) whose first m hold the
+    accumulator; m == 0 zero-fills."""
+    if acc.height * acc.width < 2 * target:
+        raise ValueError("buffer smaller than target extension elements")
+    b = u64(np.asarray(beta, dtype=np.uint64).ravel())
+    _check(lib().p3g_quotient_extend_scale(acc.dev.h, C.cast(acc.ptr, _u64p), m, target, _p(b)),
+           "quotient_extend_scale")
+
+
+def divide_by_vanishing(num: "DeviceMatrix", n: int, d: int, shift: int) -> None:
+    """divide_by_vanishing_in_place (quotient.rs:72-101) in place on n*d extension elements of the
+    quotient coset shift*J, natural order (device buffer, rows of 2 words)."""
+    if num.height * num.width < 2 * n * d:
+        raise ValueError("buffer smaller than n*d extension elements")
+    _check(lib().p3g_quotient_divide_vanishing(num.dev.h, C.cast(num.ptr, _u64p), n, d, shift),
+           "quotient_divide_vanishing")
 
 
 def commit_quotient(lmcs: LmcsConfig, q_evals, d: int, log_blowup: int) -> LiftedMerkleTree:

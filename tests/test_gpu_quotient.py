@@ -110,3 +110,86 @@ def test_commit_aux_traces_is_commit_traces_of_the_flattened_traces(device):
         assert t.root().tolist() == want.root().tolist()
         assert t.height() == (1 << 9) << 2
         t.free()
+
+
+# ---------------------------------------------------------------- accumulation helpers (quotient.rs:28-101)
+def _dev_ef(device, a, rows=None):
+    """A device buffer of `rows` extension elements (2 words each) holding a at its start."""
+    a = np.asarray(a, np.uint64).reshape(-1, 2)
+    rows = rows or a.shape[0]
+    buf = np.zeros((rows, 2), np.uint64)
+    buf[: a.shape[0]] = a
+    return device.upload(buf)
+
+
+@pytest.mark.parametrize("m,target", [(0, 8), (1, 1), (1, 64), (4, 4), (4, 1 << 12), (1 << 10, 1 << 14)])
+def test_cyclic_extend_and_scale_matches_oracle(device, m, target):
+    acc = O.rand_matrix(60 + m, m, 2) if m else np.zeros((0, 2), np.uint64)
+    beta = (123456789, O.P - 5)
+    buf = _dev_ef(device, acc, target)
+    G.cyclic_extend_and_scale(buf, m, target, beta)
+    assert np.array_equal(buf.download().reshape(-1, 2), O.cyclic_extend_scale(acc, beta, target))
+    buf.free()
+
+
+@pytest.mark.parametrize("logn,d,shift", [(0, 1, 7), (4, 2, 7), (10, 4, 49), (10, 8, 7), (16, 2, 7), (12, 1, 2401)])
+def test_divide_by_vanishing_matches_oracle(device, logn, d, shift):
+    n = 1 << logn
+    num = O.rand_matrix(70 + logn, n * d, 2)
+    buf = _dev_ef(device, num)
+    G.divide_by_vanishing(buf, n, d, shift)
+    assert np.array_equal(buf.download().reshape(-1, 2), O.divide_by_vanishing(num, d, shift))
+    buf.free()
+
+
+def test_quotient_pipeline_matches_oracle(device):
+    # prove_multi's quotient steps (prover/mod.rs:355-434) on given constraint
+    # evaluations: accumulate two instances in ascending height with Horner folding in
+    # beta, divide by Z_H on the max quotient coset (shift GENERATOR), commit the chunks
+    d, logb = 4, 2
+    n_small, n_big = 1 << 6, 1 << 9
+    beta = (987654321, 42)
+    e1 = O.rand_matrix(81, n_small * d, 2)
+    e2 = O.rand_matrix(82, n_big * d, 2)
+    def addmod(a, b):
+        return ((a.astype(object) + b.astype(object)) % O.P).astype(np.uint64)
+
+    # oracle: empty accumulator -> zeros, + e1; extend(acc * beta), + e2; divide; commit
+    acc = addmod(O.cyclic_extend_scale(np.zeros((0, 2), np.uint64), beta, n_small * d), e1)
+    acc = addmod(O.cyclic_extend_scale(acc, beta, n_big * d), e2)
+    acc = O.divide_by_vanishing(acc, d, 7)
+    want = O.commit_quotient(O.HASH_P2, acc, d, logb)
+    # GPU: the same steps on one device buffer
+    buf = _dev_ef(device, np.zeros((0, 2), np.uint64), n_big * d)
+    G.cyclic_extend_and_scale(buf, 0, n_small * d, beta)  # empty accumulator: zeros
+    host = buf.download().reshape(-1, 2)
+    host[: n_small * d] = addmod(host[: n_small * d], e1)  # instance 1's constraint evaluations
+    buf.free()
+    buf = _dev_ef(device, host[: n_small * d], n_big * d)
+    G.cyclic_extend_and_scale(buf, n_small * d, n_big * d, beta)
+    host = buf.download().reshape(-1, 2)
+    host = addmod(host, e2)  # instance 2's constraint evaluations
+    buf.free()
+    buf = device.upload(host.reshape(n_big, 2 * d))
+    G.divide_by_vanishing(buf, n_big, d, 7)
+    t = G.commit_quotient(G.LmcsConfig(device, G.HASH_POSEIDON2), buf, d, logb)
+    assert t.root().tolist() == want.root().tolist()
+    t.free()
+    buf.free()
+
+
+def test_quotient_helpers_reject_bad_arguments(device):
+    buf = _dev_ef(device, np.zeros((16, 2), np.uint64))
+    with pytest.raises(G.P3GError) as e:
+        G.divide_by_vanishing(buf, 8, 2, 1)  # shift 1: the coset meets H, Z_H = 0 there
+    assert e.value.code == 1
+    with pytest.raises(G.P3GError) as e:
+        G.divide_by_vanishing(buf, 6, 2, 7)
+    assert e.value.code == 2
+    with pytest.raises(G.P3GError) as e:
+        G.cyclic_extend_and_scale(buf, 16, 8, (1, 0))  # target below the accumulator
+    assert e.value.code == 1
+    with pytest.raises(G.P3GError) as e:
+        G.cyclic_extend_and_scale(buf, 3, 16, (1, 0))
+    assert e.value.code == 2
+    buf.free()

@@ -542,3 +542,50 @@ def test_quotient_rejects_blowup_below_degree():
         O.quotient_lde(q, 4, 1)
     with pytest.raises(ValueError):
         O.quotient_lde(np.zeros((15, 2), np.uint64), 2, 1)
+
+
+# ---------------------------------------------------------------- quotient accumulation (closed forms)
+@pytest.mark.parametrize("logn,d", [(0, 1), (2, 2), (3, 4), (4, 8), (5, 1)])
+def test_divide_by_vanishing_recovers_the_quotient(logn, d):
+    # quotient.rs:72-101 on the quotient coset sJ, |J| = N*D.  Numerators built as
+    # Z_H(x) * q(x), Z_H(X) = X^N - 1, by direct evaluation: the division must give
+    # q(x) back, whatever shortcut the oracle takes (D distinct values of Z_H)
+    rng = random.Random(500 + logn * 8 + d)
+    n = 1 << logn
+    log_d = d.bit_length() - 1
+    shift = pow(7, 1 << rng.randrange(3), P)  # g^(2^lift) as LiftedCoset::lde_shift
+    w_j = pow(7, (P - 1) >> (logn + log_d), P)
+    q = [[rng.randrange(P) for _ in range(5)] for _ in range(2)]  # EF coefficients of a small q
+    num = np.zeros((n * d, 2), np.uint64)
+    want = np.zeros((n * d, 2), np.uint64)
+    for i in range(n * d):
+        x = shift * pow(w_j, i, P) % P
+        z = (pow(x, n, P) - 1) % P
+        for c in range(2):
+            v = _poly_eval(q[c], x)
+            want[i, c] = v
+            num[i, c] = v * z % P
+    got = O.divide_by_vanishing(num, d, shift)
+    assert np.array_equal(got, want)
+
+
+def test_divide_by_vanishing_rejects_a_coset_meeting_h():
+    # shift 1: the quotient domain contains H's points, where Z_H = 0
+    with pytest.raises(ValueError):
+        O.divide_by_vanishing(np.ones((8, 2), np.uint64), 2, 1)
+
+
+def test_cyclic_extend_scale_definition():
+    # quotient.rs:40-55: empty -> zeros; else acc * beta, repeated to target_len
+    rng = random.Random(77)
+    beta = (rng.randrange(P), rng.randrange(P))
+    acc = np.array([[rng.randrange(P), rng.randrange(P)] for _ in range(4)], np.uint64)
+    out = O.cyclic_extend_scale(acc, beta, 16)
+    for i in range(16):
+        a = (int(acc[i % 4, 0]), int(acc[i % 4, 1]))
+        assert tuple(int(v) for v in out[i]) == _ef_mul(a, beta)
+    assert np.array_equal(O.cyclic_extend_scale(np.zeros((0, 2), np.uint64), beta, 8), np.zeros((8, 2), np.uint64))
+    # Horner folding over two traces: acc = (acc * beta) extended + N_2
+    assert np.array_equal(O.cyclic_extend_scale(acc, (1, 0), 4), acc)
+    with pytest.raises(ValueError):
+        O.cyclic_extend_scale(acc, beta, 2)  # target below the accumulator length
