@@ -169,12 +169,21 @@ __global__ void __launch_bounds__(T) keccak_leaf_kernel(LeafArgs a) {
   keccak_leaf_body(a, (uint64_t)blockIdx.x * T + threadIdx.x);
 }
 
-// ---- job-indexed forms for batches of same-shaped trees (blockIdx.y = job)
-__global__ void __launch_bounds__(128) p2_leaf_many_kernel(const LeafArgs* __restrict__ args) {
-  p2_leaf_body(args[blockIdx.y], (uint64_t)blockIdx.x * 128 + threadIdx.x);
+// ---- job-indexed forms for batches of same-shaped trees (blockIdx.y = job).
+// The Poseidon2 constants come as a kernel argument, not through the job's
+// descriptor: a pointer read from memory kept them in vector registers
+// (p2_leaf_many_kernel: 112 VGPRs, 4 waves per SIMD; 82 and 5 this way).
+__global__ void __launch_bounds__(128) p2_leaf_many_kernel(const LeafArgs* __restrict__ args,
+                                                        const P2Consts* __restrict__ p2) {
+  LeafArgs a = args[blockIdx.y];
+  a.p2 = p2;
+  p2_leaf_body(a, (uint64_t)blockIdx.x * 128 + threadIdx.x);
 }
-__global__ void __launch_bounds__(256) p2_leaf_coop_many_kernel(const LeafArgs* __restrict__ args) {
-  p2_leaf_coop_body(args[blockIdx.y], ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4, threadIdx.x & 15);
+__global__ void __launch_bounds__(256) p2_leaf_coop_many_kernel(const LeafArgs* __restrict__ args,
+                                                             const P2Consts* __restrict__ p2) {
+  LeafArgs a = args[blockIdx.y];
+  a.p2 = p2;
+  p2_leaf_coop_body(a, ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4, threadIdx.x & 15);
 }
 __global__ void __launch_bounds__(128) keccak_leaf_many_kernel(const LeafArgs* __restrict__ args) {
   keccak_leaf_body(args[blockIdx.y], (uint64_t)blockIdx.x * 128 + threadIdx.x);
@@ -669,9 +678,10 @@ std::vector<Tree*> lmcs_build_many(Ctx& ctx, uint32_t hash, const std::vector<st
         // while the whole launch (all jobs) is too small to fill the device
         if (hash == HASH_P2 && h * k <= COOP_LEAF_MAX)
           hipLaunchKernelGGL(p2_leaf_coop_many_kernel, dim3((uint32_t)((h * 16 + 255) / 256), (uint32_t)k), dim3(256),
-                             0, s, da);
+                             0, s, da, ctx.dev_p2);
         else if (hash == HASH_P2)
-          hipLaunchKernelGGL(p2_leaf_many_kernel, dim3((uint32_t)((h + 127) / 128), (uint32_t)k), dim3(128), 0, s, da);
+          hipLaunchKernelGGL(p2_leaf_many_kernel, dim3((uint32_t)((h + 127) / 128), (uint32_t)k), dim3(128), 0, s, da,
+                             ctx.dev_p2);
         else
           hipLaunchKernelGGL(keccak_leaf_many_kernel, dim3((uint32_t)((h + 127) / 128), (uint32_t)k), dim3(128), 0, s,
                              da);
