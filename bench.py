@@ -360,7 +360,10 @@ def bench_c5(G, W, dev, n_requests, workers):
 
 
 # ------------------------------------------------------------------ many small proofs
-def bench_small(G, W, dev, n_proofs, workers):
+BATCH_WINDOW_MS = 2.0
+
+
+def bench_small(G, W, dev, n_proofs, workers, max_batch=128):
     """n_proofs independent proofs of 2^14 x 32 traces (the CPU-sized configuration's shape):
     one after another on one context (plain) against the prove-request server."""
     n, w = 1 << W.C1_LOG_ROWS, W.C1_WIDTH
@@ -377,7 +380,7 @@ def bench_small(G, W, dev, n_proofs, workers):
     # two workers with batches of up to 128: with four (batches of 64) the
     # concurrent lock-step batches interleave their synchronised phases on the
     # GPU, and the stream measured 2,690-3,080 against 3,240-3,800 proofs/s
-    srv = G.Server(0, workers=workers, hbm_budget_bytes=64 * 10**9, max_batch=128)
+    srv = G.Server(0, workers=workers, hbm_budget_bytes=64 * 10**9, max_batch=max_batch, batch_window_ms=BATCH_WINDOW_MS)
     # warm every worker context and the shared block pool with the whole set,
     # twice: the four workers interleave differently from pass to pass, and
     # the second pass fills the size classes the first one's mix did not use
@@ -508,6 +511,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=4, help="server workers for the C5 mix")
     ap.add_argument("--small-workers", type=int, default=2, help="server workers for the small-proof stream")
+    ap.add_argument("--small-max-batch", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 4 GiB device-to-device copy (reference dispatch for HBM byte counters)")
@@ -535,7 +539,8 @@ def main():
     c3 = bench_c3(G, W, dev, max(2, args.steps // 3), args.warmup) if "c3" in sections else None
     c4 = bench_c4(G, W, dev, max(2, args.steps // 2), args.warmup) if "c4" in sections else None
     c5 = bench_c5(G, W, dev, args.serve_requests, args.serve_workers) if "c5" in sections else None
-    small = bench_small(G, W, dev, args.small_proofs, args.small_workers) if "small" in sections else None
+    small = bench_small(G, W, dev, args.small_proofs, args.small_workers, args.small_max_batch) \
+        if "small" in sections else None
     quot = bench_quotient(G, dev, max(3, args.steps // 2), 1) if "quotient" in sections else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(W, args.cpu_sample_log_rows)
 

@@ -302,6 +302,10 @@ uint64_t p3g_request_footprint(const p3g_request* req); /* device bytes reserved
 int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms, uint32_t max_batch,
                       p3g_server** out);
 int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket); /* P3G_E_RANGE: above the budget */
+/* Batch window (default 0): a worker that picks a request with fewer than max_batch
+ * queued requests of its shape holds it until it has been queued batch_window_ms, or
+ * the batch fills, so that requests arriving together are proven together. */
+int p3g_server_set_batch_window(p3g_server* s, double batch_window_ms);
 int p3g_server_wait(p3g_server* s, uint64_t ticket, p3g_result_info* info);     /* blocks; returns the proof status */
 int p3g_server_poll(p3g_server* s, uint64_t ticket, int* done);
 int p3g_server_proof(p3g_server* s, uint64_t ticket, uint64_t* fields, uint64_t* commitments);

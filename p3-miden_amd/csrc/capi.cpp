@@ -87,6 +87,8 @@ void p3g_ctx_destroy(p3g_ctx* ctx) {
     std::vector<void*> held;
     for (auto& kv : ctx->c.live_blocks) held.push_back(kv.first);
     for (void* p : held) ctx->c.release(p);  // back to the shared pool, which frees them
+    // the stream is synchronized above and destroyed below: its blocks have no pending work
+    ctx->c.pool->forget_stream(ctx->c.stream);
   } else {
     for (auto& kv : ctx->c.live_blocks) (void)hipFree(kv.first);
   }

@@ -36,13 +36,27 @@ void* SharedPool::take(std::multimap<size_t, Block>::iterator it, hipStream_t s)
   Block b = it->second;
   free_blocks.erase(it);
   cached_bytes -= b.sz;
-  if (b.ev) {
-    if (b.s != s) {
-      P3G_CHECK_HIP(hipStreamWaitEvent(s, b.ev, 0));
-      n_cross_stream++;
+  if (b.s && b.s != s) {
+    hipError_t e = hipSuccess;
+    if (b.ev) {
+      e = hipStreamWaitEvent(s, b.ev, 0);  // exactly the previous owner's work
+      (void)hipEventDestroy(b.ev);         // not recycled while the wait is pending
+      b.ev = nullptr;
+    } else {
+      // s waits for everything queued on the releasing stream so far
+      hipEvent_t ev = nullptr;
+      e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(ev, b.s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
+      if (ev) (void)hipEventDestroy(ev);
     }
-    (void)hipEventDestroy(b.ev);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(b.s);  // without the event: wait for the old owner here
+    }
+    n_cross_stream++;
   }
+  if (b.ev) spare_events.push_back(b.ev);
   live[b.p] = b.sz;
   live_bytes += b.sz;
   return b.p;
@@ -55,7 +69,9 @@ void SharedPool::free_one(std::multimap<size_t, Block>::iterator it) {
   const auto t0 = std::chrono::steady_clock::now();
   if (b.ev) {
     (void)hipEventSynchronize(b.ev);
-    (void)hipEventDestroy(b.ev);
+    spare_events.push_back(b.ev);
+  } else if (b.s) {
+    (void)hipStreamSynchronize(b.s);
   }
   (void)hipFree(b.p);
   n_free++;
@@ -77,8 +93,35 @@ void* SharedPool::alloc(size_t bytes, hipStream_t s) {
   bytes = size_class(bytes);
   std::lock_guard<std::mutex> g(mu);
   auto it = free_blocks.lower_bound(bytes);
-  if (it != free_blocks.end() && it->first <= bytes + bytes / 4) return take(it, s);
-  if (live_bytes + cached_bytes + bytes > limit) {
+  const auto hi = free_blocks.upper_bound(bytes + bytes / 4);
+  if (it != hi) {
+    // a block of this stream (or with no pending work), else one with an exact
+    // event, else one whose stream has drained, else (below) a new block
+    std::multimap<size_t, Block>::iterator evented = hi;
+    for (auto c = it; c != hi; ++c) {
+      if (!c->second.s || c->second.s == s) return take(c, s);
+      if (c->second.ev && evented == hi) evented = c;
+    }
+    if (evented != hi) return take(evented, s);
+    hipStream_t busy[8];
+    int nbusy = 0;
+    for (auto c = it; c != hi; ++c) {
+      const hipStream_t bs = c->second.s;
+      if (std::find(busy, busy + nbusy, bs) != busy + nbusy) continue;
+      if (hipStreamQuery(bs) == hipSuccess) {
+        for (auto& kv : free_blocks)
+          if (kv.second.s == bs && !kv.second.ev) kv.second.s = nullptr;  // all of bs's work is done
+        c->second.s = nullptr;
+        return take(c, s);
+      }
+      (void)hipGetLastError();  // hipErrorNotReady
+      if (nbusy < 8) busy[nbusy++] = bs;
+    }
+    // only small blocks of busy streams fit: waiting for such a stream could
+    // hold this one behind another context's long kernels, so grow the pool
+    // instead while the limit allows
+    if (live_bytes + cached_bytes + bytes > limit) return take(it, s);
+  } else if (live_bytes + cached_bytes + bytes > limit) {
     if (it != free_blocks.end()) return take(it, s);  // an oversized cached block rather than growing
     while (!free_blocks.empty() && live_bytes + cached_bytes + bytes > limit) free_one(std::prev(free_blocks.end()));
   }
@@ -107,17 +150,36 @@ void SharedPool::release(void* p, hipStream_t s) {
   auto it = live.find(p);
   if (it == live.end()) return;
   hipEvent_t ev = nullptr;
-  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, s) != hipSuccess) {
-    // without an event the block is only safe after the stream drains
-    (void)hipGetLastError();
-    if (ev) (void)hipEventDestroy(ev);
-    ev = nullptr;
-    (void)hipStreamSynchronize(s);
+  if (it->second >= EXACT_EVENT_BYTES) {
+    if (!spare_events.empty()) {
+      ev = spare_events.back();
+      spare_events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      ev = nullptr;
+    }
+    if (ev && hipEventRecord(ev, s) != hipSuccess) {
+      (void)hipGetLastError();
+      spare_events.push_back(ev);
+      ev = nullptr;  // the block falls back to the stream-based rules
+    }
   }
   free_blocks.emplace(it->second, Block{p, it->second, s, ev});
   cached_bytes += it->second;
   live_bytes -= it->second;
   live.erase(it);
+}
+
+void SharedPool::forget_stream(hipStream_t s) {
+  std::lock_guard<std::mutex> g(mu);
+  for (auto& kv : free_blocks)
+    if (kv.second.s == s) {
+      kv.second.s = nullptr;
+      if (kv.second.ev) {
+        spare_events.push_back(kv.second.ev);  // completed: the stream was synchronized
+        kv.second.ev = nullptr;
+      }
+    }
 }
 
 void SharedPool::trim() {
@@ -128,6 +190,7 @@ void SharedPool::trim() {
 SharedPool::~SharedPool() {
   trim();
   for (auto& kv : live) (void)hipFree(kv.first);
+  for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
 }
 
 uint64_t* Ctx::host_stage(size_t words) {

@@ -58,21 +58,32 @@ inline uint64_t aligned_len(uint64_t n, uint64_t a) { return a <= 1 ? n : (n + a
 
 // ------------------------------------------------------------------ shared block pool
 // Device blocks shared by several contexts (one stream each).  A released
-// block remembers its stream and an event recorded there; a context on another
-// stream that takes it makes its own stream wait for that event first, so a
-// block never changes hands while kernels of its previous owner still use it.
-// live + cached bytes stay within `limit`: beyond it cached blocks are reused
-// even when oversized, and only then freed.
+// block remembers the stream it was released on, so that it never changes
+// hands while its previous owner's kernels still use it:
+//  - taken again on that stream, stream order puts the new work after the old
+//    (no HIP call; a lock-step batch releases and retakes ~1,700 small blocks);
+//  - blocks of at least EXACT_EVENT_BYTES also get an event recorded at
+//    release, which another stream waits for;
+//  - a small block is taken on another stream only once its stream has
+//    drained (hipStreamQuery); while that stream is busy a new block is
+//    allocated instead, and only at the limit does the taker wait for an
+//    event recorded then on the releasing stream (covering the block's last
+//    uses and whatever that stream queued since).
+// alloc prefers same-stream blocks, then drained or evented ones.  live +
+// cached bytes stay within `limit`: beyond it cached blocks are reused even
+// when oversized, and only then freed.
 struct SharedPool {
+  static constexpr size_t EXACT_EVENT_BYTES = size_t(4) << 20;
   struct Block {
     void* p;
     size_t sz;
-    hipStream_t s;
-    hipEvent_t ev;
+    hipStream_t s;   // nullptr: no pending work (its stream was synchronized)
+    hipEvent_t ev;   // recorded at release (large blocks), or nullptr
   };
   std::mutex mu;
   size_t limit = 0;
   std::multimap<size_t, Block> free_blocks;
+  std::vector<hipEvent_t> spare_events;
   std::map<void*, size_t> live;
   size_t live_bytes = 0, cached_bytes = 0;
   uint64_t n_malloc = 0, n_free = 0, n_cross_stream = 0;
@@ -80,6 +91,8 @@ struct SharedPool {
 
   void* alloc(size_t bytes, hipStream_t s);
   void release(void* p, hipStream_t s);
+  // the stream has been synchronized and is about to be destroyed
+  void forget_stream(hipStream_t s);
   void trim();
   ~SharedPool();
 

@@ -70,30 +70,42 @@ struct p3g_server {
   uint32_t running = 0, peak_running = 0;
   uint64_t admitted = 0, completed = 0, batches = 0, batched_requests = 0;
   uint32_t max_batch = 1;
+  double batch_window_ms = 0;
   bool stop = false;
   clk::time_point t0;
 
   void worker_loop(uint32_t w);
-  Request* pick_locked();
+  std::deque<Request*>::iterator pick_locked();
+  double batch_wait_locked(const Request* r) const;
   void take_batch_locked(Request* first, std::vector<Request*>& batch);
   void run_one(p3g_ctx* ctx, Request* r);
   void run_batch(p3g_ctx* ctx, std::vector<Request*>& batch);
 };
 
-// First queued request that fits; the queue head may be overtaken only while
-// it is younger than the backfill window.
-Request* p3g_server::pick_locked() {
+// First queued request that fits (queue.end() if none); the queue head may be
+// overtaken only while it is younger than the backfill window.
+std::deque<Request*>::iterator p3g_server::pick_locked() {
   const auto now = clk::now();
   for (auto it = queue.begin(); it != queue.end(); ++it) {
     Request* r = *it;
-    if (inflight_bytes + r->footprint <= budget) {
-      queue.erase(it);
-      return r;
-    }
+    if (inflight_bytes + r->footprint <= budget) return it;
     const double waited = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
-    if (waited > backfill_ms) return nullptr;
+    if (waited > backfill_ms) return queue.end();
   }
-  return nullptr;
+  return queue.end();
+}
+
+// Milliseconds to hold a picked request so that more of its signature can join
+// its batch: while it is younger than the batch window and fewer than
+// max_batch requests of its signature are queued.  0 = take it now.
+double p3g_server::batch_wait_locked(const Request* r) const {
+  if (batch_window_ms <= 0 || max_batch <= 1) return 0;
+  const double age = std::chrono::duration<double, std::milli>(clk::now() - r->t_submit).count();
+  if (age >= batch_window_ms) return 0;
+  uint32_t same = 0;
+  for (const Request* q : queue)
+    if (q->sig == r->sig && ++same >= max_batch) return 0;
+  return batch_window_ms - age;
 }
 
 // Later queued requests with the first one's signature join its batch while
@@ -171,8 +183,18 @@ void p3g_server::worker_loop(uint32_t w) {
       Request* r = nullptr;
       for (;;) {
         if (stop) return;
-        r = pick_locked();
-        if (r) break;
+        auto it = pick_locked();
+        if (it != queue.end()) {
+          const double hold = batch_wait_locked(*it);
+          if (hold <= 0) {
+            r = *it;
+            queue.erase(it);
+            break;
+          }
+          // inside the batch window: let more requests of this shape arrive
+          cv_work.wait_for(lk, std::chrono::duration<double, std::milli>(hold));
+          continue;
+        }
         // re-check on every completion and submission; a timed wait lets an
         // aged queue head stop the backfill
         cv_work.wait_for(lk, std::chrono::milliseconds(5));
@@ -232,6 +254,14 @@ uint64_t p3g_request_footprint(const p3g_request* req) {
   if (hmin != hmax) bytes += 2 * L * W * 8;             // lifted sponge states
   bytes += 112 * L + 32 * (L / B);                      // DEEP evaluations, weights, FRI layers and trees
   return bytes + bytes / 8 + (64ull << 20);             // allocator rounding and tables
+}
+
+int p3g_server_set_batch_window(p3g_server* s, double batch_window_ms) {
+  if (!s || !(batch_window_ms >= 0) || batch_window_ms > 10000) return P3G_E_INVAL;
+  std::lock_guard<std::mutex> g(s->mu);
+  s->batch_window_ms = batch_window_ms;
+  s->cv_work.notify_all();
+  return P3G_OK;
 }
 
 int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, double backfill_ms, uint32_t max_batch,

@@ -186,6 +186,7 @@ def _load():
         "p3g_tree_prove_batch": (C.c_int, [_vp, _u64p, C.c_uint64, _vp]),
         "p3g_tree_free": (None, [_vp]),
         "p3g_commit_traces": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+        "p3g_server_set_batch_window": (C.c_int, [_vp, C.c_double]),
         "p3g_quotient_extend_scale": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, _u64p]),
         "p3g_quotient_divide_vanishing": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, C.c_uint64]),
         "p3g_commit_quotient": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
@@ -864,12 +865,14 @@ class Server:
     footprint within an HBM budget, continuous batching of arriving requests."""
 
     def __init__(self, device: int = 0, workers: int = 4, hbm_budget_bytes: int = 250 * 10**9,
-                 backfill_ms: float = 50.0, max_batch: int = 64):
+                 backfill_ms: float = 50.0, max_batch: int = 64, batch_window_ms: float = 0.0):
         h = _vp()
         _check(lib().p3g_server_create(device, workers, int(hbm_budget_bytes), float(backfill_ms), int(max_batch),
                                        C.byref(h)), "server_create")
         self.h = h
         self._keep = {}
+        if batch_window_ms:
+            _check(lib().p3g_server_set_batch_window(h, float(batch_window_ms)), "server_set_batch_window")
 
     def submit(self, traces, params: PcsParams, hash_kind: int = HASH_POSEIDON2) -> int:
         """Queue a proof of `traces` (host arrays or DeviceMatrix); returns a ticket.

@@ -122,3 +122,23 @@ def test_server_batches_same_shape_requests():
         assert root.tolist() == r.root.tolist() and z == r.eval_points
         assert np.array_equal(f, r.fields) and np.array_equal(c, r.commitments)
     dev.close()
+
+
+def test_server_batch_window_gathers_requests_arriving_together():
+    # with a batch window longer than the submission takes, the first request waits
+    # for the rest of its shape: every request is proven in lock-step, and each
+    # result is still the single-proof one
+    reqs = [O.rand_matrix(1300 + i, 1 << 9, 8) for i in range(24)]
+    params = (1, 1, 2, 0, 0, 20, 0)
+    srv = G.Server(0, workers=2, hbm_budget_bytes=8 << 30, max_batch=32, batch_window_ms=500.0)
+    tickets = [srv.submit([m], G.PcsParams(*params)) for m in reqs]
+    res = [srv.result(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 24 and st["batched_requests"] == 24 and st["batches"] <= 2, st
+    dev = G.Device(0)
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    for m, r in zip(reqs, res):
+        root, z, f, c = G.prove(cfg, G.PcsParams(*params), [m])
+        assert root.tolist() == r.root.tolist() and np.array_equal(f, r.fields) and np.array_equal(c, r.commitments)
+    dev.close()
