@@ -802,30 +802,37 @@ void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out) {
 // the pieces (rows, then missing siblings) in hint order, then the hints.
 size_t prove_batch_pieces(const Tree& t, const std::vector<uint64_t>& indices,
                           std::vector<std::pair<const uint64_t*, uint64_t>>& pieces) {
-  std::set<uint64_t> uniq(indices.begin(), indices.end());
-  for (uint64_t idx : uniq)
-    if (idx >= t.height) throw ApiError(E_RANGE, "query index out of range");
+  // unique indices, ascending (the BTreeSet of the reference's caller)
+  std::vector<uint64_t> known(indices);
+  std::sort(known.begin(), known.end());
+  known.erase(std::unique(known.begin(), known.end()), known.end());
+  if (!known.empty() && known.back() >= t.height) throw ApiError(E_RANGE, "query index out of range");
   // 1. rows of every matrix at every unique index, in leaf order
   const size_t first = pieces.size();
-  for (uint64_t idx : uniq)
-    for (auto& m : t.mats) {
-      uint64_t row = idx >> log2_exact(t.height / m.height);
-      pieces.emplace_back(m.ptr + row * m.width, m.width);
+  pieces.reserve(first + known.size() * (t.mats.size() + t.depth));
+  std::vector<unsigned> shift(t.mats.size());
+  for (size_t m = 0; m < t.mats.size(); m++) shift[m] = log2_exact(t.height / t.mats[m].height);
+  for (uint64_t idx : known)
+    for (size_t m = 0; m < t.mats.size(); m++) {
+      const DevMat& mm = t.mats[m];
+      pieces.emplace_back(mm.ptr + (idx >> shift[m]) * mm.width, mm.width);
     }
   const size_t nrows = pieces.size() - first;
-  // 2. missing siblings, level by level, left to right
-  std::vector<uint64_t> known(uniq.begin(), uniq.end());
+  // 2. missing siblings, level by level, left to right: in the sorted list a
+  // node's sibling is known exactly when it is the neighbouring entry
+  std::vector<uint64_t> parents;
+  parents.reserve(known.size());
   for (unsigned layer = 0; layer < t.depth; layer++) {
-    std::vector<uint64_t> parents;
+    parents.clear();
     for (size_t q = 0; q < known.size(); q++) {
-      uint64_t parent = known[q] / 2;
-      if (!parents.empty() && parents.back() == parent) continue;
+      const uint64_t node = known[q], parent = node / 2;
+      if (q + 1 < known.size() && known[q + 1] == (node ^ 1)) {
+        parents.push_back(parent);  // both children known
+        q++;
+        continue;
+      }
+      pieces.emplace_back(t.digests + (t.layer_off[layer] + (node ^ 1)) * 4, 4);
       parents.push_back(parent);
-      uint64_t left = parent * 2, right = left + 1;
-      bool hl = std::binary_search(known.begin(), known.end(), left);
-      bool hr = std::binary_search(known.begin(), known.end(), right);
-      if (hl && !hr) pieces.emplace_back(t.digests + (t.layer_off[layer] + right) * 4, 4);
-      else if (!hl && hr) pieces.emplace_back(t.digests + (t.layer_off[layer] + left) * 4, 4);
     }
     known.swap(parents);
   }

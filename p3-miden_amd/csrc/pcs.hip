@@ -1420,15 +1420,19 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
         segs[j].push_back(Seg{ftrees[r][j], first, jp[j].size(), nr});
       }
     });
-    std::vector<std::pair<const uint64_t*, uint64_t>> pieces;
-    std::vector<size_t> job_piece0(k), job_word0(k);
-    size_t words = 0;
+    // offsets per job, then every job's pieces copied into place in parallel
+    std::vector<size_t> job_piece0(k + 1, 0), job_word0(k + 1, 0), job_words(k, 0);
+    parallel_for(k, [&](size_t j) {
+      size_t wj = 0;
+      for (auto& pc : jp[j]) wj += pc.second;
+      job_words[j] = wj;
+    });
     for (size_t j = 0; j < k; j++) {
-      job_piece0[j] = pieces.size();
-      job_word0[j] = words;
-      for (auto& pc : jp[j]) words += pc.second;
-      pieces.insert(pieces.end(), jp[j].begin(), jp[j].end());
+      job_piece0[j + 1] = job_piece0[j] + jp[j].size();
+      job_word0[j + 1] = job_word0[j] + job_words[j];
     }
+    std::vector<std::pair<const uint64_t*, uint64_t>> pieces(job_piece0[k]);
+    parallel_for(k, [&](size_t j) { std::copy(jp[j].begin(), jp[j].end(), pieces.begin() + job_piece0[j]); });
     const uint64_t* raw = gather_pinned(ctx, pieces, s);
     parallel_for(k, [&](size_t j) {
       const uint64_t* base = raw + job_word0[j];
