@@ -89,40 +89,50 @@ static size_t size_class(size_t bytes) {
   return (bytes + step - 1) & ~(step - 1);
 }
 
+// The first block of [lo, hi) that another stream can take without waiting
+// for work queued after the block's release: one of stream s or with no
+// pending work, else one with an exact release event, else one whose stream
+// has drained.  hi if there is none.
+std::multimap<size_t, SharedPool::Block>::iterator SharedPool::pick(std::multimap<size_t, Block>::iterator lo,
+                                                                    std::multimap<size_t, Block>::iterator hi,
+                                                                    hipStream_t s) {
+  std::multimap<size_t, Block>::iterator evented = hi;
+  for (auto c = lo; c != hi; ++c) {
+    if (!c->second.s || c->second.s == s) return c;
+    if (c->second.ev && evented == hi) evented = c;
+  }
+  if (evented != hi) return evented;
+  hipStream_t busy[16];
+  int nbusy = 0;
+  for (auto c = lo; c != hi; ++c) {
+    const hipStream_t bs = c->second.s;
+    if (std::find(busy, busy + nbusy, bs) != busy + nbusy) continue;
+    if (hipStreamQuery(bs) == hipSuccess) {
+      for (auto& kv : free_blocks)
+        if (kv.second.s == bs && !kv.second.ev) kv.second.s = nullptr;  // all of bs's work is done
+      return c;
+    }
+    (void)hipGetLastError();  // hipErrorNotReady
+    if (nbusy < 16) busy[nbusy++] = bs;
+  }
+  return hi;
+}
+
 void* SharedPool::alloc(size_t bytes, hipStream_t s) {
   bytes = size_class(bytes);
   std::lock_guard<std::mutex> g(mu);
-  auto it = free_blocks.lower_bound(bytes);
+  const auto lo = free_blocks.lower_bound(bytes);
   const auto hi = free_blocks.upper_bound(bytes + bytes / 4);
-  if (it != hi) {
-    // a block of this stream (or with no pending work), else one with an exact
-    // event, else one whose stream has drained, else (below) a new block
-    std::multimap<size_t, Block>::iterator evented = hi;
-    for (auto c = it; c != hi; ++c) {
-      if (!c->second.s || c->second.s == s) return take(c, s);
-      if (c->second.ev && evented == hi) evented = c;
-    }
-    if (evented != hi) return take(evented, s);
-    hipStream_t busy[8];
-    int nbusy = 0;
-    for (auto c = it; c != hi; ++c) {
-      const hipStream_t bs = c->second.s;
-      if (std::find(busy, busy + nbusy, bs) != busy + nbusy) continue;
-      if (hipStreamQuery(bs) == hipSuccess) {
-        for (auto& kv : free_blocks)
-          if (kv.second.s == bs && !kv.second.ev) kv.second.s = nullptr;  // all of bs's work is done
-        c->second.s = nullptr;
-        return take(c, s);
-      }
-      (void)hipGetLastError();  // hipErrorNotReady
-      if (nbusy < 8) busy[nbusy++] = bs;
-    }
-    // only small blocks of busy streams fit: waiting for such a stream could
-    // hold this one behind another context's long kernels, so grow the pool
-    // instead while the limit allows
-    if (live_bytes + cached_bytes + bytes > limit) return take(it, s);
-  } else if (live_bytes + cached_bytes + bytes > limit) {
-    if (it != free_blocks.end()) return take(it, s);  // an oversized cached block rather than growing
+  auto c = pick(lo, hi, s);
+  if (c != hi) return take(c, s);
+  // what fits belongs to busy streams: waiting for one could hold this stream
+  // behind another context's long kernels, so grow the pool while the limit
+  // allows; at the limit take an oversized block that needs no such wait, and
+  // only then wait
+  if (live_bytes + cached_bytes + bytes > limit) {
+    c = pick(lo, free_blocks.end(), s);
+    if (c != free_blocks.end()) return take(c, s);
+    if (lo != free_blocks.end()) return take(lo, s);
     while (!free_blocks.empty() && live_bytes + cached_bytes + bytes > limit) free_one(std::prev(free_blocks.end()));
   }
   void* p = nullptr;

@@ -98,6 +98,8 @@ struct SharedPool {
 
  private:
   void* take(std::multimap<size_t, Block>::iterator it, hipStream_t s);
+  std::multimap<size_t, Block>::iterator pick(std::multimap<size_t, Block>::iterator lo,
+                                              std::multimap<size_t, Block>::iterator hi, hipStream_t s);
   void free_one(std::multimap<size_t, Block>::iterator it);
 };
 
