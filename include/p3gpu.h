@@ -113,6 +113,13 @@ uint32_t p3g_lmcs_alignment(uint32_t hash_kind);
 int p3g_lmcs_build(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* mats, uint64_t k, uint32_t alignment,
                    uint32_t flags, p3g_tree** out);
 /* Lmcs::hash / Lmcs::compress (lib.rs:203-209), host side. */
+/* Hiding LMCS (HidingLmcsConfig::build_tree / build_aligned_tree, p3-miden-lmcs/src/hiding_lmcs.rs:104-140;
+ * salt absorption lifted_tree.rs:271-279): as p3g_lmcs_build, plus a salt matrix of the tree's height and
+ * SALT columns that the caller draws from its RNG.  Each leaf absorbs its salt row after its rows; the
+ * batch opening hints the salt (unpadded) after each leaf's rows (lifted_tree.rs:187-192).  flags apply
+ * to the salt as to the matrices. */
+int p3g_lmcs_build_salted(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* mats, uint64_t k, uint32_t alignment,
+                          uint32_t flags, const p3g_matrix* salt, p3g_tree** out);
 int p3g_lmcs_hash(uint32_t hash_kind, const uint64_t* elems, const uint64_t* row_widths, uint64_t nrows,
                   uint64_t out[4]);
 int p3g_lmcs_compress(uint32_t hash_kind, const uint64_t left[4], const uint64_t right[4], uint64_t out[4]);
@@ -127,6 +134,8 @@ int p3g_tree_widths(const p3g_tree* t, uint64_t* out);
 /* rows(index): upsampled rows padded to alignment (lifted_tree.rs:127-138); *len receives the count */
 int p3g_tree_rows(p3g_tree* t, uint64_t index, uint64_t* out, uint64_t cap, uint64_t* len);
 /* digest layer l (0 = leaves): (height >> l) digests of 4 words */
+/* LiftedMerkleTree::salt(index) (lifted_tree.rs:340-360): len = SALT, 0 for a tree without salt */
+int p3g_tree_salt(const p3g_tree* t, uint64_t index, uint64_t* out, uint64_t cap, uint64_t* len);
 int p3g_tree_layer(p3g_tree* t, uint32_t layer, uint64_t* out);
 /* device address of committed matrix i (the bit-reversed LDE for commit_traces trees) */
 int p3g_tree_matrix(p3g_tree* t, uint64_t i, const uint64_t** dptr, uint64_t* height, uint64_t* width);

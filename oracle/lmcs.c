@@ -99,7 +99,14 @@ static size_t aligned_len(size_t len, size_t alignment) {
 
 orc_tree* orc_lmcs_build(int kind, const uint64_t* const* mats, const size_t* heights,
                          const size_t* widths, size_t k, size_t alignment) {
+    return orc_lmcs_build_salted(kind, mats, heights, widths, k, alignment, NULL, 0);
+}
+
+orc_tree* orc_lmcs_build_salted(int kind, const uint64_t* const* mats, const size_t* heights,
+                                const size_t* widths, size_t k, size_t alignment, const uint64_t* salt,
+                                size_t salt_w) {
     if (k == 0) return NULL;
+    if (salt && salt_w == 0) return NULL;
     for (size_t i = 0; i < k; i++) {
         if (heights[i] == 0 || (heights[i] & (heights[i] - 1))) return NULL;
         if (i && heights[i] < heights[i - 1]) return NULL;
@@ -143,6 +150,14 @@ orc_tree* orc_lmcs_build(int kind, const uint64_t* const* mats, const size_t* he
         for (size_t r = 0; r < h; r++) orc_sponge_absorb(kind, states + r * W, mat + r * w, w);
         active = h;
     }
+    /* hiding: absorb_matrix of the salt after the rows (lifted_tree.rs:271-279) */
+    if (salt) {
+        t->salt_w = salt_w;
+        t->salt = (uint64_t*)malloc(H * salt_w * sizeof(uint64_t));
+        memcpy(t->salt, salt, H * salt_w * sizeof(uint64_t));
+        #pragma omp parallel for schedule(static)
+        for (size_t r = 0; r < H; r++) orc_sponge_absorb(kind, states + r * W, t->salt + r * salt_w, salt_w);
+    }
     t->layers = (uint64_t**)malloc((t->depth + 1) * sizeof(uint64_t*));
     t->layers[0] = (uint64_t*)malloc(H * 4 * sizeof(uint64_t));
     #pragma omp parallel for
@@ -161,8 +176,15 @@ orc_tree* orc_lmcs_build(int kind, const uint64_t* const* mats, const size_t* he
     return t;
 }
 
+size_t orc_tree_salt(const orc_tree* t, size_t index, uint64_t* out) {
+    if (!t->salt) return 0;
+    if (out) memcpy(out, t->salt + index * t->salt_w, t->salt_w * sizeof(uint64_t));
+    return t->salt_w;
+}
+
 void orc_tree_free(orc_tree* t) {
     if (!t) return;
+    free(t->salt);
     for (size_t i = 0; i < t->k; i++) free(t->mats[i]);
     free(t->mats);
     free(t->heights);
@@ -232,6 +254,8 @@ void orc_tree_prove_batch(const orc_tree* t, const size_t* indices, size_t n, or
             orc_tx_hint_fields(tx, row + pos, aw);
             pos += aw;
         }
+        /* then the salt, as it is (lifted_tree.rs:187-192) */
+        if (t->salt) orc_tx_hint_fields(tx, t->salt + known[q] * t->salt_w, t->salt_w);
     }
     size_t* parents = (size_t*)malloc((nk ? nk : 1) * sizeof(size_t));
     for (unsigned layer = 0; layer < t->depth; layer++) {
@@ -256,6 +280,12 @@ void orc_tree_prove_batch(const orc_tree* t, const size_t* indices, size_t n, or
 int orc_lmcs_open_batch(int kind, const uint64_t* root, const size_t* widths, size_t nw,
                         unsigned log_max_height, const size_t* indices, size_t n, orc_tx* tx,
                         uint64_t* rows_out) {
+    return orc_lmcs_open_batch_salted(kind, root, widths, nw, log_max_height, indices, n, 0, tx, rows_out);
+}
+
+int orc_lmcs_open_batch_salted(int kind, const uint64_t* root, const size_t* widths, size_t nw,
+                               unsigned log_max_height, const size_t* indices, size_t n, size_t salt_w, orc_tx* tx,
+                               uint64_t* rows_out) {
     size_t max_h = (size_t)1 << log_max_height;
     size_t* uniq = (size_t*)malloc((n ? n : 1) * sizeof(size_t));
     memcpy(uniq, indices, n * sizeof(size_t));
@@ -275,6 +305,12 @@ int orc_lmcs_open_batch(int kind, const uint64_t* root, const size_t* widths, si
         memset(st, 0, sizeof(st));
         size_t off = 0;
         for (size_t i = 0; i < nw; i++) { orc_sponge_absorb(kind, st, elems + off, widths[i]); off += widths[i]; }
+        if (salt_w) {
+            /* hash(rows ... salt) (lmcs.rs:198-208) */
+            uint64_t sl[64];
+            if (salt_w > 64 || orc_tx_receive_hint_fields(tx, salt_w, sl)) { rc = 3; goto done; }
+            orc_sponge_absorb(kind, st, sl, salt_w);
+        }
         memcpy(hashes + 4 * q, st, 32);
         pos[q] = uniq[q];
     }

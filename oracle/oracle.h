@@ -97,11 +97,19 @@ typedef struct {
     size_t alignment;
     size_t height;
     unsigned depth;
+    uint64_t* salt;    /* hiding trees: height x salt_w, absorbed after the rows; NULL without salt */
+    size_t salt_w;
     uint64_t** layers; /* depth+1 layers, layer l holds (height >> l) digests of 4 words */
 } orc_tree;
 
 orc_tree* orc_lmcs_build(int kind, const uint64_t* const* mats, const size_t* heights,
                          const size_t* widths, size_t k, size_t alignment);
+/* HidingLmcsConfig build (hiding_lmcs.rs:104-140, lifted_tree.rs:271-279) with a caller-given salt
+ * matrix (height x salt_w) absorbed after each leaf's rows */
+orc_tree* orc_lmcs_build_salted(int kind, const uint64_t* const* mats, const size_t* heights,
+                                const size_t* widths, size_t k, size_t alignment, const uint64_t* salt,
+                                size_t salt_w);
+size_t orc_tree_salt(const orc_tree* t, size_t index, uint64_t* out);
 void orc_tree_free(orc_tree* t);
 void orc_tree_root(const orc_tree* t, uint64_t* out4);
 size_t orc_tree_height(const orc_tree* t);
@@ -109,6 +117,10 @@ void orc_tree_layer(const orc_tree* t, unsigned layer, uint64_t* out);
 size_t orc_tree_rows(const orc_tree* t, size_t index, uint64_t* out);
 void orc_tree_prove_batch(const orc_tree* t, const size_t* indices, size_t n, orc_tx* tx);
 /* Verifier side (Lmcs::open_batch).  Returns 0 ok, 1 invalid proof, 2 root mismatch, 3 transcript. */
+/* open_batch of a hiding tree: every leaf's rows are followed by salt_w hinted salt elements */
+int orc_lmcs_open_batch_salted(int kind, const uint64_t* root, const size_t* widths, size_t nw,
+                               unsigned log_max_height, const size_t* indices, size_t n, size_t salt_w, orc_tx* tx,
+                               uint64_t* rows_out);
 int orc_lmcs_open_batch(int kind, const uint64_t* root, const size_t* widths, size_t nw,
                         unsigned log_max_height, const size_t* indices, size_t n, orc_tx* tx,
                         uint64_t* rows_out /* n_unique * sum(widths), sorted */);

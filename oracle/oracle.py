@@ -71,6 +71,10 @@ def _load():
         "orc_compress": (None, [C.c_int, _u64p, _u64p, _u64p]),
         "orc_hash_rows": (None, [C.c_int, _u64p, C.c_size_t, _u64p]),
         "orc_lmcs_build": (vp, [C.c_int, C.POINTER(_u64p), _szp, _szp, C.c_size_t, C.c_size_t]),
+        "orc_lmcs_build_salted": (vp, [C.c_int, C.POINTER(_u64p), _szp, _szp, C.c_size_t, C.c_size_t, _u64p, C.c_size_t]),
+        "orc_tree_salt": (C.c_size_t, [vp, C.c_size_t, _u64p]),
+        "orc_lmcs_open_batch_salted": (C.c_int, [C.c_int, _u64p, _szp, C.c_size_t, C.c_uint, _szp, C.c_size_t, C.c_size_t,
+                                                 vp, _u64p]),
         "orc_tree_free": (None, [vp]),
         "orc_tree_root": (None, [vp, _u64p]),
         "orc_tree_height": (C.c_size_t, [vp]),
@@ -336,7 +340,8 @@ class Transcript:
 class Tree:
     """LiftedMerkleTree built by LmcsConfig::build_tree / build_aligned_tree."""
 
-    def __init__(self, kind: int, matrices, alignment: int = 1, _handle=None):
+    def __init__(self, kind: int, matrices, alignment: int = 1, _handle=None, salt=None):
+        """salt: a height x SALT matrix for a hiding tree (HidingLmcsConfig, hiding_lmcs.rs:104-140)."""
         self.kind = kind
         if _handle is not None:
             self.h = _handle
@@ -345,7 +350,14 @@ class Tree:
         ptrs = (_u64p * len(mats))(*[_p(m) for m in mats])
         hs = _sz([m.shape[0] for m in mats])
         ws = _sz([m.shape[1] for m in mats])
-        self.h = lib.orc_lmcs_build(kind, ptrs, hs, ws, len(mats), alignment)
+        if salt is None:
+            self.h = lib.orc_lmcs_build(kind, ptrs, hs, ws, len(mats), alignment)
+        else:
+            sl = u64(salt)
+            if sl.ndim != 2 or not mats or sl.shape[0] != mats[-1].shape[0] or sl.shape[1] == 0:
+                raise ValueError("salt must be a tree-height x SALT matrix")
+            self._salt_keep = sl
+            self.h = lib.orc_lmcs_build_salted(kind, ptrs, hs, ws, len(mats), alignment, _p(sl), sl.shape[1])
         if not self.h:
             raise ValueError("invalid LMCS input (empty, non power-of-two or unsorted heights)")
 
@@ -361,6 +373,13 @@ class Tree:
         return o
 
     def height(self) -> int: return int(lib.orc_tree_height(self.h))
+
+    def salt(self, index: int) -> np.ndarray:
+        n = lib.orc_tree_salt(self.h, index, None)
+        o = np.zeros(max(n, 1), np.uint64)
+        if n:
+            lib.orc_tree_salt(self.h, index, _p(o))
+        return o[:n]
 
     def layer(self, l: int) -> np.ndarray:
         n = self.height() >> l
@@ -379,14 +398,16 @@ class Tree:
         lib.orc_tree_prove_batch(self.h, idx, len(list(indices)), tx.h)
 
 
-def open_batch(kind: int, root, widths, log_max_height: int, indices, tx: Transcript):
-    """LmcsConfig::open_batch; returns (code, rows) with code 0 ok / 1 invalid / 2 root mismatch / 3 transcript."""
+def open_batch(kind: int, root, widths, log_max_height: int, indices, tx: Transcript, salt_width: int = 0):
+    """LmcsConfig::open_batch (lmcs.rs:167-250); returns (code, rows) with code 0 ok / 1 invalid / 2 root
+    mismatch / 3 transcript.  salt_width > 0: a hiding tree's openings (rows, then the salt)."""
     r = u64(root)
     uniq = sorted(set(int(i) for i in indices))
     tot = int(sum(widths))
     rows = np.zeros(max(1, len(uniq) * tot), np.uint64)
     idx = list(indices)
-    rc = lib.orc_lmcs_open_batch(kind, _p(r), _sz(widths), len(widths), log_max_height, _sz(idx), len(idx), tx.h, _p(rows))
+    rc = lib.orc_lmcs_open_batch_salted(kind, _p(r), _sz(widths), len(widths), log_max_height, _sz(idx), len(idx),
+                                        salt_width, tx.h, _p(rows))
     return int(rc), rows[: len(uniq) * tot].reshape(len(uniq), tot) if uniq else rows[:0]
 
 

@@ -301,6 +301,51 @@ int p3g_lmcs_build(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* mats, uin
   });
 }
 
+int p3g_lmcs_build_salted(p3g_ctx* ctx, uint32_t hash_kind, const p3g_matrix* mats, uint64_t k, uint32_t alignment,
+                          uint32_t flags, const p3g_matrix* salt, p3g_tree** out) {
+  if (!ctx || !out || (!mats && k) || !salt) return P3G_E_INVAL;
+  *out = nullptr;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    check_hash(hash_kind);
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    if (k == 0) throw ApiError(P3G_E_EMPTY, "cannot commit empty batch");
+    if (salt->width == 0 || salt->height != mats[k - 1].height)
+      throw ApiError(P3G_E_INVAL, "salt must have the tree height and a non-zero width");
+    std::vector<DevMat> dm = import_mats(c, mats, k, flags);
+    std::vector<DevMat> ds;
+    Tree* t = nullptr;
+    try {
+      ds = import_mats(c, salt, 1, flags);
+      t = lmcs_build(c, hash_kind, dm, alignment, c.stream, true, &ds[0]);
+    } catch (...) {
+      for (auto& m : dm)
+        if (m.owned && !(flags & P3G_DEVICE_PTRS)) c.release(m.ptr);
+      for (auto& m : ds)
+        if (m.owned && !(flags & P3G_DEVICE_PTRS)) c.release(m.ptr);
+      throw;
+    }
+    *out = new p3g_tree{t};
+  });
+}
+
+int p3g_tree_salt(const p3g_tree* t, uint64_t index, uint64_t* out, uint64_t cap, uint64_t* len) {
+  if (!t || !len) return P3G_E_INVAL;
+  return guard([&] {
+    const Tree& tr = *t->t;
+    *len = tr.salt.ptr ? tr.salt.width : 0;
+    if (!tr.salt.ptr) return;
+    if (index >= tr.height) throw ApiError(P3G_E_RANGE, "leaf index out of range");
+    if (out) {
+      if (cap < tr.salt.width) throw ApiError(P3G_E_RANGE, "output buffer too small");
+      P3G_CHECK_HIP(hipSetDevice(tr.ctx->device));
+      std::vector<uint64_t> raw;
+      gather_to_host(*tr.ctx, {{tr.salt.ptr + index * tr.salt.width, tr.salt.width}}, raw, tr.ctx->stream);
+      memcpy(out, raw.data(), tr.salt.width * 8);
+    }
+  });
+}
+
 int p3g_lmcs_hash(uint32_t hash_kind, const uint64_t* elems, const uint64_t* row_widths, uint64_t nrows,
                   uint64_t out[4]) {
   if (!out || (nrows && !row_widths)) return P3G_E_INVAL;

@@ -432,6 +432,7 @@ Tree::~Tree() {
   if (ctx) {
     for (auto& m : mats)
       if (m.owned) ctx->release(m.ptr);
+    if (salt.owned) ctx->release(salt.ptr);
     if (owns_digests) ctx->release(digests);
   }
 }

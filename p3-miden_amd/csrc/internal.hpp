@@ -208,6 +208,9 @@ struct Tree {
   unsigned depth = 0;
   uint64_t* digests = nullptr;  // all layers, layer l at layer_off[l] (in 4-word digests)
   bool owns_digests = true;
+  // hiding trees: per-leaf salt (height x salt width), absorbed after the rows
+  // and hinted after them; ptr is null for a tree without salt
+  DevMat salt;
   std::vector<uint64_t> layer_off;
   uint64_t root[4] = {0, 0, 0, 0};
   ~Tree();
@@ -301,8 +304,10 @@ size_t prove_batch_pieces(const Tree& t, const std::vector<uint64_t>& indices,
 size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_t*, uint64_t>>& pieces, size_t first,
                         size_t last, size_t nrows, const uint64_t* raw, Channel& tx);
 // read_root = false: no stream synchronisation; the root stays on the device
+// salt (optional): a tree-height x SALT matrix for a hiding tree; the tree
+// takes it over like the matrices (releasing it when owned)
 Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s,
-                 bool read_root = true);
+                 bool read_root = true, const DevMat* salt = nullptr);
 void tree_rows(const Tree& t, uint64_t index, std::vector<uint64_t>& out);
 void tree_prove_batch(const Tree& t, const std::vector<uint64_t>& indices, Channel& tx, hipStream_t s);
 // gathers the pieces into the context's pinned staging buffer; the returned

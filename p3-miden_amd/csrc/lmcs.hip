@@ -501,7 +501,7 @@ void launch_leaf_raw(uint32_t hash, int threads, const void* mats, uint32_t nmat
 }
 
 Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint32_t alignment, hipStream_t s,
-                 bool read_root) {
+                 bool read_root, const DevMat* salt) {
   if (mats.empty()) throw ApiError(E_EMPTY, "cannot commit empty batch");
   uint64_t prevh = 0;
   for (auto& m : mats) {
@@ -509,7 +509,10 @@ Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint3
     if (m.height < prevh) throw ApiError(E_UNSORTED, "matrices must be sorted by height");
     prevh = m.height;
   }
+  if (salt && (salt->height != prevh || salt->width == 0 || !salt->ptr))
+    throw ApiError(E_INVAL, "salt must have the tree height and a non-zero width");
   Tree* t = new Tree();
+  if (salt) t->salt = *salt;
   t->ctx = &ctx;
   t->hash = hash;
   t->alignment = alignment < 1 ? 1 : alignment;
@@ -543,6 +546,9 @@ Tree* lmcs_build(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& mats, uint3
       if (!states[nxt]) states[nxt] = (uint64_t*)ctx.alloc(t->height * W * sizeof(uint64_t));
       out_states = states[nxt];
     }
+    // a hiding tree absorbs each leaf's salt after its rows (build_with_alignment:
+    // absorb_matrix of the salt matrix, lifted_tree.rs:271-279)
+    if (final && salt) group.push_back(*salt);
     launch_leaf(ctx, hash, group, prev, shift, out_states, final ? t->digests : nullptr, h, s);
     cur = nxt;
     cur_h = h;
@@ -809,14 +815,16 @@ size_t prove_batch_pieces(const Tree& t, const std::vector<uint64_t>& indices,
   if (!known.empty() && known.back() >= t.height) throw ApiError(E_RANGE, "query index out of range");
   // 1. rows of every matrix at every unique index, in leaf order
   const size_t first = pieces.size();
-  pieces.reserve(first + known.size() * (t.mats.size() + t.depth));
+  pieces.reserve(first + known.size() * (t.mats.size() + 1 + t.depth));
   std::vector<unsigned> shift(t.mats.size());
   for (size_t m = 0; m < t.mats.size(); m++) shift[m] = log2_exact(t.height / t.mats[m].height);
-  for (uint64_t idx : known)
+  for (uint64_t idx : known) {
     for (size_t m = 0; m < t.mats.size(); m++) {
       const DevMat& mm = t.mats[m];
       pieces.emplace_back(mm.ptr + (idx >> shift[m]) * mm.width, mm.width);
     }
+    if (t.salt.ptr) pieces.emplace_back(t.salt.ptr + idx * t.salt.width, t.salt.width);  // hinted after the rows
+  }
   const size_t nrows = pieces.size() - first;
   // 2. missing siblings, level by level, left to right: in the sorted list a
   // node's sibling is known exactly when it is the neighbouring entry
@@ -845,9 +853,12 @@ size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_
                         size_t last, size_t nrows, const uint64_t* raw, Channel& tx) {
   size_t off = 0;
   std::vector<uint64_t> padded;
+  // per opened leaf: one row per matrix (padded to the alignment), then the salt as it is
+  const size_t per = t.mats.size() + (t.salt.ptr ? 1 : 0);
   for (size_t p = first; p < first + nrows; p++) {
     uint64_t w = pieces[p].second;
-    uint64_t aw = aligned_len(w, t.alignment);
+    const bool is_salt = t.salt.ptr && (p - first) % per == per - 1;
+    uint64_t aw = is_salt ? w : aligned_len(w, t.alignment);
     padded.assign(raw + off, raw + off + w);
     padded.resize(aw, 0);
     tx.hint_fields(padded.data(), aw);

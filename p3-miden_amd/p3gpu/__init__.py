@@ -181,6 +181,9 @@ def _load():
         "p3g_tree_num_matrices": (C.c_uint64, [_vp]),
         "p3g_tree_widths": (C.c_int, [_vp, _u64p]),
         "p3g_tree_rows": (C.c_int, [_vp, C.c_uint64, _u64p, C.c_uint64, _u64p]),
+        "p3g_lmcs_build_salted": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32,
+                                            C.POINTER(_Matrix), C.POINTER(_vp)]),
+        "p3g_tree_salt": (C.c_int, [_vp, C.c_uint64, _u64p, C.c_uint64, _u64p]),
         "p3g_tree_layer": (C.c_int, [_vp, C.c_uint32, _u64p]),
         "p3g_tree_matrix": (C.c_int, [_vp, C.c_uint64, C.POINTER(_vp), _u64p, _u64p]),
         "p3g_tree_prove_batch": (C.c_int, [_vp, _u64p, C.c_uint64, _vp]),
@@ -551,6 +554,15 @@ class LiftedMerkleTree:
         _check(lib().p3g_tree_rows(self.h, index, _p(o), o.size, _p(ln)), "tree_rows")
         return o[: int(ln[0])]
 
+    def salt(self, index: int) -> np.ndarray:
+        """LiftedMerkleTree::salt(index): the leaf's salt (empty for a tree without salt)."""
+        n = np.zeros(1, np.uint64)
+        _check(lib().p3g_tree_salt(self.h, index, None, 0, _p(n)), "tree_salt")
+        o = np.zeros(max(1, int(n[0])), np.uint64)
+        if int(n[0]):
+            _check(lib().p3g_tree_salt(self.h, index, _p(o), o.size, _p(n)), "tree_salt")
+        return o[: int(n[0])]
+
     def layer(self, l: int) -> np.ndarray:
         n = self.height() >> l
         o = np.zeros(4 * n, np.uint64)
@@ -584,21 +596,31 @@ class LmcsConfig:
     def alignment(self) -> int:
         return int(lib().p3g_lmcs_alignment(self.hash_kind))
 
-    def _build(self, mats, alignment, take_ownership=False):
+    def _build(self, mats, alignment, take_ownership=False, salt=None):
         arr, keep, dev = _matrices(mats)
         flags = (DEVICE_PTRS if dev else 0) | (TAKE_OWNERSHIP if (dev and take_ownership) else 0)
         h = _vp()
-        _check(lib().p3g_lmcs_build(self.dev.h, self.hash_kind, arr, len(mats), alignment, flags, C.byref(h)), "lmcs_build")
+        if salt is None:
+            _check(lib().p3g_lmcs_build(self.dev.h, self.hash_kind, arr, len(mats), alignment, flags, C.byref(h)),
+                   "lmcs_build")
+        else:
+            sarr, skeep, sdev = _matrices([salt])
+            if sdev != dev:
+                raise ValueError("salt and matrices must both be host arrays or both device matrices")
+            _check(lib().p3g_lmcs_build_salted(self.dev.h, self.hash_kind, arr, len(mats), alignment, flags, sarr,
+                                               C.byref(h)), "lmcs_build_salted")
         if dev and take_ownership:
-            for m in mats:
+            for m in list(mats) + ([salt] if salt is not None else []):
                 m.owned = False
         return LiftedMerkleTree(h, self.hash_kind, self.dev)
 
-    def build_tree(self, mats, take_ownership=False) -> LiftedMerkleTree:
-        return self._build(mats, 1, take_ownership)
+    def build_tree(self, mats, take_ownership=False, salt=None) -> LiftedMerkleTree:
+        """build_tree; with salt (a tree-height x SALT matrix), the hiding form of
+        HidingLmcsConfig::build_tree (hiding_lmcs.rs:104-118) with the caller's salt."""
+        return self._build(mats, 1, take_ownership, salt)
 
-    def build_aligned_tree(self, mats, take_ownership=False) -> LiftedMerkleTree:
-        return self._build(mats, self.alignment(), take_ownership)
+    def build_aligned_tree(self, mats, take_ownership=False, salt=None) -> LiftedMerkleTree:
+        return self._build(mats, self.alignment(), take_ownership, salt)
 
     def hash(self, rows) -> np.ndarray:
         rows = [u64(r).ravel() for r in rows]

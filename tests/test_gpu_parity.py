@@ -355,3 +355,57 @@ def test_prove_many_device_inputs_and_errors(device):
     with pytest.raises(G.P3GError) as e:
         G.prove_many(cfg, params, [_mats(1, [(256, 12)]), _mats(2, [(256, 13)])])
     assert e.value.code == 1  # shapes differ between jobs
+
+
+# ---------------------------------------------------------------- hiding LMCS (caller-given salt)
+@pytest.mark.parametrize("kind,ok,shapes,salt_w", [
+    (G.HASH_POSEIDON2, O.HASH_P2, [(8, 4)], 4),
+    (G.HASH_POSEIDON2, O.HASH_P2, [(4, 3), (8, 5), (16, 7)], 2),
+    (G.HASH_POSEIDON2, O.HASH_P2, [(64, 3), (4096, 40), (4096, 1)], 8),
+    (G.HASH_POSEIDON2, O.HASH_P2, [(1 << 15, 16)], 4),
+    (G.HASH_KECCAK, O.HASH_KECCAK, [(4, 17), (8, 3)], 4),
+    (G.HASH_KECCAK, O.HASH_KECCAK, [(512, 100)], 2),
+])
+def test_hiding_tree_matches_oracle(device, kind, ok, shapes, salt_w):
+    # HidingLmcsConfig::build_aligned_tree with the same salt: root, every layer,
+    # salts, and prove_batch hints (rows padded, then the salt) equal the oracle's
+    mats = _mats(700 + salt_w, shapes)
+    H = shapes[-1][0]
+    salt = O.rand_matrix(800 + salt_w, H, salt_w)
+    cfg = G.LmcsConfig(device, kind)
+    t = cfg.build_aligned_tree(mats, salt=salt)
+    want = O.Tree(ok, mats, alignment=O.hash_alignment(ok), salt=salt)
+    assert t.root().tolist() == want.root().tolist()
+    for lvl in range(0, H.bit_length(), 2):
+        assert np.array_equal(t.layer(lvl), want.layer(lvl))
+    for i in (0, H // 2, H - 1):
+        assert t.salt(i).tolist() == salt[i].tolist()
+    idx = [0, H - 1, H // 3, H // 3, 1 % H]
+    tx_g, tx_o = G.ProverTranscript(G.CHALLENGER_DUPLEX_P2), O.Transcript(O.CH_DUPLEX_P2)
+    t.prove_batch(idx, tx_g)
+    want.prove_batch(idx, tx_o)
+    fg, cg = tx_g.data()
+    fo, co = tx_o.data()
+    assert np.array_equal(fg, fo) and np.array_equal(cg, co)
+    # the oracle verifier opens the GPU's hints
+    v = O.Transcript(O.CH_DUPLEX_P2)
+    v.load(fg, cg)
+    rc, _ = O.open_batch(ok, t.root(), t.widths(), H.bit_length() - 1, idx, v, salt_width=salt_w)
+    assert rc == 0
+    t.free()
+
+
+def test_hiding_tree_device_salt_and_errors(device):
+    mats = [O.rand_matrix(900, 1 << 10, 8)]
+    salt = O.rand_matrix(901, 1 << 10, 4)
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    dm, ds = device.upload(mats[0]), device.upload(salt)
+    t = cfg.build_tree([dm], salt=ds)
+    assert t.root().tolist() == O.Tree(O.HASH_P2, mats, salt=salt).root().tolist()
+    t.free()
+    assert cfg.build_tree(mats).salt(0).size == 0  # a tree without salt
+    with pytest.raises(G.P3GError) as e:
+        cfg.build_tree(mats, salt=O.rand_matrix(902, 1 << 9, 4))  # salt height below the tree's
+    assert e.value.code == 1
+    dm.free()
+    ds.free()

@@ -324,6 +324,48 @@ def test_lmcs_roundtrip_duplicates_and_rejections():
         assert O.open_batch(O.HASH_P2, t.root(), [w for _, w in shapes], H.bit_length() - 1, idx, v4)[0] == 2
 
 
+# ---------------------------------------------------------------- hiding LMCS
+@pytest.mark.parametrize("kind,shapes,salt_w", [(O.HASH_P2, [(8, 4)], 4), (O.HASH_P2, [(4, 3), (8, 5), (16, 7)], 2),
+                                                (O.HASH_P2, [(16, 8)], 8), (O.HASH_KECCAK, [(4, 17), (8, 3)], 4)])
+def test_salt_is_absorbed_like_one_more_full_height_matrix(kind, shapes, salt_w):
+    # build_with_alignment absorbs the salt with absorb_matrix, as it absorbs a
+    # matrix, after the rows (lifted_tree.rs:271-279): the hiding root equals the
+    # root of the same tree with the salt appended as its last matrix
+    mats = [O.rand_matrix(40 + i, h, w) for i, (h, w) in enumerate(shapes)]
+    salt = O.rand_matrix(77, shapes[-1][0], salt_w)
+    for align in (1, O.hash_alignment(kind)):
+        hid = O.Tree(kind, mats, alignment=align, salt=salt)
+        plain = O.Tree(kind, mats + [salt], alignment=align)
+        assert hid.root().tolist() == plain.root().tolist()
+        assert hid.salt(3).tolist() == salt[3].tolist()
+    assert O.Tree(kind, mats, salt=O.rand_matrix(78, shapes[-1][0], salt_w)).root().tolist() != hid.root().tolist()
+
+
+def test_hiding_roundtrip_and_rejections():
+    # tests.rs:203-240 (hiding_roundtrip): openings carry each leaf's salt after its
+    # rows (lifted_tree.rs:187-192) and verify; a tampered salt fails the root
+    shapes, salt_w = [(4, 3), (8, 5), (16, 7)], 4
+    mats = [O.rand_matrix(50 + i, h, w) for i, (h, w) in enumerate(shapes)]
+    salt = O.rand_matrix(51, 16, salt_w)
+    t = O.Tree(O.HASH_P2, mats, salt=salt)
+    idx = [3, 9, 9, 0, 15]
+    tx = O.Transcript()
+    t.prove_batch(idx, tx)
+    f, c = tx.data()
+    uniq = sorted(set(idx))
+    assert f.size == len(uniq) * (3 + 5 + 7 + salt_w)
+    for k, i in enumerate(uniq):  # rows, then the salt, per leaf
+        assert f[k * 19 + 15: k * 19 + 19].tolist() == salt[i].tolist()
+    v = O.Transcript(); v.load(f, c)
+    rc, rows = O.open_batch(O.HASH_P2, t.root(), [3, 5, 7], 4, idx, v, salt_width=salt_w)
+    assert rc == 0 and v.is_empty()
+    for k, i in enumerate(uniq):
+        assert rows[k].tolist() == t.rows(i).tolist()
+    f2 = f.copy(); f2[15] = (int(f2[15]) + 1) % P  # the first leaf's salt
+    v2 = O.Transcript(); v2.load(f2, c)
+    assert O.open_batch(O.HASH_P2, t.root(), [3, 5, 7], 4, idx, v2, salt_width=salt_w)[0] == 2
+
+
 # ---------------------------------------------------------------- DEEP (closed forms)
 def _ef_add(a, b):
     return ((a[0] + b[0]) % P, (a[1] + b[1]) % P)
