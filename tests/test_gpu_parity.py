@@ -124,6 +124,11 @@ CASES = [
     (G.HASH_POSEIDON2, [(1 << 8, 5)], (3, 2, 0, 1, 2, 20, 3)),
     (G.HASH_KECCAK, [(1 << 9, 40)], (1, 1, 2, 0, 0, 50, 0)),
     (G.HASH_KECCAK, [(32, 18), (128, 17)], (2, 2, 1, 0, 1, 25, 2)),
+    # zero FRI rounds: the final polynomial is the whole DEEP polynomial (fri/tests.rs:294-340);
+    # final degree equal to and above the trace degree
+    (G.HASH_POSEIDON2, [(1 << 4, 6)], (1, 1, 4, 0, 0, 8, 0)),
+    (G.HASH_POSEIDON2, [(1 << 4, 6)], (1, 1, 6, 0, 0, 8, 0)),
+    (G.HASH_KECCAK, [(1 << 4, 6)], (2, 2, 5, 0, 0, 10, 0)),
 ]
 
 
@@ -329,6 +334,7 @@ MANY_CASES = [
     (G.HASH_KECCAK, [(32, 18), (128, 17)], (2, 2, 1, 0, 1, 25, 2), 3),
     (G.HASH_KECCAK, [(1 << 9, 40)], (1, 1, 2, 0, 0, 50, 0), 2),
     (G.HASH_POSEIDON2, [(1 << 17, 8)], (1, 1, 3, 0, 0, 20, 0), 2),
+    (G.HASH_POSEIDON2, [(1 << 4, 6)], (1, 1, 4, 0, 0, 8, 0), 3),  # zero FRI rounds
 ]
 
 
