@@ -213,6 +213,41 @@ int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t l
                           const uint64_t* eval_points, uint32_t n_points, p3g_tree* const* trees, uint64_t ntrees,
                           p3g_transcript* ch);
 
+/* ---- DEEP and FRI steps ----
+ * The pieces open_with_channel composes, for a caller that runs the protocol
+ * around them itself (its own channel order, grinding or query phase).  Each
+ * runs on the context's stream; device buffers hold extension elements as
+ * (c0, c1) words.  trees: one LDE height, one alignment (as open_with_channel).
+ *
+ * deep_batch_eval: PointQuotients::batch_eval_lifted (p3-miden-lifted-fri/src/
+ *   deep/interpolate.rs:123-201) of every committed column at the n_points
+ *   points, into host evals[column][point] (unaligned widths, tree then matrix
+ *   order): 2 * n_points * columns words, reported in *len; evals == NULL only
+ *   reports *len; cap < *len gives P3G_E_RANGE.  A point on the LDE coset gives
+ *   P3G_E_INVAL.
+ * deep_quotient: DeepPoly::from_evals after its challenges (deep/prover.rs:
+ *   106-285) -- pads evals to the aligned widths, f_red(z_j) by Horner in alpha,
+ *   column coefficients -alpha^(W-1-i), and writes the DEEP polynomial's L
+ *   evaluations (bit-reversed LDE order) to the device buffer out (2L words).
+ *   evals / len as deep_batch_eval returns them.
+ * fri_fold: FriFold::fold_matrix (fri/fold/mod.rs:119-144) for one round of
+ *   FriPolys::new (fri/prover.rs:90-223): the n bit-reversed evaluations at in
+ *   (device) fold to n >> log_arity at out (device, not overlapping in), with
+ *   s_k^-1 = w_n^-bitrev(k) for row k.  log_arity 1..3.  The round's commitment is
+ *   p3g_lmcs_build (alignment 1, P3G_DEVICE_PTRS) of in as an
+ *   (n >> log_arity) x (2 << log_arity) matrix.
+ * fri_final_poly: the final polynomial (fri/prover.rs:208-221) from the first n
+ *   evaluations (host, bit-reversed order) of the last domain to n coefficients
+ *   in descending degree (host).  n a power of two up to 2^24. */
+int p3g_deep_batch_eval(p3g_ctx* ctx, uint32_t log_blowup, const uint64_t* eval_points, uint32_t n_points,
+                        p3g_tree* const* trees, uint64_t ntrees, uint64_t* evals, uint64_t cap, uint64_t* len);
+int p3g_deep_quotient(p3g_ctx* ctx, uint32_t log_blowup, const uint64_t* eval_points, uint32_t n_points,
+                      p3g_tree* const* trees, uint64_t ntrees, const uint64_t* evals, uint64_t len,
+                      const uint64_t alpha[2], const uint64_t beta[2], uint64_t* out);
+int p3g_fri_fold(p3g_ctx* ctx, const uint64_t* in, uint64_t n, uint32_t log_arity, const uint64_t beta[2],
+                 uint64_t* out);
+int p3g_fri_final_poly(const uint64_t* evals, uint64_t n, uint64_t* coeffs);
+
 /* ---- caller-owned channel ----
  * The same operations driven through the caller's ProverChannel (for example a
  * Rust ProverTranscript over its own DuplexChallenger): the library calls back

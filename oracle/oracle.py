@@ -107,6 +107,7 @@ def _load():
         "orc_point_quotients": (None, [_u64p, C.c_size_t, _u64p, C.c_size_t, _u64p]),
         "orc_batch_eval_lifted": (None, [C.POINTER(vp), C.c_size_t, _u64p, C.c_size_t, C.c_uint, _u64p]),
         "orc_fri_fold_row": (None, [C.c_uint, _u64p, C.c_uint64, _u64p, _u64p]),
+        "orc_idft_ext": (None, [_u64p, C.c_size_t]),
         "orc_open_with_channel": (None, [vp, C.c_int, C.POINTER(vp), C.c_size_t, C.c_uint, _u64p, C.c_size_t, vp, _u64p]),
         "orc_verify": (C.c_int, [vp, C.c_int, _u64p, _szp, _szp, C.c_size_t, C.c_uint, _u64p, C.c_size_t, vp, _u64p]),
         "orc_commit_traces": (vp, [C.c_int, C.POINTER(_u64p), _szp, _szp, C.c_size_t, C.c_uint]),
@@ -511,6 +512,13 @@ def batch_eval_lifted_widths(trees, total_width: int, z_points, log_blowup: int)
     o = np.zeros(max(1, total_width * n * 2), np.uint64)
     lib.orc_batch_eval_lifted(hs, len(trees), _p(z), n, log_blowup, _p(o))
     return o[: total_width * n * 2].reshape(total_width, n, 2)
+
+
+def idft_ext(values) -> np.ndarray:
+    """Inverse DFT of n extension elements ([n][2], natural order in and out)."""
+    v = u64(values).reshape(-1).copy()
+    lib.orc_idft_ext(_p(v), v.size // 2)
+    return v.reshape(-1, 2)
 
 
 def fri_fold_row(log_arity: int, row, s_inv: int, beta) -> tuple:

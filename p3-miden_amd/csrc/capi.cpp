@@ -680,6 +680,87 @@ int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t l
   });
 }
 
+// ---- DEEP and FRI steps
+namespace {
+std::vector<ef2> ext_points(const uint64_t* v, uint32_t n) {
+  if (!canonical(v, 2ull * n)) throw ApiError(P3G_E_INVAL, "non-canonical extension element");
+  std::vector<ef2> z(n);
+  for (uint32_t j = 0; j < n; j++) z[j] = ef2{v[2 * j], v[2 * j + 1]};
+  return z;
+}
+std::vector<Tree*> tree_list(p3g_tree* const* trees, uint64_t ntrees) {
+  std::vector<Tree*> tv;
+  for (uint64_t i = 0; i < ntrees; i++) {
+    if (!trees[i]) throw ApiError(P3G_E_INVAL, "null tree");
+    tv.push_back(trees[i]->t);
+  }
+  return tv;
+}
+uint64_t eval_words(const std::vector<Tree*>& tv, uint32_t n_points) {
+  uint64_t cols = 0;
+  for (auto* t : tv)
+    for (auto& m : t->mats) cols += m.width;
+  return cols * n_points * 2;
+}
+}  // namespace
+
+int p3g_deep_batch_eval(p3g_ctx* ctx, uint32_t log_blowup, const uint64_t* eval_points, uint32_t n_points,
+                        p3g_tree* const* trees, uint64_t ntrees, uint64_t* evals, uint64_t cap, uint64_t* len) {
+  if (!ctx || !eval_points || !len || (!trees && ntrees)) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    const std::vector<Tree*> tv = tree_list(trees, ntrees);
+    const std::vector<ef2> z = ext_points(eval_points, n_points);
+    *len = eval_words(tv, n_points);
+    if (!evals) return;
+    if (cap < *len) throw ApiError(P3G_E_RANGE, "evaluation buffer too small");
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    std::vector<ef2> ev;
+    deep_batch_eval(c, log_blowup, tv, z.data(), (int)n_points, ev, c.stream);
+    memcpy(evals, ev.data(), ev.size() * sizeof(ef2));
+  });
+}
+
+int p3g_deep_quotient(p3g_ctx* ctx, uint32_t log_blowup, const uint64_t* eval_points, uint32_t n_points,
+                      p3g_tree* const* trees, uint64_t ntrees, const uint64_t* evals, uint64_t len,
+                      const uint64_t alpha[2], const uint64_t beta[2], uint64_t* out) {
+  if (!ctx || !eval_points || !evals || !alpha || !beta || !out || (!trees && ntrees)) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    const std::vector<Tree*> tv = tree_list(trees, ntrees);
+    const std::vector<ef2> z = ext_points(eval_points, n_points);
+    const std::vector<ef2> ab = ext_points(alpha, 1), bb = ext_points(beta, 1);
+    if (len != eval_words(tv, n_points)) throw ApiError(P3G_E_INVAL, "evaluation count does not match the trees");
+    if (!canonical(evals, len)) throw ApiError(P3G_E_INVAL, "non-canonical evaluation");
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    deep_quotient(c, log_blowup, tv, z.data(), (int)n_points, reinterpret_cast<const ef2*>(evals), ab[0], bb[0],
+                  reinterpret_cast<ef2*>(out), c.stream);
+  });
+}
+
+int p3g_fri_fold(p3g_ctx* ctx, const uint64_t* in, uint64_t n, uint32_t log_arity, const uint64_t beta[2],
+                 uint64_t* out) {
+  if (!ctx || !in || !out || !beta) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    const std::vector<ef2> b = ext_points(beta, 1);
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    fri_fold(c, reinterpret_cast<const ef2*>(in), n, log_arity, b[0], reinterpret_cast<ef2*>(out), c.stream);
+  });
+}
+
+int p3g_fri_final_poly(const uint64_t* evals, uint64_t n, uint64_t* coeffs) {
+  if (!evals || !coeffs) return P3G_E_INVAL;
+  if (n == 0 || (n & (n - 1))) return P3G_E_NOT_POW2;
+  if (n > (1ull << 24)) return P3G_E_RANGE;
+  return guard([&] {
+    if (!canonical(evals, 2 * n)) throw ApiError(P3G_E_INVAL, "non-canonical evaluation");
+    std::vector<ef2> out(n);
+    fri_final_poly(reinterpret_cast<const ef2*>(evals), n, out.data());
+    memcpy(coeffs, out.data(), n * sizeof(ef2));
+  });
+}
+
 namespace {
 // ProverChannel implemented by the caller through function pointers.
 struct CallbackChannel : Channel {

@@ -332,6 +332,18 @@ void quotient_divide_vanishing(Ctx& ctx, ef2* num, uint64_t n, uint64_t d, uint6
 void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned log_lde_height, const ef2* z, int npts,
                        const std::vector<Tree*>& trees, Channel& tx, hipStream_t s);
 
+// The steps open_with_channel composes.  deep_batch_eval: evals[column][point]
+// (host, unaligned widths in tree and matrix order); deep_quotient: the DEEP
+// polynomial's L evaluations into the device buffer out from such evals and
+// the two challenges; fri_fold: one round, n -> n >> la device elements;
+// fri_final_poly: host, bit-reversed evaluations -> descending coefficients.
+void deep_batch_eval(Ctx& ctx, unsigned log_blowup, const std::vector<Tree*>& trees, const ef2* z, int npts,
+                     std::vector<ef2>& evals, hipStream_t s);
+void deep_quotient(Ctx& ctx, unsigned log_blowup, const std::vector<Tree*>& trees, const ef2* z, int npts,
+                   const ef2* evals, ef2 alpha, ef2 beta, ef2* out, hipStream_t s);
+void fri_fold(Ctx& ctx, const ef2* in, uint64_t n, unsigned la, ef2 beta, ef2* out, hipStream_t s);
+void fri_final_poly(const ef2* bitrev_evals, size_t n, ef2* coeffs_desc);
+
 // f(0..n-1) on the library's host worker threads (serially when the pool is busy)
 void parallel_for(size_t n, const std::function<void(size_t)>& f);
 

@@ -698,23 +698,186 @@ static void run_quotient(Ctx& ctx, unsigned logL, const ef2* z, const std::vecto
   ctx.release(dc);
 }
 
-static void host_idft_ext(std::vector<ef2>& v) {
-  const size_t n = v.size();
-  if (n <= 1) return;
-  const unsigned ln = log2_exact(n);
-  const uint64_t winv = gl_inv(gl_two_adic_generator(ln));
-  const uint64_t ninv = gl_inv_pow2(ln);
-  std::vector<ef2> out(n);
-  for (size_t k = 0; k < n; k++) {
-    ef2 acc = ef_from(0);
-    uint64_t step = gl_pow(winv, k), x = 1;
-    for (size_t i = 0; i < n; i++) {
-      acc = ef_add(acc, ef_mul_base(v[i], x));
-      x = gl_mul(x, step);
+// The final polynomial (fri/prover.rs:208-221): n evaluations of the last FRI
+// domain in bit-reversed order to the coefficients in descending degree.  A
+// decimation-in-time inverse DFT takes the bit-reversed input as it stands and
+// yields natural-order coefficients, which are then scaled by 1/n and reversed.
+void fri_final_poly(const ef2* br, size_t n, ef2* out) {
+  std::vector<ef2> v(br, br + n);
+  for (size_t len = 2; len <= n; len <<= 1) {
+    const uint64_t w = gl_inv(gl_two_adic_generator(log2_exact(len)));
+    const size_t h = len >> 1;
+    std::vector<uint64_t> tw(h);
+    uint64_t x = 1;
+    for (size_t j = 0; j < h; j++) {
+      tw[j] = x;
+      x = gl_mul(x, w);
     }
-    out[k] = ef_mul_base(acc, ninv);
+    for (size_t i = 0; i < n; i += len)
+      for (size_t j = 0; j < h; j++) {
+        const ef2 u = v[i + j], t = ef_mul_base(v[i + j + h], tw[j]);
+        v[i + j] = ef_add(u, t);
+        v[i + j + h] = ef_sub(u, t);
+      }
   }
-  v.swap(out);
+  const uint64_t ninv = n > 1 ? gl_inv_pow2(log2_exact(n)) : 1;
+  for (size_t k = 0; k < n; k++) out[n - 1 - k] = ef_mul_base(v[k], ninv);
+}
+
+// In-row inverse DFT twiddles of the fold kernels, stage-major: stage s holds
+// w_{2^s}^{-j} for j < 2^(s-1).
+static FoldTw fold_twiddles(unsigned la) {
+  FoldTw tw{};
+  int twi = 0;
+  for (unsigned st = 1; st <= la; st++) {
+    const uint64_t wm = gl_inv(gl_two_adic_generator(st));
+    uint64_t x = 1;
+    for (uint64_t j = 0; j < (1ull << (st - 1)); j++) {
+      tw.w[twi + j] = x;
+      x = gl_mul(x, wm);
+    }
+    twi += 1 << (st - 1);
+  }
+  return tw;
+}
+
+template <int LA>
+static void launch_fold(const ef2* in, ef2* out, uint64_t fd, unsigned log_rows, const uint64_t* powinv,
+                        unsigned exp_shift, ef2 beta, const FoldTw& tw, uint64_t ainv, hipStream_t s) {
+  hipLaunchKernelGGL(fri_fold_kernel<LA>, dim3((uint32_t)((fd + 255) / 256)), dim3(256), 0, s, in, out, fd, log_rows,
+                     powinv, exp_shift, beta, tw, ainv);
+}
+
+// Aligned OOD evaluations [point][aligned column] from evals[column][point]:
+// the zero padding the prover sends for the columns the alignment adds
+// (deep/prover.rs:120-160).
+static std::vector<ef2> align_evals(const std::vector<DevMat>& mats, const std::vector<uint64_t>& awidths,
+                                    const ef2* evals, int npts, uint64_t tota) {
+  std::vector<ef2> aligned((size_t)tota * npts, ef_from(0));
+  uint64_t off = 0, aoff = 0;
+  for (size_t m = 0; m < mats.size(); m++) {
+    for (uint64_t c = 0; c < mats[m].width; c++)
+      for (int j = 0; j < npts; j++) aligned[j * tota + aoff + c] = evals[(off + c) * npts + j];
+    off += mats[m].width;
+    aoff += awidths[m];
+  }
+  return aligned;
+}
+
+// DEEP quotient on the LDE domain from the aligned evaluations and the two
+// challenges (deep/prover.rs:106-285): f_red(z_j) by Horner in alpha, column
+// coefficients -alpha^(W-1-i), then the accumulation kernels.
+static void deep_from_aligned(Ctx& ctx, unsigned logL, const ef2* z, int npts, const std::vector<DevMat>& mats,
+                              const std::vector<uint64_t>& awidths, const std::vector<ef2>& aligned, uint64_t tota,
+                              ef2 alpha, ef2 beta, ef2* out, hipStream_t s) {
+  ef2 fred[MAXPTS];
+  for (int j = 0; j < npts; j++) {
+    ef2 acc = ef_from(0);
+    for (uint64_t c = 0; c < tota; c++) acc = ef_add(ef_mul(acc, alpha), aligned[j * tota + c]);
+    fred[j] = acc;
+  }
+  std::vector<ef2> coeffs(tota);
+  {
+    ef2 pw = ef_neg(ef_from(1));
+    for (uint64_t i = tota; i-- > 0;) {
+      coeffs[i] = pw;
+      pw = ef_mul(pw, alpha);
+    }
+  }
+  switch (npts) {
+    case 1: run_quotient<1>(ctx, logL, z, mats, awidths, coeffs, fred, beta, out, s); break;
+    case 2: run_quotient<2>(ctx, logL, z, mats, awidths, coeffs, fred, beta, out, s); break;
+    case 3: run_quotient<3>(ctx, logL, z, mats, awidths, coeffs, fred, beta, out, s); break;
+    default: run_quotient<4>(ctx, logL, z, mats, awidths, coeffs, fred, beta, out, s); break;
+  }
+}
+
+static void deep_evals(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z, int npts,
+                       const std::vector<DevMat>& mats, std::vector<ef2>& evals, hipStream_t s) {
+  switch (npts) {
+    case 1: run_deep_kernels<1>(ctx, p, logL, z, mats, evals, s); break;
+    case 2: run_deep_kernels<2>(ctx, p, logL, z, mats, evals, s); break;
+    case 3: run_deep_kernels<3>(ctx, p, logL, z, mats, evals, s); break;
+    default: run_deep_kernels<4>(ctx, p, logL, z, mats, evals, s); break;
+  }
+}
+
+// Checks shared by the step-level DEEP calls: the trees' common LDE height,
+// their matrices and aligned widths; every matrix at least 2^log_blowup tall
+// and no point on the LDE coset (where 1/(z - x) does not exist).
+static unsigned deep_inputs(const std::vector<Tree*>& trees, unsigned log_blowup, const ef2* z, int npts,
+                            std::vector<DevMat>& mats, std::vector<uint64_t>& awidths, uint64_t& tota) {
+  if (trees.empty()) throw ApiError(E_EMPTY, "at least one trace tree required");
+  if (npts < 1 || npts > MAXPTS) throw ApiError(E_INVAL, "1 to 4 evaluation points supported");
+  const uint64_t L = trees[0]->height;
+  const uint32_t alignment = trees[0]->alignment;
+  for (auto* t : trees) {
+    if (t->height != L) throw ApiError(E_INVAL, "mixed LDE heights are not supported");
+    if (t->alignment != alignment) throw ApiError(E_INVAL, "mixed trace tree alignments are not supported");
+  }
+  const unsigned logL = log2_exact(L);
+  if (log_blowup > logL) throw ApiError(E_INVAL, "LDE height below blowup");
+  tota = 0;
+  for (auto* t : trees)
+    for (auto& m : t->mats) {
+      if ((m.height >> log_blowup) == 0) throw ApiError(E_INVAL, "matrix shorter than the blowup");
+      mats.push_back(m);
+      awidths.push_back(aligned_len(m.width, alignment));
+      tota += awidths.back();
+    }
+  for (int j = 0; j < npts; j++) {
+    if (z[j].c1 != 0) continue;
+    // x in g*H_L  <=>  (x / g)^L == 1
+    uint64_t t = gl_mul(z[j].c0, gl_inv(GENERATOR));
+    for (unsigned k = 0; k < logL; k++) t = gl_mul(t, t);
+    if (t == 1) throw ApiError(E_INVAL, "evaluation point lies on the LDE coset");
+  }
+  return logL;
+}
+
+void deep_batch_eval(Ctx& ctx, unsigned log_blowup, const std::vector<Tree*>& trees, const ef2* z, int npts,
+                     std::vector<ef2>& evals, hipStream_t s) {
+  std::vector<DevMat> mats;
+  std::vector<uint64_t> awidths;
+  uint64_t tota = 0;
+  const unsigned logL = deep_inputs(trees, log_blowup, z, npts, mats, awidths, tota);
+  PcsParams p{};
+  p.log_blowup = log_blowup;
+  deep_evals(ctx, p, logL, z, npts, mats, evals, s);
+}
+
+void deep_quotient(Ctx& ctx, unsigned log_blowup, const std::vector<Tree*>& trees, const ef2* z, int npts,
+                   const ef2* evals, ef2 alpha, ef2 beta, ef2* out, hipStream_t s) {
+  std::vector<DevMat> mats;
+  std::vector<uint64_t> awidths;
+  uint64_t tota = 0;
+  const unsigned logL = deep_inputs(trees, log_blowup, z, npts, mats, awidths, tota);
+  const std::vector<ef2> aligned = align_evals(mats, awidths, evals, npts, tota);
+  deep_from_aligned(ctx, logL, z, npts, mats, awidths, aligned, tota, alpha, beta, out, s);
+}
+
+// One FRI round's fold (FriFold::fold_matrix, fri/fold/mod.rs:119-144) of the
+// n evaluations of round domain D (|D| = n, bit-reversed) into n / 2^la: row k
+// of arity 2^la lies on the coset s_k <w_a> with s_k^-1 = w_n^-bitrev(k).
+void fri_fold(Ctx& ctx, const ef2* in, uint64_t n, unsigned la, ef2 beta, ef2* out, hipStream_t s) {
+  if (la < 1 || la > 3) throw ApiError(E_INVAL, "folding arity must be 2, 4 or 8");
+  if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "domain size must be a power of two");
+  const unsigned logn = log2_exact(n);
+  if (logn < la) throw ApiError(E_INVAL, "domain smaller than the folding arity");
+  if (logn > 32) throw ApiError(E_RANGE, "domain exceeds the field two-adicity");
+  const uint64_t fd = n >> la;
+  if (out < in + n && in < out + fd) throw ApiError(E_INVAL, "fold output overlaps its input");
+  const unsigned log_rows = logn - la;
+  const uint64_t* powinv = ctx.pow_table(gl_inv(gl_two_adic_generator(logn)), log_rows);
+  const FoldTw tw = fold_twiddles(la);
+  const uint64_t ainv = gl_inv_pow2(la);
+  ProfScope ps(ctx, "fri_fold", s);
+  switch (la) {
+    case 1: launch_fold<1>(in, out, fd, log_rows, powinv, 0, beta, tw, ainv, s); break;
+    case 2: launch_fold<2>(in, out, fd, log_rows, powinv, 0, beta, tw, ainv, s); break;
+    default: launch_fold<3>(in, out, fd, log_rows, powinv, 0, beta, tw, ainv, s); break;
+  }
+  P3G_CHECK_HIP(hipGetLastError());
 }
 
 void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z, int npts,
@@ -742,49 +905,16 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
 
   // ---- DEEP: evaluations at the OOD points
   std::vector<ef2> evals;  // [column][point]
-  switch (npts) {
-    case 1: run_deep_kernels<1>(ctx, p, logL, z, mats, evals, s); break;
-    case 2: run_deep_kernels<2>(ctx, p, logL, z, mats, evals, s); break;
-    case 3: run_deep_kernels<3>(ctx, p, logL, z, mats, evals, s); break;
-    default: run_deep_kernels<4>(ctx, p, logL, z, mats, evals, s); break;
-  }
+  deep_evals(ctx, p, logL, z, npts, mats, evals, s);
   uint64_t tota = 0;
   for (auto a : awidths) tota += a;
-  std::vector<ef2> aligned((size_t)tota * npts, ef_from(0));  // [point][aligned column]
-  {
-    uint64_t off = 0, aoff = 0;
-    for (size_t m = 0; m < mats.size(); m++) {
-      for (uint64_t c = 0; c < mats[m].width; c++)
-        for (int j = 0; j < npts; j++) aligned[j * tota + aoff + c] = evals[(off + c) * npts + j];
-      off += mats[m].width;
-      aoff += awidths[m];
-    }
-  }
+  const std::vector<ef2> aligned = align_evals(mats, awidths, evals.data(), npts, tota);  // [point][aligned column]
   for (int j = 0; j < npts; j++) tx.send_fields(reinterpret_cast<const uint64_t*>(aligned.data() + j * tota), 2 * tota);
   tx.grind(p.deep_pow_bits);
   const ef2 alpha = tx.sample_ef();
   const ef2 beta = tx.sample_ef();
-  ef2 fred[MAXPTS];
-  for (int j = 0; j < npts; j++) {
-    ef2 acc = ef_from(0);
-    for (uint64_t c = 0; c < tota; c++) acc = ef_add(ef_mul(acc, alpha), aligned[j * tota + c]);
-    fred[j] = acc;
-  }
-  std::vector<ef2> coeffs(tota);
-  {
-    ef2 pw = ef_neg(ef_from(1));
-    for (uint64_t i = tota; i-- > 0;) {
-      coeffs[i] = pw;
-      pw = ef_mul(pw, alpha);
-    }
-  }
   ef2* Q = (ef2*)ctx.alloc(L * sizeof(ef2));
-  switch (npts) {
-    case 1: run_quotient<1>(ctx, logL, z, mats, awidths, coeffs, fred, beta, Q, s); break;
-    case 2: run_quotient<2>(ctx, logL, z, mats, awidths, coeffs, fred, beta, Q, s); break;
-    case 3: run_quotient<3>(ctx, logL, z, mats, awidths, coeffs, fred, beta, Q, s); break;
-    default: run_quotient<4>(ctx, logL, z, mats, awidths, coeffs, fred, beta, Q, s); break;
-  }
+  deep_from_aligned(ctx, logL, z, npts, mats, awidths, aligned, tota, alpha, beta, Q, s);
 
   // ---- FRI commit phase
   const unsigned la = p.log_arity;
@@ -793,19 +923,7 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
   const uint64_t fdomain = (uint64_t)fdeg << p.log_blowup;
   const unsigned tbits = logL > la ? logL - la : 0;
   const uint64_t* powLinv = ctx.pow_table(gl_inv(gl_two_adic_generator(logL)), tbits);
-  FoldTw tw{};
-  {
-    int twi = 0;
-    for (unsigned st = 1; st <= la; st++) {
-      uint64_t wm = gl_inv(gl_two_adic_generator(st));
-      uint64_t h = 1ull << (st - 1), x = 1;
-      for (uint64_t j = 0; j < h; j++) {
-        tw.w[twi + j] = x;
-        x = gl_mul(x, wm);
-      }
-      twi += (int)h;
-    }
-  }
+  const FoldTw tw = fold_twiddles(la);
   const uint64_t ainv = gl_inv_pow2(la);
   std::vector<Tree*> ftrees;
   std::vector<ef2*> bufs;
@@ -914,12 +1032,9 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
       ctx.read_back(fin.data(), cur, fdeg * sizeof(ef2), s);
     }
     {
-      const unsigned lf = log2_exact(fdeg);
-      std::vector<ef2> nat(fdeg);
-      for (size_t i = 0; i < fdeg; i++) nat[bitrev32((uint32_t)i, lf)] = fin[i];
-      host_idft_ext(nat);
-      std::reverse(nat.begin(), nat.end());
-      tx.send_fields(reinterpret_cast<const uint64_t*>(nat.data()), 2 * fdeg);
+      std::vector<ef2> coeffs(fdeg);
+      fri_final_poly(fin.data(), fdeg, coeffs.data());
+      tx.send_fields(reinterpret_cast<const uint64_t*>(coeffs.data()), 2 * fdeg);
     }
     ctx.release(cur);
     cur = nullptr;
@@ -1238,19 +1353,7 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
     const uint64_t fdomain = (uint64_t)fdeg << p.log_blowup;
     const unsigned tbits = logL > la ? logL - la : 0;
     const uint64_t* powLinv = ctx.pow_table(gl_inv(gl_two_adic_generator(logL)), tbits);
-    FoldTw tw{};
-    {
-      int twi = 0;
-      for (unsigned st = 1; st <= la; st++) {
-        uint64_t wm = gl_inv(gl_two_adic_generator(st));
-        uint64_t h = 1ull << (st - 1), x = 1;
-        for (uint64_t q = 0; q < h; q++) {
-          tw.w[twi + q] = x;
-          x = gl_mul(x, wm);
-        }
-        twi += (int)h;
-      }
-    }
+    const FoldTw tw = fold_twiddles(la);
     const uint64_t ainv = gl_inv_pow2(la);
     std::vector<ef2*> cur = Q;
     uint64_t domain = L;
@@ -1384,14 +1487,11 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
         for (size_t j = 0; j < k; j++)
           if (!ok[j]) throw ApiError(E_TRANSCRIPT, "device transcript diverged from the host replay");
       }
-      const unsigned lf = log2_exact(fdeg);
       parallel_for(k, [&](size_t j) {
         const ef2* fin = reinterpret_cast<const ef2*>(raw.data() + j * 2 * fdeg);
-        std::vector<ef2> nat(fdeg);
-        for (size_t i = 0; i < fdeg; i++) nat[bitrev32((uint32_t)i, lf)] = fin[i];
-        host_idft_ext(nat);
-        std::reverse(nat.begin(), nat.end());
-        txs[j]->send_fields(reinterpret_cast<const uint64_t*>(nat.data()), 2 * fdeg);
+        std::vector<ef2> coeffs(fdeg);
+        fri_final_poly(fin, fdeg, coeffs.data());
+        txs[j]->send_fields(reinterpret_cast<const uint64_t*>(coeffs.data()), 2 * fdeg);
       });
     }
     // ---- query phase: indices per job, then one gather for every opening of every job

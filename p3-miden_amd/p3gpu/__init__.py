@@ -214,6 +214,12 @@ def _load():
         "p3g_proof_decode": (C.c_int, [_vp, C.c_uint64, _u64p, _u64p, _u64p, _u64p]),
         "p3g_transcript_encode": (C.c_int, [_vp, _vp, C.c_uint64, _u64p]),
         "p3g_open_with_channel": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, _vp]),
+        "p3g_deep_batch_eval": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, _u64p,
+                                          C.c_uint64, _u64p]),
+        "p3g_deep_quotient": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, _u64p,
+                                        C.c_uint64, _u64p, _u64p, _u64p]),
+        "p3g_fri_fold": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint32, _u64p, _u64p]),
+        "p3g_fri_final_poly": (C.c_int, [_u64p, C.c_uint64, _u64p]),
         "p3g_tree_prove_batch_cb": (C.c_int, [_vp, _u64p, C.c_uint64, C.POINTER(_Channel)]),
         "p3g_open_with_callbacks": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, C.POINTER(_Channel)]),
         "p3g_commit_plan_create": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
@@ -741,6 +747,63 @@ def open_with_channel(params: PcsParams, lmcs: LmcsConfig, log_lde_height: int, 
         cb = channel if isinstance(channel, CallbackChannel) else CallbackChannel(channel)
         _check(lib().p3g_open_with_callbacks(lmcs.dev.h, C.byref(params), log_lde_height, _p(z), z.size // 2, hs,
                                              len(trace_trees), C.byref(cb.struct)), "open_with_callbacks")
+
+
+def batch_eval_lifted(trace_trees, eval_points, log_blowup: int) -> np.ndarray:
+    """PointQuotients::batch_eval_lifted (p3-miden-lifted-fri/src/deep/interpolate.rs:123-201) of every
+    committed column of the trees at the points: an array [column][point][2] (unaligned widths, tree
+    then matrix order)."""
+    if not trace_trees:
+        raise ValueError("at least one tree required")
+    z = u64(np.asarray(eval_points, dtype=np.uint64).ravel())
+    hs = (_vp * len(trace_trees))(*[t.h for t in trace_trees])
+    h = trace_trees[0].dev.h
+    n = np.zeros(1, np.uint64)
+    _check(lib().p3g_deep_batch_eval(h, log_blowup, _p(z), z.size // 2, hs, len(trace_trees), None, 0, _p(n)),
+           "deep_batch_eval")
+    out = np.zeros(max(1, int(n[0])), np.uint64)
+    _check(lib().p3g_deep_batch_eval(h, log_blowup, _p(z), z.size // 2, hs, len(trace_trees), _p(out), out.size,
+                                     _p(n)), "deep_batch_eval")
+    return out[: int(n[0])].reshape(-1, z.size // 2, 2)
+
+
+def deep_quotient(trace_trees, eval_points, log_blowup: int, evals, alpha, beta) -> "DeviceMatrix":
+    """DeepPoly::from_evals after its challenges (p3-miden-lifted-fri/src/deep/prover.rs:106-285): the DEEP
+    polynomial's L evaluations (bit-reversed LDE order) as an L x 2 DeviceMatrix, from `evals` as
+    batch_eval_lifted returns them and the challenges alpha (columns) and beta (points)."""
+    if not trace_trees:
+        raise ValueError("at least one tree required")
+    z = u64(np.asarray(eval_points, dtype=np.uint64).ravel())
+    ev = u64(np.asarray(evals, dtype=np.uint64).ravel())
+    a = u64(np.asarray(alpha, dtype=np.uint64).ravel())
+    b = u64(np.asarray(beta, dtype=np.uint64).ravel())
+    hs = (_vp * len(trace_trees))(*[t.h for t in trace_trees])
+    dev = trace_trees[0].dev
+    out = dev.alloc_matrix(trace_trees[0].height(), 2)
+    _check(lib().p3g_deep_quotient(dev.h, log_blowup, _p(z), z.size // 2, hs, len(trace_trees), _p(ev), ev.size,
+                                   _p(a), _p(b), C.cast(out.ptr, _u64p)), "deep_quotient")
+    return out
+
+
+def fri_fold(evals: "DeviceMatrix", log_arity: int, beta) -> "DeviceMatrix":
+    """One FRI round's fold (FriFold::fold_matrix, p3-miden-lifted-fri/src/fri/fold/mod.rs:119-144) of the
+    n bit-reversed evaluations in an n x 2 DeviceMatrix into an (n >> log_arity) x 2 one."""
+    if evals.width != 2:
+        raise ValueError("evaluations must be an n x 2 matrix of extension elements")
+    b = u64(np.asarray(beta, dtype=np.uint64).ravel())
+    out = evals.dev.alloc_matrix(max(1, evals.height >> log_arity), 2)
+    _check(lib().p3g_fri_fold(evals.dev.h, C.cast(evals.ptr, _u64p), evals.height, log_arity, _p(b),
+                              C.cast(out.ptr, _u64p)), "fri_fold")
+    return out
+
+
+def fri_final_poly(evals) -> np.ndarray:
+    """The final polynomial (p3-miden-lifted-fri/src/fri/prover.rs:208-221): n evaluations of the last
+    domain in bit-reversed order ([n][2] host array) to n coefficients in descending degree."""
+    v = u64(np.asarray(evals, dtype=np.uint64).ravel())
+    out = np.zeros(max(2, v.size), np.uint64)
+    _check(lib().p3g_fri_final_poly(_p(v), v.size // 2, _p(out)), "fri_final_poly")
+    return out[: v.size].reshape(-1, 2)
 
 
 def prove(lmcs: LmcsConfig, params: PcsParams, traces, channel: ProverTranscript = None):

@@ -163,3 +163,47 @@ def test_proof_decoding_rejects_malformed_input():
     with pytest.raises(G.P3GError) as e:
         G.encode_proof([O.P], np.zeros((0, 4), np.uint64))
     assert e.value.code == 1
+
+
+def _bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+@pytest.mark.parametrize("logn", [0, 1, 2, 3, 5, 7])
+def test_fri_final_poly_recovers_coefficients(logn):
+    # closed form: evaluate p(x) = sum_k c_k x^k at w_n^bitrev(i) (the last FRI domain in
+    # bit-reversed order); the call must return the c_k in descending degree
+    P = 0xFFFFFFFF00000001
+    n = 1 << logn
+    rng = random.Random(40 + logn)
+    c = [(rng.randrange(P), rng.randrange(P)) for _ in range(n)]
+    w = O.two_adic_generator(logn)
+    ev = np.zeros((n, 2), np.uint64)
+    for i in range(n):
+        x = O.gl_pow(w, _bitrev(i, logn))
+        xp, a0, a1 = 1, 0, 0
+        for c0, c1 in c:
+            a0, a1 = (a0 + c0 * xp) % P, (a1 + c1 * xp) % P
+            xp = xp * x % P
+        ev[i] = (a0, a1)
+    got = G.fri_final_poly(ev)
+    assert np.array_equal(got, np.array(c[::-1], np.uint64).reshape(n, 2))
+
+
+def test_fri_final_poly_matches_oracle_idft():
+    # the oracle's inverse DFT on the natural-order values, reversed
+    n, lf = 16, 4
+    ev = O.rand_matrix(71, n, 2)
+    nat = np.zeros_like(ev)
+    for i in range(n):
+        nat[_bitrev(i, lf)] = ev[i]
+    want = O.idft_ext(nat)[::-1]
+    assert np.array_equal(G.fri_final_poly(ev), want)
+
+
+def test_fri_final_poly_rejects_bad_input():
+    P = 0xFFFFFFFF00000001
+    with pytest.raises(G.P3GError):
+        G.fri_final_poly(np.zeros((3, 2), np.uint64))
+    with pytest.raises(G.P3GError):
+        G.fri_final_poly(np.array([[P, 0], [0, 0]], np.uint64))

@@ -1,0 +1,204 @@
+"""GPU parity of the step-level DEEP and FRI calls (batch_eval_lifted,
+deep_quotient, fri_fold, fri_final_poly) through the C ABI: each against the
+oracle's own function on the same inputs, and the steps composed in the order
+of open_with_channel (p3-miden-lifted-fri/src/prover.rs:35-116) against the
+oracle's whole opening, transcript word for word.  Bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import p3gpu as G
+
+pytestmark = pytest.mark.gpu
+
+P = 0xFFFFFFFF00000001
+
+
+def _bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def _aligned_per_point(ev, widths, alignment):
+    """evals[column][point][2] -> per point the flat aligned slice the prover sends."""
+    npts = ev.shape[1]
+    out = []
+    for j in range(npts):
+        flat, off = [], 0
+        for w in widths:
+            flat.append(ev[off:off + w, j, :].reshape(-1))
+            pad = O.aligned_len(w, alignment) - w
+            if pad:
+                flat.append(np.zeros(2 * pad, np.uint64))
+            off += w
+        out.append(np.concatenate(flat) if flat else np.zeros(0, np.uint64))
+    return out
+
+
+GROUPS = [
+    # (hash, groups of trace shapes, log_blowup, points)
+    (G.HASH_POSEIDON2, [[(64, 6), (128, 10)], [(128, 3)]], 1, [(11, 12), (13, 14), (15, 16)]),
+    (G.HASH_KECCAK, [[(32, 5), (32, 1), (256, 9)]], 2, [(3, 5), (7, 9)]),
+    (G.HASH_POSEIDON2, [[(512, 17)]], 3, [(21, 22), (23, 24), (25, 26), (27, 28)]),
+    (G.HASH_POSEIDON2, [[(8, 2), (16, 4)]], 1, [(101, 0)]),  # base-field point off the coset
+]
+
+
+def _commit(device, kind, groups, lb, seed):
+    cfg = G.LmcsConfig(device, kind)
+    okind = O.HASH_P2 if kind == G.HASH_POSEIDON2 else O.HASH_KECCAK
+    gts, ots, widths = [], [], []
+    for g, shapes in enumerate(groups):
+        tr = [O.rand_matrix(seed + 10 * g + i, h, w) for i, (h, w) in enumerate(shapes)]
+        gts.append(G.commit_traces(cfg, tr, lb))
+        ots.append(O.commit_traces(okind, tr, lb))
+        widths += [w for _, w in shapes]
+    return cfg, okind, gts, ots, widths
+
+
+@pytest.mark.parametrize("case", range(len(GROUPS)))
+def test_batch_eval_lifted_matches_oracle(device, case):
+    kind, groups, lb, z = GROUPS[case]
+    _, _, gts, ots, widths = _commit(device, kind, groups, lb, 300 + case)
+    got = G.batch_eval_lifted(gts, z, lb)
+    want = O.batch_eval_lifted_widths(ots, sum(widths), z, lb)
+    assert got.shape == (sum(widths), len(z), 2)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("case", range(len(GROUPS)))
+def test_deep_quotient_matches_oracle_opening(device, case):
+    kind, groups, lb, z = GROUPS[case]
+    _, okind, gts, ots, widths = _commit(device, kind, groups, lb, 400 + case)
+    logL = int(gts[0].height()).bit_length() - 1
+    params = O.PcsParams(lb, 1, 0, 0, 0, 3, 0)
+    otx = O.Transcript(O.CH_DUPLEX_P2 if kind == G.HASH_POSEIDON2 else O.CH_KECCAK)
+    ctx = otx.clone()
+    want = O.open_with_channel(params, okind, ots, logL, z, otx, want_deep=True)
+    # the challenges the oracle's opening drew: aligned evals sent per point, then alpha and beta
+    ev = G.batch_eval_lifted(gts, z, lb)
+    for flat in _aligned_per_point(ev, widths, gts[0].alignment()):
+        ctx.send_fields(flat)
+    ctx.grind(0)
+    alpha, beta = ctx.sample_ef(), ctx.sample_ef()
+    q = G.deep_quotient(gts, z, lb, ev, alpha, beta)
+    assert np.array_equal(q.download(), want)
+
+
+@pytest.mark.parametrize("logn,la", [(1, 1), (2, 2), (3, 3), (6, 1), (9, 2), (12, 3), (13, 1), (11, 2)])
+def test_fri_fold_matches_oracle(device, logn, la):
+    n, a = 1 << logn, 1 << la
+    ev = O.rand_matrix(900 + logn * 4 + la, n, 2)
+    beta = (int(O.rand_matrix(77 + la, 1, 1)[0, 0]), 12345)
+    d = device.upload(ev)
+    got = G.fri_fold(d, la, beta).download()
+    ginv = O.gl_inv(O.two_adic_generator(logn))
+    rows = n // a
+    lr = logn - la
+    want = np.array([O.fri_fold_row(la, ev[k * a:(k + 1) * a], O.gl_pow(ginv, _bitrev(k, lr)), beta)
+                     for k in range(rows)], np.uint64)
+    assert np.array_equal(got, want.reshape(rows, 2))
+
+
+def test_fri_fold_of_a_low_degree_polynomial(device):
+    # folding p(x) = sum_j x^j p_j(x^a) by beta gives sum_j beta^j p_j(y) on the squared domain:
+    # with p of degree < a (one coefficient per p_j), every folded value equals p evaluated "at beta"
+    # in the coefficient sense, i.e. sum_j c_j beta^j, whatever the point.
+    la, logn = 2, 8
+    n, a = 1 << logn, 1 << la
+    c = [5, 9, 13, 17]  # base-field coefficients of p, degree < a
+    w = O.two_adic_generator(logn)
+    ev = np.zeros((n, 2), np.uint64)
+    for i in range(n):
+        x = O.gl_pow(w, _bitrev(i, logn))
+        ev[i, 0] = sum(cj * O.gl_pow(x, j) for j, cj in enumerate(c)) % P
+    beta = (3, 0)
+    got = G.fri_fold(device.upload(ev), la, beta).download()
+    want = sum(cj * 3 ** j for j, cj in enumerate(c)) % P
+    assert np.all(got[:, 0] == want) and np.all(got[:, 1] == 0)
+
+
+def test_fri_fold_rejects_bad_shapes(device):
+    d = device.upload(np.zeros((8, 2), np.uint64))
+    for la in (0, 4):
+        with pytest.raises(G.P3GError):
+            G.fri_fold(d, la, (1, 0))
+    with pytest.raises(G.P3GError):
+        G.fri_fold(device.upload(np.zeros((2, 2), np.uint64)), 2, (1, 0))  # domain below the arity
+    with pytest.raises(G.P3GError):
+        G.fri_fold(device.upload(np.zeros((12, 2), np.uint64)), 1, (1, 0))  # not a power of two
+    with pytest.raises(G.P3GError):
+        G.fri_fold(d, 1, (P, 0))  # non-canonical challenge
+
+
+def test_deep_steps_reject_bad_inputs(device):
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    t = G.commit_traces(cfg, [O.rand_matrix(1, 16, 3)], 1)
+    lb = 1
+    g = 7  # the LDE coset is 7 * H_32: 7 itself lies on it
+    with pytest.raises(G.P3GError):
+        G.batch_eval_lifted([t], [(g, 0)], lb)
+    with pytest.raises(G.P3GError):
+        G.batch_eval_lifted([t], [(1, 2)] * 5, lb)  # at most four points
+    with pytest.raises(G.P3GError):
+        G.batch_eval_lifted([t], [(1, 2)], 6)  # blowup above the LDE height
+    ev = G.batch_eval_lifted([t], [(1, 2)], lb)
+    with pytest.raises(G.P3GError):
+        G.deep_quotient([t], [(1, 2)], lb, ev[:-1], (1, 0), (2, 0))  # count does not match the trees
+    t2 = G.commit_traces(cfg, [O.rand_matrix(2, 8, 3)], 1)
+    with pytest.raises(G.P3GError):
+        G.batch_eval_lifted([t, t2], [(1, 2)], lb)  # mixed LDE heights
+
+
+@pytest.mark.parametrize("kind,params", [
+    (G.HASH_POSEIDON2, (1, 1, 0, 0, 0, 9, 0)),
+    (G.HASH_POSEIDON2, (2, 2, 1, 0, 2, 7, 1)),
+    (G.HASH_KECCAK, (1, 3, 2, 1, 0, 5, 0)),
+    (G.HASH_POSEIDON2, (3, 1, 6, 0, 0, 4, 0)),  # no FRI round: final polynomial only
+])
+def test_composed_steps_reproduce_open_with_channel(device, kind, params):
+    groups = [[(32, 4), (64, 7)], [(64, 2)]]
+    lb, la, _, fold_pow, deep_pow, nq, query_pow = params
+    cfg, okind, gts, ots, widths = _commit(device, kind, groups, lb, 500 + la)
+    logL = int(gts[0].height()).bit_length() - 1
+    z = [(31, 32), (33, 34)]
+    ch = G.CHALLENGER_DUPLEX_P2 if kind == G.HASH_POSEIDON2 else G.CHALLENGER_KECCAK
+    tx = G.ProverTranscript(ch)
+    # DEEP
+    ev = G.batch_eval_lifted(gts, z, lb)
+    for flat in _aligned_per_point(ev, widths, gts[0].alignment()):
+        tx.send_field_slice(flat)
+    tx.grind(deep_pow)
+    alpha = tx.sample_algebra_element()
+    beta = tx.sample_algebra_element()
+    cur = G.deep_quotient(gts, z, lb, ev, alpha, beta)
+    # FRI commit phase: commit the round as (n / a) x 2a, fold, until the final domain
+    fdeg = O.PcsParams(*params).final_degree(logL)
+    keep, ftrees = [], []
+    while cur.height > fdeg << lb:
+        view = G.DeviceMatrix(device, cur.ptr, cur.height >> la, 2 << la, owned=False)
+        t = cfg.build_tree([view])
+        tx.send_commitment(t.root())
+        tx.grind(fold_pow)
+        b = tx.sample_algebra_element()
+        keep.append(cur)
+        ftrees.append(t)
+        cur = G.fri_fold(cur, la, b)
+    tx.send_field_slice(G.fri_final_poly(cur.download()[:fdeg]).reshape(-1))
+    # query phase
+    tx.grind(query_pow)
+    idx = [_bitrev(tx.sample_bits(logL), logL) for _ in range(nq)]
+    for t in gts:
+        t.prove_batch(idx, tx)
+    for r, t in enumerate(ftrees):
+        t.prove_batch([i >> (la * (r + 1)) for i in idx], tx)
+    gf, gc = tx.data()
+    otx = O.Transcript(O.CH_DUPLEX_P2 if kind == G.HASH_POSEIDON2 else O.CH_KECCAK)
+    O.open_with_channel(O.PcsParams(*params), okind, ots, logL, z, otx)
+    of, oc = otx.data()
+    assert np.array_equal(gf, of) and np.array_equal(gc, oc)
+    # and the library's own one-call opening writes the same words
+    t2 = G.ProverTranscript(ch)
+    G.open_with_channel(G.PcsParams(*params), cfg, logL, z, gts, t2)
+    f2, c2 = t2.data()
+    assert np.array_equal(f2, of) and np.array_equal(c2, oc)
