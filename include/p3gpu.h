@@ -233,9 +233,12 @@ int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t l
  * fri_fold: FriFold::fold_matrix (fri/fold/mod.rs:119-144) for one round of
  *   FriPolys::new (fri/prover.rs:90-223): the n bit-reversed evaluations at in
  *   (device) fold to n >> log_arity at out (device, not overlapping in), with
- *   s_k^-1 = w_n^-bitrev(k) for row k.  log_arity 1..3.  The round's commitment is
- *   p3g_lmcs_build (alignment 1, P3G_DEVICE_PTRS) of in as an
- *   (n >> log_arity) x (2 << log_arity) matrix.
+ *   s_k^-1 = w_n^-bitrev(k) for row k.  log_arity 1..3.
+ * fri_commit: the round's commitment ahead of its fold (fri/prover.rs:136-150):
+ *   the unaligned tree over the n device evaluations read as the
+ *   (n >> log_arity) x (2 << log_arity) base matrix (p3g_lmcs_build, alignment 1,
+ *   of that view).  The tree refers to evals, which must outlive it; root
+ *   (optional) receives its root.
  * fri_final_poly: the final polynomial (fri/prover.rs:208-221) from the first n
  *   evaluations (host, bit-reversed order) of the last domain to n coefficients
  *   in descending degree (host).  n a power of two up to 2^24. */
@@ -246,6 +249,8 @@ int p3g_deep_quotient(p3g_ctx* ctx, uint32_t log_blowup, const uint64_t* eval_po
                       const uint64_t alpha[2], const uint64_t beta[2], uint64_t* out);
 int p3g_fri_fold(p3g_ctx* ctx, const uint64_t* in, uint64_t n, uint32_t log_arity, const uint64_t beta[2],
                  uint64_t* out);
+int p3g_fri_commit(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* evals, uint64_t n, uint32_t log_arity,
+                   p3g_tree** out, uint64_t root[4]);
 int p3g_fri_final_poly(const uint64_t* evals, uint64_t n, uint64_t* coeffs);
 
 /* ---- caller-owned channel ----

@@ -749,6 +749,28 @@ int p3g_fri_fold(p3g_ctx* ctx, const uint64_t* in, uint64_t n, uint32_t log_arit
   });
 }
 
+int p3g_fri_commit(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* evals, uint64_t n, uint32_t log_arity,
+                   p3g_tree** out, uint64_t root[4]) {
+  if (!ctx || !evals || !out) return P3G_E_INVAL;
+  *out = nullptr;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    check_hash(hash_kind);
+    if (log_arity < 1 || log_arity > 3) throw ApiError(P3G_E_INVAL, "folding arity must be 2, 4 or 8");
+    if (n == 0 || (n & (n - 1))) throw ApiError(P3G_E_NOT_POW2, "domain size must be a power of two");
+    if ((n >> log_arity) == 0) throw ApiError(P3G_E_INVAL, "domain smaller than the folding arity");
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    DevMat m;
+    m.ptr = const_cast<uint64_t*>(evals);
+    m.height = n >> log_arity;
+    m.width = 2ull << log_arity;
+    m.owned = false;
+    Tree* t = lmcs_build(c, hash_kind, {m}, 1, c.stream);
+    if (root) memcpy(root, t->root, 32);
+    *out = new p3g_tree{t};
+  });
+}
+
 int p3g_fri_final_poly(const uint64_t* evals, uint64_t n, uint64_t* coeffs) {
   if (!evals || !coeffs) return P3G_E_INVAL;
   if (n == 0 || (n & (n - 1))) return P3G_E_NOT_POW2;

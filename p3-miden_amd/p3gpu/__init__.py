@@ -220,6 +220,7 @@ def _load():
                                         C.c_uint64, _u64p, _u64p, _u64p]),
         "p3g_fri_fold": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint32, _u64p, _u64p]),
         "p3g_fri_final_poly": (C.c_int, [_u64p, C.c_uint64, _u64p]),
+        "p3g_fri_commit": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint64, C.c_uint32, C.POINTER(_vp), _u64p]),
         "p3g_tree_prove_batch_cb": (C.c_int, [_vp, _u64p, C.c_uint64, C.POINTER(_Channel)]),
         "p3g_open_with_callbacks": (C.c_int, [_vp, C.POINTER(PcsParams), C.c_uint32, _u64p, C.c_uint32, C.POINTER(_vp), C.c_uint64, C.POINTER(_Channel)]),
         "p3g_commit_plan_create": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
@@ -795,6 +796,20 @@ def fri_fold(evals: "DeviceMatrix", log_arity: int, beta) -> "DeviceMatrix":
     _check(lib().p3g_fri_fold(evals.dev.h, C.cast(evals.ptr, _u64p), evals.height, log_arity, _p(b),
                               C.cast(out.ptr, _u64p)), "fri_fold")
     return out
+
+
+def fri_commit(lmcs: "LmcsConfig", evals: "DeviceMatrix", log_arity: int) -> "LiftedMerkleTree":
+    """A FRI round's commitment (p3-miden-lifted-fri/src/fri/prover.rs:136-150): the unaligned tree over
+    the n evaluations of an n x 2 DeviceMatrix read as the (n >> log_arity) x (2 << log_arity) base
+    matrix.  The tree keeps a reference to `evals`."""
+    if evals.width != 2:
+        raise ValueError("evaluations must be an n x 2 matrix of extension elements")
+    h = _vp()
+    _check(lib().p3g_fri_commit(evals.dev.h, lmcs.hash_kind, C.cast(evals.ptr, _u64p), evals.height, log_arity,
+                                C.byref(h), None), "fri_commit")
+    t = LiftedMerkleTree(h, lmcs.hash_kind, evals.dev)
+    t._evals = evals
+    return t
 
 
 def fri_final_poly(evals) -> np.ndarray:

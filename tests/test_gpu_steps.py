@@ -1,5 +1,5 @@
 """GPU parity of the step-level DEEP and FRI calls (batch_eval_lifted,
-deep_quotient, fri_fold, fri_final_poly) through the C ABI: each against the
+deep_quotient, fri_commit, fri_fold, fri_final_poly) through the C ABI: each against the
 oracle's own function on the same inputs, and the steps composed in the order
 of open_with_channel (p3-miden-lifted-fri/src/prover.rs:35-116) against the
 oracle's whole opening, transcript word for word.  Bit-exact.
@@ -118,6 +118,27 @@ def test_fri_fold_of_a_low_degree_polynomial(device):
     assert np.all(got[:, 0] == want) and np.all(got[:, 1] == 0)
 
 
+@pytest.mark.parametrize("kind", [G.HASH_POSEIDON2, G.HASH_KECCAK])
+@pytest.mark.parametrize("logn,la", [(1, 1), (7, 1), (10, 2), (9, 3)])
+def test_fri_commit_matches_oracle_tree(device, kind, logn, la):
+    # the round tree is the unaligned tree of the (n / a) x 2a base matrix
+    n, a = 1 << logn, 1 << la
+    ev = O.rand_matrix(950 + logn, n, 2)
+    cfg = G.LmcsConfig(device, kind)
+    t = G.fri_commit(cfg, device.upload(ev), la)
+    okind = O.HASH_P2 if kind == G.HASH_POSEIDON2 else O.HASH_KECCAK
+    ot = O.Tree(okind, [ev.reshape(n // a, 2 * a)])
+    assert np.array_equal(t.root(), ot.root())
+    assert t.height() == n // a and t.widths() == [2 * a]
+    assert np.array_equal(t.rows(n // a - 1), ot.rows(n // a - 1))
+    view = G.DeviceMatrix(device, t._evals.ptr, n // a, 2 * a, owned=False)
+    assert np.array_equal(cfg.build_tree([view]).root(), ot.root())
+    d = device.upload(ev)
+    for bad in (0, 4, logn + 1):
+        with pytest.raises(G.P3GError):
+            G.fri_commit(cfg, d, bad)
+
+
 def test_fri_fold_rejects_bad_shapes(device):
     d = device.upload(np.zeros((8, 2), np.uint64))
     for la in (0, 4):
@@ -176,8 +197,7 @@ def test_composed_steps_reproduce_open_with_channel(device, kind, params):
     fdeg = O.PcsParams(*params).final_degree(logL)
     keep, ftrees = [], []
     while cur.height > fdeg << lb:
-        view = G.DeviceMatrix(device, cur.ptr, cur.height >> la, 2 << la, owned=False)
-        t = cfg.build_tree([view])
+        t = G.fri_commit(cfg, cur, la)
         tx.send_commitment(t.root())
         tx.grind(fold_pow)
         b = tx.sample_algebra_element()
