@@ -207,3 +207,27 @@ def test_fri_final_poly_rejects_bad_input():
         G.fri_final_poly(np.zeros((3, 2), np.uint64))
     with pytest.raises(G.P3GError):
         G.fri_final_poly(np.array([[P, 0], [0, 0]], np.uint64))
+
+
+def test_binding_fails_loudly_without_the_library(monkeypatch):
+    # no CPU fallback: every entry point needs libp3gpu.so
+    monkeypatch.setattr(G, "_lib", None)
+    with pytest.raises(ImportError):
+        G.fill_random(1, 4)
+    with pytest.raises(ImportError):
+        G.fri_final_poly(np.zeros((2, 2), np.uint64))
+
+
+def test_product_code_never_references_the_oracle():
+    # the oracle is test infrastructure: nothing under the package may import, link or load it
+    pkg = os.path.join(ROOT, "p3-miden_amd")
+    hits = []
+    for d, _, files in os.walk(pkg):
+        if "_build" in d:
+            continue
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")) or f == "Makefile":
+                text = open(os.path.join(d, f), errors="replace").read()
+                if re.search(r"\boracle\b|liboracle|orc_", text):
+                    hits.append(f)
+    assert not hits, hits
