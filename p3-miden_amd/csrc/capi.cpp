@@ -689,6 +689,7 @@ std::vector<ef2> ext_points(const uint64_t* v, uint32_t n) {
   return z;
 }
 std::vector<Tree*> tree_list(p3g_tree* const* trees, uint64_t ntrees) {
+  if (ntrees == 0) throw ApiError(P3G_E_EMPTY, "at least one trace tree required");
   std::vector<Tree*> tv;
   for (uint64_t i = 0; i < ntrees; i++) {
     if (!trees[i]) throw ApiError(P3G_E_INVAL, "null tree");
@@ -697,6 +698,7 @@ std::vector<Tree*> tree_list(p3g_tree* const* trees, uint64_t ntrees) {
   return tv;
 }
 uint64_t eval_words(const std::vector<Tree*>& tv, uint32_t n_points) {
+  if (n_points < 1 || n_points > 4) throw ApiError(P3G_E_INVAL, "1 to 4 evaluation points supported");
   uint64_t cols = 0;
   for (auto* t : tv)
     for (auto& m : t->mats) cols += m.width;

@@ -222,3 +222,16 @@ def test_composed_steps_reproduce_open_with_channel(device, kind, params):
     G.open_with_channel(G.PcsParams(*params), cfg, logL, z, gts, t2)
     f2, c2 = t2.data()
     assert np.array_equal(f2, of) and np.array_equal(c2, oc)
+
+
+def test_deep_size_query_validates_like_the_call(device):
+    # asking for the evaluation count checks the trees and the point count as the call does
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    t = G.commit_traces(cfg, [O.rand_matrix(3, 16, 5)], 1)
+    n = np.zeros(1, np.uint64)
+    z = np.array([1, 2] * 5, np.uint64)
+    hs = (G.C.c_void_p * 1)(t.h)
+    assert G.lib().p3g_deep_batch_eval(device.h, 1, G._p(z), 5, hs, 1, None, 0, G._p(n)) == 1  # INVAL
+    assert G.lib().p3g_deep_batch_eval(device.h, 1, G._p(z), 2, hs, 0, None, 0, G._p(n)) == 6  # EMPTY
+    assert G.lib().p3g_deep_batch_eval(device.h, 1, G._p(z), 2, hs, 1, None, 0, G._p(n)) == 0
+    assert int(n[0]) == 5 * 2 * 2
