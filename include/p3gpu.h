@@ -70,6 +70,11 @@ const char* p3g_last_error(void); /* thread-local detail of the last failure */
 
 /* ---- context (one per device; all work runs on its stream) ---- */
 int p3g_ctx_create(int device, p3g_ctx** out);
+/* the same on a stream the caller created (a hipStream_t of that device from the HIP
+ * runtime this library links, not the null stream; no reference counterpart): the
+ * library's work is ordered with the caller's own work on it, and p3g_ctx_destroy
+ * synchronizes the stream but leaves it alive */
+int p3g_ctx_create_on_stream(int device, void* stream, p3g_ctx** out);
 /* frees all device memory of the context: free its trees and plans first (they refer to it) */
 void p3g_ctx_destroy(p3g_ctx* ctx);
 int p3g_ctx_synchronize(p3g_ctx* ctx);

@@ -55,7 +55,7 @@ const char* p3g_status_string(int status) {
 
 const char* p3g_last_error(void) { return g_last_error.c_str(); }
 
-int p3g_ctx_create(int device, p3g_ctx** out) {
+static int ctx_create(int device, hipStream_t external, p3g_ctx** out) {
   if (!out) return P3G_E_INVAL;
   *out = nullptr;
   p3g_ctx* c = new p3g_ctx();
@@ -65,7 +65,15 @@ int p3g_ctx_create(int device, p3g_ctx** out) {
     if (gl_pow(2, OMEGA16_LOG2) != gl_two_adic_generator(4))
       throw ApiError(P3G_E_INVAL, "omega_16 is not 2^OMEGA16_LOG2");
     P3G_CHECK_HIP(hipSetDevice(device));
-    P3G_CHECK_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+    if (external) {
+      int sdev = -1;
+      P3G_CHECK_HIP(hipStreamGetDevice(external, &sdev));
+      if (sdev != device) throw ApiError(P3G_E_INVAL, "stream belongs to another device");
+      c->c.stream = external;
+      c->c.owns_stream = false;
+    } else {
+      P3G_CHECK_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+    }
     p2_generate_constants(2025, &c->c.host_p2);
     P3G_CHECK_HIP(hipMalloc(&c->c.dev_p2, sizeof(P2Consts)));
     P3G_CHECK_HIP(hipMemcpy(c->c.dev_p2, &c->c.host_p2, sizeof(P2Consts), hipMemcpyHostToDevice));
@@ -76,6 +84,13 @@ int p3g_ctx_create(int device, p3g_ctx** out) {
   }
   *out = c;
   return P3G_OK;
+}
+
+int p3g_ctx_create(int device, p3g_ctx** out) { return ctx_create(device, nullptr, out); }
+
+int p3g_ctx_create_on_stream(int device, void* stream, p3g_ctx** out) {
+  if (!stream) return P3G_E_INVAL;  // the null stream would mark released blocks as idle
+  return ctx_create(device, static_cast<hipStream_t>(stream), out);
 }
 
 void p3g_ctx_destroy(p3g_ctx* ctx) {
@@ -96,7 +111,7 @@ void p3g_ctx_destroy(p3g_ctx* ctx) {
   if (ctx->c.dev_p2) (void)hipFree(ctx->c.dev_p2);
   if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
   ctx->c.free_upload_ring();
-  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  if (ctx->c.stream && ctx->c.owns_stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }
 

@@ -107,6 +107,7 @@ struct SharedPool {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool owns_stream = true;  // false for a stream the caller created and keeps
   P2Consts host_p2{};
   P2Consts* dev_p2 = nullptr;
   std::mutex mu;

@@ -157,6 +157,7 @@ def _load():
         "p3g_status_string": (C.c_char_p, [C.c_int]),
         "p3g_last_error": (C.c_char_p, []),
         "p3g_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+        "p3g_ctx_create_on_stream": (C.c_int, [C.c_int, _vp, C.POINTER(_vp)]),
         "p3g_ctx_destroy": (None, [_vp]),
         "p3g_ctx_synchronize": (C.c_int, [_vp]),
         "p3g_ctx_trim": (C.c_int, [_vp]),
@@ -286,9 +287,15 @@ def fill_random(seed: int, n: int) -> np.ndarray:
 class Device:
     """One context per GPU: stream, constants, cached tables and device memory."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, stream: int = None):
+        """stream: optional hipStream_t handle that the context runs on instead of its own.  It must
+        come from the HIP runtime this library links (libamdhip64.so.7 of the ROCm install; a runtime
+        bundled elsewhere, such as inside a Python wheel, has its own handles) and outlive the Device."""
         h = _vp()
-        _check(lib().p3g_ctx_create(device, C.byref(h)), "ctx_create")
+        if stream is None:
+            _check(lib().p3g_ctx_create(device, C.byref(h)), "ctx_create")
+        else:
+            _check(lib().p3g_ctx_create_on_stream(device, _vp(stream), C.byref(h)), "ctx_create_on_stream")
         self.h = h
         self.index = device
         # trees, plans and matrices living in this context: freed before the context is
