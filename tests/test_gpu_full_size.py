@@ -6,6 +6,8 @@ oracle to redo the whole computation:
     fold at all 100 queries) and rejects it after one word is changed;
   - a shift-1 LDE of a 2^22-row trace holds the trace itself in its first
     2^22 (bit-reversed) rows: the iNTT -> scale -> NTT round trip at full size;
+  - the step-level DEEP and FRI calls, composed in the opening's order, write
+    the same 2^22 x 64 proof as the one-call opening;
   - the LDE is linear (LDE(a) + LDE(b) == LDE(a + b) mod p);
   - the batched commit plan of the 12-trace mixed-height workload gives the
     same root, layers and opened rows as the per-trace path, and its proof
@@ -42,6 +44,27 @@ def test_c2_proof_accepted_by_oracle_verifier(device):
     bad = f.copy()
     bad[len(bad) // 2] = (int(bad[len(bad) // 2]) + 1) % P
     assert O.pcs_trace_verify(op, O.HASH_P2, root, [w], logL, bad, c) != 0
+
+
+def test_c2_composed_steps_equal_one_call_opening(device):
+    # at the full 2^22 x 64 shape: the step-level calls, composed in the opening's order, write
+    # the same transcript as the one-call opening (whose proofs the oracle verifier accepts)
+    from compose import compose_open
+    n, w = 1 << W.C2_LOG_ROWS, W.C2_WIDTH
+    params = W.C2_PARAMS
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    tr = device.alloc_matrix(n, w)
+    tr.fill_random(2026)
+    t = G.commit_traces(cfg, [tr], params[0])
+    tx = G.ProverTranscript()
+    tx.observe_commitment(t.root())
+    z = [tx.sample_algebra_element(), tx.sample_algebra_element()]
+    one_call = tx.clone()
+    compose_open(cfg, params, [t], [w], z, tx)
+    G.open_with_channel(G.PcsParams(*params), cfg, int(t.height()).bit_length() - 1, z, [t], one_call)
+    f1, c1 = tx.data()
+    f2, c2 = one_call.data()
+    assert f1.size > 0 and np.array_equal(f1, f2) and np.array_equal(c1, c2)
 
 
 def test_lde_shift_one_contains_trace_full_size(device):

@@ -9,30 +9,11 @@ import pytest
 
 import oracle as O
 import p3gpu as G
+from compose import aligned_per_point as _aligned_per_point, bitrev as _bitrev, compose_open
 
 pytestmark = pytest.mark.gpu
 
 P = 0xFFFFFFFF00000001
-
-
-def _bitrev(i, bits):
-    return int(format(i, f"0{bits}b")[::-1], 2) if bits else 0
-
-
-def _aligned_per_point(ev, widths, alignment):
-    """evals[column][point][2] -> per point the flat aligned slice the prover sends."""
-    npts = ev.shape[1]
-    out = []
-    for j in range(npts):
-        flat, off = [], 0
-        for w in widths:
-            flat.append(ev[off:off + w, j, :].reshape(-1))
-            pad = O.aligned_len(w, alignment) - w
-            if pad:
-                flat.append(np.zeros(2 * pad, np.uint64))
-            off += w
-        out.append(np.concatenate(flat) if flat else np.zeros(0, np.uint64))
-    return out
 
 
 GROUPS = [
@@ -179,39 +160,13 @@ def test_deep_steps_reject_bad_inputs(device):
 ])
 def test_composed_steps_reproduce_open_with_channel(device, kind, params):
     groups = [[(32, 4), (64, 7)], [(64, 2)]]
-    lb, la, _, fold_pow, deep_pow, nq, query_pow = params
+    lb, la = params[0], params[1]
     cfg, okind, gts, ots, widths = _commit(device, kind, groups, lb, 500 + la)
     logL = int(gts[0].height()).bit_length() - 1
     z = [(31, 32), (33, 34)]
     ch = G.CHALLENGER_DUPLEX_P2 if kind == G.HASH_POSEIDON2 else G.CHALLENGER_KECCAK
     tx = G.ProverTranscript(ch)
-    # DEEP
-    ev = G.batch_eval_lifted(gts, z, lb)
-    for flat in _aligned_per_point(ev, widths, gts[0].alignment()):
-        tx.send_field_slice(flat)
-    tx.grind(deep_pow)
-    alpha = tx.sample_algebra_element()
-    beta = tx.sample_algebra_element()
-    cur = G.deep_quotient(gts, z, lb, ev, alpha, beta)
-    # FRI commit phase: commit the round as (n / a) x 2a, fold, until the final domain
-    fdeg = O.PcsParams(*params).final_degree(logL)
-    keep, ftrees = [], []
-    while cur.height > fdeg << lb:
-        t = G.fri_commit(cfg, cur, la)
-        tx.send_commitment(t.root())
-        tx.grind(fold_pow)
-        b = tx.sample_algebra_element()
-        keep.append(cur)
-        ftrees.append(t)
-        cur = G.fri_fold(cur, la, b)
-    tx.send_field_slice(G.fri_final_poly(cur.download()[:fdeg]).reshape(-1))
-    # query phase
-    tx.grind(query_pow)
-    idx = [_bitrev(tx.sample_bits(logL), logL) for _ in range(nq)]
-    for t in gts:
-        t.prove_batch(idx, tx)
-    for r, t in enumerate(ftrees):
-        t.prove_batch([i >> (la * (r + 1)) for i in idx], tx)
+    compose_open(cfg, params, gts, widths, z, tx)
     gf, gc = tx.data()
     otx = O.Transcript(O.CH_DUPLEX_P2 if kind == G.HASH_POSEIDON2 else O.CH_KECCAK)
     O.open_with_channel(O.PcsParams(*params), okind, ots, logL, z, otx)
