@@ -86,6 +86,25 @@ def timed(fn, steps, dev):
     return (time.perf_counter() - t0) * 1e3 / steps, out
 
 
+# ------------------------------------------------------------------ measured peaks
+def bench_calibrate(dev):
+    """Sustained VALU issue rates per instruction class and device copy bandwidth, measured on this
+    GPU: the ceilings the kernels' roofline fractions are quoted against beside the published ones."""
+    valu = [dev.calibrate_valu(k) for k in range(5)]
+    copy = dev.calibrate_copy(4 << 30, 5)
+    add = valu[0]
+    return {
+        "valu": {v["kind"]: {"ms": round(v["ms"], 3), "wave_insts_per_s": round(v["wave_insts_per_s"] / 1e12, 4),
+                             "lane_ops_tops": round(v["lane_ops_per_s"] / 1e12, 2)} for v in valu},
+        "valu_issue_peak_tops": round(add["lane_ops_per_s"] / 1e12, 2),
+        "copy_kernel_gbs": round(copy["kernel_gbs"], 1),
+        "copy_memcpy_gbs": round(copy["memcpy_gbs"], 1),
+        "note": "valu: 8192 x 256 lanes, independent chains of one instruction (inline asm); wave_insts_per_s in "
+                "10^12 wave64 instructions/s, lane_ops = 64 x that.  copy: 4 GiB device-to-device, read + write "
+                "bytes per second",
+    }
+
+
 # ------------------------------------------------------------------ headline: plain single-trace proof
 def bench_c2(G, W, dev, steps, warmup):
     n, w, b = 1 << W.C2_LOG_ROWS, W.C2_WIDTH, W.C2_PARAMS[0]
@@ -505,8 +524,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--sections", default="c2,c3,c4,c5,small,quotient",
-                    help="comma list of c2 (headline), c3, c4, c5, small, quotient")
+    ap.add_argument("--sections", default="calibrate,c2,c3,c4,c5,small,quotient",
+                    help="comma list of calibrate (measured peaks), c2 (headline), c3, c4, c5, small, quotient")
     ap.add_argument("--small-proofs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=4, help="server workers for the C5 mix")
@@ -533,6 +552,7 @@ def main():
         b.copy_from(a)
         a.free()
         b.free()
+    peaks = bench_calibrate(dev) if "calibrate" in sections else None
     # the headline section always runs in the default configuration; leaving it
     # out of --sections is for profiling one other workload on its own
     c2 = bench_c2(G, W, dev, args.steps, args.warmup) if "c2" in sections else None
@@ -579,6 +599,7 @@ def main():
         "small_proofs": small,
         "quotient_commit": quot,
         "cpu_baseline": cpu,
+        "measured_peaks": peaks,
     }
     print(json.dumps(line), flush=True)
     dev.close()

@@ -92,6 +92,21 @@ int p3g_profile_enable(p3g_ctx* ctx, int on);
 int p3g_profile_reset(p3g_ctx* ctx);
 int p3g_profile_read(p3g_ctx* ctx, const char* name, double* total_ms, uint64_t* launches);
 
+/* ---- peak calibration (instrumentation; no reference counterpart) ----
+ * The measured ceilings the roofline figures are quoted against.
+ * calibrate_valu: one launch of blocks x 256 threads in which every lane runs
+ *   independent chains of one VALU instruction (kind 0: v_add_u32, 1: v_mad_u64_u32,
+ *   2: v_fma_f64, 3: v_mul_hi_u32, 4: a 64-bit add on a carry pair, v_add_co_u32 +
+ *   v_addc_co_u32) for iters iterations (a positive multiple of 4); *ms is the launch
+ *   time (HIP events on the context's stream), *insts_per_wave the instructions of
+ *   that class each wave issues.
+ * calibrate_copy: a device-to-device copy of bytes (a multiple of 16) with a
+ *   16-bytes-per-lane copy kernel and with hipMemcpyAsync, reps times each after a
+ *   warm-up; each *ms is the average time of one copy. */
+int p3g_calibrate_valu(p3g_ctx* ctx, uint32_t kind, uint32_t iters, uint32_t blocks, double* ms,
+                       uint64_t* insts_per_wave);
+int p3g_calibrate_copy(p3g_ctx* ctx, uint64_t bytes, uint32_t reps, double* kernel_ms, double* memcpy_ms);
+
 /* ---- device memory (resident inputs, benchmarks, serving) ---- */
 int p3g_alloc(p3g_ctx* ctx, uint64_t bytes, void** dptr);
 int p3g_free(p3g_ctx* ctx, void* dptr);

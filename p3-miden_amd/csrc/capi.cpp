@@ -12,6 +12,9 @@ using namespace p3g;
 
 namespace p3g {
 void fill_random_device(Ctx& ctx, uint64_t seed, uint64_t n, uint64_t* out, hipStream_t s);
+uint64_t calib_valu_insts_per_wave(unsigned kind, uint32_t iters);
+double calib_valu(Ctx& ctx, unsigned kind, uint32_t iters, uint32_t blocks);
+void calib_copy(Ctx& ctx, uint64_t bytes, uint32_t reps, double* kernel_ms, double* memcpy_ms);
 }
 
 static thread_local std::string g_last_error;
@@ -79,6 +82,9 @@ static int ctx_create(int device, hipStream_t external, p3g_ctx** out) {
     P3G_CHECK_HIP(hipMemcpy(c->c.dev_p2, &c->c.host_p2, sizeof(P2Consts), hipMemcpyHostToDevice));
   });
   if (rc != P3G_OK) {
+    // undo what was set up before the failure: the constants' block and a stream this call created
+    if (c->c.dev_p2) (void)hipFree(c->c.dev_p2);
+    if (c->c.stream && c->c.owns_stream) (void)hipStreamDestroy(c->c.stream);
     delete c;
     return rc;
   }
@@ -175,6 +181,24 @@ int p3g_profile_read(p3g_ctx* ctx, const char* name, double* total_ms, uint64_t*
       }
     *total_ms = tot;
     *launches = n;
+  });
+}
+
+int p3g_calibrate_valu(p3g_ctx* ctx, uint32_t kind, uint32_t iters, uint32_t blocks, double* ms,
+                       uint64_t* insts_per_wave) {
+  if (!ctx || !ms || !insts_per_wave) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
+    *ms = calib_valu(ctx->c, kind, iters, blocks);
+    *insts_per_wave = calib_valu_insts_per_wave(kind, iters);
+  });
+}
+
+int p3g_calibrate_copy(p3g_ctx* ctx, uint64_t bytes, uint32_t reps, double* kernel_ms, double* memcpy_ms) {
+  if (!ctx || !kernel_ms || !memcpy_ms) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
+    calib_copy(ctx->c, bytes, reps, kernel_ms, memcpy_ms);
   });
 }
 

@@ -165,6 +165,9 @@ def _load():
         "p3g_profile_enable": (C.c_int, [_vp, C.c_int]),
         "p3g_profile_reset": (C.c_int, [_vp]),
         "p3g_profile_read": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), _u64p]),
+        "p3g_calibrate_valu": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double), _u64p]),
+        "p3g_calibrate_copy": (C.c_int, [_vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
         "p3g_alloc": (C.c_int, [_vp, C.c_uint64, C.POINTER(_vp)]),
         "p3g_free": (C.c_int, [_vp, _vp]),
         "p3g_upload": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
@@ -346,6 +349,28 @@ class Device:
         n = np.zeros(1, np.uint64)
         _check(lib().p3g_profile_read(self.h, name.encode(), C.byref(ms), _p(n)), "profile_read")
         return float(ms.value), int(n[0])
+
+    VALU_KINDS = ("v_add_u32", "v_mad_u64_u32", "v_fma_f64", "v_mul_hi_u32", "add64_carry_pair")
+
+    def calibrate_valu(self, kind: int, iters: int = 16384, blocks: int = 8192) -> dict:
+        """Sustained issue rate of one VALU instruction class (independent chains on every lane):
+        launch time, instructions per wave, and wave64 instructions and lane-ops per second."""
+        ms = C.c_double(0)
+        n = np.zeros(1, np.uint64)
+        _check(lib().p3g_calibrate_valu(self.h, kind, iters, blocks, C.byref(ms), _p(n)), "calibrate_valu")
+        waves = blocks * 256 // 64
+        insts = int(n[0]) * waves
+        return {"kind": self.VALU_KINDS[kind], "ms": float(ms.value), "insts_per_wave": int(n[0]), "waves": waves,
+                "wave_insts_per_s": insts / (ms.value * 1e-3), "lane_ops_per_s": 64 * insts / (ms.value * 1e-3)}
+
+    def calibrate_copy(self, nbytes: int = 4 << 30, reps: int = 5) -> dict:
+        """Device-to-device copy bandwidth (read + write bytes per second) of a copy kernel and of
+        hipMemcpyAsync over nbytes."""
+        k = C.c_double(0)
+        m = C.c_double(0)
+        _check(lib().p3g_calibrate_copy(self.h, nbytes, reps, C.byref(k), C.byref(m)), "calibrate_copy")
+        return {"bytes": nbytes, "kernel_ms": float(k.value), "memcpy_ms": float(m.value),
+                "kernel_gbs": 2 * nbytes / (k.value * 1e-3) / 1e9, "memcpy_gbs": 2 * nbytes / (m.value * 1e-3) / 1e9}
 
     def alloc_matrix(self, height: int, width: int) -> "DeviceMatrix":
         """Uninitialised height x width matrix in HBM."""
