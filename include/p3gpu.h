@@ -124,6 +124,20 @@ int p3g_fill_random(uint64_t seed, uint64_t n, uint64_t* host_out);
 int p3g_coset_lde_batch(p3g_ctx* ctx, const uint64_t* trace, uint64_t n, uint64_t w, uint32_t added_bits,
                         uint64_t shift, uint32_t flags, uint64_t* out);
 
+/* coset_lde_batch + bit_reverse_rows on device buffers, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same values as
+ * p3g_coset_lde_batch with P3G_DEVICE_PTRS | P3G_BITREV_ROWS: d_trace (n x w) and d_out
+ * ((n << added_bits) x w) are device addresses.  The twiddle and coset tables live in one
+ * static 768 KiB array per GPU, held until the process ends; every call on a GPU shares it,
+ * so calls that overlap on one GPU race.  The work is enqueued on the context's stream and
+ * the call returns without waiting (p3g_ctx_synchronize waits).  Shapes whose tables need
+ * more than the array -- 2 (2048 + 2^(log n - 12)) + 2^added_bits (2048 + 2^(log n - 11))
+ * words > 98,304 -- or added_bits > 3 give P3G_E_RANGE; every trace up to 2^24 rows with
+ * added_bits <= 3 fits. */
+int p3g_coset_lde_batch_noalloc(p3g_ctx* ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, uint32_t added_bits,
+                                uint64_t shift, uint64_t* d_out);
+
 /* ---- Lmcs ----
  * LmcsConfig::build_tree (alignment = 1) / build_aligned_tree (alignment = p3g_lmcs_alignment)
  * (p3-miden-lmcs/src/lmcs.rs:102-130, lifted_tree.rs:244-310).  Matrices sorted by

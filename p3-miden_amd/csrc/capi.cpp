@@ -77,6 +77,7 @@ static int ctx_create(int device, hipStream_t external, p3g_ctx** out) {
     } else {
       P3G_CHECK_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
     }
+    lde_scratch_init(device);  // the no-alloc LDE's table array on this device
     p2_generate_constants(2025, &c->c.host_p2);
     P3G_CHECK_HIP(hipMalloc(&c->c.dev_p2, sizeof(P2Consts)));
     P3G_CHECK_HIP(hipMemcpy(c->c.dev_p2, &c->c.host_p2, sizeof(P2Consts), hipMemcpyHostToDevice));
@@ -283,6 +284,17 @@ int p3g_coset_lde_batch(p3g_ctx* ctx, const uint64_t* trace, uint64_t n, uint64_
       c.release((void*)dtrace);
       c.release(dout);
     }
+  });
+}
+
+int p3g_coset_lde_batch_noalloc(p3g_ctx* ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, uint32_t added_bits,
+                                uint64_t shift, uint64_t* d_out) {
+  if (!ctx || !d_trace || !d_out) return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    if (shift >= P) throw ApiError(P3G_E_INVAL, "shift is not a canonical field element");
+    lde_coset_batch_noalloc(c, d_trace, n, w, added_bits, shift, d_out, c.stream);
   });
 }
 

@@ -288,6 +288,14 @@ struct ColTwist {
 void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
                      uint64_t shift, uint64_t* d_out, hipStream_t s, const ColTwist* twist = nullptr);
 void bit_reverse_rows(Ctx& ctx, const uint64_t* src, uint64_t* dst, uint64_t h, uint64_t w, hipStream_t s);
+// The same LDE (bit-reversed rows out) with its tables in a static per-GPU
+// scratch array: no allocation, nothing waited for.  lde_scratch_init fetches
+// the array's device address once per device (that device current).
+void lde_scratch_init(int device);
+uint64_t lde_scratch_words(unsigned logn, unsigned log_blowup);
+uint64_t lde_scratch_capacity();
+void lde_coset_batch_noalloc(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
+                             uint64_t shift, uint64_t* d_out, hipStream_t s);
 // one transform of a batch of same-shaped coset LDEs (bit-reversed rows out)
 struct LdeJob {
   const uint64_t* src;

@@ -481,35 +481,35 @@ __global__ void coef_twist_kernel(uint64_t* __restrict__ coef, uint64_t n, uint3
   if (e) coef[i] = gl_mul(coef[i], pw(tab, e));
 }
 
+// Twiddle and coset-scale tables of one LDE shape, wherever they live.
+struct LdeTables {
+  const uint64_t* tw_inv;  // power table of omega_n^-1
+  const uint64_t* tw_fwd;  // power table of omega_n
+  const uint64_t* scale;   // B power tables of c_t = shift * omega_nB^t with 1/n folded into lo
+};
+
+static void lde_check_shape(uint64_t n, unsigned log_blowup) {
+  if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
+  if (log2_exact(n) + log_blowup > 32) throw ApiError(E_RANGE, "LDE height exceeds the field two-adicity");
+}
+
+static void lde_coset_launches(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
+                               uint64_t* d_out, hipStream_t s, const ColTwist* twist, const LdeTables& tabs);
+
 void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
                      uint64_t shift, uint64_t* d_out, hipStream_t s, const ColTwist* twist) {
-  if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
+  lde_check_shape(n, log_blowup);
   if (w == 0) return;
   const unsigned logn = log2_exact(n);
-  if (logn + log_blowup > 32) throw ApiError(E_RANGE, "LDE height exceeds the field two-adicity");
-  uint32_t logC = 0;
-  while ((2u << logC) <= NTT_C && (2ull << logC) <= w) logC++;
-  if (logC == 3) {
-    // widths 8..15: register tiles of 16 columns, those past the width masked,
-    // beat the per-stage LDS kernel on 8-column tiles (a 2^22 x 8 -> 2^24 LDE
-    // and its tree: 18.0 -> 17.6 ms); at widths 4..7 three quarters of the lanes
-    // would idle and the LDS kernel stays faster.  Every pass must be 5..8 stages.
-    bool reg = true;
-    for (uint32_t k : ntt_pass_split(logn, NTT_K)) reg = reg && ntt_reg_ok(k, 4);
-    if (reg) logC = 4;
-  }
-  const uint32_t C = 1u << logC;
-  const uint32_t ncol = (uint32_t)((w + C - 1) / C);
   const uint32_t B = 1u << log_blowup;
   const uint32_t twbits = logn ? logn - 1 : 0;
-
   const uint64_t* tw_inv = ctx.pow_table(gl_inv(gl_two_adic_generator(logn)), twbits);
   const uint64_t* tw_fwd = ctx.pow_table(gl_two_adic_generator(logn), twbits);
   // scale tables for the B cosets, stacked in one allocation
   const uint64_t ninv = gl_inv_pow2(logn);
   const uint64_t stride = POW_LO + pow_table_hi_len(logn);
   uint64_t* scale = (uint64_t*)ctx.alloc(stride * B * sizeof(uint64_t));
-  {
+  try {
     std::vector<uint64_t> host(stride * B);
     const uint64_t om = gl_two_adic_generator(logn + log_blowup);
     uint64_t ct = shift;
@@ -528,7 +528,35 @@ void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, 
       ct = gl_mul(ct, om);
     }
     ctx.upload_async(scale, host.data(), host.size() * 8, s);
+    lde_coset_launches(ctx, d_trace, n, w, log_blowup, d_out, s, twist, LdeTables{tw_inv, tw_fwd, scale});
+  } catch (...) {
+    ctx.release(scale);
+    throw;
   }
+  ctx.release(scale);
+}
+
+static void lde_coset_launches(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
+                               uint64_t* d_out, hipStream_t s, const ColTwist* twist, const LdeTables& tabs) {
+  const unsigned logn = log2_exact(n);
+  uint32_t logC = 0;
+  while ((2u << logC) <= NTT_C && (2ull << logC) <= w) logC++;
+  if (logC == 3) {
+    // widths 8..15: register tiles of 16 columns, those past the width masked,
+    // beat the per-stage LDS kernel on 8-column tiles (a 2^22 x 8 -> 2^24 LDE
+    // and its tree: 18.0 -> 17.6 ms); at widths 4..7 three quarters of the lanes
+    // would idle and the LDS kernel stays faster.  Every pass must be 5..8 stages.
+    bool reg = true;
+    for (uint32_t k : ntt_pass_split(logn, NTT_K)) reg = reg && ntt_reg_ok(k, 4);
+    if (reg) logC = 4;
+  }
+  const uint32_t C = 1u << logC;
+  const uint32_t ncol = (uint32_t)((w + C - 1) / C);
+  const uint32_t B = 1u << log_blowup;
+  const uint64_t* tw_inv = tabs.tw_inv;
+  const uint64_t* tw_fwd = tabs.tw_fwd;
+  const uint64_t* scale = tabs.scale;
+  const uint64_t stride = POW_LO + pow_table_hi_len(logn);
 
   auto launch = [&](uint32_t mode, uint32_t s0, uint32_t K, uint64_t nblk) {
     NttArgs a{};
@@ -585,7 +613,106 @@ void lde_coset_batch(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, 
       s0 += split[i];
     }
   }
-  ctx.release(scale);
+}
+
+// ------------------------------------------------------------------ the LDE without allocation
+// One array per GPU for the twiddle and coset-scale tables of the no-alloc
+// entry point, so that it makes no allocation of its own.  In host code the
+// name `scratch` is only a stand-in: its device address is fetched once per
+// device (lde_scratch_init, called when a context is created with that device
+// current) and only that address is handed to kernels.  Every call on one GPU
+// shares the array, so calls that overlap on one GPU race.
+constexpr uint64_t LDE_SCRATCH_WORDS = 96 * 1024;  // 768 KiB
+static __device__ __align__(256) uint64_t scratch[LDE_SCRATCH_WORDS];
+
+static std::mutex g_scratch_mu;
+static std::map<int, uint64_t*> g_scratch_ptr;
+
+void lde_scratch_init(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_ptr.count(device)) return;
+  void* ptr = nullptr;
+  P3G_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<uint64_t*>(ptr);
+}
+
+static uint64_t* lde_scratch(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it == g_scratch_ptr.end()) throw ApiError(E_INVAL, "no scratch address for this device (context not set up)");
+  return it->second;
+}
+
+// Words of table space an LDE of 2^logn rows and blowup 2^log_blowup needs.
+uint64_t lde_scratch_words(unsigned logn, unsigned log_blowup) {
+  const unsigned twbits = logn ? logn - 1 : 0;
+  return 2 * (POW_LO + pow_table_hi_len(twbits)) + (uint64_t(1) << log_blowup) * (POW_LO + pow_table_hi_len(logn));
+}
+
+uint64_t lde_scratch_capacity() { return LDE_SCRATCH_WORDS; }
+
+// Up to 8 power tables of `len` words at dst + t * stride, in the layout of
+// Ctx::pow_table: [lo_mul * base^i, i < POW_LO][base^(POW_LO j)].  Each entry
+// is one exponentiation, so the tables come out of one launch with no host data.
+struct PowFill {
+  uint64_t base[8];
+  uint64_t lo_mul;
+  uint64_t* dst;
+  uint64_t stride, len;
+};
+
+__global__ void pow_fill_kernel(PowFill f) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.y;
+  if (i >= f.len) return;
+  const uint64_t b = f.base[t];
+  f.dst[t * f.stride + i] = i < POW_LO ? gl_mul(f.lo_mul, gl_pow(b, i)) : gl_pow(b, uint64_t(POW_LO) * (i - POW_LO));
+}
+
+static void pow_fill(const PowFill& f, uint32_t ntab, hipStream_t s) {
+  hipLaunchKernelGGL(pow_fill_kernel, dim3((uint32_t)((f.len + 255) / 256), ntab), dim3(256), 0, s, f);
+  P3G_CHECK_HIP(hipGetLastError());
+}
+
+void lde_coset_batch_noalloc(Ctx& ctx, const uint64_t* d_trace, uint64_t n, uint64_t w, unsigned log_blowup,
+                             uint64_t shift, uint64_t* d_out, hipStream_t s) {
+  lde_check_shape(n, log_blowup);
+  const unsigned logn = log2_exact(n);
+  const uint64_t need = lde_scratch_words(logn, log_blowup);
+  if (log_blowup > 3 || need > LDE_SCRATCH_WORDS)
+    throw ApiError(E_RANGE, "coset_lde_batch_noalloc: this shape needs " + std::to_string(need) +
+                                " table words (added_bits <= 3); the scratch array holds " +
+                                std::to_string(LDE_SCRATCH_WORDS));
+  if (w == 0) return;
+  uint64_t* base = lde_scratch(ctx.device);
+  const unsigned twbits = logn ? logn - 1 : 0;
+  const uint64_t twlen = POW_LO + pow_table_hi_len(twbits);
+  const uint64_t stride = POW_LO + pow_table_hi_len(logn);
+  const uint32_t B = 1u << log_blowup;
+  uint64_t* tw_inv = base;
+  uint64_t* tw_fwd = base + twlen;
+  uint64_t* scale = base + 2 * twlen;
+  PowFill f{};
+  f.lo_mul = 1;
+  f.dst = tw_inv;
+  f.stride = twlen;  // tw_fwd follows tw_inv
+  f.len = twlen;
+  f.base[0] = gl_inv(gl_two_adic_generator(logn));
+  f.base[1] = gl_two_adic_generator(logn);
+  pow_fill(f, 2, s);
+  PowFill g{};
+  g.lo_mul = gl_inv_pow2(logn);
+  g.dst = scale;
+  g.stride = stride;
+  g.len = stride;
+  const uint64_t om = gl_two_adic_generator(logn + log_blowup);
+  uint64_t ct = shift;
+  for (uint32_t t = 0; t < B; t++) {
+    g.base[t] = ct;
+    ct = gl_mul(ct, om);
+  }
+  pow_fill(g, B, s);
+  lde_coset_launches(ctx, d_trace, n, w, log_blowup, d_out, s, nullptr, LdeTables{tw_inv, tw_fwd, scale});
 }
 
 }  // namespace p3g

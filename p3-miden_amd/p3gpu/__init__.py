@@ -175,6 +175,7 @@ def _load():
         "p3g_copy_device": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
         "p3g_fill_random": (C.c_int, [C.c_uint64, C.c_uint64, _u64p]),
         "p3g_coset_lde_batch": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, _u64p]),
+        "p3g_coset_lde_batch_noalloc": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, _vp]),
         "p3g_lmcs_alignment": (C.c_uint32, [C.c_uint32]),
         "p3g_lmcs_build": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
         "p3g_lmcs_hash": (C.c_int, [C.c_uint32, _u64p, _u64p, C.c_uint64, _u64p]),
