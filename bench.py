@@ -447,6 +447,55 @@ def bench_small(G, W, dev, n_proofs, workers, max_batch=128):
     }
 
 
+# ------------------------------------------------------------------ grinding (the reference binary's parameters)
+def bench_grind(G, dev, steps, warmup):
+    """PcsParams(3, 1, 0, 16, 0, 100, 0) of the reference's end-to-end Goldilocks binary
+    (p3-miden-lifted-examples/src/bin/lifted_miden.rs:34-54) on its two trace shapes,
+    2^18 x 51 and 2^19 x 20 (LDE 2^22 rows): 19 FRI commit rounds, each grinding 16 bits,
+    with the witness search on the GPU inside the device Fiat-Shamir rounds."""
+    shapes = [(1 << 18, 51), (1 << 19, 20)]
+    traces = []
+    for i, (h, w) in enumerate(shapes):
+        m = dev.alloc_matrix(h, w)
+        m.fill_random(SEED + i)
+        traces.append(m)
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    params = G.PcsParams(3, 1, 0, 16, 0, 100, 0)
+
+    def step():
+        return G.prove(cfg, params, traces)
+
+    first = None
+    for _ in range(max(1, warmup)):
+        first = step()
+    dev.profile(True)
+    dev.profile_reset()
+    ms, out = timed(step, steps, dev)
+    assert np.array_equal(first[2], out[2]), "proofs differ between runs"
+    g_ms, g_n = dev.profile_read("grind")
+    dev.profile(False)
+    for m in traces:
+        m.free()
+    dev.trim()
+    # the same 16-bit search serially on the host (this library's transcript, one thread)
+    tx = G.ProverTranscript()
+    tx.observe(list(range(5)))
+    h0 = time.perf_counter()
+    tx.grind(16)
+    host_ms = (time.perf_counter() - h0) * 1e3
+    rounds = g_n // max(1, steps)
+    return {
+        "workload": "PcsParams(3, 1, 0, 16, 0, 100, 0) (the reference binary's), traces 2^18 x 51 and 2^19 x 20 "
+                    "(LDE 2^22 rows), Poseidon2 LMCS: 19 FRI rounds with 16-bit folding grinding",
+        "proofs_per_s": round(1000.0 / ms, 3),
+        "ms_per_proof": round(ms, 3),
+        "grind_ms_per_proof": round(g_ms / max(1, steps), 3),
+        "grind_searches_per_proof": rounds,
+        "host_serial_grind_ms_16_bits": round(host_ms, 2),
+        "host_serial_grind_ms_per_proof_est": round(host_ms * rounds, 1),
+    }
+
+
 # ------------------------------------------------------------------ quotient commitment
 def bench_quotient(G, dev, steps, warmup):
     """commit_quotient (quotient.rs:133-238) on quotient evaluations resident in HBM:
@@ -524,8 +573,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--sections", default="calibrate,c2,c3,c4,c5,small,quotient",
-                    help="comma list of calibrate (measured peaks), c2 (headline), c3, c4, c5, small, quotient")
+    ap.add_argument("--sections", default="calibrate,c2,c3,c4,c5,small,quotient,grind",
+                    help="comma list of calibrate (measured peaks), c2 (headline), c3, c4, c5, small, quotient, "
+                         "grind (the reference binary's parameters, 16-bit folding grinding)")
     ap.add_argument("--small-proofs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=4, help="server workers for the C5 mix")
@@ -562,6 +612,7 @@ def main():
     small = bench_small(G, W, dev, args.small_proofs, args.small_workers, args.small_max_batch) \
         if "small" in sections else None
     quot = bench_quotient(G, dev, max(3, args.steps // 2), 1) if "quotient" in sections else None
+    grind = bench_grind(G, dev, max(2, args.steps // 2), 1) if "grind" in sections else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(W, args.cpu_sample_log_rows)
 
     ms = c2["ms"] if c2 else None
@@ -598,6 +649,7 @@ def main():
         "serving": c5,
         "small_proofs": small,
         "quotient_commit": quot,
+        "grinding": grind,
         "cpu_baseline": cpu,
         "measured_peaks": peaks,
     }

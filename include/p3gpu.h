@@ -87,7 +87,7 @@ int p3g_ctx_memory(p3g_ctx* ctx, uint64_t* live, uint64_t* cached, uint64_t* pea
  * When enabled, HIP events are recorded on the context's stream around each
  * launch of a named kernel family ("lde_inverse_pass", "lde_forward_pass",
  * "p2_leaf", "keccak_leaf", "compress_layer", "compress_top", "deep_weights",
- * "deep_colsum", "deep_quotient", "fri_fold"). */
+ * "deep_colsum", "deep_quotient", "fri_fold", "grind"). */
 int p3g_profile_enable(p3g_ctx* ctx, int on);
 int p3g_profile_reset(p3g_ctx* ctx);
 int p3g_profile_read(p3g_ctx* ctx, const char* name, double* total_ms, uint64_t* launches);
@@ -242,7 +242,12 @@ int p3g_transcript_encode(const p3g_transcript* ch, uint8_t* out, uint64_t cap, 
  * open_with_channel(params, lmcs, log_lde_height, eval_points, trace_trees, channel)
  * (p3-miden-lifted-fri/src/prover.rs:35-116): DEEP quotient, FRI commit phase,
  * query grinding and sampling, batch openings of every trace tree and FRI round.
- * eval_points: n_points extension elements (2 words each), 1 <= n_points <= 4. */
+ * eval_points: n_points extension elements (2 words each), 1 <= n_points <= 4.
+ * Grinding on this library's Poseidon2 duplex transcript searches witnesses on the
+ * GPU (every candidate below the returned witness is tested, so it is the serial
+ * search's witness); the parameters are checked as PcsParams::new does, and
+ * proof-of-work bits of 32 or more give P3G_E_INVAL (here and in p3g_prove,
+ * p3g_prove_many, p3g_prove_planned and p3g_server_submit). */
 int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t log_lde_height,
                           const uint64_t* eval_points, uint32_t n_points, p3g_tree* const* trees, uint64_t ntrees,
                           p3g_transcript* ch);

@@ -19,6 +19,14 @@ void calib_copy(Ctx& ctx, uint64_t bytes, uint32_t reps, double* kernel_ms, doub
 
 static thread_local std::string g_last_error;
 
+// the caller's parameters, checked once for every open and prove path
+static PcsParams pcs_params(const p3g_pcs_params* q) {
+  const PcsParams p{q->log_blowup,   q->log_folding_arity, q->log_final_degree, q->folding_pow_bits,
+                    q->deep_pow_bits, q->num_queries,       q->query_pow_bits};
+  check_pcs_params(p);
+  return p;
+}
+
 template <class F>
 static int guard(F&& f) {
   try {
@@ -717,8 +725,7 @@ int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t l
   return guard([&] {
     Ctx& c = ctx->c;
     P3G_CHECK_HIP(hipSetDevice(c.device));
-    PcsParams p{params->log_blowup,   params->log_folding_arity, params->log_final_degree, params->folding_pow_bits,
-                params->deep_pow_bits, params->num_queries,      params->query_pow_bits};
+    const PcsParams p = pcs_params(params);
     if (!canonical(eval_points, 2ull * n_points)) throw ApiError(P3G_E_INVAL, "non-canonical evaluation point");
     std::vector<ef2> z(n_points);
     for (uint32_t j = 0; j < n_points; j++) z[j] = ef2{eval_points[2 * j], eval_points[2 * j + 1]};
@@ -888,8 +895,7 @@ int p3g_open_with_callbacks(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t
   return guard([&] {
     Ctx& c = ctx->c;
     P3G_CHECK_HIP(hipSetDevice(c.device));
-    PcsParams p{params->log_blowup,   params->log_folding_arity, params->log_final_degree, params->folding_pow_bits,
-                params->deep_pow_bits, params->num_queries,      params->query_pow_bits};
+    const PcsParams p = pcs_params(params);
     if (!canonical(eval_points, 2ull * n_points)) throw ApiError(P3G_E_INVAL, "non-canonical evaluation point");
     std::vector<ef2> z(n_points);
     for (uint32_t j = 0; j < n_points; j++) z[j] = ef2{eval_points[2 * j], eval_points[2 * j + 1]};
@@ -978,6 +984,8 @@ int p3g_prove_planned(p3g_ctx* ctx, p3g_commit_plan* plan, const p3g_pcs_params*
                       uint64_t root[4], uint64_t eval_points_out[4]) {
   if (!ctx || !plan || !params || !ch) return P3G_E_INVAL;
   if (params->log_blowup != plan->p->log_blowup) return P3G_E_INVAL;
+  // reject bad parameters before the commitment touches the transcript
+  if (int rc0 = guard([&] { (void)pcs_params(params); })) return rc0;
   p3g_tree* tree = nullptr;
   int rc = p3g_commit_plan_run(plan, &tree);
   if (rc != P3G_OK) return rc;
@@ -1009,8 +1017,7 @@ int p3g_prove_many(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* param
     check_hash(hash_kind);
     P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
     Ctx& c = ctx->c;
-    PcsParams p{params->log_blowup,   params->log_folding_arity, params->log_final_degree, params->folding_pow_bits,
-                params->deep_pow_bits, params->num_queries,      params->query_pow_bits};
+    const PcsParams p = pcs_params(params);
     std::vector<std::vector<DevMat>> tr(k);
     std::vector<uint64_t*> uploaded;
     try {
@@ -1055,6 +1062,8 @@ int p3g_prove_many(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* param
 int p3g_prove(p3g_ctx* ctx, uint32_t hash_kind, const p3g_pcs_params* params, const p3g_matrix* traces, uint64_t k,
               uint32_t flags, p3g_transcript* ch, uint64_t root[4], uint64_t eval_points_out[4]) {
   if (!ctx || !params || !ch || (!traces && k)) return P3G_E_INVAL;
+  // reject bad parameters before the commitment touches the transcript
+  if (int rc0 = guard([&] { (void)pcs_params(params); })) return rc0;
   p3g_tree* tree = nullptr;
   int rc = p3g_commit_traces(ctx, hash_kind, traces, k, params->log_blowup, flags, &tree);
   if (rc != P3G_OK) return rc;

@@ -437,6 +437,15 @@ Tree::~Tree() {
   }
 }
 
+void check_pcs_params(const PcsParams& p) {
+  if (p.log_arity < 1 || p.log_arity > 3) throw ApiError(E_INVAL, "invalid folding arity: log_arity must be 1, 2 or 3");
+  if (p.log_blowup == 0) throw ApiError(E_INVAL, "log_blowup must be > 0");
+  if (uint64_t(p.log_final_degree) + p.log_blowup > 64) throw ApiError(E_INVAL, "log_final_degree + log_blowup exceeds 64");
+  if (p.num_queries == 0) throw ApiError(E_INVAL, "num_queries must be > 0");
+  if (p.folding_pow_bits >= MAX_POW_BITS || p.deep_pow_bits >= MAX_POW_BITS || p.query_pow_bits >= MAX_POW_BITS)
+    throw ApiError(E_INVAL, "proof-of-work bits must be below 32");
+}
+
 size_t fri_num_rounds(const PcsParams& p, unsigned log_domain) {
   unsigned lmf = p.log_final_degree + p.log_blowup;
   unsigned diff = log_domain > lmf ? log_domain - lmf : 0;
@@ -445,6 +454,10 @@ size_t fri_num_rounds(const PcsParams& p, unsigned log_domain) {
 
 size_t fri_final_degree(const PcsParams& p, unsigned log_domain) {
   size_t rounds = fri_num_rounds(p, log_domain);
+  // the reference subtracts in usize here (fri/mod.rs:84-92) and fails when the
+  // last round folds past the domain (an arity too large for a tiny domain)
+  if (rounds * p.log_arity > log_domain)
+    throw ApiError(E_INVAL, "FRI folding overshoots the LDE domain: log_arity too large for this height");
   unsigned lfs = log_domain - (unsigned)(rounds * p.log_arity);
   unsigned e = lfs > p.log_blowup ? lfs - p.log_blowup : 0;
   return (size_t)1 << e;

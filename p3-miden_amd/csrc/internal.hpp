@@ -259,6 +259,9 @@ struct Transcript : Channel {
   void hint_commitment(const uint64_t* c4) override;
   uint64_t grind(unsigned bits) override;
   bool check_witness(unsigned bits, uint64_t w);
+  // grind with a witness found elsewhere (the device search): observes and
+  // checks it as grind would, and records it; throws if it fails the check
+  void accept_witness(unsigned bits, uint64_t w);
 
  private:
   void duplex();
@@ -273,6 +276,10 @@ void keccak256_host(const uint8_t* in, size_t len, uint8_t* out32);
 struct PcsParams {
   uint32_t log_blowup, log_arity, log_final_degree, folding_pow_bits, deep_pow_bits, num_queries, query_pow_bits;
 };
+// PcsParams::new's checks (p3-miden-lifted-fri/src/params.rs:54-88), plus this
+// library's bound on grinding: proof-of-work bits below MAX_POW_BITS.
+constexpr uint32_t MAX_POW_BITS = 32;
+void check_pcs_params(const PcsParams& p);
 size_t fri_num_rounds(const PcsParams& p, unsigned log_domain);
 size_t fri_final_degree(const PcsParams& p, unsigned log_domain);
 

@@ -474,13 +474,16 @@ struct DevSponge {
   uint64_t in_len, out_len;
 };
 
-// One FRI commit round of the transcript on the device (16 threads, one DPP
-// row): send_commitment(root) then beta = sample_ef(), with exactly the
-// host's Transcript::observe / sample / duplex steps (grind with 0 bits adds
-// nothing to the sponge).  Lane l holds sponge word l; the buffers and their
-// lengths are uniform.
-__device__ __forceinline__ void fs_root_beta(DevSponge* __restrict__ sp, const uint64_t* __restrict__ root,
-                                             ef2* __restrict__ beta, const P2Consts* __restrict__ p2) {
+// Transcript steps on the device (16 threads, one DPP row), with exactly the
+// host's Transcript::observe / sample / duplex steps: observe the nw words at
+// `words`, draw `discard` samples (the sample_bits check of a grind), then, if
+// beta is set, beta = sample_ef().  Lane l holds sponge word l; the buffers and
+// their lengths are uniform.  An FRI commit round is send_commitment(root),
+// grind, sample beta (p3-miden-lifted-fri/src/fri/prover.rs:149-156): without
+// grinding one call (root, no discard, beta); with grinding one call for the
+// root, the witness search, then one call for the witness (one discard, beta).
+__device__ __forceinline__ void fs_steps(DevSponge* __restrict__ sp, const uint64_t* __restrict__ words, int nw,
+                                         int discard, ef2* __restrict__ beta, const P2Consts* __restrict__ p2) {
   __shared__ uint64_t IN[8], OUT[8];
   const uint32_t l = threadIdx.x;
   uint64_t x = l < 12 ? sp->st[l] : 0;
@@ -498,21 +501,25 @@ __device__ __forceinline__ void fs_root_beta(DevSponge* __restrict__ sp, const u
     outl = 8;
     __syncthreads();
   };
-  for (int k = 0; k < 4; k++) {
-    const uint64_t v = root[k];
+  for (int k = 0; k < nw; k++) {
+    const uint64_t v = words[k];
     outl = 0;
     if (l == 0) IN[inl] = v;
     __syncthreads();
     inl++;
     if (inl == 8) duplex();
   }
-  uint64_t b[2];
-  for (int q = 0; q < 2; q++) {
+  auto sample = [&]() {
     if (inl > 0 || outl == 0) duplex();
-    b[q] = OUT[--outl];
+    return OUT[--outl];
+  };
+  for (int q = 0; q < discard; q++) (void)sample();
+  if (beta) {
+    uint64_t b[2];
+    for (int q = 0; q < 2; q++) b[q] = sample();
+    if (l == 0) *beta = ef2{b[0], b[1]};
   }
   __syncthreads();
-  if (l == 0) *beta = ef2{b[0], b[1]};
   if (l < 12) sp->st[l] = x;
   if (l < 8) {
     sp->in[l] = IN[l];
@@ -526,15 +533,171 @@ __device__ __forceinline__ void fs_root_beta(DevSponge* __restrict__ sp, const u
 __global__ void __launch_bounds__(16) fs_root_beta_kernel(DevSponge* __restrict__ sp,
                                                           const uint64_t* __restrict__ root, ef2* __restrict__ beta,
                                                           const P2Consts* __restrict__ p2) {
-  fs_root_beta(sp, root, beta, p2);
+  fs_steps(sp, root, 4, 0, beta, p2);
 }
 // one 16-lane workgroup per job: sponge j, root roots[j], beta betas[j]
 __global__ void __launch_bounds__(16) fs_root_beta_many_kernel(DevSponge* __restrict__ sps,
                                                                const uint64_t* const* __restrict__ roots,
                                                                ef2* __restrict__ betas,
                                                                const P2Consts* __restrict__ p2) {
-  fs_root_beta(sps + blockIdx.x, roots[blockIdx.x], betas + blockIdx.x, p2);
+  fs_steps(sps + blockIdx.x, roots[blockIdx.x], 4, 0, betas + blockIdx.x, p2);
 }
+// a grinding round: observe the root, and reset the witness slot the search fills
+__global__ void __launch_bounds__(16) fs_root_kernel(DevSponge* __restrict__ sp, const uint64_t* __restrict__ root,
+                                                     unsigned long long* __restrict__ wit,
+                                                     const P2Consts* __restrict__ p2) {
+  fs_steps(sp, root, 4, 0, nullptr, p2);
+  if (threadIdx.x == 0) *wit = ~0ull;
+}
+__global__ void __launch_bounds__(16) fs_root_many_kernel(DevSponge* __restrict__ sps,
+                                                          const uint64_t* const* __restrict__ roots,
+                                                          unsigned long long* __restrict__ wits,
+                                                          const P2Consts* __restrict__ p2) {
+  fs_steps(sps + blockIdx.x, roots[blockIdx.x], 4, 0, nullptr, p2);
+  if (threadIdx.x == 0) wits[blockIdx.x] = ~0ull;
+}
+// after the search: observe the witness, draw the sample it was checked against, sample beta
+__global__ void __launch_bounds__(16) fs_witness_beta_kernel(DevSponge* __restrict__ sp,
+                                                             const unsigned long long* __restrict__ wit,
+                                                             ef2* __restrict__ beta, const P2Consts* __restrict__ p2) {
+  fs_steps(sp, reinterpret_cast<const uint64_t*>(wit), 1, 1, beta, p2);
+}
+__global__ void __launch_bounds__(16) fs_witness_beta_many_kernel(DevSponge* __restrict__ sps,
+                                                                  const unsigned long long* __restrict__ wits,
+                                                                  ef2* __restrict__ betas,
+                                                                  const P2Consts* __restrict__ p2) {
+  fs_steps(sps + blockIdx.x, reinterpret_cast<const uint64_t*>(wits + blockIdx.x), 1, 1, betas + blockIdx.x, p2);
+}
+
+// ------------------------------------------------------------------ proof-of-work grinding
+// DuplexChallenger::grind(bits) (called at p3-miden-transcript/src/prover.rs:140-144):
+// the smallest witness i with observe(i); sample_bits(bits) == 0.  With k < 8
+// input words pending, observing i and sampling is one permutation of the state
+// whose rate words 0..k-1 are the pending input and word k is i; the sample is
+// word 7 of the result (observing the 8th word duplexes at once, and the sample
+// then pops the same word).  Each thread tests one candidate per sweep, and a
+// sweep of the grid covers gridDim.x * GRIND_T consecutive candidates.  A
+// workgroup stops before a sweep that starts above the smallest witness found
+// so far, so every candidate below the final minimum has been tested: the
+// result is the witness of the reference's serial search.  *wit keeps its
+// value (~0: none) when [start, start + count) holds no witness.
+constexpr int GRIND_T = 256;
+constexpr uint32_t GRIND_BLOCKS = 1024;
+
+__global__ void __launch_bounds__(GRIND_T) grind_p2_kernel(const DevSponge* __restrict__ sps, uint32_t bits,
+                                                           uint64_t start, uint64_t count,
+                                                           unsigned long long* __restrict__ wits,
+                                                           const P2Consts* __restrict__ p2) {
+  // job blockIdx.y: sponge sps[y], witness wits[y]
+  const DevSponge* __restrict__ sp = sps + blockIdx.y;
+  unsigned long long* __restrict__ wit = wits + blockIdx.y;
+  __shared__ uint64_t S[12];
+  __shared__ unsigned long long seen;
+  const uint32_t t = threadIdx.x;
+  const uint32_t k = (uint32_t)sp->in_len;
+  if (t < 12) S[t] = t < k ? sp->in[t] : sp->st[t];
+  __syncthreads();
+  const uint64_t mask = (1ull << bits) - 1;
+  const uint64_t sweep = uint64_t(gridDim.x) * GRIND_T;
+  for (uint64_t off = 0; off < count; off += sweep) {
+    if (t == 0) seen = __hip_atomic_load(wit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const bool stop = seen < start + off;
+    __syncthreads();
+    if (stop) break;
+    const uint64_t j = off + uint64_t(blockIdx.x) * GRIND_T + t;
+    if (j < count) {
+      const uint64_t i = start + j;
+      uint64_t st[12];
+#pragma unroll
+      for (int q = 0; q < 12; q++) st[q] = (uint32_t)q == k ? i : S[q];
+      p2_permute_fast(st, p2);
+      if ((st[7] & mask) == 0) atomicMin(wit, (unsigned long long)i);
+    }
+  }
+}
+
+__global__ void grind_reset_kernel(unsigned long long* __restrict__ wits, uint32_t k) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < k) wits[j] = ~0ull;
+}
+
+// Candidates per search launch: a witness of `bits` bits is missing from
+// 2^(bits + 8) consecutive candidates with probability about e^-256.
+static uint64_t grind_span(unsigned bits) { return 1ull << (bits + 8); }
+
+// k searches side by side (sponge j, witness j), about GRIND_BLOCKS workgroups in all
+static void launch_grind(Ctx& ctx, const DevSponge* dsp, unsigned bits, uint64_t start, unsigned long long* wit,
+                         hipStream_t s, uint32_t k = 1) {
+  ProfScope ps(ctx, "grind", s);
+  const uint32_t per_job = std::max<uint32_t>(16, GRIND_BLOCKS / k);
+  hipLaunchKernelGGL(grind_p2_kernel, dim3(per_job, k), dim3(GRIND_T), 0, s, dsp, bits, start, grind_span(bits), wit,
+                     ctx.dev_p2);
+  P3G_CHECK_HIP(hipGetLastError());
+}
+
+static DevSponge device_image(const Transcript& t) {
+  DevSponge hs{};
+  memcpy(hs.st, t.st, sizeof(hs.st));
+  memcpy(hs.in, t.in, sizeof(hs.in));
+  memcpy(hs.out, t.out, sizeof(hs.out));
+  hs.in_len = (uint64_t)t.in_len;
+  hs.out_len = (uint64_t)t.out_len;
+  return hs;
+}
+
+// Channel::grind(bits) for k transcripts at once, with the witness searches on
+// the device when every channel is this library's Poseidon2 duplex transcript;
+// the host then observes each witness and checks it, as its own grind would.
+// Other channels grind themselves.
+static void grind_all(Ctx& ctx, const std::vector<Channel*>& txs, unsigned bits, hipStream_t s) {
+  const size_t k = txs.size();
+  std::vector<Transcript*> ts(k);
+  bool dev = bits > 0;
+  for (size_t j = 0; j < k; j++) {
+    ts[j] = dynamic_cast<Transcript*>(txs[j]);
+    dev = dev && ts[j] && ts[j]->kind == CH_DUPLEX_P2;
+  }
+  if (!dev) {
+    parallel_for(k, [&](size_t j) { txs[j]->grind(bits); });
+    return;
+  }
+  if (bits >= MAX_POW_BITS) throw ApiError(E_INVAL, "grinding: bits must be below 32");
+  std::vector<DevSponge> hs(k);
+  for (size_t j = 0; j < k; j++) {
+    if (ts[j]->in_len >= 8) throw ApiError(E_TRANSCRIPT, "grinding: transcript input buffer is full");
+    hs[j] = device_image(*ts[j]);
+  }
+  DevSponge* dsp = (DevSponge*)ctx.upload(hs.data(), k * sizeof(DevSponge), s);
+  unsigned long long* wit = (unsigned long long*)ctx.alloc(k * sizeof(unsigned long long));
+  std::vector<unsigned long long> w(k, ~0ull), found(k, ~0ull);
+  try {
+    // a bounded number of spans: each misses a witness with probability about e^-256
+    bool all = false;
+    for (uint64_t start = 0, span = 0; !all && span < 16; start += grind_span(bits), span++) {
+      hipLaunchKernelGGL(grind_reset_kernel, dim3((uint32_t)((k + 63) / 64)), dim3(64), 0, s, wit, (uint32_t)k);
+      P3G_CHECK_HIP(hipGetLastError());
+      launch_grind(ctx, dsp, bits, start, wit, s, (uint32_t)k);
+      ctx.read_back(w.data(), wit, k * sizeof(unsigned long long), s);
+      all = true;
+      for (size_t j = 0; j < k; j++) {
+        if (found[j] == ~0ull) found[j] = w[j];  // the first span with a witness holds the smallest
+        all = all && found[j] != ~0ull;
+      }
+    }
+  } catch (...) {
+    ctx.release(dsp);
+    ctx.release(wit);
+    throw;
+  }
+  ctx.release(dsp);
+  ctx.release(wit);
+  for (size_t j = 0; j < k; j++)
+    if (found[j] == ~0ull) throw ApiError(E_TRANSCRIPT, "grinding found no witness");
+  parallel_for(k, [&](size_t j) { ts[j]->accept_witness(bits, found[j]); });
+}
+
+static void grind(Ctx& ctx, Channel& tx, unsigned bits, hipStream_t s) { grind_all(ctx, {&tx}, bits, s); }
 
 struct FoldJob {
   const ef2* in;
@@ -910,7 +1073,7 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
   for (auto a : awidths) tota += a;
   const std::vector<ef2> aligned = align_evals(mats, awidths, evals.data(), npts, tota);  // [point][aligned column]
   for (int j = 0; j < npts; j++) tx.send_fields(reinterpret_cast<const uint64_t*>(aligned.data() + j * tota), 2 * tota);
-  tx.grind(p.deep_pow_bits);
+  grind(ctx, tx, p.deep_pow_bits, s);
   const ef2 alpha = tx.sample_ef();
   const ef2 beta = tx.sample_ef();
   ef2* Q = (ef2*)ctx.alloc(L * sizeof(ef2));
@@ -930,25 +1093,25 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
   ef2* cur = Q;
   uint64_t domain = L;
   // Fiat-Shamir of the commit rounds on the device, for this library's own
-  // Poseidon2 duplex transcript without folding grinding: no host round trip
-  // per round.  The host replays the rounds afterwards from the read-back
-  // roots and checks every beta against the device's.
+  // Poseidon2 duplex transcript: no host round trip per round.  With folding
+  // grinding each round observes its root, searches the witness on the device
+  // and observes it before sampling beta.  The host replays the rounds
+  // afterwards from the read-back roots and witnesses (checking each witness)
+  // and checks every beta against the device's.
   Transcript* dtx = dynamic_cast<Transcript*>(&tx);
-  const bool dev_fs = dtx && dtx->kind == CH_DUPLEX_P2 && p.folding_pow_bits == 0;
+  const bool dev_fs = dtx && dtx->kind == CH_DUPLEX_P2;
+  const unsigned fbits = p.folding_pow_bits;
   DevSponge* dsp = nullptr;
   ef2* dbeta = nullptr;
+  unsigned long long* dwit = nullptr;
   try {
     if (dev_fs) {
       size_t nr = 0;
       for (uint64_t d = L; d > fdomain; d >>= la) nr++;
-      DevSponge hs{};
-      memcpy(hs.st, dtx->st, sizeof(hs.st));
-      memcpy(hs.in, dtx->in, sizeof(hs.in));
-      memcpy(hs.out, dtx->out, sizeof(hs.out));
-      hs.in_len = (uint64_t)dtx->in_len;
-      hs.out_len = (uint64_t)dtx->out_len;
+      const DevSponge hs = device_image(*dtx);
       dsp = (DevSponge*)ctx.upload(&hs, sizeof(hs), s);
       dbeta = (ef2*)ctx.alloc((nr ? nr : 1) * sizeof(ef2));
+      if (fbits) dwit = (unsigned long long*)ctx.alloc((nr ? nr : 1) * sizeof(unsigned long long));
     }
     size_t r = 0;
     while (domain > fdomain) {
@@ -963,12 +1126,19 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
       bufs.push_back(cur);
       ef2 b{0, 0};
       if (dev_fs) {
-        hipLaunchKernelGGL(fs_root_beta_kernel, dim3(1), dim3(16), 0, s, dsp, t->digests + t->layer_off[t->depth] * 4,
-                           dbeta + r, ctx.dev_p2);
+        const uint64_t* droot = t->digests + t->layer_off[t->depth] * 4;
+        if (fbits == 0) {
+          hipLaunchKernelGGL(fs_root_beta_kernel, dim3(1), dim3(16), 0, s, dsp, droot, dbeta + r, ctx.dev_p2);
+        } else {
+          hipLaunchKernelGGL(fs_root_kernel, dim3(1), dim3(16), 0, s, dsp, droot, dwit + r, ctx.dev_p2);
+          P3G_CHECK_HIP(hipGetLastError());
+          launch_grind(ctx, dsp, fbits, 0, dwit + r, s);
+          hipLaunchKernelGGL(fs_witness_beta_kernel, dim3(1), dim3(16), 0, s, dsp, dwit + r, dbeta + r, ctx.dev_p2);
+        }
         P3G_CHECK_HIP(hipGetLastError());
       } else {
         tx.send_commitment(t->root);
-        tx.grind(p.folding_pow_bits);
+        grind(ctx, tx, fbits, s);
         b = tx.sample_ef();
       }
       ef2* nxt = (ef2*)ctx.alloc(fd * sizeof(ef2));
@@ -1017,11 +1187,19 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
       for (auto* t : ftrees) pieces.emplace_back(t->digests + t->layer_off[t->depth] * 4, 4);
       pieces.emplace_back(reinterpret_cast<const uint64_t*>(dbeta), 2 * nr);
       pieces.emplace_back(reinterpret_cast<const uint64_t*>(cur), 2 * fdeg);
+      if (fbits) pieces.emplace_back(reinterpret_cast<const uint64_t*>(dwit), nr);
       const uint64_t* raw = gather_pinned(ctx, pieces, s);
       for (size_t q = 0; q < nr; q++) {
         memcpy(ftrees[q]->root, raw + 4 * q, 32);
         tx.send_commitment(ftrees[q]->root);
-        tx.grind(p.folding_pow_bits);
+        if (fbits) {
+          // the device searched every candidate below its witness (the serial search's answer)
+          const uint64_t w = raw[6 * nr + 2 * fdeg + q];
+          if (w == ~0ull) throw ApiError(E_TRANSCRIPT, "device grinding found no witness");
+          dtx->accept_witness(fbits, w);
+        } else {
+          tx.grind(0);
+        }
         const ef2 hb = tx.sample_ef();
         const uint64_t* dbq = raw + 4 * nr + 2 * q;
         if (hb.c0 != dbq[0] || hb.c1 != dbq[1])
@@ -1040,7 +1218,7 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
     cur = nullptr;
 
     // ---- query phase
-    tx.grind(p.query_pow_bits);
+    grind(ctx, tx, p.query_pow_bits, s);
     std::vector<uint64_t> idx(p.num_queries);
     for (uint32_t q = 0; q < p.num_queries; q++) idx[q] = bitrev32((uint32_t)tx.sample_bits(logL), logL);
     // every opening of every tree in one gather, then the hints tree by tree
@@ -1070,12 +1248,14 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
     if (cur) ctx.release(cur);
     ctx.release(dsp);
     ctx.release(dbeta);
+    ctx.release(dwit);
     throw;
   }
   for (auto* t : ftrees) delete t;
   for (auto* b : bufs) ctx.release(b);
   ctx.release(dsp);
   ctx.release(dbeta);
+  ctx.release(dwit);
 }
 
 
@@ -1278,6 +1458,9 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
     ef2* dcoef = (ef2*)ctx.alloc(coeffs_all.size() * sizeof(ef2));
     tmp.push_back(dcoef);
     const uint64_t* powQ = ctx.pow_table(gl_two_adic_generator(logL), logL);
+    // per job: scale the evaluations, align and send them; then the DEEP grinds of
+    // all jobs (one device search); then per job the challenges and quotient arguments
+    std::vector<std::vector<ef2>> aligned_all(k);
     parallel_for(k, [&](size_t j) {
       ef2* ev = evals_host.data() + j * totw * N;
       ef2 sc[N];
@@ -1300,7 +1483,12 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
       }
       Transcript& tx = *txs[j];
       for (int q = 0; q < N; q++) tx.send_fields(reinterpret_cast<const uint64_t*>(aligned.data() + q * tota), 2 * tota);
-      tx.grind(p.deep_pow_bits);
+      aligned_all[j] = std::move(aligned);
+    });
+    grind_all(ctx, std::vector<Channel*>(txs.begin(), txs.end()), p.deep_pow_bits, s);
+    parallel_for(k, [&](size_t j) {
+      Transcript& tx = *txs[j];
+      const std::vector<ef2>& aligned = aligned_all[j];
       const ef2 alpha = tx.sample_ef();
       const ef2 beta = tx.sample_ef();
       QuotArgs a{};
@@ -1358,10 +1546,13 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
     std::vector<ef2*> cur = Q;
     uint64_t domain = L;
     // Fiat-Shamir of the commit rounds on the device (one sponge and one
-    // 16-lane row per job), as in the single-proof path; the hosts replay
-    // every job's rounds afterwards and check every beta.
-    bool dev_fs = p.folding_pow_bits == 0;
+    // 16-lane row per job, and with folding grinding one witness search per
+    // job and round), as in the single-proof path; the hosts replay every
+    // job's rounds afterwards, checking every witness and beta.
+    bool dev_fs = true;
     for (size_t j = 0; j < k; j++) dev_fs = dev_fs && txs[j]->kind == CH_DUPLEX_P2;
+    const unsigned fbits = p.folding_pow_bits;
+    unsigned long long* dwit = nullptr;
     size_t nr = 0;
     for (uint64_t d = L; d > fdomain; d >>= la) nr++;
     DevSponge* dsp = nullptr;
@@ -1379,6 +1570,10 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
       dsp = (DevSponge*)upload(hs.data(), k * sizeof(DevSponge));
       dbeta = (ef2*)ctx.alloc((nr ? nr : 1) * k * sizeof(ef2));
       tmp.push_back(dbeta);
+      if (fbits) {
+        dwit = (unsigned long long*)ctx.alloc((nr ? nr : 1) * k * sizeof(unsigned long long));
+        tmp.push_back(dwit);
+      }
     }
     size_t r = 0;
     while (domain > fdomain) {
@@ -1408,15 +1603,21 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
           fj[j] = FoldJob{cur[j], nxt[j], ef2{0, 0}};
         }
         const uint64_t* const* drp = (const uint64_t* const*)upload(rp.data(), k * sizeof(uint64_t*));
-        hipLaunchKernelGGL(fs_root_beta_many_kernel, dim3((uint32_t)k), dim3(16), 0, s, dsp, drp, db, ctx.dev_p2);
+        if (fbits == 0) {
+          hipLaunchKernelGGL(fs_root_beta_many_kernel, dim3((uint32_t)k), dim3(16), 0, s, dsp, drp, db, ctx.dev_p2);
+        } else {
+          unsigned long long* wr = dwit + r * k;
+          hipLaunchKernelGGL(fs_root_many_kernel, dim3((uint32_t)k), dim3(16), 0, s, dsp, drp, wr, ctx.dev_p2);
+          P3G_CHECK_HIP(hipGetLastError());
+          launch_grind(ctx, dsp, fbits, 0, wr, s, (uint32_t)k);
+          hipLaunchKernelGGL(fs_witness_beta_many_kernel, dim3((uint32_t)k), dim3(16), 0, s, dsp, wr, db,
+                             ctx.dev_p2);
+        }
         P3G_CHECK_HIP(hipGetLastError());
       } else {
-        parallel_for(k, [&](size_t j) {
-          Transcript& tx = *txs[j];
-          tx.send_commitment(ft[j]->root);
-          tx.grind(p.folding_pow_bits);
-          fj[j] = FoldJob{cur[j], nxt[j], tx.sample_ef()};
-        });
+        parallel_for(k, [&](size_t j) { txs[j]->send_commitment(ft[j]->root); });
+        grind_all(ctx, std::vector<Channel*>(txs.begin(), txs.end()), fbits, s);
+        parallel_for(k, [&](size_t j) { fj[j] = FoldJob{cur[j], nxt[j], txs[j]->sample_ef()}; });
       }
       const FoldJob* dfj = (const FoldJob*)upload(fj.data(), k * sizeof(FoldJob));
       const unsigned log_rows = log2_exact(fd);
@@ -1465,12 +1666,14 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
             pieces.emplace_back(t->digests + t->layer_off[t->depth] * 4, 4);
           }
         pieces.emplace_back(reinterpret_cast<const uint64_t*>(dbeta), 2 * rounds * k);
+        if (fbits) pieces.emplace_back(reinterpret_cast<const uint64_t*>(dwit), rounds * k);
       }
       std::vector<uint64_t> raw;
       gather_to_host(ctx, pieces, raw, s);
       if (dev_fs) {
         const uint64_t* rr = raw.data() + 2 * fdeg * k;  // roots [round][job], then betas [round][job]
         const uint64_t* bb = rr + 4 * rounds * k;
+        const uint64_t* ww = bb + 2 * rounds * k;  // witnesses [round][job] (folding grinding)
         std::vector<int> ok(k, 1);
         parallel_for(k, [&](size_t j) {
           Transcript& tx = *txs[j];
@@ -1478,7 +1681,16 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
             Tree* t = ftrees[q][j];
             memcpy(t->root, rr + 4 * (q * k + j), 32);
             tx.send_commitment(t->root);
-            tx.grind(p.folding_pow_bits);
+            if (fbits) {
+              const uint64_t w = ww[q * k + j];
+              if (w == ~0ull || w >= P || !tx.check_witness(fbits, w)) {
+                ok[j] = 0;
+                return;
+              }
+              tx.fields.push_back(w);
+            } else {
+              tx.grind(0);
+            }
             const ef2 hb = tx.sample_ef();
             const uint64_t* dbq = bb + 2 * (q * k + j);
             if (hb.c0 != dbq[0] || hb.c1 != dbq[1]) ok[j] = 0;
@@ -1503,9 +1715,9 @@ void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<s
     // gather for all jobs, then the hints per job (in parallel)
     std::vector<std::vector<std::pair<const uint64_t*, uint64_t>>> jp(k);
     std::vector<std::vector<Seg>> segs(k);
+    grind_all(ctx, std::vector<Channel*>(txs.begin(), txs.end()), p.query_pow_bits, s);
     parallel_for(k, [&](size_t j) {
       Transcript& tx = *txs[j];
-      tx.grind(p.query_pow_bits);
       std::vector<uint64_t> idx(p.num_queries);
       for (uint32_t q = 0; q < p.num_queries; q++) idx[q] = bitrev32((uint32_t)tx.sample_bits(logL), logL);
       size_t first = 0;

@@ -304,6 +304,19 @@ int p3g_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, d
 int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket) {
   if (!s || !req || !ticket || !req->traces || req->k == 0) return P3G_E_INVAL;
   if (req->hash_kind > P3G_HASH_KECCAK_GL) return P3G_E_INVAL;
+  // a request is checked alone here, so that a bad one never reaches a lock-step batch
+  // (where its error would be reported to its batch-mates too)
+  for (uint64_t i = 0; i < req->k; i++)
+    if (!req->traces[i].data && req->traces[i].height * req->traces[i].width) return P3G_E_INVAL;
+  {
+    const p3g_pcs_params& q = req->params;
+    try {
+      check_pcs_params(PcsParams{q.log_blowup, q.log_folding_arity, q.log_final_degree, q.folding_pow_bits,
+                                 q.deep_pow_bits, q.num_queries, q.query_pow_bits});
+    } catch (const ApiError&) {
+      return P3G_E_INVAL;
+    }
+  }
   const uint64_t fp = p3g_request_footprint(req);
   if (fp > s->budget) return P3G_E_RANGE;
   Request* r = new Request();

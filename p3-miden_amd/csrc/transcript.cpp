@@ -129,6 +129,11 @@ bool Transcript::check_witness(unsigned bits, uint64_t w) {
   return sample_bits(bits) == 0;
 }
 
+void Transcript::accept_witness(unsigned bits, uint64_t w) {
+  if (w >= P || !check_witness(bits, w)) throw ApiError(E_TRANSCRIPT, "grinding witness fails the transcript's check");
+  fields.push_back(w);
+}
+
 uint64_t Transcript::grind(unsigned bits) {
   uint64_t w = 0;
   if (bits > 0) {

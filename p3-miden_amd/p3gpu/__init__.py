@@ -73,6 +73,8 @@ class PcsParams(C.Structure):
             raise ValueError("log_final_degree + log_blowup exceeds 64")
         if num_queries == 0:
             raise ValueError("num_queries must be > 0")
+        if max(folding_pow_bits, deep_pow_bits, query_pow_bits) >= 32:
+            raise ValueError("proof-of-work bits must be below 32 (the library's grinding bound)")
         super().__init__(log_blowup, log_folding_arity, log_final_degree, folding_pow_bits, deep_pow_bits,
                          num_queries, query_pow_bits)
 
