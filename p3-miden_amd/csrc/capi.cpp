@@ -726,6 +726,7 @@ int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t l
     Ctx& c = ctx->c;
     P3G_CHECK_HIP(hipSetDevice(c.device));
     const PcsParams p = pcs_params(params);
+    if (n_points < 1 || n_points > 4) throw ApiError(P3G_E_INVAL, "1 to 4 evaluation points supported");
     if (!canonical(eval_points, 2ull * n_points)) throw ApiError(P3G_E_INVAL, "non-canonical evaluation point");
     std::vector<ef2> z(n_points);
     for (uint32_t j = 0; j < n_points; j++) z[j] = ef2{eval_points[2 * j], eval_points[2 * j + 1]};
@@ -741,6 +742,8 @@ int p3g_open_with_channel(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t l
 // ---- DEEP and FRI steps
 namespace {
 std::vector<ef2> ext_points(const uint64_t* v, uint32_t n) {
+  // the count first: the caller's buffer holds 2 n words only when n is valid
+  if (n < 1 || n > 4) throw ApiError(P3G_E_INVAL, "1 to 4 evaluation points supported");
   if (!canonical(v, 2ull * n)) throw ApiError(P3G_E_INVAL, "non-canonical extension element");
   std::vector<ef2> z(n);
   for (uint32_t j = 0; j < n; j++) z[j] = ef2{v[2 * j], v[2 * j + 1]};
@@ -772,6 +775,8 @@ int p3g_deep_batch_eval(p3g_ctx* ctx, uint32_t log_blowup, const uint64_t* eval_
     const std::vector<Tree*> tv = tree_list(trees, ntrees);
     const std::vector<ef2> z = ext_points(eval_points, n_points);
     *len = eval_words(tv, n_points);
+    // the size query makes the same checks of the inputs as the call
+    deep_check_inputs(tv, log_blowup, z.data(), (int)n_points);
     if (!evals) return;
     if (cap < *len) throw ApiError(P3G_E_RANGE, "evaluation buffer too small");
     P3G_CHECK_HIP(hipSetDevice(c.device));
@@ -896,6 +901,7 @@ int p3g_open_with_callbacks(p3g_ctx* ctx, const p3g_pcs_params* params, uint32_t
     Ctx& c = ctx->c;
     P3G_CHECK_HIP(hipSetDevice(c.device));
     const PcsParams p = pcs_params(params);
+    if (n_points < 1 || n_points > 4) throw ApiError(P3G_E_INVAL, "1 to 4 evaluation points supported");
     if (!canonical(eval_points, 2ull * n_points)) throw ApiError(P3G_E_INVAL, "non-canonical evaluation point");
     std::vector<ef2> z(n_points);
     for (uint32_t j = 0; j < n_points; j++) z[j] = ef2{eval_points[2 * j], eval_points[2 * j + 1]};

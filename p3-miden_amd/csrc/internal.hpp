@@ -353,6 +353,8 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned log_lde_height, co
 // polynomial's L evaluations into the device buffer out from such evals and
 // the two challenges; fri_fold: one round, n -> n >> la device elements;
 // fri_final_poly: host, bit-reversed evaluations -> descending coefficients.
+// the checks deep_batch_eval and deep_quotient make of their inputs, alone
+void deep_check_inputs(const std::vector<Tree*>& trees, unsigned log_blowup, const ef2* z, int npts);
 void deep_batch_eval(Ctx& ctx, unsigned log_blowup, const std::vector<Tree*>& trees, const ef2* z, int npts,
                      std::vector<ef2>& evals, hipStream_t s);
 void deep_quotient(Ctx& ctx, unsigned log_blowup, const std::vector<Tree*>& trees, const ef2* z, int npts,

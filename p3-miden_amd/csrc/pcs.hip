@@ -998,6 +998,13 @@ static unsigned deep_inputs(const std::vector<Tree*>& trees, unsigned log_blowup
   return logL;
 }
 
+void deep_check_inputs(const std::vector<Tree*>& trees, unsigned log_blowup, const ef2* z, int npts) {
+  std::vector<DevMat> mats;
+  std::vector<uint64_t> awidths;
+  uint64_t tota = 0;
+  (void)deep_inputs(trees, log_blowup, z, npts, mats, awidths, tota);
+}
+
 void deep_batch_eval(Ctx& ctx, unsigned log_blowup, const std::vector<Tree*>& trees, const ef2* z, int npts,
                      std::vector<ef2>& evals, hipStream_t s) {
   std::vector<DevMat> mats;

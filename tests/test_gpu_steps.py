@@ -190,3 +190,22 @@ def test_deep_size_query_validates_like_the_call(device):
     assert G.lib().p3g_deep_batch_eval(device.h, 1, G._p(z), 2, hs, 0, None, 0, G._p(n)) == 6  # EMPTY
     assert G.lib().p3g_deep_batch_eval(device.h, 1, G._p(z), 2, hs, 1, None, 0, G._p(n)) == 0
     assert int(n[0]) == 5 * 2 * 2
+
+
+def test_deep_size_query_rejects_what_the_call_rejects(device):
+    # the size query runs the call's input checks too: a point on the LDE coset, a blowup above
+    # the LDE height, mixed heights; and a bad point count is refused before the points are read
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    t = G.commit_traces(cfg, [O.rand_matrix(4, 16, 3)], 1)
+    t2 = G.commit_traces(cfg, [O.rand_matrix(5, 8, 3)], 1)
+    n = np.zeros(1, np.uint64)
+    hs = (G.C.c_void_p * 2)(t.h, t2.h)
+    on_coset = np.array([7, 0], np.uint64)
+    ok = np.array([1, 2], np.uint64)
+    lib = G.lib()
+    assert lib.p3g_deep_batch_eval(device.h, 1, G._p(on_coset), 1, hs, 1, None, 0, G._p(n)) == 1
+    assert lib.p3g_deep_batch_eval(device.h, 6, G._p(ok), 1, hs, 1, None, 0, G._p(n)) == 1
+    assert lib.p3g_deep_batch_eval(device.h, 1, G._p(ok), 1, hs, 2, None, 0, G._p(n)) == 1
+    # n_points 0: rejected on the count before the point buffer is read
+    assert lib.p3g_deep_batch_eval(device.h, 1, G._p(ok), 0, hs, 1, None, 0, G._p(n)) == 1
+    assert lib.p3g_deep_batch_eval(device.h, 1, G._p(ok), 1, hs, 1, None, 0, G._p(n)) == 0

@@ -231,9 +231,13 @@ def test_poseidon2_linear_layers_structure():
     assert inn.tolist() == [(d[i] * x[i] + sum(x)) % P for i in range(12)]
 
 
-def test_poseidon2_round_counts_and_constant_draw_order():
-    # new_from_rng_128 draws 4 x [F;12] initial, 4 x [F;12] terminal, 22 x F internal
-    # constants from SmallRng(2025), in that order (parity unpinned: Plonky3 is not vendored).
+def test_poseidon2_constant_draw_order_assumption():
+    # Not a pin: a statement of the draw order the oracle assumes for new_from_rng_128
+    # (4 x [F;12] initial, 4 x [F;12] terminal, then 22 x F internal constants, all from
+    # SmallRng(2025) by rejection below p).  It checks the oracle against its own generator,
+    # so it only fixes that assumption; the constants stay parity unpinned (Plonky3 and
+    # rand are not vendored and no reference test holds them).  The round numbers and the
+    # internal matrix are pinned from the design rules in test_poseidon2_params.py.
     c = O.p2_constants()
     assert c.size == 118
     assert c.tolist() == O.rand_matrix(2025, 1, 118).ravel().tolist()
