@@ -43,8 +43,9 @@ SEED = 2025
 
 # MI355X published peaks
 HBM_PEAK_GBS = 8000.0
-# 32-bit VALU lane-ops/s: 2 wave64 instructions per clock per CU (the published 157.3 TFLOP/s FP32
-# vector rate / 2 flops per FMA) = 256 CUs x 128 lanes x 2.4 GHz
+# 32-bit VALU lane-ops/s at 2 wave64 instructions per SIMD per quad-cycle (the published 157.3 TFLOP/s
+# FP32 vector rate / 2 flops per FMA) = 256 CUs x 128 lanes x 2.4 GHz; the measured peak replaces it
+# when the calibrate section runs
 VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 
 
@@ -65,12 +66,12 @@ def leaf_perms(shapes, log_blowup, rate):
 
 
 def read_profile_constants():
-    """Per-permutation VALU work and measured HBM traffic of the leaf kernel (rocprofv3 PMC runs)."""
+    """Per-launch VALU instructions, issue occupancy and HBM traffic of the leaf kernel (rocprofv3 PMC runs)."""
     out = {}
-    path = os.path.join(ROOT, "profiles", "p2_leaf_c2.md")
+    path = os.path.join(ROOT, "profiles", "valu_issue_c2_pmc.md")
     if os.path.exists(path):
         for line in open(path):
-            for key in ("valu_lane_ops_per_perm", "hbm_bytes_per_launch"):
+            for key in ("valu_insts_per_launch", "valu_issue_busy", "valu_paired_share", "hbm_bytes_per_launch"):
                 if line.startswith(key + ":"):
                     out[key] = float(line.split(":")[1])
     return out
@@ -90,7 +91,7 @@ def timed(fn, steps, dev):
 def bench_calibrate(dev):
     """Sustained VALU issue rates per instruction class and device copy bandwidth, measured on this
     GPU: the ceilings the kernels' roofline fractions are quoted against beside the published ones."""
-    valu = [dev.calibrate_valu(k) for k in range(5)]
+    valu = [dev.calibrate_valu(k) for k in range(len(dev.VALU_KINDS))]
     copy = dev.calibrate_copy(4 << 30, 5)
     add = valu[0]
     return {
@@ -106,7 +107,7 @@ def bench_calibrate(dev):
 
 
 # ------------------------------------------------------------------ headline: plain single-trace proof
-def bench_c2(G, W, dev, steps, warmup):
+def bench_c2(G, W, dev, steps, warmup, peaks=None):
     n, w, b = 1 << W.C2_LOG_ROWS, W.C2_WIDTH, W.C2_PARAMS[0]
     log(f"[c2] generating trace 2^{W.C2_LOG_ROWS} x {w} from SmallRng({SEED})")
     host = G.fill_random(SEED, n * w).reshape(n, w)
@@ -158,24 +159,34 @@ def bench_c2(G, W, dev, steps, warmup):
     perms = leaf_perms(shapes, b, 8)
     lde_bytes = 8 * n * w * (1 + (1 << b))
     pc = read_profile_constants()
+    leaf_bytes = 8 * (n << b) * w + 32 * (n << b)  # read every LDE word once, write the 2^24 leaf digests
+    peak_tops = (peaks or {}).get("valu_issue_peak_tops") or VALU_PEAK_TOPS
     roof = {
         "kernel": "p2_leaf (Poseidon2-GL12 LMCS leaf hashing of the 2^24 x 64 LDE: 2^24 leaves x 8 permutations)",
         "bound": "valu",
         "achieved": None,
-        "peak": round(VALU_PEAK_TOPS, 2),
-        "unit": "TOP/s",
+        "peak": round(peak_tops, 2),
+        "unit": "T lane-ops/s",
         "frac": None,
         "traffic": pc.get("hbm_bytes_per_launch"),
         "launch_ms": round(leaf_ms, 3),
         "permutations_per_s": round(perms / (leaf_ms * 1e-3), 1) if leaf_ms else None,
-        "hbm_view": {"algorithmic_bytes": 8 * (n << b) * w + 32 * (n << b),
-                     "achieved": round((8 * (n << b) * w + 32 * (n << b)) / (leaf_ms * 1e-3) / 1e9, 1) if leaf_ms else None,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+        "peak_source": "measured on this GPU in this run (v_add_u32 issue rate, bench.py calibrate)" if peaks
+                       else "published FP32 vector rate / 2 (calibrate section not run)",
+        "valu_issue_busy": pc.get("valu_issue_busy"),
+        "valu_paired_share": pc.get("valu_paired_share"),
+        "hbm": {"algorithmic_bytes": leaf_bytes,
+                "achieved": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9, 1) if leaf_ms else None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if leaf_ms else None,
+                "measured_copy_gbs": (peaks or {}).get("copy_kernel_gbs")},
+        "note": "VALU instructions per launch from SQ_INSTS_VALU (exact, calibrated) x 64 lanes / launch time; "
+                "valu_issue_busy = share of SIMD quad-cycles with a VALU issue (PMC), the kernel's real limit",
     }
-    if pc.get("valu_lane_ops_per_perm") and leaf_ms:
-        ach = perms * pc["valu_lane_ops_per_perm"] / (leaf_ms * 1e-3) / 1e12
+    if pc.get("valu_insts_per_launch") and leaf_ms:
+        ach = pc["valu_insts_per_launch"] * 64 / (leaf_ms * 1e-3) / 1e12
         roof["achieved"] = round(ach, 2)
-        roof["frac"] = round(ach / VALU_PEAK_TOPS, 3)
+        roof["frac"] = round(ach / peak_tops, 3)
     others = {
         "lde (inverse + forward passes)": {"ms": round(lde_ms, 3), "bytes": lde_bytes,
                                            "gbps": round(lde_bytes / (lde_ms * 1e-3) / 1e9, 1) if lde_ms else None,
@@ -548,24 +559,67 @@ def bench_quotient(G, dev, steps, warmup):
             "runs": out}
 
 
-def cpu_baseline(W, log_rows):
+def cpu_baseline(W, log_rows, full_c2=False):
+    """The oracle (the C restatement of the reference CPU prover, OpenMP) on the host cores, timed as
+    p3-miden-lifted-examples/src/stats.rs:185-190 does (one warm-up, then BENCH_ITERS runs):
+      - C1 (2^14 x 32, the workload the reference runs on CPU) at full size: 1 + 5 runs on all
+        threads and 1 + 3 runs on one thread, median and p99;
+      - the headline workload (2^22 x 64, C2 parameters) on a 2^log_rows-row sample, scaled by
+        the row ratio (the prover is linear in rows at fixed width), which is `value`;
+      - with full_c2, one C2 proof at full size as well (about a minute of CPU work)."""
     import oracle as O
 
+    def pct(v, q):
+        v = sorted(v)
+        return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+
+    threads = O.get_threads()
     w = W.C2_WIDTH
-    log(f"cpu baseline: oracle on 2^{log_rows} x {w} (same parameters), {O.get_threads()} threads")
+    out = {"unit": "proofs/s", "cores": threads, "kind": "port"}
+    c1 = [O.rand_matrix(SEED, 1 << W.C1_LOG_ROWS, W.C1_WIDTH)]
+    c1p = O.PcsParams(*W.C1_PARAMS)
+
+    def c1_runs(n):
+        ts = []
+        for i in range(n + 1):
+            t0 = time.perf_counter()
+            O.pcs_trace_prove(c1p, O.HASH_P2, c1)
+            if i:
+                ts.append(time.perf_counter() - t0)
+        return ts
+
+    log(f"cpu baseline: C1 2^{W.C1_LOG_ROWS} x {W.C1_WIDTH}, {threads} threads (1 + 5 runs) and 1 thread (1 + 3 runs)")
+    ta = c1_runs(5)
+    O.set_threads(1)
+    try:
+        t1 = c1_runs(3)
+    finally:
+        O.set_threads(threads)
+    out["c1_full_size"] = {
+        "workload": f"2^{W.C1_LOG_ROWS} x {W.C1_WIDTH}, PcsParams{W.C1_PARAMS}, Poseidon2",
+        "all_threads": {"threads": threads, "median_s": round(pct(ta, 0.5), 4), "p99_s": round(pct(ta, 0.99), 4),
+                        "proofs_per_s": round(1.0 / pct(ta, 0.5), 3)},
+        "one_thread": {"median_s": round(pct(t1, 0.5), 4), "p99_s": round(pct(t1, 0.99), 4),
+                       "proofs_per_s": round(1.0 / pct(t1, 0.5), 3)},
+    }
+    log(f"cpu baseline: oracle on 2^{log_rows} x {w} (C2 parameters), {threads} threads")
     tr = [O.rand_matrix(SEED, 1 << log_rows, w)]
     c0 = time.perf_counter()
     O.pcs_trace_prove(O.PcsParams(*W.C2_PARAMS), O.HASH_P2, tr)
-    c1 = time.perf_counter()
+    c1t = time.perf_counter()
+    del tr
     scale = (1 << W.C2_LOG_ROWS) // (1 << log_rows)
-    return {
-        "value": round(1.0 / ((c1 - c0) * scale), 6),
-        "unit": "proofs/s",
-        "cores": O.get_threads(),
-        "kind": "port",
-        "sample": f"oracle proof of a 2^{log_rows} x {w} trace with the same parameters took {c1 - c0:.2f} s; "
-                  f"scaled x{scale} (linear in rows) to the 2^{W.C2_LOG_ROWS}-row workload",
-    }
+    out["value"] = round(1.0 / ((c1t - c0) * scale), 6)
+    out["sample"] = (f"oracle proof of a 2^{log_rows} x {w} trace with the C2 parameters took {c1t - c0:.2f} s on "
+                     f"{threads} threads; scaled x{scale} (linear in rows) to the 2^{W.C2_LOG_ROWS}-row workload")
+    if full_c2:
+        log("cpu baseline: one C2 proof at full size")
+        tr = [O.rand_matrix(SEED, 1 << W.C2_LOG_ROWS, w)]
+        f0 = time.perf_counter()
+        O.pcs_trace_prove(O.PcsParams(*W.C2_PARAMS), O.HASH_P2, tr)
+        out["c2_full_size_s"] = round(time.perf_counter() - f0, 2)
+        del tr
+    return out
 
 
 def main():
@@ -584,7 +638,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 4 GiB device-to-device copy (reference dispatch for HBM byte counters)")
-    ap.add_argument("--cpu-sample-log-rows", type=int, default=17)
+    ap.add_argument("--cpu-sample-log-rows", type=int, default=19)
+    ap.add_argument("--cpu-full-c2", action="store_true", help="also time one C2 proof at full size on the CPU")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
@@ -605,7 +660,7 @@ def main():
     peaks = bench_calibrate(dev) if "calibrate" in sections else None
     # the headline section always runs in the default configuration; leaving it
     # out of --sections is for profiling one other workload on its own
-    c2 = bench_c2(G, W, dev, args.steps, args.warmup) if "c2" in sections else None
+    c2 = bench_c2(G, W, dev, args.steps, args.warmup, peaks) if "c2" in sections else None
     c3 = bench_c3(G, W, dev, max(2, args.steps // 3), args.warmup) if "c3" in sections else None
     c4 = bench_c4(G, W, dev, max(2, args.steps // 2), args.warmup) if "c4" in sections else None
     c5 = bench_c5(G, W, dev, args.serve_requests, args.serve_workers) if "c5" in sections else None
@@ -613,7 +668,7 @@ def main():
         if "small" in sections else None
     quot = bench_quotient(G, dev, max(3, args.steps // 2), 1) if "quotient" in sections else None
     grind = bench_grind(G, dev, max(2, args.steps // 2), 1) if "grind" in sections else None
-    cpu = None if args.no_cpu_baseline else cpu_baseline(W, args.cpu_sample_log_rows)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(W, args.cpu_sample_log_rows, args.cpu_full_c2)
 
     ms = c2["ms"] if c2 else None
     c2 = c2 or {}

@@ -144,6 +144,76 @@ __global__ __launch_bounds__(CAL_THREADS) void cal_add64_kernel(uint32_t iters, 
   out[t] = r;
 }
 
+// kinds 5..11: one more instruction class each, 16 independent chains per lane
+//   5 v_mov_b32 (a rotation of the chains), 6 v_cndmask_b32 on an SGPR mask, 7 v_add_co_u32
+//   (VOP3, a carry-out SGPR pair per chain), 8 v_lshl_add_u64 (64-bit add, 8 chains),
+//   9 v_add3_u32, 10 v_xor_b32, 11 v_alignbit_b32
+template <int KIND>
+__global__ __launch_bounds__(CAL_THREADS) void cal_misc_kernel(uint32_t iters, uint32_t k, uint64_t* out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t a[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) a[i] = t ^ (0x9e3779b9u * (i + 1));
+  uint64_t q[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) q[i] = (uint64_t(a[2 * i]) << 32) | a[2 * i + 1];
+  const uint64_t k64 = (uint64_t(k) << 32) | (k * 3u);
+  for (uint32_t it = 0; it < iters; it += CAL_UNROLL) {
+#pragma unroll
+    for (int u = 0; u < CAL_UNROLL; u++) {
+      if constexpr (KIND == 5) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) asm volatile("v_mov_b32 %0, %1" : "=v"(a[i]) : "v"(a[(i + 1) & 15]));
+      } else if constexpr (KIND == 6) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) asm volatile("v_cndmask_b32_e64 %0, %0, %1, %2" : "+v"(a[i]) : "v"(k), "s"(k64));
+      } else if constexpr (KIND == 7) {
+        asm volatile(
+            "v_add_co_u32_e64 %0, s[40:41], %0, %16\n\t"
+            "v_add_co_u32_e64 %1, s[42:43], %1, %16\n\t"
+            "v_add_co_u32_e64 %2, s[44:45], %2, %16\n\t"
+            "v_add_co_u32_e64 %3, s[46:47], %3, %16\n\t"
+            "v_add_co_u32_e64 %4, s[48:49], %4, %16\n\t"
+            "v_add_co_u32_e64 %5, s[50:51], %5, %16\n\t"
+            "v_add_co_u32_e64 %6, s[52:53], %6, %16\n\t"
+            "v_add_co_u32_e64 %7, s[54:55], %7, %16\n\t"
+            "v_add_co_u32_e64 %8, s[56:57], %8, %16\n\t"
+            "v_add_co_u32_e64 %9, s[58:59], %9, %16\n\t"
+            "v_add_co_u32_e64 %10, s[60:61], %10, %16\n\t"
+            "v_add_co_u32_e64 %11, s[62:63], %11, %16\n\t"
+            "v_add_co_u32_e64 %12, s[64:65], %12, %16\n\t"
+            "v_add_co_u32_e64 %13, s[66:67], %13, %16\n\t"
+            "v_add_co_u32_e64 %14, s[68:69], %14, %16\n\t"
+            "v_add_co_u32_e64 %15, s[70:71], %15, %16"
+            : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]),
+              "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+            : "v"(k)
+            : "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53",
+              "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67",
+              "s68", "s69", "s70", "s71");
+      } else if constexpr (KIND == 8) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(q[i]) : "v"(k64));
+      } else if constexpr (KIND == 9) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) asm volatile("v_add3_u32 %0, %0, %1, %1" : "+v"(a[i]) : "v"(k));
+      } else if constexpr (KIND == 10) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(a[i]) : "v"(k));
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) asm volatile("v_alignbit_b32 %0, %0, %1, 7" : "+v"(a[i]) : "v"(k));
+      }
+    }
+  }
+  uint64_t x = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) x ^= a[i];
+#pragma unroll
+  for (int i = 0; i < 8; i++) x ^= q[i];
+  out[t] = x;
+}
+
 __global__ __launch_bounds__(CAL_THREADS) void cal_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                               uint64_t n16) {
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
@@ -163,12 +233,12 @@ __global__ __launch_bounds__(CAL_THREADS) void cal_copy_kernel(const uint4* __re
 // Instructions of the measured class that one wave issues for `iters` (a
 // multiple of CAL_UNROLL) iterations.
 uint64_t calib_valu_insts_per_wave(unsigned kind, uint32_t iters) {
-  static const uint32_t per_iter[5] = {16, 8, 8, 16, 16};
+  static const uint32_t per_iter[12] = {16, 8, 8, 16, 16, 16, 16, 16, 8, 16, 16, 16};
   return uint64_t(iters) * per_iter[kind];
 }
 
 double calib_valu(Ctx& ctx, unsigned kind, uint32_t iters, uint32_t blocks) {
-  if (kind > 4) throw ApiError(E_INVAL, "calibrate_valu: kind is 0..4");
+  if (kind > 11) throw ApiError(E_INVAL, "calibrate_valu: kind is 0..11");
   if (iters == 0 || iters % CAL_UNROLL) throw ApiError(E_INVAL, "calibrate_valu: iters must be a positive multiple of 4");
   if (blocks == 0 || blocks > (1u << 20)) throw ApiError(E_INVAL, "calibrate_valu: blocks must be 1..2^20");
   hipStream_t s = ctx.stream;
@@ -179,7 +249,14 @@ double calib_valu(Ctx& ctx, unsigned kind, uint32_t iters, uint32_t blocks) {
       case 1: cal_mad64_kernel<<<blocks, CAL_THREADS, 0, s>>>(n, 0x9e3779b9u, out); break;
       case 2: cal_fma64_kernel<<<blocks, CAL_THREADS, 0, s>>>(n, 5u, out); break;
       case 3: cal_mulhi_kernel<<<blocks, CAL_THREADS, 0, s>>>(n, 0xfffffff1u, out); break;
-      default: cal_add64_kernel<<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 4: cal_add64_kernel<<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 5: cal_misc_kernel<5><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 6: cal_misc_kernel<6><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 7: cal_misc_kernel<7><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 8: cal_misc_kernel<8><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 9: cal_misc_kernel<9><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 10: cal_misc_kernel<10><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      default: cal_misc_kernel<11><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
     }
     P3G_CHECK_HIP(hipGetLastError());
   };
