@@ -30,6 +30,7 @@ extern "C" {
 #define P3G_E_EMPTY 6      /* empty batch */
 #define P3G_E_TRANSCRIPT 7 /* transcript misuse */
 #define P3G_E_RANGE 8      /* index / size out of range */
+#define P3G_E_CANCELED 9   /* prove request canceled (p3g_server_cancel) */
 
 /* ---- hash and challenger kinds ---- */
 #define P3G_HASH_POSEIDON2_GL12 0 /* StatefulSponge<Poseidon2Goldilocks<12>,12,8,4> + TruncatedPermutation */
@@ -368,8 +369,12 @@ typedef struct {
   p3g_pcs_params params;
   const p3g_matrix* traces; /* ascending power-of-two heights */
   uint64_t k;
-  uint32_t flags; /* P3G_DEVICE_PTRS */
+  uint32_t flags; /* P3G_DEVICE_PTRS, P3G_INJECT_ALLOC_FAILURE */
 } p3g_request;
+/* Failure-handling test hook (request flag): the request's third device allocation fails
+ * with P3G_E_NOMEM, as if HBM ran out in mid-proof.  In a lock-step batch this fails the
+ * batch, whose members are then proven alone, so that only this request reports the error. */
+#define P3G_INJECT_ALLOC_FAILURE 8u
 typedef struct {
   int status;
   uint64_t root[4];
@@ -400,6 +405,12 @@ int p3g_server_wait(p3g_server* s, uint64_t ticket, p3g_result_info* info);     
 int p3g_server_poll(p3g_server* s, uint64_t ticket, int* done);
 int p3g_server_proof(p3g_server* s, uint64_t ticket, uint64_t* fields, uint64_t* commitments);
 int p3g_server_release(p3g_server* s, uint64_t ticket);
+/* Cancels a request: a queued one completes at once with P3G_E_CANCELED; a running one
+ * stops at its next phase boundary (after its commitment, its DEEP quotient, each FRI
+ * round, before its queries) and completes with P3G_E_CANCELED, its device memory
+ * released.  A member of a lock-step batch does not stop its batch-mates: its result is
+ * dropped when the batch ends.  A request already done keeps its result (P3G_OK either way). */
+int p3g_server_cancel(p3g_server* s, uint64_t ticket);
 int p3g_server_stats(p3g_server* s, p3g_server_counters* out);
 int p3g_server_trim(p3g_server* s);     /* frees the pool's cached (unused) device blocks */
 void p3g_server_destroy(p3g_server* s); /* finishes running requests, drops queued ones */

@@ -60,6 +60,7 @@ const char* p3g_status_string(int status) {
     case P3G_E_EMPTY: return "empty batch";
     case P3G_E_TRANSCRIPT: return "transcript error";
     case P3G_E_RANGE: return "index or size out of range";
+    case P3G_E_CANCELED: return "request canceled";
     default: return "unknown status";
   }
 }

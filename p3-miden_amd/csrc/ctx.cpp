@@ -122,7 +122,10 @@ void* SharedPool::alloc(size_t bytes, hipStream_t s) {
   bytes = size_class(bytes);
   std::lock_guard<std::mutex> g(mu);
   const auto lo = free_blocks.lower_bound(bytes);
-  const auto hi = free_blocks.upper_bound(bytes + bytes / 4);
+  // a cached block up to 25% larger serves a request; for large requests (whose
+  // hipMalloc costs milliseconds and whose sizes vary continuously between the
+  // traces of a served mix) up to twice as large
+  const auto hi = free_blocks.upper_bound(bytes >= LARGE_BLOCK_BYTES ? 2 * bytes : bytes + bytes / 4);
   auto c = pick(lo, hi, s);
   if (c != hi) return take(c, s);
   // what fits belongs to busy streams: waiting for one could hold this stream
@@ -328,6 +331,8 @@ void Ctx::free_upload_ring() {
 }
 
 void* Ctx::alloc(size_t bytes) {
+  if (fail_alloc_after >= 0 && fail_alloc_after-- == 0)
+    throw ApiError(E_NOMEM, "device allocation failed (injected for a failure-handling test)");
   if (bytes == 0) bytes = 8;
   bytes = (bytes + 255) & ~size_t(255);
   if (pool) {

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <deque>
 #include <functional>
 #include <map>
@@ -32,6 +33,7 @@ enum Status : int {
   E_EMPTY = 6,
   E_TRANSCRIPT = 7,
   E_RANGE = 8,
+  E_CANCELED = 9,
 };
 
 struct DeviceError {
@@ -74,6 +76,7 @@ inline uint64_t aligned_len(uint64_t n, uint64_t a) { return a <= 1 ? n : (n + a
 // when oversized, and only then freed.
 struct SharedPool {
   static constexpr size_t EXACT_EVENT_BYTES = size_t(4) << 20;
+  static constexpr size_t LARGE_BLOCK_BYTES = size_t(64) << 20;
   struct Block {
     void* p;
     size_t sz;
@@ -128,6 +131,15 @@ struct Ctx {
 
   // when set, device blocks come from this pool (shared with other contexts)
   SharedPool* pool = nullptr;
+  // Set by the prove-request server for the request it runs: a nonzero value
+  // makes the proof stop at its next phase boundary with E_CANCELED.
+  const std::atomic<int>* cancel = nullptr;
+  void check_cancel() const {
+    if (cancel && cancel->load(std::memory_order_relaxed)) throw ApiError(E_CANCELED, "request canceled");
+  }
+  // Fault injection for failure-handling tests: when >= 0, the allocation after
+  // that many more fails with E_NOMEM (as if HBM were exhausted), then it resets to -1.
+  int fail_alloc_after = -1;
   // pinned host staging for gathers, grown on demand; contents valid until the
   // next host_stage call on this context
   uint64_t* pinned = nullptr;

@@ -920,6 +920,7 @@ Tree* commit_traces(Ctx& ctx, uint32_t hash, const std::vector<DevMat>& traces, 
       ldes.push_back(m);
       lde_coset_batch(ctx, tr.ptr, tr.height, tr.width, log_blowup, shift, m.ptr, s);
     }
+    ctx.check_cancel();
     return lmcs_build(ctx, hash, ldes, hash_alignment(hash), s);
   } catch (...) {
     for (auto& m : ldes) ctx.release(m.ptr);

@@ -1074,6 +1074,7 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
     }
 
   // ---- DEEP: evaluations at the OOD points
+  ctx.check_cancel();
   std::vector<ef2> evals;  // [column][point]
   deep_evals(ctx, p, logL, z, npts, mats, evals, s);
   uint64_t tota = 0;
@@ -1085,6 +1086,7 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
   const ef2 beta = tx.sample_ef();
   ef2* Q = (ef2*)ctx.alloc(L * sizeof(ef2));
   deep_from_aligned(ctx, logL, z, npts, mats, awidths, aligned, tota, alpha, beta, Q, s);
+  ctx.check_cancel();
 
   // ---- FRI commit phase
   const unsigned la = p.log_arity;
@@ -1122,6 +1124,7 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
     }
     size_t r = 0;
     while (domain > fdomain) {
+      ctx.check_cancel();
       const uint64_t fd = domain >> la;
       DevMat m;
       m.ptr = reinterpret_cast<uint64_t*>(cur);
@@ -1225,6 +1228,7 @@ void open_with_channel(Ctx& ctx, const PcsParams& p, unsigned logL, const ef2* z
     cur = nullptr;
 
     // ---- query phase
+    ctx.check_cancel();
     grind(ctx, tx, p.query_pow_bits, s);
     std::vector<uint64_t> idx(p.num_queries);
     for (uint32_t q = 0; q < p.num_queries; q++) idx[q] = bitrev32((uint32_t)tx.sample_bits(logL), logL);

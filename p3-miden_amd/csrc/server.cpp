@@ -15,6 +15,7 @@
 // holds more than the device can give.  A proof depends only on its request,
 // never on what runs beside it, so each result is byte-identical to running
 // the request alone.
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <condition_variable>
@@ -46,6 +47,8 @@ struct Request {
   clk::time_point t_submit, t_admit, t_done;
   bool admitted = false, done = false;
   int status = P3G_OK;
+  std::atomic<int> cancel{0};  // set by p3g_server_cancel while the request runs
+  bool inject_alloc_failure = false;
   uint32_t worker = 0;
   uint64_t root[4] = {0, 0, 0, 0}, z[4] = {0, 0, 0, 0};
   std::vector<uint64_t> fields, comms;
@@ -130,8 +133,15 @@ void p3g_server::run_one(p3g_ctx* ctx, Request* r) {
   p3g_transcript* tx = nullptr;
   int rc = p3g_transcript_create(r->hash == P3G_HASH_POSEIDON2_GL12 ? P3G_CHALLENGER_DUPLEX_P2 : P3G_CHALLENGER_KECCAK,
                                  &tx);
+  // the proof stops at its next phase boundary once the request is canceled
+  ctx->c.cancel = &r->cancel;
+  if (r->inject_alloc_failure) ctx->c.fail_alloc_after = 2;
   if (rc == P3G_OK)
-    rc = p3g_prove(ctx, r->hash, &r->params, r->traces.data(), r->traces.size(), r->flags, tx, r->root, r->z);
+    rc = p3g_prove(ctx, r->hash, &r->params, r->traces.data(), r->traces.size(), r->flags & P3G_DEVICE_PTRS, tx,
+                   r->root, r->z);
+  ctx->c.cancel = nullptr;
+  ctx->c.fail_alloc_after = -1;
+  if (rc == P3G_OK && r->cancel.load()) rc = P3G_E_CANCELED;  // canceled after its last checkpoint
   if (rc == P3G_OK) {
     r->fields = std::move(tx->tx.fields);  // the transcript is freed below
     r->comms = std::move(tx->tx.comms);
@@ -154,20 +164,31 @@ void p3g_server::run_batch(p3g_ctx* ctx, std::vector<Request*>& batch) {
     flat.insert(flat.end(), batch[j]->traces.begin(), batch[j]->traces.end());
   }
   std::vector<uint64_t> roots(4 * k), zs(4 * k);
+  // a failure injected into one member fails the lock-step batch, as a real one would
+  for (auto* r : batch)
+    if (r->inject_alloc_failure) ctx->c.fail_alloc_after = 2;
   if (rc == P3G_OK)
-    rc = p3g_prove_many(ctx, batch[0]->hash, &batch[0]->params, flat.data(), k, t, batch[0]->flags, txs.data(),
-                        roots.data(), zs.data());
-  const std::string err = rc == P3G_OK ? std::string() : std::string(p3g_last_error());
+    rc = p3g_prove_many(ctx, batch[0]->hash, &batch[0]->params, flat.data(), k, t, batch[0]->flags & P3G_DEVICE_PTRS,
+                        txs.data(), roots.data(), zs.data());
+  ctx->c.fail_alloc_after = -1;
+  if (rc != P3G_OK) {
+    // a result depends only on its own request: when the batch fails, every member
+    // runs alone, so that only the requests that fail by themselves report an error
+    for (size_t j = 0; j < k; j++) p3g_transcript_free(txs[j]);
+    for (Request* r : batch) run_one(ctx, r);
+    return;
+  }
   for (size_t j = 0; j < k; j++) {
     Request* r = batch[j];
-    r->status = rc;
-    if (rc == P3G_OK) {
+    // a member canceled while the batch ran keeps no result (the others are unaffected)
+    r->status = r->cancel.load() ? P3G_E_CANCELED : P3G_OK;
+    if (r->status == P3G_OK) {
       memcpy(r->root, roots.data() + 4 * j, 32);
       memcpy(r->z, zs.data() + 4 * j, 32);
       r->fields = std::move(txs[j]->tx.fields);  // the transcript is freed below
       r->comms = std::move(txs[j]->tx.comms);
     } else {
-      r->error = err;
+      r->error = "request canceled";
     }
     p3g_transcript_free(txs[j]);
   }
@@ -324,6 +345,7 @@ int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket) {
   r->params = req->params;
   r->traces.assign(req->traces, req->traces + req->k);
   r->flags = req->flags;
+  r->inject_alloc_failure = (req->flags & P3G_INJECT_ALLOC_FAILURE) != 0;
   r->footprint = fp;
   {
     char buf[96];
@@ -393,6 +415,33 @@ int p3g_server_proof(p3g_server* s, uint64_t ticket, uint64_t* fields, uint64_t*
   if (r->status != P3G_OK) return r->status;
   if (fields && !r->fields.empty()) memcpy(fields, r->fields.data(), r->fields.size() * 8);
   if (commitments && !r->comms.empty()) memcpy(commitments, r->comms.data(), r->comms.size() * 8);
+  return P3G_OK;
+}
+
+int p3g_server_cancel(p3g_server* s, uint64_t ticket) {
+  if (!s) return P3G_E_INVAL;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return P3G_E_RANGE;
+    Request* r = it->second;
+    if (r->done) return P3G_OK;  // finished first: its result stands
+    if (!r->admitted) {
+      for (auto q = s->queue.begin(); q != s->queue.end(); ++q)
+        if (*q == r) {
+          s->queue.erase(q);
+          break;
+        }
+      r->status = P3G_E_CANCELED;
+      r->error = "request canceled";
+      r->done = true;
+      r->t_admit = r->t_done = clk::now();
+    } else {
+      r->cancel.store(1);
+    }
+  }
+  s->cv_done.notify_all();
+  s->cv_work.notify_all();
   return P3G_OK;
 }
 

@@ -244,6 +244,7 @@ def _load():
         "p3g_server_wait": (C.c_int, [_vp, C.c_uint64, C.POINTER(_ResultInfo)]),
         "p3g_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(C.c_int)]),
         "p3g_server_proof": (C.c_int, [_vp, C.c_uint64, _u64p, _u64p]),
+        "p3g_server_cancel": (C.c_int, [_vp, C.c_uint64]),
         "p3g_server_release": (C.c_int, [_vp, C.c_uint64]),
         "p3g_server_stats": (C.c_int, [_vp, C.POINTER(_ServerCounters)]),
         "p3g_server_destroy": (None, [_vp]),
@@ -972,9 +973,13 @@ def fill_random_device(device: Device, seed: int, n: int, ptr) -> None:
     _check(lib().p3g_fill_random_device(device.h, seed, n, ptr), "fill_random_device")
 
 
-def _request(hash_kind, params: PcsParams, traces):
+INJECT_ALLOC_FAILURE = 8  # request flag: failure-handling test hook (the request's third device allocation fails)
+
+
+def _request(hash_kind, params: PcsParams, traces, extra_flags: int = 0):
     arr, keep, dev = _matrices(traces)
-    req = _Request(hash_kind, params, C.cast(arr, C.POINTER(_Matrix)), len(traces), DEVICE_PTRS if dev else 0)
+    req = _Request(hash_kind, params, C.cast(arr, C.POINTER(_Matrix)), len(traces),
+                   (DEVICE_PTRS if dev else 0) | extra_flags)
     return req, (arr, keep, traces)
 
 
@@ -1010,10 +1015,11 @@ class Server:
         if batch_window_ms:
             _check(lib().p3g_server_set_batch_window(h, float(batch_window_ms)), "server_set_batch_window")
 
-    def submit(self, traces, params: PcsParams, hash_kind: int = HASH_POSEIDON2) -> int:
+    def submit(self, traces, params: PcsParams, hash_kind: int = HASH_POSEIDON2, inject_alloc_failure=False) -> int:
         """Queue a proof of `traces` (host arrays or DeviceMatrix); returns a ticket.
-        The arrays are kept alive here until the result is collected."""
-        req, keep = _request(hash_kind, params, traces)
+        The arrays are kept alive here until the result is collected.  inject_alloc_failure:
+        test hook making one of the request's device allocations fail (P3G_E_NOMEM)."""
+        req, keep = _request(hash_kind, params, traces, INJECT_ALLOC_FAILURE if inject_alloc_failure else 0)
         t = np.zeros(1, np.uint64)
         _check(lib().p3g_server_submit(self.h, C.byref(req), _p(t)), "server_submit")
         self._keep[int(t[0])] = keep
@@ -1037,6 +1043,11 @@ class Server:
         finally:
             lib().p3g_server_release(self.h, ticket)
             self._keep.pop(ticket, None)
+
+    def cancel(self, ticket: int):
+        """Cancel a request: queued, it completes at once; running, at its next phase boundary.
+        Its result() then raises P3GError with code 9 (P3G_E_CANCELED), unless it had already finished."""
+        _check(lib().p3g_server_cancel(self.h, ticket), "server_cancel")
 
     def stats(self) -> dict:
         s = _ServerCounters()

@@ -142,3 +142,117 @@ def test_server_batch_window_gathers_requests_arriving_together():
         root, z, f, c = G.prove(cfg, G.PcsParams(*params), [m])
         assert root.tolist() == r.root.tolist() and np.array_equal(f, r.fields) and np.array_equal(c, r.commitments)
     dev.close()
+
+
+def _check_oracle(proof, kind, params, traces):
+    root_o, z_o, f_o, c_o, _ = O.pcs_trace_prove(O.PcsParams(*params), kind, traces)
+    assert proof.root.tolist() == root_o.tolist() and proof.eval_points == tuple(z_o)
+    assert np.array_equal(proof.fields, f_o) and np.array_equal(proof.commitments, c_o)
+
+
+def _wait_running(srv, n=1, timeout_s=30.0):
+    import time
+    t0 = time.perf_counter()
+    while srv.stats()["running"] < n:
+        assert time.perf_counter() - t0 < timeout_s, "request never started"
+        time.sleep(0.0005)
+
+
+def test_canceled_requests_leave_the_others_bit_exact(device):
+    # one worker: a large proof runs (2^22 x 64 at blowup 8, about 0.2 s of GPU work) while two
+    # small ones wait behind it.  Canceling the queued one completes it at once; canceling the
+    # running one stops it at its next phase boundary; the remaining request is untouched.
+    big = device.alloc_matrix(1 << 22, 64)
+    big.fill_random(77)
+    small = [O.rand_matrix(500 + i, 1 << 10, 12) for i in range(2)]
+    params = (1, 1, 2, 0, 0, 24, 0)
+    srv = G.Server(0, workers=1, hbm_budget_bytes=64 << 30, max_batch=1)
+    t_big = srv.submit([big], G.PcsParams(3, 1, 4, 0, 0, 40, 0))
+    t_keep = srv.submit([small[0]], G.PcsParams(*params))
+    t_drop = srv.submit([small[1]], G.PcsParams(*params))
+    _wait_running(srv)
+    print("cancel: queued request, then the running 2^22 x 64 proof", flush=True)
+    srv.cancel(t_drop)
+    srv.cancel(t_big)
+    for t in (t_drop, t_big):
+        with pytest.raises(G.P3GError) as e:
+            srv.result(t)
+        assert e.value.code == 9, e.value  # P3G_E_CANCELED
+    keep = srv.result(t_keep)
+    _check_oracle(keep, G.HASH_POSEIDON2, params, [small[0]])
+    st = srv.stats()
+    assert st["running"] == 0 and st["inflight_bytes"] == 0 and st["queued"] == 0
+    # the server keeps serving after the cancellations
+    again = srv.result(srv.submit([small[1]], G.PcsParams(*params)))
+    _check_oracle(again, G.HASH_POSEIDON2, params, [small[1]])
+    srv.close()
+    big.free()
+
+
+def test_canceled_batch_member_leaves_its_batch_mates_bit_exact(device):
+    # same-shaped requests queue behind a running proof and would batch together; one is
+    # canceled while queued: the others are proven in lock-step and stay bit-exact
+    big = device.alloc_matrix(1 << 21, 64)
+    big.fill_random(78)
+    params = (1, 2, 1, 0, 0, 16, 0)
+    jobs = [[O.rand_matrix(600 + i, 1 << 11, 9)] for i in range(5)]
+    srv = G.Server(0, workers=1, hbm_budget_bytes=64 << 30, max_batch=8)
+    t_big = srv.submit([big], G.PcsParams(2, 1, 4, 0, 0, 40, 0))
+    _wait_running(srv)
+    ts = [srv.submit(j, G.PcsParams(*params)) for j in jobs]
+    srv.cancel(ts[2])
+    with pytest.raises(G.P3GError) as e:
+        srv.result(ts[2])
+    assert e.value.code == 9
+    srv.result(t_big)
+    for i, t in enumerate(ts):
+        if i != 2:
+            _check_oracle(srv.result(t), G.HASH_POSEIDON2, params, jobs[i])
+    assert srv.stats()["batches"] >= 1
+    srv.close()
+    big.free()
+
+
+def test_injected_allocation_failure_is_isolated():
+    # A request whose device allocation fails in mid-proof reports P3G_E_NOMEM; its lock-step
+    # batch-mates (same shape, proven together) and an unrelated request still finish
+    # bit-exact, and the server keeps serving.
+    params = (1, 1, 2, 0, 0, 20, 0)
+    jobs = [[O.rand_matrix(700 + i, 1 << 10, 10)] for i in range(4)]
+    other = [O.rand_matrix(800, 1 << 9, 30), O.rand_matrix(801, 1 << 11, 5)]
+    srv = G.Server(0, workers=2, hbm_budget_bytes=8 << 30, max_batch=8, batch_window_ms=200)
+    ts = [srv.submit(j, G.PcsParams(*params), inject_alloc_failure=(i == 1)) for i, j in enumerate(jobs)]
+    t_other = srv.submit(other, G.PcsParams(2, 2, 0, 0, 0, 12, 0))
+    t_alone = srv.submit([O.rand_matrix(900, 1 << 8, 7)], G.PcsParams(1, 1, 0, 0, 0, 8, 0), inject_alloc_failure=True)
+    for i, t in enumerate(ts):
+        if i == 1:
+            with pytest.raises(G.P3GError) as e:
+                srv.result(t)
+            assert e.value.code == 4, e.value  # P3G_E_NOMEM
+        else:
+            _check_oracle(srv.result(t), G.HASH_POSEIDON2, params, jobs[i])
+    with pytest.raises(G.P3GError) as e:
+        srv.result(t_alone)
+    assert e.value.code == 4
+    _check_oracle(srv.result(t_other), G.HASH_POSEIDON2, (2, 2, 0, 0, 0, 12, 0), other)
+    again = srv.result(srv.submit(jobs[1], G.PcsParams(*params)))
+    _check_oracle(again, G.HASH_POSEIDON2, params, jobs[1])
+    st = srv.stats()
+    assert st["running"] == 0 and st["inflight_bytes"] == 0
+    srv.close()
+
+
+def test_submit_rejects_null_trace_data_and_bad_params():
+    import ctypes as C
+    srv = G.Server(0, workers=1, hbm_budget_bytes=1 << 30)
+    m = (G._Matrix * 1)(G._Matrix(None, 64, 4))
+    req = G._Request(G.HASH_POSEIDON2, G.PcsParams(1, 1, 0, 0, 0, 4, 0), C.cast(m, C.POINTER(G._Matrix)), 1, 0)
+    t = np.zeros(1, np.uint64)
+    assert G.lib().p3g_server_submit(srv.h, C.byref(req), G._p(t)) == 1  # P3G_E_INVAL
+    tr = O.rand_matrix(1, 64, 4)
+    m2 = (G._Matrix * 1)(G._Matrix(tr.ctypes.data_as(G._u64p), 64, 4))
+    bad = G.PcsParams.__new__(G.PcsParams)
+    C.Structure.__init__(bad, 1, 1, 0, 0, 40, 4, 0)  # 40 DEEP grinding bits
+    req2 = G._Request(G.HASH_POSEIDON2, bad, C.cast(m2, C.POINTER(G._Matrix)), 1, 0)
+    assert G.lib().p3g_server_submit(srv.h, C.byref(req2), G._p(t)) == 1
+    srv.close()
