@@ -30,6 +30,26 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_rust_shim_declarations_match_the_header():
+    # the Rust shim's extern block (p3-miden_amd/rust/src/ffi.rs) declares only functions of
+    # include/p3gpu.h, with the header's parameter counts, and the library exports them all
+    text = open(os.path.join(ROOT, "p3-miden_amd", "rust", "src", "ffi.rs")).read()
+    block = text[text.index('unsafe extern "C" {'):]
+    block = block[:block.index("\n}\n")]
+    rust = dict(re.findall(r"pub fn (p3g_[a-z0-9_]+)\(([^)]*)\)", block, flags=re.S))
+    assert len(rust) > 15
+    header = open(os.path.join(ROOT, "include", "p3gpu.h")).read()
+    header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
+    lib = C.CDLL(G.LIB_PATH)
+    for name, args in rust.items():
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", header, flags=re.S)
+        assert m, name
+        nh = 0 if m.group(1).strip() in ("", "void") else m.group(1).count(",") + 1
+        nr = 0 if not args.strip() else args.rstrip().rstrip(",").count(",") + 1
+        assert nh == nr, (name, nh, nr)
+        assert hasattr(lib, name), name
+
+
 def test_fill_random_matches_oracle_generator():
     assert G.fill_random(2025, 1000).tolist() == O.rand_matrix(2025, 1, 1000).ravel().tolist()
 
