@@ -206,6 +206,57 @@ int p3g_commit_quotient(p3g_ctx* ctx, uint32_t hash_kind, const uint64_t* q_eval
 int p3g_quotient_extend_scale(p3g_ctx* ctx, uint64_t* acc, uint64_t m, uint64_t target, const uint64_t beta[2]);
 int p3g_quotient_divide_vanishing(p3g_ctx* ctx, uint64_t* num, uint64_t n, uint64_t d, uint64_t shift);
 
+/* ---- constraint evaluation on the quotient domain (evaluate_constraints_into,
+ * p3-miden-lifted-stark/src/prover/constraints/mod.rs:63-179) ----
+ * An AIR's constraints as a straight-line program: what a symbolic AIR builder records
+ * when the AIR's eval runs (p3-air's SymbolicAirBuilder).  Op i's value is slot i; operands
+ * name earlier slots.  Values are extension elements (base values have c1 = 0).
+ *   CONST imm | MAIN col=a, row=b (0 current, 1 next) | AUX ext column a, row b |
+ *   PERIODIC col a | PUBLIC index a | EXT_INPUT index a (extension inputs: the permutation
+ *   randomness, then the permutation values) | IS_FIRST_ROW | IS_LAST_ROW | IS_TRANSITION |
+ *   ADD a b | SUB a b | MUL a b | NEG a | ASSERT_ZERO a | ASSERT_ZERO_EXT a
+ * The constraints are the ASSERT ops in program order, C_0 .. C_{K-1}; the folded value is
+ * sum_k alpha^(K-1-k) C_k (constraints/mod.rs:53-58). */
+#define P3G_AIR_CONST 0u
+#define P3G_AIR_MAIN 1u
+#define P3G_AIR_AUX 2u
+#define P3G_AIR_PERIODIC 3u
+#define P3G_AIR_PUBLIC 4u
+#define P3G_AIR_EXT_INPUT 5u
+#define P3G_AIR_IS_FIRST_ROW 6u
+#define P3G_AIR_IS_LAST_ROW 7u
+#define P3G_AIR_IS_TRANSITION 8u
+#define P3G_AIR_ADD 9u
+#define P3G_AIR_SUB 10u
+#define P3G_AIR_MUL 11u
+#define P3G_AIR_NEG 12u
+#define P3G_AIR_ASSERT_ZERO 13u
+#define P3G_AIR_ASSERT_ZERO_EXT 14u
+typedef struct {
+  uint32_t op;
+  uint32_t a, b;
+  uint32_t reserved; /* 0 */
+  uint64_t imm;      /* P3G_AIR_CONST: a canonical value */
+} p3g_air_op;
+/* accumulator[i] += sum_k alpha^(K-1-k) C_k(x_i) at every point x_i = shift w^i of the
+ * quotient domain gJ (natural order i < n d, |gJ| = n d, d = 2^log_d), on the context's stream.
+ *  main, aux: device matrices holding a committed LDE in bit-reversed row order, at least n d
+ *    rows (the first n d rows are gJ, as Committed::evals_on_quotient_domain views them,
+ *    commit.rs:134-144); the next row of point i is point (i + d) mod n d.  aux holds 2 base
+ *    words per extension column (c0, c1); NULL when the AIR has none.
+ *  periodic: host table of the periodic columns' LDE on gJ in natural order (PeriodicLde,
+ *    periodic.rs:29-99), rows x columns, read at row i mod rows; NULL when none.
+ *  publics, ext_inputs: host values (ext_inputs as c0, c1 pairs).
+ *  Selectors are the reference's unnormalised ones at x: Z_H(x)/(x - 1), Z_H(x)/(x - w_n^-1)
+ *    and x - w_n^-1 with Z_H(x) = x^n - 1 (coset.rs:210-247).
+ *  accumulator: device, n d extension elements.  A program whose live values exceed 64
+ *    slots, or an operand out of range, gives P3G_E_RANGE / P3G_E_INVAL before any launch. */
+int p3g_eval_constraints(p3g_ctx* ctx, const p3g_air_op* ops, uint64_t nops, const p3g_matrix* main,
+                         const p3g_matrix* aux, const uint64_t* periodic, uint64_t periodic_rows,
+                         uint64_t periodic_cols, const uint64_t* publics, uint64_t num_publics,
+                         const uint64_t* ext_inputs, uint64_t num_ext_inputs, uint64_t n, uint32_t log_d,
+                         uint64_t shift, const uint64_t alpha[2], uint64_t* accumulator);
+
 /* ---- transcript (ProverChannel, p3-miden-transcript/src/prover.rs:53-145) ---- */
 int p3g_transcript_create(uint32_t challenger_kind, p3g_transcript** out);
 void p3g_transcript_free(p3g_transcript* ch);

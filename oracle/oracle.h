@@ -202,6 +202,16 @@ int orc_cyclic_extend_scale(const uint64_t* acc, size_t m, const uint64_t* beta,
 /* divide_by_vanishing_in_place (quotient.rs:72-101): n*d EF, coset shift s; -1 if Z_H vanishes */
 int orc_divide_by_vanishing(uint64_t* num, size_t n, size_t d, uint64_t shift);
 
+/* evaluate_constraints_into (p3-miden-lifted-stark/src/prover/constraints/mod.rs:63-179) for an
+ * AIR given as a straight-line program (the op list of include/p3gpu.h); main / aux are committed
+ * LDEs with bit-reversed rows (at least n*2^log_d rows), periodic a natural-order table on gJ;
+ * acc (n*2^log_d EF) += the folded constraints.  -1 on a malformed program or shape. */
+typedef struct { uint32_t op, a, b, reserved; uint64_t imm; } orc_air_op;
+int orc_eval_constraints(const orc_air_op* ops, size_t nops, const uint64_t* main, size_t main_h, size_t main_w,
+                         const uint64_t* aux, size_t aux_h, size_t aux_w, const uint64_t* periodic, size_t prow,
+                         size_t pcols, const uint64_t* publics, size_t npub, const uint64_t* ext, size_t next,
+                         size_t n, unsigned log_d, uint64_t shift, const uint64_t* alpha, uint64_t* acc);
+
 void orc_set_threads(int n);
 int orc_get_threads(void);
 

@@ -60,6 +60,10 @@ def _load():
         "orc_ef_mul": (None, [_u64p, _u64p, _u64p]),
         "orc_ef_inv": (None, [_u64p, _u64p]),
         "orc_ntt": (None, [_u64p, C.c_size_t, C.c_int]),
+        "orc_eval_constraints": (C.c_int, [C.c_void_p, C.c_size_t, _u64p, C.c_size_t, C.c_size_t, _u64p,
+                                           C.c_size_t, C.c_size_t, _u64p, C.c_size_t, C.c_size_t, _u64p,
+                                           C.c_size_t, _u64p, C.c_size_t, C.c_size_t, C.c_uint, C.c_uint64,
+                                           _u64p, _u64p]),
         "orc_coset_lde_batch": (None, [_u64p, C.c_size_t, C.c_size_t, C.c_uint, C.c_uint64, _u64p]),
         "orc_bit_reverse_rows": (None, [_u64p, C.c_size_t, C.c_size_t]),
         "orc_naive_coset_dft": (None, [_u64p, C.c_size_t, C.c_size_t, C.c_uint64, _u64p]),
@@ -464,6 +468,36 @@ def divide_by_vanishing(num: np.ndarray, d: int, shift: int) -> np.ndarray:
     if lib.orc_divide_by_vanishing(_p(a), a.shape[0] // d, d, shift) != 0:
         raise ValueError("divide_by_vanishing: bad shape or the vanishing polynomial is zero on the coset")
     return a
+
+
+# ---------------------------------------------------------------- constraint evaluation
+AIR_OP_DTYPE = np.dtype([("op", "<u4"), ("a", "<u4"), ("b", "<u4"), ("reserved", "<u4"), ("imm", "<u8")])
+
+
+def eval_constraints(ops, main, n: int, log_d: int, shift: int, alpha, acc=None, aux=None, periodic=None,
+                     publics=(), ext_inputs=()) -> np.ndarray:
+    """evaluate_constraints_into (p3-miden-lifted-stark/src/prover/constraints/mod.rs:63-179) for a
+    straight-line constraint program (structured array of AIR_OP_DTYPE).  main / aux: committed LDEs
+    with bit-reversed rows (aux columns as c0, c1 pairs); periodic: natural-order table on gJ.
+    Returns acc + the folded constraints, an (n*d, 2) array."""
+    o = np.ascontiguousarray(np.asarray(ops, dtype=AIR_OP_DTYPE))
+    m = u64(main)
+    M = n << log_d
+    out = np.zeros((M, 2), np.uint64) if acc is None else u64(acc).reshape(M, 2).copy()
+    x = u64(aux) if aux is not None else None
+    pr = u64(periodic) if periodic is not None else None
+    pub = u64(np.atleast_1d(np.asarray(publics, dtype=np.uint64)))
+    ext = u64(np.asarray(ext_inputs, dtype=np.uint64).reshape(-1))
+    al = u64(alpha)
+    rc = lib.orc_eval_constraints(o.ctypes.data_as(C.c_void_p), o.size, _p(m), m.shape[0], m.shape[1],
+                                  _p(x) if x is not None else None, x.shape[0] if x is not None else 0,
+                                  x.shape[1] if x is not None else 0,
+                                  _p(pr) if pr is not None else None, pr.shape[0] if pr is not None else 0,
+                                  pr.shape[1] if pr is not None else 0, _p(pub), len(publics), _p(ext),
+                                  ext.size // 2, n, log_d, shift, _p(al), _p(out))
+    if rc != 0:
+        raise ValueError("eval_constraints: malformed program or shape")
+    return out
 
 
 # ---------------------------------------------------------------- PCS

@@ -2,6 +2,7 @@
 // library and HIP failures and turns them into status codes; nothing throws
 // across the boundary.
 #include <array>
+#include <cstddef>
 #include <cstring>
 #include <string>
 
@@ -304,6 +305,32 @@ int p3g_coset_lde_batch_noalloc(p3g_ctx* ctx, const uint64_t* d_trace, uint64_t 
     P3G_CHECK_HIP(hipSetDevice(c.device));
     if (shift >= P) throw ApiError(P3G_E_INVAL, "shift is not a canonical field element");
     lde_coset_batch_noalloc(c, d_trace, n, w, added_bits, shift, d_out, c.stream);
+  });
+}
+
+static_assert(sizeof(p3g_air_op) == sizeof(AirOpIn) && offsetof(p3g_air_op, imm) == offsetof(AirOpIn, imm),
+              "p3g_air_op layout");
+
+int p3g_eval_constraints(p3g_ctx* ctx, const p3g_air_op* ops, uint64_t nops, const p3g_matrix* main,
+                         const p3g_matrix* aux, const uint64_t* periodic, uint64_t periodic_rows,
+                         uint64_t periodic_cols, const uint64_t* publics, uint64_t num_publics,
+                         const uint64_t* ext_inputs, uint64_t num_ext_inputs, uint64_t n, uint32_t log_d,
+                         uint64_t shift, const uint64_t alpha[2], uint64_t* accumulator) {
+  if (!ctx || !ops || !main || !alpha || !accumulator) return P3G_E_INVAL;
+  if ((num_publics && !publics) || (num_ext_inputs && !ext_inputs) || (periodic_rows && periodic_cols && !periodic))
+    return P3G_E_INVAL;
+  return guard([&] {
+    Ctx& c = ctx->c;
+    P3G_CHECK_HIP(hipSetDevice(c.device));
+    if (alpha[0] >= P || alpha[1] >= P) throw ApiError(P3G_E_INVAL, "non-canonical alpha");
+    DevMat m{const_cast<uint64_t*>(main->data), main->height, main->width, false};
+    DevMat x{};
+    if (aux) x = DevMat{const_cast<uint64_t*>(aux->data), aux->height, aux->width, false};
+    std::vector<ef2> ext(num_ext_inputs);
+    for (uint64_t i = 0; i < num_ext_inputs; i++) ext[i] = ef2{ext_inputs[2 * i], ext_inputs[2 * i + 1]};
+    eval_constraints(c, reinterpret_cast<const AirOpIn*>(ops), nops, m, aux ? &x : nullptr, periodic, periodic_rows,
+                     periodic_cols, publics, num_publics, ext.data(), num_ext_inputs, n, log_d, shift,
+                     ef2{alpha[0], alpha[1]}, reinterpret_cast<ef2*>(accumulator), c.stream);
   });
 }
 

@@ -377,6 +377,17 @@ void fri_final_poly(const ef2* bitrev_evals, size_t n, ef2* coeffs_desc);
 // f(0..n-1) on the library's host worker threads (serially when the pool is busy)
 void parallel_for(size_t n, const std::function<void(size_t)>& f);
 
+// Constraint evaluation on the quotient domain (evaluate_constraints_into): one
+// straight-line program op, laid out as the C ABI's p3g_air_op
+struct AirOpIn {
+  uint32_t op, a, b, reserved;
+  uint64_t imm;
+};
+void eval_constraints(Ctx& ctx, const AirOpIn* ops, uint64_t nops, const DevMat& main, const DevMat* aux,
+                      const uint64_t* periodic, uint64_t prow, uint64_t pcols, const uint64_t* publics, uint64_t npub,
+                      const ef2* ext_inputs, uint64_t next, uint64_t n, unsigned log_d, uint64_t shift, ef2 alpha,
+                      ef2* acc, hipStream_t s);
+
 // k same-shaped proofs in lock-step (the p3g_prove flow per job)
 void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<std::vector<DevMat>>& traces,
                 const std::vector<Transcript*>& txs, std::vector<std::array<uint64_t, 4>>& roots,

@@ -178,6 +178,9 @@ def _load():
         "p3g_fill_random": (C.c_int, [C.c_uint64, C.c_uint64, _u64p]),
         "p3g_coset_lde_batch": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, _u64p]),
         "p3g_coset_lde_batch_noalloc": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, _vp]),
+        "p3g_eval_constraints": (C.c_int, [_vp, _vp, C.c_uint64, C.POINTER(_Matrix), C.POINTER(_Matrix), _u64p,
+                                           C.c_uint64, C.c_uint64, _u64p, C.c_uint64, _u64p, C.c_uint64,
+                                           C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _vp]),
         "p3g_lmcs_alignment": (C.c_uint32, [C.c_uint32]),
         "p3g_lmcs_build": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
         "p3g_lmcs_hash": (C.c_int, [C.c_uint32, _u64p, _u64p, C.c_uint64, _u64p]),
@@ -1087,3 +1090,31 @@ class PinnedArray:
 
     def __del__(self):
         self.free()
+
+
+from .air import AirProgram  # noqa: E402
+
+
+def eval_constraints(program, main: "DeviceMatrix", n: int, log_d: int, shift: int, alpha, accumulator: "DeviceMatrix",
+                     aux: "DeviceMatrix" = None, periodic=None, publics=(), ext_inputs=()) -> None:
+    """evaluate_constraints_into (p3-miden-lifted-stark/src/prover/constraints/mod.rs:63-179) on the GPU:
+    accumulator (n*2^log_d extension elements, an (n*d) x 2 DeviceMatrix) += the program's constraints
+    folded with powers of alpha, at every point of the quotient domain shift*<w_{n d}>.  main / aux are
+    committed LDEs with bit-reversed rows (a tree's matrix, or any device matrix of that layout); periodic
+    is the host table of the periodic columns' LDE on gJ (natural order, PeriodicLde)."""
+    ops = program.ops if isinstance(program, AirProgram) else np.asarray(program)
+    ops = np.ascontiguousarray(ops)
+    if accumulator.height * accumulator.width < 2 * (n << log_d):
+        raise ValueError("accumulator smaller than n*d extension elements")
+    dev = main.dev
+    mm = _Matrix(C.cast(main.ptr, _u64p), main.height, main.width)
+    am = _Matrix(C.cast(aux.ptr, _u64p), aux.height, aux.width) if aux is not None else None
+    per = u64(periodic) if periodic is not None else None
+    pub = u64(np.atleast_1d(np.asarray(publics, dtype=np.uint64)))
+    ext = u64(np.asarray(ext_inputs, dtype=np.uint64).reshape(-1))
+    al = u64(alpha)
+    _check(lib().p3g_eval_constraints(dev.h, ops.ctypes.data_as(_vp), ops.size, C.byref(mm),
+                                      C.byref(am) if am is not None else None,
+                                      _p(per) if per is not None else None, per.shape[0] if per is not None else 0,
+                                      per.shape[1] if per is not None else 0, _p(pub), len(publics), _p(ext),
+                                      ext.size // 2, n, log_d, shift, _p(al), accumulator.ptr), "eval_constraints")

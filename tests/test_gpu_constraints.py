@@ -1,0 +1,151 @@
+"""Constraint evaluation on the GPU (p3g_eval_constraints) against the oracle's
+restatement of evaluate_constraints_into (p3-miden-lifted-stark/src/prover/
+constraints/mod.rs:63-179), word for word:
+
+  - three AIRs (Fibonacci with public values, the reference's DummyMidenAir, a
+    running sum over an extension aux column with periodic and permutation
+    inputs), several trace heights and quotient degrees, a nonzero accumulator;
+  - reading main and aux in place from committed trees (the bit-reversed LDEs
+    of commit_traces), then the rest of the quotient pipeline of prove_multi
+    (prover/mod.rs:370-432): divide_by_vanishing and commit_quotient, whose root
+    must equal the oracle's;
+  - the DummyMidenAir at the reference binary's shape (2^19 x 20, log_blowup 3,
+    quotient degree 8: 2^22 points);
+  - malformed programs and shapes rejected before anything is launched.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import p3gpu as G
+import airs
+
+pytestmark = pytest.mark.gpu
+
+P = airs.P
+G7 = 7
+
+
+def _run_both(device, prog, main, n, logb, log_d, alpha, aux=None, periodic=None, publics=(), ext=(), seed=0):
+    M = n << log_d
+    rng = np.random.default_rng(seed)
+    acc0 = rng.integers(0, P, size=(M, 2), dtype=np.uint64)
+    main_lde = O.coset_lde_batch(main, logb, G7)
+    aux_lde = O.coset_lde_batch(aux, logb, G7) if aux is not None else None
+    want = O.eval_constraints(prog.ops, main_lde, n, log_d, G7, alpha, acc=acc0, aux=aux_lde, periodic=periodic,
+                              publics=publics, ext_inputs=ext)
+    dm = device.upload(main_lde)
+    da = device.upload(aux_lde) if aux_lde is not None else None
+    acc = device.upload(acc0)
+    print(f"eval_constraints: n={n} log_blowup={logb} log_d={log_d} ops={len(prog.ops)}", flush=True)
+    G.eval_constraints(prog, dm, n, log_d, G7, alpha, acc, aux=da, periodic=periodic, publics=publics,
+                       ext_inputs=ext)
+    got = acc.download()
+    for m in (dm, da, acc):
+        if m is not None:
+            m.free()
+    return got, want
+
+
+@pytest.mark.parametrize("n,logb,log_d", [(16, 1, 1), (256, 2, 1), (1 << 12, 3, 2)])
+def test_fibonacci_matches_oracle(device, n, logb, log_d):
+    trace, pub = airs.fibonacci_trace(n, 3, 4)
+    got, want = _run_both(device, airs.fibonacci_program(), trace, n, logb, log_d, (11, 12), publics=pub, seed=n)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n,width,logb,log_d", [(64, 9, 3, 3), (1 << 10, 20, 3, 3), (256, 51, 4, 4)])
+def test_miden_dummy_matches_oracle(device, n, width, logb, log_d):
+    trace = airs.miden_trace(n, width, width)
+    got, want = _run_both(device, airs.miden_program(), trace, n, logb, log_d, (7, 9), seed=width)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n,logb", [(16, 1), (1 << 11, 2)])
+def test_running_sum_matches_oracle(device, n, logb):
+    r = (99, 100)
+    main, aux, final = airs.running_trace(n, r, 5)
+    per = np.array(airs.PERIODIC, dtype=np.uint64).reshape(4, 1)
+    table = O.coset_lde_batch(per, 1, pow(G7, n // 4, P), bit_reverse=False)
+    got, want = _run_both(device, airs.running_program(), main, n, logb, 1, (13, 14), aux=aux, periodic=table,
+                          ext=[r[0], r[1], final[0], final[1]], seed=3)
+    assert np.array_equal(got, want)
+
+
+def test_quotient_pipeline_from_committed_trees(device):
+    # prove_multi's quotient steps on the GPU: commit the trace, evaluate the constraints on
+    # the committed LDE in place, divide by Z_H, commit the quotient; the root must equal the
+    # oracle's for the same steps
+    n, logb, log_d = 1 << 10, 2, 1
+    trace, pub = airs.fibonacci_trace(n, 1, 2)
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    tree = G.commit_traces(cfg, [trace], logb)
+    ptr, h, w = tree.matrix(0)
+    main = G.DeviceMatrix(device, ptr, h, w, owned=False)
+    acc = device.alloc_matrix(n, 2 << log_d)  # n*d extension elements, read by commit_quotient as n x 2d
+    G.cyclic_extend_and_scale(acc, 0, n << log_d, (0, 0))  # an empty accumulator becomes zeros
+    alpha = (17, 19)
+    G.eval_constraints(airs.fibonacci_program(), main, n, log_d, G7, alpha, acc, publics=pub)
+    G.divide_by_vanishing(acc, n, 1 << log_d, G7)
+    qt = G.commit_quotient(cfg, acc, 1 << log_d, logb)
+    num = O.eval_constraints(airs.fibonacci_program().ops, O.coset_lde_batch(trace, logb, G7), n, log_d, G7, alpha,
+                             publics=pub)
+    q = O.divide_by_vanishing(num, 1 << log_d, G7)
+    assert np.array_equal(acc.download().reshape(-1, 2), q)
+    assert qt.root().tolist() == O.commit_quotient(O.HASH_P2, q, 1 << log_d, logb).root().tolist()
+    qt.free()
+    acc.free()
+    tree.free()
+
+
+def test_miden_dummy_at_the_reference_binary_shape(device):
+    # lifted_miden.rs: trace 2 is 2^19 x 20 with log_blowup 3 and quotient degree 8 (2^22 points)
+    n, width, logb, log_d = 1 << 19, 20, 3, 3
+    trace = airs.miden_trace(n, width, 42)
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    tree = G.commit_traces(cfg, [trace], logb)
+    ptr, h, w = tree.matrix(0)
+    main = G.DeviceMatrix(device, ptr, h, w, owned=False)
+    acc = device.alloc_matrix(n << log_d, 2)
+    G.cyclic_extend_and_scale(acc, 0, n << log_d, (0, 0))
+    print("eval_constraints: DummyMidenAir 2^19 x 20, 2^22 points", flush=True)
+    G.eval_constraints(airs.miden_program(), main, n, log_d, G7, (5, 8), acc)
+    got = acc.download().reshape(-1, 2)
+    want = O.eval_constraints(airs.miden_program().ops, main.download(), n, log_d, G7, (5, 8))
+    assert np.array_equal(got, want)
+    acc.free()
+    tree.free()
+
+
+def test_bad_programs_and_shapes_are_rejected(device):
+    n = 64
+    main = device.alloc_matrix(2 * n, 2)
+    main.fill_random(1)
+    acc = device.alloc_matrix(4 * n, 2)  # room for n d points at d = 4 (the short-main case below)
+    ops = airs.fibonacci_program().ops
+    bad = ops.copy()
+    bad[9]["b"] = 50  # a product that reads a value defined later
+    for prog, kw, code in [
+        (bad, dict(publics=[0, 0, 0]), 1),
+        (np.concatenate([ops[:1], ops[1:2]]).copy(), dict(), 6),  # asserts nothing
+        (ops, dict(publics=[0]), 1),                       # a public value that is not supplied
+    ]:
+        with pytest.raises(G.P3GError) as e:
+            G.eval_constraints(prog, main, n, 1, G7, (1, 0), acc, **kw)
+        assert e.value.code == code, (code, e.value)
+    # more than 64 values alive at once
+    p = G.AirProgram()
+    vals = [p.main(0) + k for k in range(70)]
+    total = vals[0]
+    for v in vals[1:]:
+        total = total * v
+    p.assert_zero(total)
+    with pytest.raises(G.P3GError) as e:
+        G.eval_constraints(p, main, n, 1, G7, (1, 0), acc)
+    assert e.value.code == 8
+    # the main matrix must hold the n d points
+    with pytest.raises(G.P3GError) as e:
+        G.eval_constraints(ops, main, n, 2, G7, (1, 0), acc, publics=[0, 0, 0])
+    assert e.value.code == 1
+    main.free()
+    acc.free()
