@@ -249,13 +249,22 @@ typedef struct {
  *  publics, ext_inputs: host values (ext_inputs as c0, c1 pairs).
  *  Selectors are the reference's unnormalised ones at x: Z_H(x)/(x - 1), Z_H(x)/(x - w_n^-1)
  *    and x - w_n^-1 with Z_H(x) = x^n - 1 (coset.rs:210-247).
- *  accumulator: device, n d extension elements.  A program whose live values exceed 64
- *    slots, or an operand out of range, gives P3G_E_RANGE / P3G_E_INVAL before any launch. */
+ *  shift: outside the subgroup of order n d (P3G_E_INVAL otherwise: Z_H would vanish).
+ *  accumulator: device, n d extension elements.
+ * Each program is compiled once into its own gfx950 kernel (values typed base or extension,
+ * kept in registers) and cached for the process; a malformed program or an index out of
+ * range gives P3G_E_INVAL, an empty or assertion-free one P3G_E_EMPTY, more than 2^20 ops
+ * P3G_E_RANGE, all before any launch. */
 int p3g_eval_constraints(p3g_ctx* ctx, const p3g_air_op* ops, uint64_t nops, const p3g_matrix* main,
                          const p3g_matrix* aux, const uint64_t* periodic, uint64_t periodic_rows,
                          uint64_t periodic_cols, const uint64_t* publics, uint64_t num_publics,
                          const uint64_t* ext_inputs, uint64_t num_ext_inputs, uint64_t n, uint32_t log_d,
                          uint64_t shift, const uint64_t alpha[2], uint64_t* accumulator);
+/* Check a constraint program and compile its kernel ahead of the first p3g_eval_constraints
+ * (the first call otherwise pays the compile, ~0.3-2 s); needs no device.  *code_bytes is
+ * the size of the gfx950 code object.  No counterpart in the reference (its AIRs are
+ * compiled with the prover). */
+int p3g_air_compile(const p3g_air_op* ops, uint64_t nops, uint64_t* code_bytes);
 
 /* ---- transcript (ProverChannel, p3-miden-transcript/src/prover.rs:53-145) ---- */
 int p3g_transcript_create(uint32_t challenger_kind, p3g_transcript** out);

@@ -37,23 +37,32 @@ int orc_eval_constraints(const orc_air_op* ops, size_t nops, const uint64_t* mai
     const unsigned logn = log2_exact(n), logM = logn + log_d;
     const size_t M = (size_t)1 << logM, d = (size_t)1 << log_d;
     if (main_h < M || (aux && aux_h < M)) return -1;
-    /* check the program and count the constraints */
+    /* the coset must miss the subgroup of order M, where Z_H and the selector
+       denominators vanish */
+    if (shift >= GL_P || gl_pow(shift, M) == 1) return -1;
+    /* check the program and count the constraints; operands name earlier values */
     size_t K = 0;
+    unsigned char* is_assert = (unsigned char*)calloc(nops, 1);
+#define OPERAND(v) if ((v) >= k || is_assert[(v)]) { free(is_assert); return -1; }
     for (size_t k = 0; k < nops; k++) {
         const orc_air_op* o = &ops[k];
+        is_assert[k] = o->op == A_ASSERT || o->op == A_ASSERT_EXT;
         switch (o->op) {
-            case A_MAIN: if (o->a >= main_w || o->b > 1) return -1; break;
-            case A_AUX: if (!aux || 2 * (size_t)o->a + 1 >= aux_w || o->b > 1) return -1; break;
-            case A_PERIODIC: if (!periodic || o->a >= pcols || prow == 0) return -1; break;
-            case A_PUBLIC: if (o->a >= npub) return -1; break;
-            case A_EXT: if (o->a >= next) return -1; break;
-            case A_ADD: case A_SUB: case A_MUL: if (o->a >= k || o->b >= k) return -1; break;
-            case A_NEG: if (o->a >= k) return -1; break;
-            case A_ASSERT: case A_ASSERT_EXT: if (o->a >= k) return -1; K++; break;
-            case A_CONST: case A_FIRST: case A_LAST: case A_TRANS: break;
-            default: return -1;
+            case A_MAIN: if (o->a >= main_w || o->b > 1) { free(is_assert); return -1; } break;
+            case A_AUX: if (!aux || 2 * (size_t)o->a + 1 >= aux_w || o->b > 1) { free(is_assert); return -1; } break;
+            case A_PERIODIC: if (!periodic || o->a >= pcols || prow == 0) { free(is_assert); return -1; } break;
+            case A_PUBLIC: if (o->a >= npub) { free(is_assert); return -1; } break;
+            case A_EXT: if (o->a >= next) { free(is_assert); return -1; } break;
+            case A_ADD: case A_SUB: case A_MUL: OPERAND(o->a) OPERAND(o->b) break;
+            case A_NEG: OPERAND(o->a) break;
+            case A_ASSERT: case A_ASSERT_EXT: OPERAND(o->a) K++; break;
+            case A_CONST: if (o->imm >= GL_P) { free(is_assert); return -1; } break;
+            case A_FIRST: case A_LAST: case A_TRANS: break;
+            default: free(is_assert); return -1;
         }
     }
+#undef OPERAND
+    free(is_assert);
     if (K == 0) return -1;
     /* alpha^(K-1-k) for the k-th constraint */
     const ef2 al = ef_make(alpha[0], alpha[1]);

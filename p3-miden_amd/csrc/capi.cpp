@@ -334,6 +334,11 @@ int p3g_eval_constraints(p3g_ctx* ctx, const p3g_air_op* ops, uint64_t nops, con
   });
 }
 
+int p3g_air_compile(const p3g_air_op* ops, uint64_t nops, uint64_t* code_bytes) {
+  if (!ops || !code_bytes) return P3G_E_INVAL;
+  return guard([&] { *code_bytes = air_compile(reinterpret_cast<const AirOpIn*>(ops), nops); });
+}
+
 uint32_t p3g_lmcs_alignment(uint32_t hash_kind) { return hash_alignment(hash_kind); }
 
 static std::vector<DevMat> import_mats(Ctx& c, const p3g_matrix* mats, uint64_t k, uint32_t flags) {

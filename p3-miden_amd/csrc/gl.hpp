@@ -6,8 +6,16 @@
 // All values are kept canonical (< p) so that every stored word equals the
 // reference's canonical serialisation.
 #pragma once
+#if defined(__HIPCC_RTC__)
+// compiled at run time (hiprtc): no system headers; the types as <stdint.h> has them
+typedef unsigned long uint64_t;
+typedef unsigned int uint32_t;
+typedef long int64_t;
+typedef int int32_t;
+#else
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #define P3G_HD __host__ __device__ __forceinline__
 
@@ -171,7 +179,26 @@ P3G_HD uint64_t gl_pow(uint64_t a, uint64_t e) {
   return r;
 }
 
-P3G_HD uint64_t gl_inv(uint64_t a) { return gl_pow(a, P - 2); }
+// a^k squared s times, times b
+P3G_HD uint64_t gl_sqr_n_mul(uint64_t a, unsigned s, uint64_t b) {
+  for (unsigned i = 0; i < s; i++) a = gl_mul(a, a);
+  return gl_mul(a, b);
+}
+
+// a^(p-2) (0 for 0) by an addition chain: with t_k = a^(2^k - 1),
+// p - 2 = (2^31 - 1) 2^33 + (2^32 - 1), so a^(p-2) = t_31^(2^33) t_32.
+// 64 squarings and 9 products (square-and-multiply takes 63 and 62).
+P3G_HD uint64_t gl_inv(uint64_t a) {
+  const uint64_t t2 = gl_sqr_n_mul(a, 1, a);
+  const uint64_t t3 = gl_sqr_n_mul(t2, 1, a);
+  const uint64_t t6 = gl_sqr_n_mul(t3, 3, t3);
+  const uint64_t t12 = gl_sqr_n_mul(t6, 6, t6);
+  const uint64_t t24 = gl_sqr_n_mul(t12, 12, t12);
+  const uint64_t t30 = gl_sqr_n_mul(t24, 6, t6);
+  const uint64_t t31 = gl_sqr_n_mul(t30, 1, a);
+  const uint64_t t32 = gl_sqr_n_mul(t31, 1, a);
+  return gl_sqr_n_mul(t31, 33, t32);
+}
 
 P3G_HD uint64_t gl_two_adic_generator(unsigned bits) {
   uint64_t g = OMEGA32;

@@ -387,6 +387,8 @@ void eval_constraints(Ctx& ctx, const AirOpIn* ops, uint64_t nops, const DevMat&
                       const uint64_t* periodic, uint64_t prow, uint64_t pcols, const uint64_t* publics, uint64_t npub,
                       const ef2* ext_inputs, uint64_t next, uint64_t n, unsigned log_d, uint64_t shift, ef2 alpha,
                       ef2* acc, hipStream_t s);
+// checks a program and compiles its kernel into the process cache; the code object's size
+uint64_t air_compile(const AirOpIn* ops, uint64_t nops);
 
 // k same-shaped proofs in lock-step (the p3g_prove flow per job)
 void prove_many(Ctx& ctx, uint32_t hash, const PcsParams& p, const std::vector<std::vector<DevMat>>& traces,

@@ -181,6 +181,7 @@ def _load():
         "p3g_eval_constraints": (C.c_int, [_vp, _vp, C.c_uint64, C.POINTER(_Matrix), C.POINTER(_Matrix), _u64p,
                                            C.c_uint64, C.c_uint64, _u64p, C.c_uint64, _u64p, C.c_uint64,
                                            C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _vp]),
+        "p3g_air_compile": (C.c_int, [_vp, C.c_uint64, _u64p]),
         "p3g_lmcs_alignment": (C.c_uint32, [C.c_uint32]),
         "p3g_lmcs_build": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
         "p3g_lmcs_hash": (C.c_int, [C.c_uint32, _u64p, _u64p, C.c_uint64, _u64p]),
@@ -1092,7 +1093,23 @@ class PinnedArray:
         self.free()
 
 
-from .air import AirProgram  # noqa: E402
+from .air import AIR_OP_DTYPE, AirProgram  # noqa: E402
+
+
+def _air_ops(program) -> np.ndarray:
+    ops = program.ops if isinstance(program, AirProgram) else np.asarray(program)
+    if ops.dtype != AIR_OP_DTYPE:
+        raise TypeError("a constraint program is an AirProgram or an array of AIR_OP_DTYPE")
+    return np.ascontiguousarray(ops)
+
+
+def air_compile(program) -> int:
+    """Check a constraint program and compile its kernel into the process cache ahead of
+    eval_constraints (needs no GPU); returns the size of the gfx950 code object."""
+    ops = _air_ops(program)
+    size = C.c_uint64(0)
+    _check(lib().p3g_air_compile(ops.ctypes.data_as(_vp), ops.size, C.byref(size)), "air_compile")
+    return size.value
 
 
 def eval_constraints(program, main: "DeviceMatrix", n: int, log_d: int, shift: int, alpha, accumulator: "DeviceMatrix",
@@ -1102,8 +1119,7 @@ def eval_constraints(program, main: "DeviceMatrix", n: int, log_d: int, shift: i
     folded with powers of alpha, at every point of the quotient domain shift*<w_{n d}>.  main / aux are
     committed LDEs with bit-reversed rows (a tree's matrix, or any device matrix of that layout); periodic
     is the host table of the periodic columns' LDE on gJ (natural order, PeriodicLde)."""
-    ops = program.ops if isinstance(program, AirProgram) else np.asarray(program)
-    ops = np.ascontiguousarray(ops)
+    ops = _air_ops(program)
     if accumulator.height * accumulator.width < 2 * (n << log_d):
         raise ValueError("accumulator smaller than n*d extension elements")
     dev = main.dev

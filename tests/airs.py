@@ -88,3 +88,53 @@ def running_trace(n, r, seed):
         s = ((s[0] + rv[0]) % P, (s[1] + rv[1]) % P)
         aux[i] = s
     return main, aux, s
+
+
+def random_program(seed, nops, main_w, aux_cols=0, npub=0, n_ext=0, per_cols=0, selectors=True):
+    """A random straight-line program over every kind of input: leaves (main and aux
+    columns on both rows, periodic columns, public values, extension inputs, constants,
+    selectors) and ADD/SUB/MUL/NEG of earlier values (mixed base and extension operands),
+    with an assertion about every eighth op."""
+    rng = np.random.default_rng(seed)
+    p = G.AirProgram()
+    vals = []
+    leaves = [lambda: p.main(int(rng.integers(main_w)), next=bool(rng.integers(2))),
+              lambda: p.const(int(rng.integers(0, P, dtype=np.uint64)))]
+    if aux_cols:
+        leaves.append(lambda: p.aux(int(rng.integers(aux_cols)), next=bool(rng.integers(2))))
+    if npub:
+        leaves.append(lambda: p.public(int(rng.integers(npub))))
+    if n_ext:
+        leaves.append(lambda: p.ext_input(int(rng.integers(n_ext))))
+    if per_cols:
+        leaves.append(lambda: p.periodic(int(rng.integers(per_cols))))
+    if selectors:
+        leaves += [p.is_first_row, p.is_last_row, p.is_transition]
+    while len(p._ops) < nops:
+        if len(vals) < 4 or rng.random() < 0.3:
+            vals.append(leaves[int(rng.integers(len(leaves)))]())
+            continue
+        # operands mostly from the last 32 values (as an AIR's constraints use their own
+        # subexpressions), one in twenty from anywhere earlier (long live ranges)
+        def pick():
+            far = rng.random() < 0.05
+            return vals[int(rng.integers(len(vals)))] if far else vals[-1 - int(rng.integers(min(32, len(vals))))]
+        a, b = pick(), pick()
+        kind = int(rng.integers(4))
+        vals.append(a + b if kind == 0 else a - b if kind == 1 else a * b if kind == 2 else -a)
+        if rng.random() < 0.125:
+            (p.assert_zero if rng.integers(2) else p.assert_zero_ext)(vals[-1])
+    p.assert_zero(vals[-1])
+    return p
+
+
+def wide_program(live):
+    """`live` values alive at once: a product over main(0) + k for k < live, formed
+    only after every factor exists."""
+    p = G.AirProgram()
+    vals = [p.main(0) + k for k in range(live)]
+    total = vals[0]
+    for v in vals[1:]:
+        total = total * v
+    p.assert_zero(total)
+    return p

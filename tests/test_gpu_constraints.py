@@ -117,6 +117,34 @@ def test_miden_dummy_at_the_reference_binary_shape(device):
     tree.free()
 
 
+@pytest.mark.parametrize("seed,n,logb,log_d,nops", [(1, 64, 2, 2, 400), (2, 1 << 10, 1, 1, 1500), (3, 4, 1, 0, 200),
+                                                     (4, 8, 1, 1, 300), (5, 2, 3, 3, 300), (6, 1 << 9, 3, 3, 3000)])
+def test_random_programs_match_oracle(device, seed, n, logb, log_d, nops):
+    # every input kind, mixed base/extension arithmetic, long live ranges; the domains
+    # run from 4 points (smaller than one 32 x 32 tile) to 2^12
+    main_w, aux_w, npub, n_ext, per_cols = 7, 4, 3, 2, 2
+    prog = airs.random_program(seed, nops, main_w, aux_w // 2, npub, n_ext, per_cols)
+    rng = np.random.default_rng(seed)
+    main = rng.integers(0, P, size=(n, main_w), dtype=np.uint64)
+    aux = rng.integers(0, P, size=(n, aux_w), dtype=np.uint64)
+    prow = max(1, min(n, 4))
+    per = rng.integers(0, P, size=(prow, per_cols), dtype=np.uint64)
+    table = O.coset_lde_batch(per, log_d, pow(G7, n // prow, P), bit_reverse=False)
+    pub = [int(v) for v in rng.integers(0, P, size=npub, dtype=np.uint64)]
+    ext = [int(v) for v in rng.integers(0, P, size=2 * n_ext, dtype=np.uint64)]
+    got, want = _run_both(device, prog, main, n, logb, log_d, tuple(int(v) for v in rng.integers(0, P, size=2, dtype=np.uint64)), aux=aux,
+                          periodic=table, publics=pub, ext=ext, seed=seed)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("live", [70, 300])
+def test_many_live_values_match_oracle(device, live):
+    n = 256
+    trace = np.random.default_rng(live).integers(0, P, size=(n, 3), dtype=np.uint64)
+    got, want = _run_both(device, airs.wide_program(live), trace, n, 2, 2, (3, 4), seed=live)
+    assert np.array_equal(got, want)
+
+
 def test_bad_programs_and_shapes_are_rejected(device):
     n = 64
     main = device.alloc_matrix(2 * n, 2)
@@ -133,16 +161,22 @@ def test_bad_programs_and_shapes_are_rejected(device):
         with pytest.raises(G.P3GError) as e:
             G.eval_constraints(prog, main, n, 1, G7, (1, 0), acc, **kw)
         assert e.value.code == code, (code, e.value)
-    # more than 64 values alive at once
-    p = G.AirProgram()
-    vals = [p.main(0) + k for k in range(70)]
-    total = vals[0]
-    for v in vals[1:]:
-        total = total * v
-    p.assert_zero(total)
+    # an operand that names an assertion
+    bad = ops.copy()
+    bad[12]["b"] = 10
+    assert bad[10]["op"] == G.air.ASSERT_ZERO and bad[12]["op"] == G.air.SUB
     with pytest.raises(G.P3GError) as e:
-        G.eval_constraints(p, main, n, 1, G7, (1, 0), acc)
-    assert e.value.code == 8
+        G.eval_constraints(bad, main, n, 1, G7, (1, 0), acc, publics=[0, 0, 0])
+    assert e.value.code == 1
+    # a shift inside the subgroup of order n d (Z_H vanishes there)
+    for shift in (1, pow(G7, (P - 1) // 2, P)):
+        with pytest.raises(G.P3GError) as e:
+            G.eval_constraints(ops, main, n, 1, shift, (1, 0), acc, publics=[0, 0, 0])
+        assert e.value.code == 1
+    # inputs the program reads but the call does not supply
+    with pytest.raises(G.P3GError) as e:
+        G.eval_constraints(airs.running_program(), main, n, 1, G7, (1, 0), acc, ext_inputs=[1, 2, 3, 4])
+    assert e.value.code == 1  # no aux matrix
     # the main matrix must hold the n d points
     with pytest.raises(G.P3GError) as e:
         G.eval_constraints(ops, main, n, 2, G7, (1, 0), acc, publics=[0, 0, 0])

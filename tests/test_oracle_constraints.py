@@ -144,3 +144,10 @@ def test_malformed_programs_are_rejected():
         O.eval_constraints(ops, lde, 8, 1, G7, (1, 0), publics=[0, 0, 0])
     with pytest.raises(ValueError):  # a public value that is not supplied
         O.eval_constraints(p.ops, lde, 8, 1, G7, (1, 0), publics=[0])
+    ops = p.ops.copy()
+    ops[12]["b"] = 10  # an operand that is an assertion
+    with pytest.raises(ValueError):
+        O.eval_constraints(ops, lde, 8, 1, G7, (1, 0), publics=[0, 0, 0])
+    for shift in (1, P - 1):  # inside the subgroup of order n d: Z_H vanishes on the coset
+        with pytest.raises(ValueError):
+            O.eval_constraints(p.ops, lde, 8, 1, shift, (1, 0), publics=[0, 0, 0])
