@@ -614,6 +614,121 @@ def bench_quotient(G, dev, steps, warmup):
             "runs": out}
 
 
+# ------------------------------------------------------------------ constraint evaluation
+def air_bytes_per_point(ops):
+    """HBM bytes one point must move at least: every distinct main column it reads (8 B; a
+    next-row value is another point's current row), every distinct aux extension column
+    (16 B), and the accumulator read and written (32 B)."""
+    from p3gpu import air
+    main = {int(o["a"]) for o in ops if o["op"] == air.MAIN}
+    aux = {int(o["a"]) for o in ops if o["op"] == air.AUX}
+    return 8 * len(main) + 16 * len(aux) + 32
+
+
+def bench_constraints(G, dev, steps, warmup, cpu=True):
+    """evaluate_constraints_into (constraints/mod.rs:63-179) on committed LDEs resident in HBM,
+    one kernel per AIR compiled at run time:
+      - DummyMidenAir at the reference binary's shape (lifted_miden.rs: 2^19 x 20, log_blowup 3,
+        quotient degree 8: 2^22 points), with the oracle on the host cores beside it;
+      - Fibonacci with public values (selectors) and the running-sum AIR (aux extension column,
+        periodic column, extension inputs) at 2^22 points;
+      - a 3000-op random program (every input kind) at 2^20 points, bound by arithmetic."""
+    from p3gpu import example_airs as A
+    cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
+    rng = np.random.default_rng(SEED)
+    out = []
+
+    def case(name, prog, trace, logb, log_d, aux=None, cpu_check=False, **kw):
+        n = trace.shape[0]
+        t0 = time.perf_counter()
+        G.air_compile(prog)
+        compile_s = time.perf_counter() - t0
+        tree = G.commit_traces(cfg, [trace], logb)
+        ptr, h, w = tree.matrix(0)
+        main = G.DeviceMatrix(dev, ptr, h, w, owned=False)
+        atree, da = None, None
+        if aux is not None:
+            atree = G.commit_traces(cfg, [aux], logb)
+            p2, h2, w2 = atree.matrix(0)
+            da = G.DeviceMatrix(dev, p2, h2, w2, owned=False)
+        M = n << log_d
+        acc = dev.alloc_matrix(M, 2)
+        G.cyclic_extend_and_scale(acc, 0, M, (0, 0))
+        alpha = (5, 8)
+
+        def run():
+            G.eval_constraints(prog, main, n, log_d, 7, alpha, acc, aux=da, **kw)
+
+        for _ in range(warmup):
+            run()
+        dev.synchronize()
+        reps = max(5, steps)
+        dev.profile(True)
+        dev.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        dev.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / reps
+        kms, cnt = dev.profile_read("eval_constraints")
+        dev.profile(False)
+        kms /= cnt
+        ops = prog.ops
+        bpp = air_bytes_per_point(ops)
+        row = {"air": name, "trace": f"2^{n.bit_length() - 1} x {trace.shape[1]}", "log_blowup": logb,
+               "quotient_degree": 1 << log_d, "points": M, "ops": int(len(ops)),
+               "constraints": int(sum(1 for o in ops if o["op"] >= 13)), "compile_s": round(compile_s, 2),
+               "kernel_ms": round(kms, 4), "wall_ms": round(wall_ms, 4),
+               "points_per_s": round(M / (kms * 1e-3), 1), "algorithmic_bytes_per_point": bpp,
+               "algorithmic_gbps": round(bpp * M / (kms * 1e-3) / 1e9, 1)}
+        if cpu_check:
+            # the oracle (the C restatement, OpenMP on the host cores) on the same inputs:
+            # it must agree word for word, and its time is the CPU figure beside the kernel's
+            import oracle as O
+            lde = main.download()
+            acc0 = np.zeros((M, 2), np.uint64)
+            G.cyclic_extend_and_scale(acc, 0, M, (0, 0))
+            run()
+            got = acc.download().reshape(-1, 2)
+            c0 = time.perf_counter()
+            want = O.eval_constraints(ops, lde, n, log_d, 7, alpha, acc=acc0)
+            row["oracle_cpu_s"] = round(time.perf_counter() - c0, 3)
+            row["oracle_threads"] = O.get_threads()
+            row["bit_exact_vs_oracle"] = bool(np.array_equal(got, want))
+            del lde, want
+        out.append(row)
+        log(f"constraints: {name} {row['kernel_ms']} ms ({row['algorithmic_gbps']} GB/s), compile {compile_s:.2f} s")
+        acc.free()
+        tree.free()
+        if atree is not None:
+            atree.free()
+
+    case("DummyMidenAir (miden.rs:38-64)", A.miden_program(), A.miden_trace(1 << 19, 20, SEED), 3, 3, cpu_check=cpu)
+    t, pub = A.fibonacci_trace(1 << 21, 1, 2)
+    case("Fibonacci with public values", A.fibonacci_program(), t, 1, 1, publics=pub)
+    n = 1 << 21
+    main = rng.integers(0, A.P, size=(n, 2), dtype=np.uint64)
+    aux = rng.integers(0, A.P, size=(n, 2), dtype=np.uint64)
+    per = rng.integers(0, A.P, size=(8, 1), dtype=np.uint64)
+    case("running sum (aux, periodic, extension inputs)", A.running_program(), main, 1, 1, aux=aux, periodic=per,
+         ext_inputs=[3, 4, 5, 6])
+    n = 1 << 18
+    prog = A.random_program(SEED, 3000, 9, 2, 3, 2, 2)
+    case("random program, 3000 ops", prog, rng.integers(0, A.P, size=(n, 9), dtype=np.uint64), 2, 2,
+         aux=rng.integers(0, A.P, size=(n, 4), dtype=np.uint64),
+         periodic=rng.integers(0, A.P, size=(16, 2), dtype=np.uint64), publics=[1, 2, 3], ext_inputs=[4, 5, 6, 7])
+    dev.trim()
+    hbm_peak = 8000.0
+    m = out[0]
+    return {"workload": "constraint evaluation on the quotient domain, committed LDE in HBM (bit-reversed rows), "
+                        "accumulator of n*d extension elements",
+            "runs": out,
+            "roofline": {"kernel": "p3g_air_kernel (DummyMidenAir, 2^22 points)", "bound": "hbm",
+                         "achieved": m["algorithmic_gbps"], "peak": hbm_peak, "unit": "GB/s",
+                         "frac": round(m["algorithmic_gbps"] / hbm_peak, 3),
+                         "bytes_per_point": m["algorithmic_bytes_per_point"]}}
+
+
 def cpu_baseline(W, log_rows, full_c2=False):
     """The oracle (the C restatement of the reference CPU prover, OpenMP) on the host cores, timed as
     p3-miden-lifted-examples/src/stats.rs:185-190 does (one warm-up, then BENCH_ITERS runs):
@@ -682,9 +797,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--sections", default="calibrate,c2,c2host,c3,c4,c5,small,quotient,grind",
+    ap.add_argument("--sections", default="calibrate,c2,c2host,c3,c4,c5,small,quotient,grind,constraints",
                     help="comma list of calibrate (measured peaks), c2 (headline), c2host (headline from host "
-                         "memory), c3, c4, c5, small, quotient, grind (the reference binary's parameters)")
+                         "memory), c3, c4, c5, small, quotient, grind (the reference binary's parameters), "
+                         "constraints (constraint evaluation)")
     ap.add_argument("--small-proofs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=4, help="server workers for the C5 mix")
@@ -724,6 +840,8 @@ def main():
         if "small" in sections else None
     quot = bench_quotient(G, dev, max(3, args.steps // 2), 1) if "quotient" in sections else None
     grind = bench_grind(G, dev, max(2, args.steps // 2), 1) if "grind" in sections else None
+    cons = bench_constraints(G, dev, args.steps, 1, cpu=not args.no_cpu_baseline) \
+        if "constraints" in sections else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(W, args.cpu_sample_log_rows, args.cpu_full_c2)
 
     ms = c2["ms"] if c2 else None
@@ -762,6 +880,7 @@ def main():
         "small_proofs": small,
         "quotient_commit": quot,
         "grinding": grind,
+        "constraints": cons,
         "cpu_baseline": cpu,
         "measured_peaks": peaks,
     }

@@ -38,7 +38,8 @@ struct AirKArgs {
   uint64_t prow, pcols;    // periodic table (prow < 2^32)
   uint64_t omega_tile;     // w^(2^k): x at tile index mid carries w^(mid 2^k)
   uint64_t omega_n_inv;    // w_n^-1, the last row of H
-  uint32_t logM, k, log_d, pad;
+  uint32_t logM, k, log_d;
+  uint32_t blocks_on;      // all ones: the guard of every basic block of a compiled program
 };
 
 // What air_point reads at one point.

@@ -56,7 +56,8 @@ enum AirOp : uint32_t {
 };
 
 constexpr uint64_t AIR_MAX_OPS = uint64_t(1) << 20;
-constexpr uint64_t AIR_CHUNK = 128;  // ops per device function of a compiled program
+constexpr uint64_t AIR_BLOCK = 128;   // ops per basic block of a compiled program
+constexpr uint64_t AIR_FUNC = 1024;   // ops per device function (a multiple of AIR_BLOCK)
 
 bool is_assert(uint32_t op) { return op == OP_ASSERT || op == OP_ASSERT_EXT; }
 
@@ -178,16 +179,23 @@ std::string air_expr(const AirOpIn& op, const AirShape& s) {
   return o.str();
 }
 
-// The kernel source of a program.  The ops are cut into chunks of AIR_CHUNK consecutive
-// ops, each a device function that returns its constraints folded with their powers of
-// alpha; air_point sums them.  A chunk re-emits the leaves (loads, inputs, constants,
-// selectors) it reads from earlier chunks; a computed value read by a later chunk goes
-// through a per-thread slot array (slots reused once their last reader has run).  The
-// compiler's scheduling and register allocation work on one chunk at a time, so compile
-// time stays linear in the program; a program of one chunk is inlined whole.
+// The kernel source of a program, in two levels:
+//   - blocks of AIR_BLOCK consecutive ops, each a basic block of its own (entered on a
+//     kernel-argument bit that is always set): the compiler's instruction selection and
+//     scheduling work on one block at a time.  Values read by a later block are declared
+//     at function scope and stay in registers; a block re-emits the leaves (loads, inputs,
+//     constants, selectors) it reads from elsewhere.
+//   - functions of AIR_FUNC ops (whole blocks), which bound the work of register
+//     allocation.  A computed value read by a later function goes through a per-thread
+//     slot array (slots reused once their last reader has run).  Each function returns
+//     its constraints folded with their powers of alpha; air_point sums them.  A program
+//     of one function is inlined whole.
+// Compile time is linear in the program; the calls and slots cost little next to the
+// arithmetic of AIR_FUNC ops.
 std::string air_source(const AirOpIn* ops, uint64_t nops, const AirShape& s) {
-  const uint64_t nchunks = (nops + AIR_CHUNK - 1) / AIR_CHUNK;
-  auto chunk_of = [](uint64_t k) { return k / AIR_CHUNK; };
+  const uint64_t nfn = (nops + AIR_FUNC - 1) / AIR_FUNC;
+  auto fn_of = [](uint64_t k) { return k / AIR_FUNC; };
+  auto blk_of = [](uint64_t k) { return k / AIR_BLOCK; };
   auto operands = [&](uint64_t k, auto&& f) {
     const uint32_t op = ops[k].op;
     if (op == OP_ADD || op == OP_SUB || op == OP_MUL) {
@@ -197,33 +205,42 @@ std::string air_source(const AirOpIn* ops, uint64_t nops, const AirShape& s) {
       f(ops[k].a);
     }
   };
-  // the last chunk reading each value, and each chunk's operands from earlier chunks
-  std::vector<uint64_t> last_chunk(nops, 0), seen(nops, ~uint64_t(0));
-  std::vector<std::vector<uint32_t>> imports(nchunks);
+  const uint64_t nblk = (nops + AIR_BLOCK - 1) / AIR_BLOCK;
+  // readers: the last function and block reading each value; what each function reads
+  // from earlier functions (computed values) and each block from elsewhere (leaves)
+  std::vector<uint64_t> last_fn(nops, 0), last_blk(nops, 0), seen_fn(nops, ~uint64_t(0)),
+      seen_blk(nops, ~uint64_t(0));
+  std::vector<std::vector<uint32_t>> fn_imports(nfn), blk_leaves(nblk);
   for (uint64_t k = 0; k < nops; k++) {
-    const uint64_t c = chunk_of(k);
+    const uint64_t f = fn_of(k), b = blk_of(k);
     operands(k, [&](uint32_t v) {
-      last_chunk[v] = std::max(last_chunk[v], c);
-      if (chunk_of(v) < c && seen[v] != c) {
-        seen[v] = c;
-        imports[c].push_back(v);
+      last_fn[v] = std::max(last_fn[v], f);
+      last_blk[v] = std::max(last_blk[v], b);
+      if (is_leaf(ops[v].op)) {
+        if (blk_of(v) != b && seen_blk[v] != b) {
+          seen_blk[v] = b;
+          blk_leaves[b].push_back(v);
+        }
+      } else if (fn_of(v) < f && seen_fn[v] != f) {
+        seen_fn[v] = f;
+        fn_imports[f].push_back(v);
       }
     });
   }
-  // slots (two words each) for computed values read in a later chunk
+  // slots (two words each) for computed values read in a later function
   std::vector<int64_t> slot(nops, -1);
-  std::vector<std::vector<uint32_t>> frees(nchunks), exports(nchunks);
+  std::vector<std::vector<uint32_t>> frees(nfn), exports(nfn);
   std::vector<uint32_t> free_slots;
   uint32_t nslots = 0;
-  for (uint64_t c = 0; c < nchunks; c++) {
-    for (uint32_t v : frees[c]) free_slots.push_back((uint32_t)slot[v]);  // read at the top of c
-    for (uint64_t k = c * AIR_CHUNK; k < std::min(nops, (c + 1) * AIR_CHUNK); k++) {
-      if (is_leaf(ops[k].op) || is_assert(ops[k].op) || last_chunk[k] <= c) continue;
+  for (uint64_t f = 0; f < nfn; f++) {
+    for (uint32_t v : frees[f]) free_slots.push_back((uint32_t)slot[v]);  // read at the top of f
+    for (uint64_t k = f * AIR_FUNC; k < std::min(nops, (f + 1) * AIR_FUNC); k++) {
+      if (is_leaf(ops[k].op) || is_assert(ops[k].op) || last_fn[k] <= f) continue;
       if (free_slots.empty()) free_slots.push_back(nslots++);
       slot[k] = free_slots.back();
       free_slots.pop_back();
-      frees[last_chunk[k]].push_back((uint32_t)k);
-      exports[c].push_back((uint32_t)k);
+      frees[last_fn[k]].push_back((uint32_t)k);
+      exports[f].push_back((uint32_t)k);
     }
   }
 
@@ -234,57 +251,65 @@ std::string air_source(const AirOpIn* ops, uint64_t nops, const AirShape& s) {
     << "#include \"gl.hpp\"\n"
     << "#include \"air_kernel.hpp\"\n"
     << "namespace p3g {\n";
-  auto decl = [&](uint64_t k) { return std::string(s.ext[k] ? "  const ef2 v" : "  const uint64_t v") + std::to_string(k); };
+  auto type = [&](uint64_t k) { return s.ext[k] ? "ef2 " : "uint64_t "; };
   uint64_t cons = 0;  // constraints emitted
-  for (uint64_t c = 0; c < nchunks; c++) {
-    o << "__device__ " << (nchunks > 1 ? "__noinline__" : "__forceinline__") << " ef2 air_chunk_" << c
+  for (uint64_t f = 0; f < nfn; f++) {
+    const uint64_t k0 = f * AIR_FUNC, k1 = std::min(nops, (f + 1) * AIR_FUNC);
+    o << "__device__ " << (nfn > 1 ? "__noinline__" : "__forceinline__") << " ef2 air_fn_" << f
       << "(const uint64_t* __restrict__ ap, const uint64_t* __restrict__ pub, const ef2* __restrict__ ext,"
       << " const uint64_t* mc, const uint64_t* mn, const uint64_t* ac, const uint64_t* an, const uint64_t* per,"
-      << " uint64_t first, uint64_t last, uint64_t trans, uint64_t* live) {\n"
+      << " uint64_t first, uint64_t last, uint64_t trans, uint64_t* live, uint32_t on) {\n"
       << "  Acc192 s0 = acc192_zero(), s1 = acc192_zero();\n";
-    for (uint32_t v : imports[c]) {
-      if (is_leaf(ops[v].op)) {
-        o << decl(v) << " = " << air_expr(ops[v], s) << ";\n";
-      } else if (s.ext[v]) {
-        o << decl(v) << " = ef2{live[" << 2 * slot[v] << "], live[" << 2 * slot[v] + 1 << "]};\n";
-      } else {
-        o << decl(v) << " = live[" << 2 * slot[v] << "];\n";
-      }
+    for (uint32_t v : fn_imports[f]) {
+      if (s.ext[v]) o << "  const ef2 v" << v << " = ef2{live[" << 2 * slot[v] << "], live[" << 2 * slot[v] + 1 << "]};\n";
+      else o << "  const uint64_t v" << v << " = live[" << 2 * slot[v] << "];\n";
     }
-    for (uint64_t k = c * AIR_CHUNK; k < std::min(nops, (c + 1) * AIR_CHUNK); k++) {
-      const auto& op = ops[k];
-      if (is_assert(op.op)) {
-        const uint64_t pw = 3 * (s.nasserts - 1 - cons++);  // alpha^(K-1-c) as (c0, c1, 7 c1)
-        const std::string x = "v" + std::to_string(op.a);
-        if (s.ext[op.a]) {
-          // (x0 + x1 X)(a0 + a1 X) = (x0 a0 + x1 (7 a1)) + (x1 a0 + x0 a1) X
-          o << "  acc192_mad(s0, " << x << ".c0, ap[" << pw << "]); acc192_mad(s0, " << x << ".c1, ap[" << pw + 2
-            << "]); acc192_mad(s1, " << x << ".c1, ap[" << pw << "]); acc192_mad(s1, " << x << ".c0, ap["
-            << pw + 1 << "]);\n";
-        } else {
-          o << "  acc192_mad(s0, " << x << ", ap[" << pw << "]); acc192_mad(s1, " << x << ", ap[" << pw + 1
-            << "]);\n";
+    // computed values of this function read outside their own block
+    std::vector<uint8_t> fscope(k1 - k0, 0);
+    for (uint64_t k = k0; k < k1; k++) {
+      if (is_leaf(ops[k].op) || is_assert(ops[k].op) || !s.used[k] || last_blk[k] <= blk_of(k)) continue;
+      fscope[k - k0] = 1;
+      o << "  " << type(k) << "v" << k << ";\n";
+    }
+    const bool guard = k1 - k0 > AIR_BLOCK;  // a lone block needs no guard
+    for (uint64_t b = blk_of(k0); b * AIR_BLOCK < k1; b++) {
+      o << (guard ? "  if (on >> " + std::to_string(b % 32) + " & 1) {\n" : "  {\n");
+      for (uint32_t v : blk_leaves[b]) o << "  const " << type(v) << "v" << v << " = " << air_expr(ops[v], s) << ";\n";
+      for (uint64_t k = b * AIR_BLOCK; k < std::min(k1, (b + 1) * AIR_BLOCK); k++) {
+        const auto& op = ops[k];
+        if (is_assert(op.op)) {
+          const uint64_t pw = 3 * (s.nasserts - 1 - cons++);  // alpha^(K-1-c) as (c0, c1, 7 c1)
+          const std::string x = "v" + std::to_string(op.a);
+          if (s.ext[op.a]) {
+            // (x0 + x1 X)(a0 + a1 X) = (x0 a0 + x1 (7 a1)) + (x1 a0 + x0 a1) X
+            o << "  acc192_mad(s0, " << x << ".c0, ap[" << pw << "]); acc192_mad(s0, " << x << ".c1, ap["
+              << pw + 2 << "]); acc192_mad(s1, " << x << ".c1, ap[" << pw << "]); acc192_mad(s1, " << x
+              << ".c0, ap[" << pw + 1 << "]);\n";
+          } else {
+            o << "  acc192_mad(s0, " << x << ", ap[" << pw << "]); acc192_mad(s1, " << x << ", ap[" << pw + 1
+              << "]);\n";
+          }
+        } else if (s.used[k]) {
+          if (fscope[k - k0]) o << "  v" << k << " = " << air_expr(op, s) << ";\n";
+          else o << "  const " << type(k) << "v" << k << " = " << air_expr(op, s) << ";\n";
         }
-      } else if (s.used[k]) {
-        o << decl(k) << " = " << air_expr(op, s) << ";\n";
       }
+      o << "  }\n";
     }
-    for (uint32_t v : exports[c]) {
-      if (s.ext[v]) {
-        o << "  live[" << 2 * slot[v] << "] = v" << v << ".c0; live[" << 2 * slot[v] + 1 << "] = v" << v << ".c1;\n";
-      } else {
-        o << "  live[" << 2 * slot[v] << "] = v" << v << ";\n";
-      }
+    for (uint32_t v : exports[f]) {
+      if (s.ext[v]) o << "  live[" << 2 * slot[v] << "] = v" << v << ".c0; live[" << 2 * slot[v] + 1 << "] = v" << v << ".c1;\n";
+      else o << "  live[" << 2 * slot[v] << "] = v" << v << ";\n";
     }
     o << "  return ef2{acc192_reduce(s0), acc192_reduce(s1)};\n}\n";
   }
   o << "__device__ __forceinline__ ef2 air_point(const AirKArgs& a, const AirRow& r) {\n";
   if (nslots) o << "  uint64_t live[" << 2 * nslots << "];\n";
   else o << "  uint64_t* live = nullptr;\n";
-  for (uint64_t c = 0; c < nchunks; c++) {
-    const std::string call = "air_chunk_" + std::to_string(c) +
-                             "(a.apow, a.pub, a.ext, r.mc, r.mn, r.ac, r.an, r.per, r.first, r.last, r.trans, live)";
-    o << (c ? "  f = ef_add(f, " + call + ");\n" : "  ef2 f = " + call + ";\n");
+  for (uint64_t f = 0; f < nfn; f++) {
+    const std::string call = "air_fn_" + std::to_string(f) +
+                             "(a.apow, a.pub, a.ext, r.mc, r.mn, r.ac, r.an, r.per, r.first, r.last, r.trans, live, "
+                             "a.blocks_on)";
+    o << (f ? "  f = ef_add(f, " + call + ");\n" : "  ef2 f = " + call + ";\n");
   }
   o << "  return f;\n}\n}  // namespace p3g\n";
   return o.str();
@@ -445,6 +470,7 @@ void eval_constraints(Ctx& ctx, const AirOpIn* ops, uint64_t nops, const DevMat&
     a.logM = L;
     a.k = k;
     a.log_d = log_d;
+    a.blocks_on = ~0u;
     void* params[] = {&a};
     const uint32_t blocks = (uint32_t)(M >> (2 * k));
     ProfScope ps(ctx, "eval_constraints", s);

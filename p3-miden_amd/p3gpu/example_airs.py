@@ -1,0 +1,140 @@
+"""Example AIRs written against p3gpu.AirProgram (the reference's AirBuilder vocabulary),
+with trace generators: the constraint-evaluation tests and the bench use them.
+
+  fibonacci: two columns, a' = b, b' = a + b; a_0, b_0 and the last b are public values
+             (degree 2, quotient degree 2);
+  miden:     the reference's DummyMidenAir (p3-miden-lifted-examples/src/miden.rs:38-64):
+             local[0] * ... * local[8] == 0 on every row (degree 9, quotient degree 8),
+             column 0 zero, the rest random (generate_dummy_trace, :122-144);
+  running:   a main column v, a column w = v * p with p a period-4 periodic column, and an
+             extension aux column s, the running sum of r * v with r the permutation
+             randomness, ending in the permutation value (degree 2).
+"""
+import numpy as np
+
+from .air import AirProgram
+
+P = (1 << 64) - (1 << 32) + 1
+W = 7  # the extension's non-residue: F[x]/(x^2 - 7)
+
+
+def fibonacci_program():
+    p = AirProgram()
+    a, b = p.main(0), p.main(1)
+    a2, b2 = p.main(0, next=True), p.main(1, next=True)
+    first, last, trans = p.is_first_row(), p.is_last_row(), p.is_transition()
+    p.assert_zero(first * (a - p.public(0)))
+    p.assert_zero(first * (b - p.public(1)))
+    p.assert_zero(trans * (a2 - b))
+    p.assert_zero(trans * (b2 - a - b))
+    p.assert_zero(last * (b - p.public(2)))
+    return p
+
+
+def fibonacci_trace(n, a0=1, b0=1):
+    t = np.zeros((n, 2), np.uint64)
+    a, b = a0, b0
+    for i in range(n):
+        t[i] = (a, b)
+        a, b = b, (a + b) % P
+    return t, [a0, b0, int(t[n - 1, 1])]
+
+
+def miden_program():
+    p = AirProgram()
+    prod = p.main(0)
+    for j in range(1, 9):
+        prod = prod * p.main(j)
+    p.assert_zero(prod)
+    return p
+
+
+def miden_trace(n, width, seed):
+    rng = np.random.default_rng(seed)
+    t = rng.integers(0, P, size=(n, width), dtype=np.uint64)
+    t[:, 0] = 0
+    return t
+
+
+PERIODIC = [3, 5, 7, 11]
+
+
+def running_program():
+    p = AirProgram()
+    v, w, v2 = p.main(0), p.main(1), p.main(0, next=True)
+    s, s2 = p.aux(0), p.aux(0, next=True)
+    r, final = p.ext_input(0), p.ext_input(1)
+    p.assert_zero(w - v * p.periodic(0))
+    p.assert_zero_ext(p.is_first_row() * (s - r * v))
+    p.assert_zero_ext(p.is_transition() * (s2 - s - r * v2))
+    p.assert_zero_ext(p.is_last_row() * (s - final))
+    return p
+
+
+def ef_mul(a, b):
+    return ((a[0] * b[0] + W * a[1] * b[1]) % P, (a[0] * b[1] + a[1] * b[0]) % P)
+
+
+def running_trace(n, r, seed):
+    """main (n x 2), aux (n x 2: s as c0, c1), final value"""
+    rng = np.random.default_rng(seed)
+    v = rng.integers(0, P, size=n, dtype=np.uint64)
+    main = np.zeros((n, 2), np.uint64)
+    aux = np.zeros((n, 2), np.uint64)
+    s = (0, 0)
+    for i in range(n):
+        main[i] = (int(v[i]), int(v[i]) * PERIODIC[i % 4] % P)
+        rv = ef_mul(r, (int(v[i]), 0))
+        s = ((s[0] + rv[0]) % P, (s[1] + rv[1]) % P)
+        aux[i] = s
+    return main, aux, s
+
+
+def random_program(seed, nops, main_w, aux_cols=0, npub=0, n_ext=0, per_cols=0, selectors=True):
+    """A random straight-line program over every kind of input: leaves (main and aux
+    columns on both rows, periodic columns, public values, extension inputs, constants,
+    selectors) and ADD/SUB/MUL/NEG of earlier values (mixed base and extension operands),
+    with an assertion about every eighth op."""
+    rng = np.random.default_rng(seed)
+    p = AirProgram()
+    vals = []
+    leaves = [lambda: p.main(int(rng.integers(main_w)), next=bool(rng.integers(2))),
+              lambda: p.const(int(rng.integers(0, P, dtype=np.uint64)))]
+    if aux_cols:
+        leaves.append(lambda: p.aux(int(rng.integers(aux_cols)), next=bool(rng.integers(2))))
+    if npub:
+        leaves.append(lambda: p.public(int(rng.integers(npub))))
+    if n_ext:
+        leaves.append(lambda: p.ext_input(int(rng.integers(n_ext))))
+    if per_cols:
+        leaves.append(lambda: p.periodic(int(rng.integers(per_cols))))
+    if selectors:
+        leaves += [p.is_first_row, p.is_last_row, p.is_transition]
+    while len(p._ops) < nops:
+        if len(vals) < 4 or rng.random() < 0.3:
+            vals.append(leaves[int(rng.integers(len(leaves)))]())
+            continue
+        # operands mostly from the last 32 values (as an AIR's constraints use their own
+        # subexpressions), one in twenty from anywhere earlier (long live ranges)
+        def pick():
+            far = rng.random() < 0.05
+            return vals[int(rng.integers(len(vals)))] if far else vals[-1 - int(rng.integers(min(32, len(vals))))]
+        a, b = pick(), pick()
+        kind = int(rng.integers(4))
+        vals.append(a + b if kind == 0 else a - b if kind == 1 else a * b if kind == 2 else -a)
+        if rng.random() < 0.125:
+            (p.assert_zero if rng.integers(2) else p.assert_zero_ext)(vals[-1])
+    p.assert_zero(vals[-1])
+    return p
+
+
+def wide_program(live):
+    """`live` values alive at once: a product over main(0) + k for k < live, formed
+    only after every factor exists."""
+    p = AirProgram()
+    vals = [p.main(0) + k for k in range(live)]
+    total = vals[0]
+    for v in vals[1:]:
+        total = total * v
+    p.assert_zero(total)
+    return p
