@@ -51,6 +51,33 @@ pub struct p3g_pcs_params {
     pub query_pow_bits: u32,
 }
 
+/// One op of a constraint program (`p3g_air_op` in the header).
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default, PartialEq, Eq)]
+pub struct p3g_air_op {
+    pub op: u32,
+    pub a: u32,
+    pub b: u32,
+    pub reserved: u32,
+    pub imm: u64,
+}
+
+pub const P3G_AIR_CONST: u32 = 0;
+pub const P3G_AIR_MAIN: u32 = 1;
+pub const P3G_AIR_AUX: u32 = 2;
+pub const P3G_AIR_PERIODIC: u32 = 3;
+pub const P3G_AIR_PUBLIC: u32 = 4;
+pub const P3G_AIR_EXT_INPUT: u32 = 5;
+pub const P3G_AIR_IS_FIRST_ROW: u32 = 6;
+pub const P3G_AIR_IS_LAST_ROW: u32 = 7;
+pub const P3G_AIR_IS_TRANSITION: u32 = 8;
+pub const P3G_AIR_ADD: u32 = 9;
+pub const P3G_AIR_SUB: u32 = 10;
+pub const P3G_AIR_MUL: u32 = 11;
+pub const P3G_AIR_NEG: u32 = 12;
+pub const P3G_AIR_ASSERT_ZERO: u32 = 13;
+pub const P3G_AIR_ASSERT_ZERO_EXT: u32 = 14;
+
 /// The caller's ProverChannel as callbacks (`p3g_channel` in the header).
 #[repr(C)]
 pub struct p3g_channel {
@@ -114,6 +141,32 @@ unsafe extern "C" {
     -> c_int;
     pub fn p3g_tree_free(t: *mut p3g_tree);
     pub fn p3g_download(ctx: *mut p3g_ctx, dst: *mut c_void, src: *const c_void, bytes: u64) -> c_int;
+    pub fn p3g_alloc(ctx: *mut p3g_ctx, bytes: u64, dptr: *mut *mut c_void) -> c_int;
+    pub fn p3g_free(ctx: *mut p3g_ctx, dptr: *mut c_void) -> c_int;
+
+    pub fn p3g_quotient_extend_scale(ctx: *mut p3g_ctx, acc: *mut u64, m: u64, target: u64, beta: *const u64)
+    -> c_int;
+    pub fn p3g_quotient_divide_vanishing(ctx: *mut p3g_ctx, num: *mut u64, n: u64, d: u64, shift: u64) -> c_int;
+    pub fn p3g_air_compile(ops: *const p3g_air_op, nops: u64, code_bytes: *mut u64) -> c_int;
+    pub fn p3g_eval_constraints(
+        ctx: *mut p3g_ctx,
+        ops: *const p3g_air_op,
+        nops: u64,
+        main: *const p3g_matrix,
+        aux: *const p3g_matrix,
+        periodic: *const u64,
+        periodic_rows: u64,
+        periodic_cols: u64,
+        publics: *const u64,
+        num_publics: u64,
+        ext_inputs: *const u64,
+        num_ext_inputs: u64,
+        n: u64,
+        log_d: u32,
+        shift: u64,
+        alpha: *const u64,
+        accumulator: *mut u64,
+    ) -> c_int;
 
     pub fn p3g_open_with_callbacks(
         ctx: *mut p3g_ctx,

@@ -7,7 +7,10 @@
 //!   config's `Lmcs`), wrapping the CPU `LmcsConfig` for the verifier side;
 //! - [`open_with_channel`] is the PCS opening of p3-miden-lifted-fri/src/prover.rs:35-116
 //!   with the same arguments, every transcript operation going to the caller's channel;
-//! - [`commit_traces`] is commit.rs:175-228 in one call, the LDEs staying in HBM.
+//! - [`commit_traces`] is commit.rs:175-228 in one call, the LDEs staying in HBM;
+//! - [`GpuQuotient`] holds the quotient accumulator of `prove_multi` in HBM and
+//!   evaluates an AIR's constraints on it ([`AirProgram`], constraints/mod.rs:63-179),
+//!   reading the committed LDEs in place.
 //!
 //! ```ignore
 //! let ctx = GpuContext::new(0);
@@ -21,6 +24,7 @@
 //! here with the library's message.
 
 mod channel;
+mod constraints;
 mod context;
 mod dft;
 pub mod ffi;
@@ -28,6 +32,7 @@ mod lmcs;
 mod open;
 
 pub use channel::{Digest, GpuChannel};
+pub use constraints::{AirProgram, AirValue, GpuQuotient};
 pub use context::GpuContext;
 pub use dft::GpuDft;
 pub use lmcs::{GpuLmcs, GpuTree};
