@@ -183,3 +183,43 @@ def test_bad_programs_and_shapes_are_rejected(device):
     assert e.value.code == 1
     main.free()
     acc.free()
+
+
+def test_two_instances_as_prove_multi(device):
+    # prove_multi's quotient loop (prover/mod.rs:355-434) over two AIRs in ascending height,
+    # all on the device: one commit_traces tree holds both LDEs (the smaller on the nested
+    # shift g^(N/n), commit.rs:205-221); the accumulator is extended to each instance's
+    # n*D (scaled by beta), each instance's constraints are evaluated in place on its tree
+    # matrix with its own coset shift, then Z_H is divided out on the largest quotient
+    # coset and the quotient committed.  Every step must equal the oracle's.
+    D, logD, logb = 8, 3, 3
+    n0, n1 = 1 << 6, 1 << 8
+    fib, pub = airs.fibonacci_trace(n0, 3, 5)
+    miden = airs.miden_trace(n1, 12, 77)
+    alpha, beta = (1234, 5678), (9012, 3456)
+    shift0 = pow(G7, n1 // n0, P)
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    tree = G.commit_traces(cfg, [fib, miden], logb)
+    mats = [G.DeviceMatrix(device, *tree.matrix(j), owned=False) for j in range(2)]
+    acc = device.alloc_matrix(n1 * D, 2)
+    G.cyclic_extend_and_scale(acc, 0, n0 * D, beta)
+    G.eval_constraints(airs.fibonacci_program(), mats[0], n0, logD, shift0, alpha, acc, publics=pub)
+    G.cyclic_extend_and_scale(acc, n0 * D, n1 * D, beta)
+    G.eval_constraints(airs.miden_program(), mats[1], n1, logD, G7, alpha, acc)
+    G.divide_by_vanishing(acc, n1, D, G7)
+    got = acc.download().reshape(-1, 2)
+    q = device.upload(got.reshape(n1, 2 * D))
+    qt = G.commit_quotient(cfg, q, D, logb)
+
+    lde0 = O.coset_lde_batch(fib, logb, shift0)
+    lde1 = O.coset_lde_batch(miden, logb, G7)
+    assert np.array_equal(mats[0].download(), lde0) and np.array_equal(mats[1].download(), lde1)
+    want = O.cyclic_extend_scale(np.zeros((0, 2), np.uint64), beta, n0 * D)
+    want = O.eval_constraints(airs.fibonacci_program().ops, lde0, n0, logD, shift0, alpha, acc=want, publics=pub)
+    want = O.cyclic_extend_scale(want, beta, n1 * D)
+    want = O.eval_constraints(airs.miden_program().ops, lde1, n1, logD, G7, alpha, acc=want)
+    want = O.divide_by_vanishing(want, D, G7)
+    assert np.array_equal(got, want)
+    assert qt.root().tolist() == O.commit_quotient(O.HASH_P2, want, D, logb).root().tolist()
+    for x in (qt, q, acc, tree):
+        x.free()
