@@ -632,7 +632,8 @@ def bench_constraints(G, dev, steps, warmup, cpu=True):
         quotient degree 8: 2^22 points), with the oracle on the host cores beside it;
       - Fibonacci with public values (selectors) and the running-sum AIR (aux extension column,
         periodic column, extension inputs) at 2^22 points;
-      - a 3000-op random program (every input kind) at 2^20 points, bound by arithmetic."""
+      - a Poseidon2-GL12 AIR (3,047 ops over 248 columns) at 2^19 points and a 3000-op random
+        program (every input kind) at 2^20 points, bound by arithmetic."""
     from p3gpu import example_airs as A
     cfg = G.LmcsConfig(dev, G.HASH_POSEIDON2)
     rng = np.random.default_rng(SEED)
@@ -712,7 +713,13 @@ def bench_constraints(G, dev, steps, warmup, cpu=True):
     per = rng.integers(0, A.P, size=(8, 1), dtype=np.uint64)
     case("running sum (aux, periodic, extension inputs)", A.running_program(), main, 1, 1, aux=aux, periodic=per,
          ext_inputs=[3, 4, 5, 6])
+    # Poseidon2-GL12 AIR (one permutation per row; round constants drawn here, which does
+    # not change the work): 3,047 ops, 236 degree-3 constraints over 248 columns
     n = 1 << 18
+    p2 = A.poseidon2_program(rng.integers(0, A.P, size=118, dtype=np.uint64),
+                             rng.integers(0, A.P, size=12, dtype=np.uint64))
+    case("Poseidon2-GL12 AIR (one permutation per row, 248 columns)", p2,
+         rng.integers(0, A.P, size=(n, A.P2_COLS), dtype=np.uint64), 1, 1, cpu_check=cpu)
     prog = A.random_program(SEED, 3000, 9, 2, 3, 2, 2)
     case("random program, 3000 ops", prog, rng.integers(0, A.P, size=(n, 9), dtype=np.uint64), 2, 2,
          aux=rng.integers(0, A.P, size=(n, 4), dtype=np.uint64),

@@ -138,3 +138,143 @@ def wide_program(live):
         total = total * v
     p.assert_zero(total)
     return p
+
+
+# --------------------------------------------------------------------------- Poseidon2-GL12
+P2_FULL, P2_PARTIAL, P2_W = 4, 22, 12  # half of the full rounds, partial rounds, width
+P2_COLS = P2_W + 2 * P2_FULL * 2 * P2_W + 2 * P2_PARTIAL  # 248
+
+
+def _p2_columns():
+    """Column offsets: inputs | per beginning full round: x^3 registers, post-round state |
+    per partial round: x^3 register, lane-0 S-box output | per ending full round: as the
+    beginning."""
+    begin = [P2_W + 2 * P2_W * r for r in range(P2_FULL)]
+    part0 = P2_W + 2 * P2_W * P2_FULL
+    partial = [part0 + 2 * r for r in range(P2_PARTIAL)]
+    end0 = part0 + 2 * P2_PARTIAL
+    end = [end0 + 2 * P2_W * r for r in range(P2_FULL)]
+    return begin, partial, end
+
+
+def _m4(p, x):
+    a, b, c, d = x
+    ab, cd = a + b, c + d
+    abcd = ab + cd
+    return [abcd + a + b + b, abcd + b + c + c, abcd + c + d + d, abcd + d + a + a]
+
+
+def _external(p, s):
+    """circ(2 M4, M4, M4) on 12 lanes: M4 on each block of 4, then every lane plus the sum of
+    its position across the blocks"""
+    y = _m4(p, s[0:4]) + _m4(p, s[4:8]) + _m4(p, s[8:12])
+    sums = [y[k] + y[4 + k] + y[8 + k] for k in range(4)]
+    return [y[i] + sums[i & 3] for i in range(12)]
+
+
+def _internal(p, s, diag):
+    """diag(d) + 1 1^T"""
+    total = s[0]
+    for v in s[1:]:
+        total = total + v
+    return [s[i] * p.const(int(diag[i])) + total for i in range(12)]
+
+
+def poseidon2_program(rc, diag):
+    """One Poseidon2-GL12 permutation per row (S-box x^7, 8 full and 22 partial rounds, the
+    external layer circ(2 M4, M4, M4) first and after every full round, the internal layer
+    diag(d) + 1 1^T after every partial round), shaped like Plonky3's Poseidon2Air (the
+    reference's poseidon2.rs instantiates it over BabyBear): one x^3 register per S-box,
+    the state stored after every full round and the lane-0 S-box output of every partial
+    round, the other partial-round lanes carried as expressions.  Every constraint has
+    degree 3: register = x^3, output = register^2 * x.  rc: 118 round constants (4 x 12
+    beginning, 4 x 12 ending, 22 partial), diag: the internal diagonal."""
+    p = AirProgram()
+    begin, partial, end = _p2_columns()
+    rc = [int(v) for v in rc]
+    s = _external(p, [p.main(i) for i in range(P2_W)])
+
+    def full(s, base, consts):
+        ys = []
+        for i in range(P2_W):
+            x = s[i] + p.const(consts[i])
+            cube = p.main(base + i)
+            p.assert_zero(cube - x * x * x)
+            ys.append(cube * cube * x)
+        out = _external(p, ys)
+        post = [p.main(base + P2_W + i) for i in range(P2_W)]
+        for i in range(P2_W):
+            p.assert_zero(post[i] - out[i])
+        return post
+
+    for r in range(P2_FULL):
+        s = full(s, begin[r], rc[P2_W * r:P2_W * (r + 1)])
+    for r in range(P2_PARTIAL):
+        x = s[0] + p.const(rc[2 * P2_W * P2_FULL + r])
+        cube, out = p.main(partial[r]), p.main(partial[r] + 1)
+        p.assert_zero(cube - x * x * x)
+        p.assert_zero(out - cube * cube * x)
+        s = _internal(p, [out] + s[1:], diag)
+    off = P2_W * P2_FULL
+    for r in range(P2_FULL):
+        s = full(s, end[r], rc[off + P2_W * r:off + P2_W * (r + 1)])
+    return p
+
+
+def poseidon2_trace(inputs, rc, diag):
+    """The trace of poseidon2_program for rows of 12 inputs (n x 12); the state after the
+    last round is the permutation of the row's inputs."""
+    inputs = np.asarray(inputs, dtype=np.uint64)
+    n = inputs.shape[0]
+    t = np.zeros((n, P2_COLS), dtype=object)
+    rc = [int(v) for v in rc]
+    d = [int(v) for v in diag]
+    s = [inputs[:, i].astype(object) for i in range(P2_W)]
+    t[:, :P2_W] = inputs.astype(object)
+
+    def m4(x):
+        a, b, c, e = x
+        abcd = a + b + c + e
+        return [abcd + a + 2 * b, abcd + b + 2 * c, abcd + c + 2 * e, abcd + e + 2 * a]
+
+    def ext(s):
+        y = m4(s[0:4]) + m4(s[4:8]) + m4(s[8:12])
+        sums = [y[k] + y[4 + k] + y[8 + k] for k in range(4)]
+        return [(y[i] + sums[i & 3]) % P for i in range(12)]
+
+    begin, partial, end = _p2_columns()
+    s = ext(s)
+
+    def full(s, base, consts):
+        ys = []
+        for i in range(P2_W):
+            x = (s[i] + consts[i]) % P
+            cube = x * x % P * x % P
+            t[:, base + i] = cube
+            ys.append(cube * cube % P * x % P)
+        s = ext(ys)
+        for i in range(P2_W):
+            t[:, base + P2_W + i] = s[i]
+        return s
+
+    for r in range(P2_FULL):
+        s = full(s, begin[r], rc[P2_W * r:P2_W * (r + 1)])
+    for r in range(P2_PARTIAL):
+        x = (s[0] + rc[2 * P2_W * P2_FULL + r]) % P
+        cube = x * x % P * x % P
+        out = cube * cube % P * x % P
+        t[:, partial[r]] = cube
+        t[:, partial[r] + 1] = out
+        s = [out] + s[1:]
+        total = sum(s) % P
+        s = [(s[i] * d[i] + total) % P for i in range(12)]
+    off = P2_W * P2_FULL
+    for r in range(P2_FULL):
+        s = full(s, end[r], rc[off + P2_W * r:off + P2_W * (r + 1)])
+    return t.astype(np.uint64)
+
+
+def poseidon2_output_columns():
+    """the columns holding the permutation's output (the last round's post state)"""
+    _, _, end = _p2_columns()
+    return list(range(end[-1] + P2_W, end[-1] + 2 * P2_W))

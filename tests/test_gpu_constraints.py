@@ -223,3 +223,24 @@ def test_two_instances_as_prove_multi(device):
     assert qt.root().tolist() == O.commit_quotient(O.HASH_P2, want, D, logb).root().tolist()
     for x in (qt, q, acc, tree):
         x.free()
+
+
+@pytest.mark.parametrize("n,logb,log_d", [(1 << 8, 2, 2), (1 << 12, 1, 1)])
+def test_poseidon2_air_matches_oracle(device, n, logb, log_d):
+    # a 3,047-op AIR over 248 columns (several 1,024-op device functions), on a valid trace:
+    # word for word against the oracle, and numerator / Z_H is of degree <= 2n - 3
+    rc, dg = O.p2_constants(), O.p2_diag()
+    inp = np.random.default_rng(n).integers(0, P, size=(n, 12), dtype=np.uint64)
+    trace = airs.poseidon2_trace(inp, rc, dg)
+    prog = airs.poseidon2_program(rc, dg)
+    got, want = _run_both(device, prog, trace, n, logb, log_d, (31, 37), seed=n)
+    assert np.array_equal(got, want)
+    acc = device.upload(np.zeros((n << log_d, 2), np.uint64))
+    main = device.upload(O.coset_lde_batch(trace, logb, G7))
+    G.eval_constraints(prog, main, n, log_d, G7, (31, 37), acc)
+    G.divide_by_vanishing(acc, n, 1 << log_d, G7)
+    q = acc.download().reshape(-1, 2)
+    co = np.stack([O.ntt(np.ascontiguousarray(q[:, c]), inverse=True) for c in range(2)], axis=1)
+    assert not co[2 * n - 2:].any()
+    acc.free()
+    main.free()

@@ -151,3 +151,25 @@ def test_malformed_programs_are_rejected():
     for shift in (1, P - 1):  # inside the subgroup of order n d: Z_H vanishes on the coset
         with pytest.raises(ValueError):
             O.eval_constraints(p.ops, lde, 8, 1, shift, (1, 0), publics=[0, 0, 0])
+
+
+def test_poseidon2_air_trace_is_the_permutation_and_its_quotient_is_low_degree():
+    # a Poseidon2-GL12 permutation per row (p3gpu.example_airs.poseidon2_program, 3,047 ops,
+    # 236 constraints of degree 3 over 248 columns): the trace's last round is the oracle's
+    # permutation of each row, and the quotient of a valid trace has degree <= 2n - 3
+    n, logb, log_d = 32, 2, 2
+    rc, dg = O.p2_constants(), O.p2_diag()
+    rng = np.random.default_rng(7)
+    inp = rng.integers(0, P, size=(n, 12), dtype=np.uint64)
+    trace = airs.poseidon2_trace(inp, rc, dg)
+    out = airs.poseidon2_output_columns()
+    for r in (0, 5, n - 1):
+        assert trace[r, out].tolist() == O.p2_permute(inp[r]).tolist()
+    prog = airs.poseidon2_program(rc, dg)
+    num = O.eval_constraints(prog.ops, _lde(trace, logb), n, log_d, G7, (3, 5))
+    assert _quotient_top_zero(num, n, 1 << log_d, 2 * n - 2)
+    for col in (20, 150, 247):  # an x^3 register, a partial-round output, an output cell
+        bad = trace.copy()
+        bad[9, col] = (int(bad[9, col]) + 1) % P
+        num_bad = O.eval_constraints(prog.ops, _lde(bad, logb), n, log_d, G7, (3, 5))
+        assert not _quotient_top_zero(num_bad, n, 1 << log_d, 2 * n - 2), col
