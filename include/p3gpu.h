@@ -253,8 +253,8 @@ typedef struct {
  *  accumulator: device, n d extension elements.
  * Each program is compiled once into its own gfx950 kernel (values typed base or extension,
  * kept in registers) and cached for the process; a malformed program or an index out of
- * range gives P3G_E_INVAL, an empty or assertion-free one P3G_E_EMPTY, more than 2^20 ops
- * P3G_E_RANGE, all before any launch. */
+ * range gives P3G_E_INVAL, an empty or assertion-free one P3G_E_EMPTY, more than 2^20 ops,
+ * log_d > 20 or n d > 2^31 P3G_E_RANGE, all before any launch. */
 int p3g_eval_constraints(p3g_ctx* ctx, const p3g_air_op* ops, uint64_t nops, const p3g_matrix* main,
                          const p3g_matrix* aux, const uint64_t* periodic, uint64_t periodic_rows,
                          uint64_t periodic_cols, const uint64_t* publics, uint64_t num_publics,

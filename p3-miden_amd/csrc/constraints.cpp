@@ -56,6 +56,7 @@ enum AirOp : uint32_t {
 };
 
 constexpr uint64_t AIR_MAX_OPS = uint64_t(1) << 20;
+constexpr unsigned AIR_MAX_LOG_D = 20;
 constexpr uint64_t AIR_BLOCK = 128;   // ops per basic block of a compiled program
 constexpr uint64_t AIR_FUNC = 1024;   // ops per device function (a multiple of AIR_BLOCK)
 
@@ -393,7 +394,8 @@ void eval_constraints(Ctx& ctx, const AirOpIn* ops, uint64_t nops, const DevMat&
                       uint64_t shift, ef2 alpha, ef2* acc, hipStream_t s) {
   if (n == 0 || (n & (n - 1))) throw ApiError(E_NOT_POW2, "trace height must be a power of two");
   const unsigned logn = log2_exact(n);
-  if (log_d > 31 || logn + log_d > 31) throw ApiError(E_RANGE, "quotient domain above 2^31 points");
+  if (log_d > AIR_MAX_LOG_D) throw ApiError(E_RANGE, "quotient degree above 2^20");  // Z_H takes d values
+  if (logn + log_d > 31) throw ApiError(E_RANGE, "quotient domain above 2^31 points");
   const unsigned L = logn + log_d;
   const uint64_t M = uint64_t(1) << L, d = uint64_t(1) << log_d;
   const AirShape sh = analyze(ops, nops);

@@ -177,6 +177,13 @@ def test_bad_programs_and_shapes_are_rejected(device):
     with pytest.raises(G.P3GError) as e:
         G.eval_constraints(airs.running_program(), main, n, 1, G7, (1, 0), acc, ext_inputs=[1, 2, 3, 4])
     assert e.value.code == 1  # no aux matrix
+    # quotient degree and domain limits: rejected on the host before anything touches the
+    # accumulator (a view that claims enough rows passes the binding's own size check)
+    claimed = G.DeviceMatrix(device, acc.ptr, n << 26, 2, owned=False)
+    for log_d in (21, 26):
+        with pytest.raises(G.P3GError) as e:
+            G.eval_constraints(ops, main, n, log_d, G7, (1, 0), claimed, publics=[0, 0, 0])
+        assert e.value.code == 8
     # the main matrix must hold the n d points
     with pytest.raises(G.P3GError) as e:
         G.eval_constraints(ops, main, n, 2, G7, (1, 0), acc, publics=[0, 0, 0])
