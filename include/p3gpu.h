@@ -461,6 +461,13 @@ int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket); 
  * queued requests of its shape holds it until it has been queued batch_window_ms, or
  * the batch fills, so that requests arriving together are proven together. */
 int p3g_server_set_batch_window(p3g_server* s, double batch_window_ms);
+/* Upload ahead: a stage with a stream of its own copies the host traces of up to `requests`
+ * queued requests (0 = off, the default; at most 16) into device memory, in arrival order,
+ * while the workers prove earlier ones, so that PCIe copies overlap proving.  A request's
+ * whole footprint is reserved in the budget when its copies start, and a worker then proves
+ * from the copies; results are unchanged.  No counterpart in the reference (its prover
+ * reads host matrices); for callers that hand over host traces. */
+int p3g_server_set_upload_ahead(p3g_server* s, uint32_t requests);
 int p3g_server_wait(p3g_server* s, uint64_t ticket, p3g_result_info* info);     /* blocks; returns the proof status */
 int p3g_server_poll(p3g_server* s, uint64_t ticket, int* done);
 int p3g_server_proof(p3g_server* s, uint64_t ticket, uint64_t* fields, uint64_t* commitments);

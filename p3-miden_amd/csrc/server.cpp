@@ -53,6 +53,13 @@ struct Request {
   uint64_t root[4] = {0, 0, 0, 0}, z[4] = {0, 0, 0, 0};
   std::vector<uint64_t> fields, comms;
   std::string error;
+  // device copies of host traces made ahead of admission by the upload stage
+  uint64_t trace_bytes = 0;         // host trace bytes (0 for device inputs)
+  bool reserved = false;            // footprint counted in flight ahead of admission (upload stage)
+  bool staging = false;             // the upload stage is enqueuing its copies
+  bool staged = false;              // copies enqueued: staged_ptrs and staged_ev are set
+  std::vector<void*> staged_ptrs;   // one per trace (nullptr for an empty one)
+  hipEvent_t staged_ev = nullptr;   // recorded on the upload stream after the copies
 };
 
 }  // namespace
@@ -76,8 +83,19 @@ struct p3g_server {
   double batch_window_ms = 0;
   bool stop = false;
   clk::time_point t0;
+  // upload stage: copies the host traces of up to upload_ahead queued requests into
+  // pool blocks on its own stream while the workers prove earlier requests
+  uint32_t upload_ahead = 0;
+  hipStream_t upload_stream = nullptr;
+  std::thread stager;
+  uint32_t n_staged = 0;  // staged or staging, not yet admitted
 
   void worker_loop(uint32_t w);
+  void stager_loop();
+  // gives a staged request's copies back to the pool (s: the stream whose work last used them)
+  void unstage_locked(Request* r, hipStream_t s);
+  // the device-pointer form of a staged request's traces
+  std::vector<p3g_matrix> staged_traces(const Request* r) const;
   std::deque<Request*>::iterator pick_locked();
   double batch_wait_locked(const Request* r) const;
   void take_batch_locked(Request* first, std::vector<Request*>& batch);
@@ -91,7 +109,8 @@ std::deque<Request*>::iterator p3g_server::pick_locked() {
   const auto now = clk::now();
   for (auto it = queue.begin(); it != queue.end(); ++it) {
     Request* r = *it;
-    if (inflight_bytes + r->footprint <= budget) return it;
+    // the upload stage reserved this one's memory already: it only needs a worker
+    if (r->reserved || inflight_bytes + r->footprint <= budget) return it;
     const double waited = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
     if (waited > backfill_ms) return queue.end();
   }
@@ -129,6 +148,22 @@ void p3g_server::take_batch_locked(Request* first, std::vector<Request*>& batch)
   }
 }
 
+std::vector<p3g_matrix> p3g_server::staged_traces(const Request* r) const {
+  std::vector<p3g_matrix> t = r->traces;
+  for (size_t i = 0; i < t.size(); i++)
+    if (r->staged_ptrs[i]) t[i].data = static_cast<const uint64_t*>(r->staged_ptrs[i]);
+  return t;
+}
+
+void p3g_server::unstage_locked(Request* r, hipStream_t s) {
+  for (void* p : r->staged_ptrs)
+    if (p) pool.release(p, s);
+  r->staged_ptrs.clear();
+  if (r->staged_ev) (void)hipEventDestroy(r->staged_ev);
+  r->staged_ev = nullptr;
+  r->staged = false;
+}
+
 void p3g_server::run_one(p3g_ctx* ctx, Request* r) {
   p3g_transcript* tx = nullptr;
   int rc = p3g_transcript_create(r->hash == P3G_HASH_POSEIDON2_GL12 ? P3G_CHALLENGER_DUPLEX_P2 : P3G_CHALLENGER_KECCAK,
@@ -136,9 +171,21 @@ void p3g_server::run_one(p3g_ctx* ctx, Request* r) {
   // the proof stops at its next phase boundary once the request is canceled
   ctx->c.cancel = &r->cancel;
   if (r->inject_alloc_failure) ctx->c.fail_alloc_after = 2;
-  if (rc == P3G_OK)
-    rc = p3g_prove(ctx, r->hash, &r->params, r->traces.data(), r->traces.size(), r->flags & P3G_DEVICE_PTRS, tx,
-                   r->root, r->z);
+  if (rc == P3G_OK) {
+    if (r->staged && hipStreamWaitEvent(ctx->c.stream, r->staged_ev, 0) == hipSuccess) {
+      // the traces were copied ahead: prove from the device copies
+      const std::vector<p3g_matrix> t = staged_traces(r);
+      rc = p3g_prove(ctx, r->hash, &r->params, t.data(), t.size(), P3G_DEVICE_PTRS, tx, r->root, r->z);
+    } else {
+      (void)hipGetLastError();
+      rc = p3g_prove(ctx, r->hash, &r->params, r->traces.data(), r->traces.size(), r->flags & P3G_DEVICE_PTRS, tx,
+                     r->root, r->z);
+    }
+  }
+  if (r->staged) {
+    std::lock_guard<std::mutex> g(mu);
+    unstage_locked(r, ctx->c.stream);
+  }
   ctx->c.cancel = nullptr;
   ctx->c.fail_alloc_after = -1;
   if (rc == P3G_OK && r->cancel.load()) rc = P3G_E_CANCELED;  // canceled after its last checkpoint
@@ -167,10 +214,28 @@ void p3g_server::run_batch(p3g_ctx* ctx, std::vector<Request*>& batch) {
   // a failure injected into one member fails the lock-step batch, as a real one would
   for (auto* r : batch)
     if (r->inject_alloc_failure) ctx->c.fail_alloc_after = 2;
+  // members copied ahead by the upload stage: when all were, prove from the copies
+  bool all_staged = true;
+  for (auto* r : batch) all_staged = all_staged && r->staged;
+  uint32_t in_flags = batch[0]->flags & P3G_DEVICE_PTRS;
+  if (all_staged) {
+    flat.clear();
+    for (auto* r : batch) {
+      if (hipStreamWaitEvent(ctx->c.stream, r->staged_ev, 0) != hipSuccess) rc = P3G_E_DEVICE;
+      const std::vector<p3g_matrix> tr = staged_traces(r);
+      flat.insert(flat.end(), tr.begin(), tr.end());
+    }
+    in_flags = P3G_DEVICE_PTRS;
+  }
   if (rc == P3G_OK)
-    rc = p3g_prove_many(ctx, batch[0]->hash, &batch[0]->params, flat.data(), k, t, batch[0]->flags & P3G_DEVICE_PTRS,
-                        txs.data(), roots.data(), zs.data());
+    rc = p3g_prove_many(ctx, batch[0]->hash, &batch[0]->params, flat.data(), k, t, in_flags, txs.data(), roots.data(),
+                        zs.data());
   ctx->c.fail_alloc_after = -1;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto* r : batch)
+      if (r->staged) unstage_locked(r, ctx->c.stream);
+  }
   if (rc != P3G_OK) {
     // a result depends only on its own request: when the batch fails, every member
     // runs alone, so that only the requests that fail by themselves report an error
@@ -226,7 +291,13 @@ void p3g_server::worker_loop(uint32_t w) {
         b->admitted = true;
         b->t_admit = now;
         b->worker = w;
-        inflight_bytes += b->footprint;
+        if (!b->reserved) inflight_bytes += b->footprint;
+      }
+      // a member whose copies the upload stage is enqueuing right now: wait for them
+      // (the worker would otherwise copy the same traces itself)
+      for (auto* b : batch) {
+        cv_work.wait(lk, [&] { return !b->staging; });
+        if (b->reserved) n_staged--;
       }
       running += (uint32_t)batch.size();
       admitted += batch.size();
@@ -252,6 +323,81 @@ void p3g_server::worker_loop(uint32_t w) {
     }
     cv_done.notify_all();
     cv_work.notify_all();
+  }
+}
+
+void p3g_server::stager_loop() {
+  (void)hipSetDevice(device);
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      for (;;) {
+        if (stop) return;
+        if (n_staged < upload_ahead) {
+          // the first queued request with host traces and no copies yet, in arrival
+          // order, once its copies fit in the budget beside what is held already
+          for (Request* q : queue) {
+            if (q->reserved || (q->flags & P3G_DEVICE_PTRS) || !q->trace_bytes) continue;
+            if (inflight_bytes + q->footprint <= budget) r = q;
+            break;
+          }
+        }
+        if (r) break;
+        cv_work.wait_for(lk, std::chrono::milliseconds(5));
+      }
+      // admitted as far as memory goes: its whole footprint is reserved now
+      r->reserved = true;
+      r->staging = true;
+      n_staged++;
+      inflight_bytes += r->footprint;
+      if (inflight_bytes > peak_inflight_bytes) peak_inflight_bytes = inflight_bytes;
+    }
+    // enqueue the copies outside the lock; a failure leaves the request to its worker
+    std::vector<void*> ptrs(r->traces.size(), nullptr);
+    hipEvent_t ev = nullptr;
+    bool ok = true;
+    try {
+      for (size_t i = 0; i < r->traces.size(); i++) {
+        const uint64_t bytes = 8 * r->traces[i].height * r->traces[i].width;
+        if (!bytes) continue;
+        ptrs[i] = pool.alloc(bytes, upload_stream);
+        if (hipMemcpyAsync(ptrs[i], r->traces[i].data, bytes, hipMemcpyHostToDevice, upload_stream) != hipSuccess)
+          ok = false;
+      }
+      if (ok && (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+                 hipEventRecord(ev, upload_stream) != hipSuccess))
+        ok = false;
+    } catch (...) {
+      ok = false;
+    }
+    if (!ok) (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> g(mu);
+      r->staging = false;
+      r->staged_ptrs = ptrs;
+      r->staged_ev = ev;
+      r->staged = true;
+      // without copies (a failure) the worker uploads the traces itself, within the
+      // reservation
+      if (!ok || r->cancel.load()) unstage_locked(r, upload_stream);
+      if (r->cancel.load() && !r->admitted) {
+        // canceled while its copies were being enqueued: complete the cancellation
+        for (auto q = queue.begin(); q != queue.end(); ++q)
+          if (*q == r) {
+            queue.erase(q);
+            break;
+          }
+        r->status = P3G_E_CANCELED;
+        r->error = "request canceled";
+        r->done = true;
+        r->t_admit = r->t_done = clk::now();
+        n_staged--;
+        inflight_bytes -= r->footprint;
+      }
+    }
+    cv_work.notify_all();
+    cv_done.notify_all();
   }
 }
 
@@ -281,6 +427,25 @@ int p3g_server_set_batch_window(p3g_server* s, double batch_window_ms) {
   if (!s || !(batch_window_ms >= 0) || batch_window_ms > 10000) return P3G_E_INVAL;
   std::lock_guard<std::mutex> g(s->mu);
   s->batch_window_ms = batch_window_ms;
+  s->cv_work.notify_all();
+  return P3G_OK;
+}
+
+int p3g_server_set_upload_ahead(p3g_server* s, uint32_t requests) {
+  if (!s || requests > 16) return P3G_E_INVAL;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (requests && !s->upload_stream) {
+      (void)hipSetDevice(s->device);
+      if (hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        s->upload_stream = nullptr;
+        return P3G_E_DEVICE;
+      }
+      s->stager = std::thread([s] { s->stager_loop(); });
+    }
+    s->upload_ahead = requests;
+  }
   s->cv_work.notify_all();
   return P3G_OK;
 }
@@ -347,6 +512,8 @@ int p3g_server_submit(p3g_server* s, const p3g_request* req, uint64_t* ticket) {
   r->flags = req->flags;
   r->inject_alloc_failure = (req->flags & P3G_INJECT_ALLOC_FAILURE) != 0;
   r->footprint = fp;
+  if (!(req->flags & P3G_DEVICE_PTRS))
+    for (uint64_t i = 0; i < req->k; i++) r->trace_bytes += 8 * req->traces[i].height * req->traces[i].width;
   {
     char buf[96];
     const p3g_pcs_params& q = req->params;
@@ -426,12 +593,19 @@ int p3g_server_cancel(p3g_server* s, uint64_t ticket) {
     if (it == s->reqs.end()) return P3G_E_RANGE;
     Request* r = it->second;
     if (r->done) return P3G_OK;  // finished first: its result stands
-    if (!r->admitted) {
+    if (!r->admitted && r->staging) {
+      r->cancel.store(1);  // the upload stage completes the cancellation when its copies are enqueued
+    } else if (!r->admitted) {
       for (auto q = s->queue.begin(); q != s->queue.end(); ++q)
         if (*q == r) {
           s->queue.erase(q);
           break;
         }
+      if (r->staged) s->unstage_locked(r, s->upload_stream);
+      if (r->reserved) {
+        s->n_staged--;
+        s->inflight_bytes -= r->footprint;
+      }
       r->status = P3G_E_CANCELED;
       r->error = "request canceled";
       r->done = true;
@@ -487,8 +661,15 @@ void p3g_server_destroy(p3g_server* s) {
   if (!s) return;
   {
     std::unique_lock<std::mutex> lk(s->mu);
-    // queued requests are dropped; running ones finish first
+    // queued requests are dropped; running ones (and copies being enqueued) finish first
+    s->cv_work.wait(lk, [&] {
+      for (auto* r : s->queue)
+        if (r->staging) return false;
+      return true;
+    });
     for (auto* r : s->queue) {
+      if (r->staged) s->unstage_locked(r, s->upload_stream);
+      if (r->reserved) s->inflight_bytes -= r->footprint;
       r->status = P3G_E_INVAL;
       r->done = true;
     }
@@ -499,6 +680,12 @@ void p3g_server_destroy(p3g_server* s) {
   s->cv_work.notify_all();
   s->cv_done.notify_all();
   for (auto& t : s->threads) t.join();
+  if (s->stager.joinable()) s->stager.join();
+  if (s->upload_stream) {
+    (void)hipStreamSynchronize(s->upload_stream);
+    s->pool.forget_stream(s->upload_stream);
+    (void)hipStreamDestroy(s->upload_stream);
+  }
   for (auto& kv : s->reqs) delete kv.second;
   for (auto* c : s->ctxs) p3g_ctx_destroy(c);
   delete s;

@@ -201,6 +201,7 @@ def _load():
         "p3g_tree_free": (None, [_vp]),
         "p3g_commit_traces": (C.c_int, [_vp, C.c_uint32, C.POINTER(_Matrix), C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
         "p3g_server_set_batch_window": (C.c_int, [_vp, C.c_double]),
+        "p3g_server_set_upload_ahead": (C.c_int, [_vp, C.c_uint32]),
         "p3g_quotient_extend_scale": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, _u64p]),
         "p3g_quotient_divide_vanishing": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint64, C.c_uint64]),
         "p3g_commit_quotient": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
@@ -1010,7 +1011,10 @@ class Server:
     footprint within an HBM budget, continuous batching of arriving requests."""
 
     def __init__(self, device: int = 0, workers: int = 4, hbm_budget_bytes: int = 250 * 10**9,
-                 backfill_ms: float = 50.0, max_batch: int = 64, batch_window_ms: float = 0.0):
+                 backfill_ms: float = 50.0, max_batch: int = 64, batch_window_ms: float = 0.0,
+                 upload_ahead: int = 0):
+        """upload_ahead: how many queued requests' host traces a stage of their own copies to the
+        device ahead of admission, overlapping PCIe copies with proving (0: off)."""
         h = _vp()
         _check(lib().p3g_server_create(device, workers, int(hbm_budget_bytes), float(backfill_ms), int(max_batch),
                                        C.byref(h)), "server_create")
@@ -1018,6 +1022,8 @@ class Server:
         self._keep = {}
         if batch_window_ms:
             _check(lib().p3g_server_set_batch_window(h, float(batch_window_ms)), "server_set_batch_window")
+        if upload_ahead:
+            _check(lib().p3g_server_set_upload_ahead(h, int(upload_ahead)), "server_set_upload_ahead")
 
     def submit(self, traces, params: PcsParams, hash_kind: int = HASH_POSEIDON2, inject_alloc_failure=False) -> int:
         """Queue a proof of `traces` (host arrays or DeviceMatrix); returns a ticket.
@@ -1049,7 +1055,8 @@ class Server:
             self._keep.pop(ticket, None)
 
     def cancel(self, ticket: int):
-        """Cancel a request: queued, it completes at once; running, at its next phase boundary.
+        """Cancel a request: queued, it completes at once (or, while the upload stage is enqueuing
+        its copies, as soon as they are enqueued); running, at its next phase boundary.
         Its result() then raises P3GError with code 9 (P3G_E_CANCELED), unless it had already finished."""
         _check(lib().p3g_server_cancel(self.h, ticket), "server_cancel")
 

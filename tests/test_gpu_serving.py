@@ -256,3 +256,66 @@ def test_submit_rejects_null_trace_data_and_bad_params():
     req2 = G._Request(G.HASH_POSEIDON2, bad, C.cast(m2, C.POINTER(G._Matrix)), 1, 0)
     assert G.lib().p3g_server_submit(srv.h, C.byref(req2), G._p(t)) == 1
     srv.close()
+
+
+@pytest.mark.parametrize("workers,max_batch", [(1, 1), (3, 1), (2, 8)])
+def test_upload_ahead_results_identical_to_the_oracle(device, workers, max_batch):
+    # the upload stage copies queued host traces to the device ahead of admission; device-
+    # input requests skip it.  Every result must be the oracle's, and the budget must hold.
+    reqs = _requests(11, 10)
+    # two requests with device inputs among the host ones
+    dev_reqs = [(G.HASH_POSEIDON2, (1, 1, 2, 0, 0, 20, 0), [O.rand_matrix(950 + i, 1 << 11, 10)]) for i in range(2)]
+    dev_in = [device.upload(tr[0]) for _, _, tr in dev_reqs]
+    reqs = reqs[:3] + dev_reqs[:1] + reqs[3:6] + dev_reqs[1:] + reqs[6:]
+    srv = G.Server(0, workers=workers, hbm_budget_bytes=8 << 30, max_batch=max_batch, upload_ahead=3)
+    tickets = []
+    for kind, p, tr in reqs:
+        d = [m for (_, _, t), m in zip(dev_reqs, dev_in) if t is tr]
+        tickets.append(srv.submit(d if d else tr, G.PcsParams(*p), kind))
+    # same-shaped host requests, which batch together when max_batch > 1
+    same = [O.rand_matrix(900 + i, 1 << 10, 16) for i in range(6)]
+    sp = (1, 1, 2, 0, 0, 20, 0)
+    tickets += [srv.submit([t], G.PcsParams(*sp)) for t in same]
+    print(f"upload ahead: workers={workers} max_batch={max_batch}", flush=True)
+    res = [srv.result(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(tickets) and st["peak_inflight_bytes"] <= st["budget_bytes"]
+    assert st["inflight_bytes"] == 0
+    for (kind, p, tr), r in zip(reqs, res):
+        _check_oracle(r, kind, p, tr)
+    for t, r in zip(same, res[len(reqs):]):
+        _check_oracle(r, G.HASH_POSEIDON2, sp, [t])
+    for m in dev_in:
+        m.free()
+
+
+def test_upload_ahead_within_a_tight_budget_and_with_cancels():
+    # a budget for two requests at a time and a lookahead of four: staged requests reserve
+    # their footprint, so the in-flight bytes never pass the budget; canceled queued requests
+    # (staged or not) give their reservation back and the rest finish bit-exact
+    reqs = [(G.HASH_POSEIDON2, (2, 2, 2, 0, 0, 30, 0), [O.rand_matrix(70 + i, 1 << 14, 24)]) for i in range(8)]
+    fp = G.request_footprint(G.HASH_POSEIDON2, G.PcsParams(*reqs[0][1]), reqs[0][2])
+    budget = int(2.5 * fp)
+    srv = G.Server(0, workers=2, hbm_budget_bytes=budget, max_batch=1, upload_ahead=4)
+    tickets = [srv.submit(tr, G.PcsParams(*p), kind) for kind, p, tr in reqs]
+    srv.cancel(tickets[5])
+    srv.cancel(tickets[7])
+    out = {}
+    for i, t in enumerate(tickets):
+        if i in (5, 7):
+            try:
+                srv.result(t)
+                out[i] = "finished"  # it may have completed before the cancel
+            except G.P3GError as e:
+                assert e.code == 9, e
+                out[i] = "canceled"
+        else:
+            out[i] = srv.result(t)
+    st = srv.stats()
+    srv.close()
+    assert st["peak_inflight_bytes"] <= budget and st["inflight_bytes"] == 0 and st["queued"] == 0
+    assert st["peak_running"] <= 2
+    for i, (kind, p, tr) in enumerate(reqs):
+        if i not in (5, 7):
+            _check_oracle(out[i], kind, p, tr)
