@@ -472,7 +472,8 @@ int p3g_server_wait(p3g_server* s, uint64_t ticket, p3g_result_info* info);     
 int p3g_server_poll(p3g_server* s, uint64_t ticket, int* done);
 int p3g_server_proof(p3g_server* s, uint64_t ticket, uint64_t* fields, uint64_t* commitments);
 int p3g_server_release(p3g_server* s, uint64_t ticket);
-/* Cancels a request: a queued one completes at once with P3G_E_CANCELED; a running one
+/* Cancels a request: a queued one completes at once with P3G_E_CANCELED (one whose copies
+ * the upload stage is enqueuing, as soon as they are enqueued); a running one
  * stops at its next phase boundary (after its commitment, its DEEP quotient, each FRI
  * round, before its queries) and completes with P3G_E_CANCELED, its device memory
  * released.  A member of a lock-step batch does not stop its batch-mates: its result is
