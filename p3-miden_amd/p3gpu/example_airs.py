@@ -8,7 +8,9 @@ with trace generators: the constraint-evaluation tests and the bench use them.
              column 0 zero, the rest random (generate_dummy_trace, :122-144);
   running:   a main column v, a column w = v * p with p a period-4 periodic column, and an
              extension aux column s, the running sum of r * v with r the permutation
-             randomness, ending in the permutation value (degree 2).
+             randomness, ending in the permutation value (degree 2);
+  tiny:      the reference's TinyAir test AIR (x -> x^4 on main and aux, periodic columns);
+  poseidon2: a Poseidon2-GL12 permutation per row (3,047 ops over 248 columns).
 """
 import numpy as np
 
@@ -278,3 +280,52 @@ def poseidon2_output_columns():
     """the columns holding the permutation's output (the last round's post state)"""
     _, _, end = _p2_columns()
     return list(range(end[-1] + P2_W, end[-1] + 2 * P2_W))
+
+
+# --------------------------------------------------------------------------- TinyAir
+def tiny_program(num_periodic):
+    """The reference's TinyAir (p3-miden-lifted-stark/tests/tiny_air.rs:27-118), over
+    Goldilocks: main[0] starts at public value 0 and each row is the previous one^4; every
+    periodic column (pattern [1, 0, ..., 0, 1] per period) is 1 on the first and the last
+    row; the aux column starts at the permutation randomness and each row is the previous
+    one^4.  Constraints in the AIR's emission order."""
+    p = AirProgram()
+    local, nxt = p.main(0), p.main(0, next=True)
+    first, last, trans = p.is_first_row(), p.is_last_row(), p.is_transition()
+    p.assert_zero(first * (local - p.public(0)))
+    sq = local * local
+    p.assert_zero(trans * (nxt - sq * sq))
+    for c in range(num_periodic):
+        v = p.periodic(c)
+        p.assert_zero(first * (v - 1))
+        p.assert_zero(last * (v - 1))
+    a, a2 = p.aux(0), p.aux(0, next=True)
+    p.assert_zero_ext(first * (a - p.ext_input(0)))
+    sq = a * a
+    p.assert_zero_ext(trans * (a2 - sq * sq))
+    return p
+
+
+def tiny_periodic_columns(periods):
+    """TinyAir::new's periodic columns: 1 at the first and last position of each period."""
+    cols = []
+    for q in periods:
+        c = [0] * q
+        c[0] = c[q - 1] = 1
+        cols.append(c)
+    return cols
+
+
+def tiny_trace(n, start, challenge):
+    """main (n x 1): start^(4^i); aux (n x 2, c0 and c1): challenge^(4^i) in F[x]/(x^2 - 7)
+    (TinyAuxBuilder, tiny_air.rs:121-140)."""
+    main = np.zeros((n, 1), np.uint64)
+    aux = np.zeros((n, 2), np.uint64)
+    x, e = int(start) % P, (int(challenge[0]) % P, int(challenge[1]) % P)
+    for i in range(n):
+        main[i, 0] = x
+        aux[i] = e
+        x = pow(x, 4, P)
+        e = ef_mul(e, e)
+        e = ef_mul(e, e)
+    return main, aux

@@ -251,3 +251,44 @@ def test_poseidon2_air_matches_oracle(device, n, logb, log_d):
     assert not co[2 * n - 2:].any()
     acc.free()
     main.free()
+
+
+@pytest.mark.parametrize("periods,heights", airs.TINY_CASES)
+def test_tiny_air_cases_match_oracle(device, periods, heights):
+    # the reference's TinyAir cases (tiny_air.rs) through prove_multi's quotient loop on the
+    # device: main traces and aux traces in one tree each (nested coset shifts), every
+    # instance evaluated in place with its periodic table, one division by Z_H.  Word for
+    # word the oracle's, and the quotient is of degree < 4N (valid traces).
+    log_d, logb = 3, 3
+    D, N = 1 << log_d, max(heights)
+    challenge = (5, 9)
+    alpha, beta = (321, 123), (987, 654)
+    prog = airs.tiny_program(len(periods))
+    traces = [airs.tiny_trace(h, idx + 2, challenge) for idx, h in enumerate(heights)]
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    mtree = G.commit_traces(cfg, [m for m, _ in traces], logb)
+    atree = G.commit_traces(cfg, [a for _, a in traces], logb)
+    acc = device.alloc_matrix(N * D, 2)
+    want = np.zeros((0, 2), np.uint64)
+    m = 0
+    for idx, (h, (main, aux)) in enumerate(zip(heights, traces)):
+        shift = pow(G7, N // h, P)
+        G.cyclic_extend_and_scale(acc, m, h * D, beta)
+        want = O.cyclic_extend_scale(want, beta, h * D)
+        per = airs.tiny_periodic_table(periods, h, log_d, shift) if periods else None
+        dm = G.DeviceMatrix(device, *mtree.matrix(idx), owned=False)
+        da = G.DeviceMatrix(device, *atree.matrix(idx), owned=False)
+        G.eval_constraints(prog, dm, h, log_d, shift, alpha, acc, aux=da, periodic=per, publics=[idx + 2],
+                           ext_inputs=list(challenge))
+        want = O.eval_constraints(prog.ops, O.coset_lde_batch(main, logb, shift), h, log_d, shift, alpha, acc=want,
+                                  aux=O.coset_lde_batch(aux, logb, shift), periodic=per, publics=[idx + 2],
+                                  ext_inputs=list(challenge))
+        m = h * D
+    G.divide_by_vanishing(acc, N, D, G7)
+    want = O.divide_by_vanishing(want, D, G7)
+    got = acc.download().reshape(-1, 2)
+    assert np.array_equal(got, want)
+    co = np.stack([O.ntt(np.ascontiguousarray(got[:, c]), inverse=True) for c in range(2)], axis=1)
+    assert not co[4 * N:].any()
+    for x in (acc, mtree, atree):
+        x.free()

@@ -173,3 +173,43 @@ def test_poseidon2_air_trace_is_the_permutation_and_its_quotient_is_low_degree()
         bad[9, col] = (int(bad[9, col]) + 1) % P
         num_bad = O.eval_constraints(prog.ops, _lde(bad, logb), n, log_d, G7, (3, 5))
         assert not _quotient_top_zero(num_bad, n, 1 << log_d, 2 * n - 2), col
+
+
+def _tiny_quotient(periods, heights, publics=None, log_d=3, logb=3, challenge=(5, 9)):
+    """prove_multi's quotient loop (prover/mod.rs:355-432) over TinyAir instances with the
+    oracle: nested coset shifts, beta extension, evaluation, one division by Z_H"""
+    D, N = 1 << log_d, max(heights)
+    prog = airs.tiny_program(len(periods))
+    acc = np.zeros((0, 2), np.uint64)
+    for idx, h in enumerate(heights):
+        main, aux = airs.tiny_trace(h, idx + 2, challenge)
+        shift = pow(G7, N // h, P)
+        acc = O.cyclic_extend_scale(acc, (987, 654), h * D)
+        per = airs.tiny_periodic_table(periods, h, log_d, shift) if periods else None
+        pub = [idx + 2] if publics is None else publics[idx]
+        acc = O.eval_constraints(prog.ops, _lde(main, logb, shift), h, log_d, shift, (321, 123), acc=acc,
+                                 aux=_lde(aux, logb, shift), periodic=per, publics=pub, ext_inputs=list(challenge))
+    return O.divide_by_vanishing(acc, D, G7), N, D
+
+
+@pytest.mark.parametrize("periods,heights", airs.TINY_CASES)
+def test_tiny_air_cases_give_a_low_degree_quotient(periods, heights):
+    # the reference's TinyAir cases: valid traces, so the combined quotient of the instances
+    # (lifted to the largest height) is a polynomial of degree < 4N (constraint degree 5)
+    q, N, D = _tiny_quotient(periods, heights)
+    co = _coset_coeffs(q, G7)
+    assert not co[4 * N:].any()
+
+
+def test_tiny_air_wrong_public_value_or_periodic_pattern_is_caught():
+    # a wrong start value in the second instance
+    q, N, D = _tiny_quotient([2, 4], [4, 8], publics=[[2], [4]])
+    assert _coset_coeffs(q, G7)[4 * N:].any()
+    # a periodic column that is 0 on the last row (period 4 pattern [1, 0, 0, 0])
+    orig = airs.tiny_periodic_columns
+    try:
+        airs.tiny_periodic_columns = lambda periods: [[1] + [0] * (p - 1) for p in periods]
+        q, N, D = _tiny_quotient([4], [8])
+    finally:
+        airs.tiny_periodic_columns = orig
+    assert _coset_coeffs(q, G7)[4 * N:].any()
