@@ -10,6 +10,7 @@ with trace generators: the constraint-evaluation tests and the bench use them.
              extension aux column s, the running sum of r * v with r the permutation
              randomness, ending in the permutation value (degree 2);
   tiny:      the reference's TinyAir test AIR (x -> x^4 on main and aux, periodic columns);
+  bus:       the reference's BusTestAir (two aux columns, randomness and aux values);
   poseidon2: a Poseidon2-GL12 permutation per row (3,047 ops over 248 columns).
 """
 import numpy as np
@@ -329,3 +330,51 @@ def tiny_trace(n, start, challenge):
         e = ef_mul(e, e)
         e = ef_mul(e, e)
     return main, aux
+
+
+# --------------------------------------------------------------------------- BusTestAir
+def bus_program():
+    """The reference's BusTestAir (p3-miden-lifted-stark/tests/bus_test.rs:43-150) over
+    Goldilocks: main[0] starts at public value 0 and each row is the previous one^4; two aux
+    columns with two randomness values c0, c1 and two aux values: on the first row
+    aux0 (pi_0 + c0) = 1 and aux1 = pi_1 + c1 (pi_0, pi_1 public values 1 and 2), both
+    columns constant, and equal to the aux values on the last row.  Extension inputs:
+    c0, c1, then the aux values."""
+    p = AirProgram()
+    pv0, pv1, pv2 = p.public(0), p.public(1), p.public(2)
+    local, nxt = p.main(0), p.main(0, next=True)
+    first, last, trans = p.is_first_row(), p.is_last_row(), p.is_transition()
+    p.assert_zero(first * (local - pv0))
+    sq = local * local
+    p.assert_zero(trans * (nxt - sq * sq))
+    c0, c1 = p.ext_input(0), p.ext_input(1)
+    av0, av1 = p.ext_input(2), p.ext_input(3)
+    a0, a1 = p.aux(0), p.aux(1)
+    p.assert_zero_ext(first * (a0 * (pv1 + c0) - 1))
+    p.assert_zero_ext(first * (a1 - (pv2 + c1)))
+    p.assert_zero_ext(trans * (p.aux(0, next=True) - a0))
+    p.assert_zero_ext(trans * (p.aux(1, next=True) - a1))
+    p.assert_zero_ext(last * (a0 - av0))
+    p.assert_zero_ext(last * (a1 - av1))
+    return p
+
+
+def ef_inv(a):
+    """1 / (a0 + a1 X) in F[x]/(x^2 - 7): (a0 - a1 X) / (a0^2 - 7 a1^2)"""
+    norm = (a[0] * a[0] - W * a[1] * a[1]) % P
+    ni = pow(norm, P - 2, P)
+    return (a[0] * ni % P, (P - a[1]) % P * ni % P)
+
+
+def bus_trace(n, start, pi0, pi1, c0, c1):
+    """main (n x 1): start^(4^i); aux (n x 4): the constant columns 1/(pi_0 + c0) and
+    pi_1 + c1 as (c0, c1) pairs (BusTestAuxBuilder, bus_test.rs:154-183); the aux values."""
+    main = np.zeros((n, 1), np.uint64)
+    x = int(start) % P
+    for i in range(n):
+        main[i, 0] = x
+        x = pow(x, 4, P)
+    col0 = ef_inv(((int(pi0) + c0[0]) % P, c0[1] % P))
+    col1 = ((int(pi1) + c1[0]) % P, c1[1] % P)
+    aux = np.tile(np.array([col0[0], col0[1], col1[0], col1[1]], dtype=np.uint64), (n, 1))
+    return main, aux, [col0, col1]

@@ -292,3 +292,28 @@ def test_tiny_air_cases_match_oracle(device, periods, heights):
     assert not co[4 * N:].any()
     for x in (acc, mtree, atree):
         x.free()
+
+
+@pytest.mark.parametrize("n", [8, 1 << 12])
+def test_bus_air_matches_oracle(device, n):
+    # the reference's BusTestAir (bus_test.rs, bus_identity_check's values) on committed
+    # main and aux trees: word for word the oracle's, and the quotient is of degree < 4n
+    log_d, logb = 3, 3
+    main, aux, vals = airs.bus_trace(n, 2, 42, 67, (11, 13), (17, 19))
+    ext = [11, 13, 17, 19, vals[0][0], vals[0][1], vals[1][0], vals[1][1]]
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    mt, at = G.commit_traces(cfg, [main], logb), G.commit_traces(cfg, [aux], logb)
+    dm = G.DeviceMatrix(device, *mt.matrix(0), owned=False)
+    da = G.DeviceMatrix(device, *at.matrix(0), owned=False)
+    acc = device.upload(np.zeros((n << log_d, 2), np.uint64))
+    G.eval_constraints(airs.bus_program(), dm, n, log_d, G7, (5, 6), acc, aux=da, publics=[2, 42, 67], ext_inputs=ext)
+    want = O.eval_constraints(airs.bus_program().ops, O.coset_lde_batch(main, logb, G7), n, log_d, G7, (5, 6),
+                              aux=O.coset_lde_batch(aux, logb, G7), publics=[2, 42, 67], ext_inputs=ext)
+    got = acc.download().reshape(-1, 2)
+    assert np.array_equal(got, want)
+    G.divide_by_vanishing(acc, n, 1 << log_d, G7)
+    q = acc.download().reshape(-1, 2)
+    co = np.stack([O.ntt(np.ascontiguousarray(q[:, c]), inverse=True) for c in range(2)], axis=1)
+    assert not co[4 * n:].any()
+    for x in (acc, mt, at):
+        x.free()

@@ -213,3 +213,25 @@ def test_tiny_air_wrong_public_value_or_periodic_pattern_is_caught():
     finally:
         airs.tiny_periodic_columns = orig
     assert _coset_coeffs(q, G7)[4 * N:].any()
+
+
+def _bus_quotient(n, publics, ext_fix=None, log_d=3, logb=3):
+    main, aux, vals = airs.bus_trace(n, 2, 42, 67, (11, 13), (17, 19))
+    ext = [11, 13, 17, 19, vals[0][0], vals[0][1], vals[1][0], vals[1][1]]
+    if ext_fix:
+        ext = ext_fix(ext)
+    num = O.eval_constraints(airs.bus_program().ops, _lde(main, logb), n, log_d, G7, (5, 6), aux=_lde(aux, logb),
+                             publics=publics, ext_inputs=ext)
+    return O.divide_by_vanishing(num, 1 << log_d, G7)
+
+
+def test_bus_air_gives_a_low_degree_quotient_and_catches_wrong_inputs():
+    # the reference's BusTestAir (bus_test.rs) with its bus_identity_check values: start 2,
+    # pi_0 = 42, pi_1 = 67, height 8 (and 64).  Valid: degree < 4n.  A wrong pi_0, a wrong
+    # aux value or a wrong challenge breaks a constraint.
+    for n in (8, 64):
+        assert not _coset_coeffs(_bus_quotient(n, [2, 42, 67]), G7)[4 * n:].any()
+    n = 8
+    assert _coset_coeffs(_bus_quotient(n, [2, 99, 67]), G7)[4 * n:].any()
+    assert _coset_coeffs(_bus_quotient(n, [2, 42, 67], lambda e: e[:6] + [(e[6] + 1) % P, e[7]]), G7)[4 * n:].any()
+    assert _coset_coeffs(_bus_quotient(n, [2, 42, 67], lambda e: [12] + e[1:]), G7)[4 * n:].any()
