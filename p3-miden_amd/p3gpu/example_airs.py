@@ -11,6 +11,7 @@ with trace generators: the constraint-evaluation tests and the bench use them.
              randomness, ending in the permutation value (degree 2);
   tiny:      the reference's TinyAir test AIR (x -> x^4 on main and aux, periodic columns);
   bus:       the reference's BusTestAir (two aux columns, randomness and aux values);
+  padding:   the reference's PaddingAir (widths one more than the LMCS alignment);
   poseidon2: a Poseidon2-GL12 permutation per row (3,047 ops over 248 columns).
 """
 import numpy as np
@@ -378,3 +379,30 @@ def bus_trace(n, start, pi0, pi1, c0, c1):
     col1 = ((int(pi1) + c1[0]) % P, c1[1] % P)
     aux = np.tile(np.array([col0[0], col0[1], col1[0], col1[1]], dtype=np.uint64), (n, 1))
     return main, aux, [col0, col1]
+
+
+# --------------------------------------------------------------------------- PaddingAir
+def padding_program():
+    """The reference's PaddingAir (p3-miden-lifted-stark/tests/multi_aux_alignment.rs:23-80):
+    main[0] equals public value 0 on the first row and is constant; aux column 0 equals the
+    randomness on the first row and is constant.  The other columns (the AIR is made one
+    wider than the LMCS alignment) are unconstrained."""
+    p = AirProgram()
+    first, trans = p.is_first_row(), p.is_transition()
+    local = p.main(0)
+    p.assert_zero(first * (local - p.public(0)))
+    p.assert_zero(trans * (p.main(0, next=True) - local))
+    a = p.aux(0)
+    p.assert_zero_ext(first * (a - p.ext_input(0)))
+    p.assert_zero_ext(trans * (p.aux(0, next=True) - a))
+    return p
+
+
+def padding_trace(n, width, aux_width, start, challenge):
+    """main (n x width): start then zeros on every row; aux (n x 2 aux_width): the
+    randomness then zeros on every row (PaddingAuxBuilder, multi_aux_alignment.rs:82-102)."""
+    main = np.zeros((n, width), np.uint64)
+    main[:, 0] = int(start) % P
+    aux = np.zeros((n, 2 * aux_width), np.uint64)
+    aux[:, 0], aux[:, 1] = int(challenge[0]) % P, int(challenge[1]) % P
+    return main, aux

@@ -317,3 +317,41 @@ def test_bus_air_matches_oracle(device, n):
     assert not co[4 * n:].any()
     for x in (acc, mt, at):
         x.free()
+
+
+def test_padding_air_multi_trace_matches_oracle(device):
+    # the reference's multi_trace_with_aux_padding (multi_aux_alignment.rs:118-150): widths one
+    # more than the LMCS alignment (main 9 columns, aux 9 extension columns), instances of
+    # heights 8 and 16 through the quotient loop, read in place from the aligned trees
+    cfg = G.LmcsConfig(device, G.HASH_POSEIDON2)
+    width = aux_width = cfg.alignment() + 1
+    log_d, logb = 1, 1
+    D, N = 1 << log_d, 16
+    challenge, alpha, beta = (3, 4), (8, 9), (10, 11)
+    heights = [8, 16]
+    traces = [airs.padding_trace(h, width, aux_width, idx + 2, challenge) for idx, h in enumerate(heights)]
+    mtree = G.commit_traces(cfg, [m for m, _ in traces], logb)
+    atree = G.commit_traces(cfg, [a for _, a in traces], logb)
+    prog = airs.padding_program()
+    acc = device.alloc_matrix(N * D, 2)
+    want = np.zeros((0, 2), np.uint64)
+    m = 0
+    for idx, (h, (main, aux)) in enumerate(zip(heights, traces)):
+        shift = pow(G7, N // h, P)
+        G.cyclic_extend_and_scale(acc, m, h * D, beta)
+        want = O.cyclic_extend_scale(want, beta, h * D)
+        dm = G.DeviceMatrix(device, *mtree.matrix(idx), owned=False)
+        da = G.DeviceMatrix(device, *atree.matrix(idx), owned=False)
+        G.eval_constraints(prog, dm, h, log_d, shift, alpha, acc, aux=da, publics=[idx + 2], ext_inputs=list(challenge))
+        want = O.eval_constraints(prog.ops, O.coset_lde_batch(main, logb, shift), h, log_d, shift, alpha, acc=want,
+                                  aux=O.coset_lde_batch(aux, logb, shift), publics=[idx + 2], ext_inputs=list(challenge))
+        m = h * D
+    G.divide_by_vanishing(acc, N, D, G7)
+    want = O.divide_by_vanishing(want, D, G7)
+    got = acc.download().reshape(-1, 2)
+    assert np.array_equal(got, want)
+    # constraint degree 2: the quotient has degree < N
+    co = np.stack([O.ntt(np.ascontiguousarray(got[:, c]), inverse=True) for c in range(2)], axis=1)
+    assert not co[N:].any()
+    for x in (acc, mtree, atree):
+        x.free()
