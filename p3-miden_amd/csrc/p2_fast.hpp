@@ -71,6 +71,89 @@ P3G_HD uint64_t sbox_nc(uint64_t x) {
   return mul_nc(x3, x4);
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// a*b (+ s) mod p as some value < 2^64, in 12 (13) single-issue VALU
+// instructions instead of 16 (19): the carry-outs and 64-bit addends of
+// v_mad_u64_u32 do the carry propagation that otherwise takes add/addc pairs
+// and selects.  With a = a0 + 2^32 a1, b = b0 + 2^32 b1:
+//   P + cp 2^64 = a0 b0 + s                (cp = 0 without s)
+//   M           = a1 b0 + P1 + 2^32 cp     (< 2^64: P1 <= 2^32 - 2 when cp = 1)
+//   N + cn 2^64 = a0 b1 + M
+//   R           = a1 b1 + N1               (<= 2^64 - 2^32)
+// so a b + s = lo + 2^64 R + 2^96 cn with lo = P0 + 2^32 N0, and since
+// 2^64 == eps and 2^96 == -1 (mod p):  a b + s == lo + eps R0 - R1 - cn.
+//   T + c 2^64 = lo + eps R0               (v_mad_u64_u32 with carry-out c)
+//   U - b 2^64 = T - R1 - cn               (borrow chain, borrow b)
+// and the result is U + (c - b) eps mod 2^64, which lies in [0, 2^64) in all
+// four cases of (c, b) (T < 2^64 - 2^33 when c = 1, U >= 2^64 - 2^32 when b = 1).
+// The fold is one 64-bit add of adj = {c eps + b, (b & !c) eps}.
+// The assembly is arithmetic only; every SGPR a VALU instruction writes is read
+// two or more wait states later (s_nop), as gfx950 requires, and no SGPR
+// written here is read outside these statements.
+__device__ __forceinline__ uint64_t mul_tail(uint32_t a0, uint32_t a1, uint32_t b1, uint32_t P0, uint64_t M) {
+  uint64_t N, cn;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(N), "=s"(cn) : "v"(a0), "v"(b1), "v"(M));
+  const uint64_t R = (uint64_t)a1 * b1 + (N >> 32);
+  const uint64_t lo = (N << 32) | P0;
+  uint64_t T, c;
+  asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(T), "=s"(c) : "v"((uint32_t)R), "v"(lo));
+  uint32_t u0, u1, al, ah;
+  uint64_t sb, sx;
+  asm("s_nop 1\n\t"
+      "v_subb_co_u32 %0, %4, %6, %8, %9\n\t"
+      "v_cndmask_b32 %2, 0, -1, %10\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32 %1, %4, %7, 0, %4\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %2, %5, %2, 0, %4\n\t"
+      "v_cndmask_b32 %3, 0, -1, %4\n\t"
+      "v_cndmask_b32 %3, %3, 0, %10"
+      : "=&v"(u0), "=&v"(u1), "=&v"(al), "=&v"(ah), "=&s"(sb), "=&s"(sx)
+      : "v"((uint32_t)T), "v"((uint32_t)(T >> 32)), "v"((uint32_t)(R >> 32)), "s"(cn), "s"(c));
+  const uint64_t U = ((uint64_t)u1 << 32) | u0;
+  const uint64_t adj = ((uint64_t)ah << 32) | al;
+  uint64_t r;
+  asm("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(r) : "v"(U), "v"(adj));
+  return r;
+}
+
+__device__ __forceinline__ uint64_t mul_fast(uint64_t a, uint64_t b) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t P = (uint64_t)a0 * b0;
+  const uint64_t M = (uint64_t)a1 * b0 + (P >> 32);
+  return mul_tail(a0, a1, b1, (uint32_t)P, M);
+}
+
+__device__ __forceinline__ uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_t s) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  uint64_t P, cp;
+  uint32_t cpv;
+  asm("v_mad_u64_u32 %0, %1, %3, %4, %5\n\t"
+      "s_nop 1\n\t"
+      "v_cndmask_b32 %2, 0, 1, %1"
+      : "=&v"(P), "=&s"(cp), "=v"(cpv)
+      : "v"(a0), "v"(b0), "v"(s));
+  const uint64_t M = (uint64_t)a1 * b0 + (((uint64_t)cpv << 32) | (P >> 32));
+  return mul_tail(a0, a1, b1, (uint32_t)P, M);
+}
+#else
+P3G_HD uint64_t mul_fast(uint64_t a, uint64_t b) { return mul_nc(a, b); }
+P3G_HD uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_t s) {
+  uint64_t lo, hi;
+  mul128(a, b, lo, hi);
+  unsigned int carry;
+  lo = add64c(lo, s, carry);
+  return red_nc(lo, hi + carry);
+}
+#endif
+
+P3G_HD uint64_t sbox_fast(uint64_t x) {
+  const uint64_t x2 = mul_fast(x, x);
+  const uint64_t x3 = mul_fast(x2, x);
+  const uint64_t x4 = mul_fast(x2, x2);
+  return mul_fast(x3, x4);
+}
+
 // External layer: y = circ(2 M4, M4, M4) x with M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]].
 P3G_HD void ext_linear_nc(uint64_t* s) {
   Acc o[12];
@@ -102,29 +185,23 @@ P3G_HD void p2_permute_fast(uint64_t* s, const P2Consts* __restrict__ c) {
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = sbox_nc(add_canon_nc(s[i], c->rc_init[r][i]));
+    for (int i = 0; i < 12; i++) s[i] = sbox_fast(add_canon_nc(s[i], c->rc_init[r][i]));
     ext_linear_nc(s);
   }
 #pragma unroll
   for (int r = 0; r < 22; r++) {
-    s[0] = sbox_nc(add_canon_nc(s[0], c->rc_int[r]));
+    s[0] = sbox_fast(add_canon_nc(s[0], c->rc_int[r]));
     Acc sum = acc_of(s[0]);
 #pragma unroll
     for (int i = 1; i < 12; i++) sum = acc_add(sum, s[i]);
     const uint64_t sm = acc_fold(sum);
 #pragma unroll
-    for (int i = 0; i < 12; i++) {
-      uint64_t lo, hi;
-      mul128(s[i], c->diag[i], lo, hi);
-      unsigned int carry;
-      lo = add64c(lo, sm, carry);
-      s[i] = red_nc(lo, hi + carry);
-    }
+    for (int i = 0; i < 12; i++) s[i] = mul_add_fast(s[i], c->diag[i], sm);
   }
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = sbox_nc(add_canon_nc(s[i], c->rc_term[r][i]));
+    for (int i = 0; i < 12; i++) s[i] = sbox_fast(add_canon_nc(s[i], c->rc_term[r][i]));
     ext_linear_nc(s);
   }
 #pragma unroll
