@@ -72,7 +72,7 @@ P3G_HD uint64_t sbox_nc(uint64_t x) {
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// a*b (+ s) mod p as some value < 2^64, in 12 (13) single-issue VALU
+// a*b (+ s) mod p as some value < 2^64, in 11 (12) single-issue VALU
 // instructions instead of 16 (19): the carry-outs and 64-bit addends of
 // v_mad_u64_u32 do the carry propagation that otherwise takes add/addc pairs
 // and selects.  With a = a0 + 2^32 a1, b = b0 + 2^32 b1:
@@ -86,9 +86,11 @@ P3G_HD uint64_t sbox_nc(uint64_t x) {
 //   U - b 2^64 = T - R1 - cn               (borrow chain, borrow b)
 // and the result is U + (c - b) eps mod 2^64, which lies in [0, 2^64) in all
 // four cases of (c, b) (T < 2^64 - 2^33 when c = 1, U >= 2^64 - 2^32 when b = 1).
-// The fold is one 64-bit add of adj = {c eps + b, (b & !c) eps}.
-// The assembly is arithmetic only; every SGPR a VALU instruction writes is read
-// two or more wait states later (s_nop), as gfx950 requires, and no SGPR
+// The fold is one 64-bit add of adj: eps when c & !b, -eps = {1, eps} when
+// b & !c, else 0 (the two masks are combined on the scalar unit).
+// The assembly is arithmetic only.  An SGPR that a VALU instruction writes is
+// read by VALU instructions two or more wait states later (s_nop), as gfx950
+// requires; VALU-to-SALU and SALU-to-VALU reads of an SGPR need none.  No SGPR
 // written here is read outside these statements.
 __device__ __forceinline__ uint64_t mul_tail(uint32_t a0, uint32_t a1, uint32_t b1, uint32_t P0, uint64_t M) {
   uint64_t N, cn;
@@ -98,18 +100,19 @@ __device__ __forceinline__ uint64_t mul_tail(uint32_t a0, uint32_t a1, uint32_t 
   uint64_t T, c;
   asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(T), "=s"(c) : "v"((uint32_t)R), "v"(lo));
   uint32_t u0, u1, al, ah;
-  uint64_t sb, sx;
+  uint64_t sb, f, g;
   asm("s_nop 1\n\t"
-      "v_subb_co_u32 %0, %4, %6, %8, %9\n\t"
-      "v_cndmask_b32 %2, 0, -1, %10\n\t"
-      "s_nop 0\n\t"
-      "v_subb_co_u32 %1, %4, %7, 0, %4\n\t"
+      "v_subb_co_u32 %0, %4, %7, %9, %10\n\t"
       "s_nop 1\n\t"
-      "v_addc_co_u32 %2, %5, %2, 0, %4\n\t"
-      "v_cndmask_b32 %3, 0, -1, %4\n\t"
-      "v_cndmask_b32 %3, %3, 0, %10"
-      : "=&v"(u0), "=&v"(u1), "=&v"(al), "=&v"(ah), "=&s"(sb), "=&s"(sx)
-      : "v"((uint32_t)T), "v"((uint32_t)(T >> 32)), "v"((uint32_t)(R >> 32)), "s"(cn), "s"(c));
+      "v_subb_co_u32 %1, %4, %8, 0, %4\n\t"
+      "s_andn2_b64 %5, %4, %11\n\t"
+      "s_andn2_b64 %6, %11, %4\n\t"
+      "v_cndmask_b32 %2, 0, -1, %6\n\t"
+      "v_cndmask_b32 %3, 0, -1, %5\n\t"
+      "v_cndmask_b32 %2, %2, 1, %5"
+      : "=&v"(u0), "=&v"(u1), "=&v"(al), "=&v"(ah), "=&s"(sb), "=&s"(f), "=&s"(g)
+      : "v"((uint32_t)T), "v"((uint32_t)(T >> 32)), "v"((uint32_t)(R >> 32)), "s"(cn), "s"(c)
+      : "scc");
   const uint64_t U = ((uint64_t)u1 << 32) | u0;
   const uint64_t adj = ((uint64_t)ah << 32) | al;
   uint64_t r;
@@ -179,31 +182,145 @@ P3G_HD void ext_linear_nc(uint64_t* s) {
   }
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Linear layers on 32-bit halves.  A sum of words x_i = xl_i + 2^32 xh_i with
+// small coefficients is kept as two 64-bit sums L = sum(k_i xl_i) and
+// H = sum(k_i xh_i), each far below 2^64, so every step is one instruction
+// with no carry to track: v_mad_u64_u32 adds k * (32-bit half) to a 64-bit sum,
+// v_lshl_add_u64 adds two sums.  One reduction per output turns L + 2^32 H into
+// a word (red_halves).  The carry-outs of these mads are never read (vcc).
+__device__ __forceinline__ uint64_t mad1(uint32_t x, uint64_t y) {
+  uint64_t r;
+  asm("v_mad_u64_u32 %0, vcc, %1, 1, %2" : "=v"(r) : "v"(x), "v"(y) : "vcc");
+  return r;
+}
+__device__ __forceinline__ uint64_t mad2(uint32_t x, uint64_t y) {
+  uint64_t r;
+  asm("v_mad_u64_u32 %0, vcc, %1, 2, %2" : "=v"(r) : "v"(x), "v"(y) : "vcc");
+  return r;
+}
+__device__ __forceinline__ uint64_t add64(uint64_t x, uint64_t y) {
+  uint64_t r;
+  asm("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(r) : "v"(x), "v"(y));
+  return r;
+}
+
+// (L + 2^32 H) mod p as some value < 2^64, for L, H < 2^40.  With
+// G = H + (L >> 32) the value is L0 + 2^32 G0 + 2^64 G1 == W + eps G1 where
+// W = L0 + 2^32 G0; T + c 2^64 = W + eps G1 and T + c eps < 2^64 (T < eps G1
+// < 2^40 when c = 1).
+__device__ __forceinline__ uint64_t red_halves(uint64_t L, uint64_t H) {
+  const uint64_t G = mad1((uint32_t)(L >> 32), H);
+  const uint64_t W = (G << 32) | (uint32_t)L;
+  uint64_t T, c;
+  uint32_t e;
+  asm("v_mad_u64_u32 %0, %1, %3, -1, %4\n\t"
+      "s_nop 1\n\t"
+      "v_cndmask_b32 %2, 0, -1, %1"
+      : "=&v"(T), "=&s"(c), "=v"(e)
+      : "v"((uint32_t)(G >> 32)), "v"(W));
+  return add64(T, (uint64_t)e);
+}
+
+// Halves of the four M4 outputs of one block (a, b, c, d: one half of each word).
+__device__ __forceinline__ void m4_halves(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint64_t* o) {
+  const uint64_t ab = mad1(a, (uint64_t)b);
+  const uint64_t cd = mad1(c, (uint64_t)d);
+  const uint64_t t = add64(ab, cd);   // a+b+c+d
+  const uint64_t tb = mad1(b, t);      // a+2b+c+d
+  const uint64_t td = mad1(d, t);      // a+b+c+2d
+  o[0] = add64(tb, ab);                // 2a+3b+c+d
+  o[1] = mad2(c, tb);                  // a+2b+3c+d
+  o[2] = add64(td, cd);                // a+b+2c+3d
+  o[3] = mad2(a, td);                  // 3a+b+c+2d
+}
+
+// s <- circ(2 M4, M4, M4) s (+ rc when given: the next round's constants)
+template <bool RC>
+__device__ __forceinline__ void ext_linear_fast(uint64_t* s, const uint64_t* rc) {
+  uint64_t ol[12], oh[12];
+#pragma unroll
+  for (int j = 0; j < 12; j += 4) {
+    m4_halves((uint32_t)s[j], (uint32_t)s[j + 1], (uint32_t)s[j + 2], (uint32_t)s[j + 3], ol + j);
+    m4_halves((uint32_t)(s[j] >> 32), (uint32_t)(s[j + 1] >> 32), (uint32_t)(s[j + 2] >> 32),
+              (uint32_t)(s[j + 3] >> 32), oh + j);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint64_t sl = add64(add64(ol[k], ol[4 + k]), ol[8 + k]);
+    const uint64_t sh = add64(add64(oh[k], oh[4 + k]), oh[8 + k]);
+#pragma unroll
+    for (int j = 0; j < 12; j += 4) {
+      uint64_t L = add64(ol[j + k], sl), H = add64(oh[j + k], sh);
+      if (RC) {
+        L = mad1((uint32_t)rc[j + k], L);
+        H = mad1((uint32_t)(rc[j + k] >> 32), H);
+      }
+      s[j + k] = red_halves(L, H);
+    }
+  }
+}
+
+// sum of the twelve state words
+__device__ __forceinline__ uint64_t sum12_fast(const uint64_t* s) {
+  uint64_t L = mad1((uint32_t)s[0], (uint64_t)(uint32_t)s[1]);
+  uint64_t H = mad1((uint32_t)(s[0] >> 32), (uint64_t)(uint32_t)(s[1] >> 32));
+#pragma unroll
+  for (int i = 2; i < 12; i++) {
+    L = mad1((uint32_t)s[i], L);
+    H = mad1((uint32_t)(s[i] >> 32), H);
+  }
+  return red_halves(L, H);
+}
+#endif
+
 template <bool UNROLL_INTERNAL = true>
 P3G_HD void p2_permute_fast(uint64_t* s, const P2Consts* __restrict__ c) {
-  ext_linear_nc(s);
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Each external layer that precedes a full round also adds that round's
+  // constants (in its reductions), so the full rounds start at the S-boxes.
+  ext_linear_fast<true>(s, c->rc_init[0]);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = sbox_fast(add_canon_nc(s[i], c->rc_init[r][i]));
-    ext_linear_nc(s);
+    for (int i = 0; i < 12; i++) s[i] = sbox_fast(s[i]);
+    if (r < 3) ext_linear_fast<true>(s, c->rc_init[r + 1]);
+    else ext_linear_fast<false>(s, nullptr);
   }
 #pragma unroll
   for (int r = 0; r < 22; r++) {
     s[0] = sbox_fast(add_canon_nc(s[0], c->rc_int[r]));
-    Acc sum = acc_of(s[0]);
-#pragma unroll
-    for (int i = 1; i < 12; i++) sum = acc_add(sum, s[i]);
-    const uint64_t sm = acc_fold(sum);
+    const uint64_t sm = sum12_fast(s);
 #pragma unroll
     for (int i = 0; i < 12; i++) s[i] = mul_add_fast(s[i], c->diag[i], sm);
   }
 #pragma unroll
+  for (int i = 0; i < 12; i++) s[i] = add_canon_nc(s[i], c->rc_term[0][i]);
+#pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = sbox_fast(add_canon_nc(s[i], c->rc_term[r][i]));
+    for (int i = 0; i < 12; i++) s[i] = sbox_fast(s[i]);
+    if (r < 3) ext_linear_fast<true>(s, c->rc_term[r + 1]);
+    else ext_linear_fast<false>(s, nullptr);
+  }
+#else
+  ext_linear_nc(s);
+  for (int r = 0; r < 4; r++) {
+    for (int i = 0; i < 12; i++) s[i] = sbox_nc(add_canon_nc(s[i], c->rc_init[r][i]));
     ext_linear_nc(s);
   }
+  for (int r = 0; r < 22; r++) {
+    s[0] = sbox_nc(add_canon_nc(s[0], c->rc_int[r]));
+    Acc sum = acc_of(s[0]);
+    for (int i = 1; i < 12; i++) sum = acc_add(sum, s[i]);
+    const uint64_t sm = acc_fold(sum);
+    for (int i = 0; i < 12; i++) s[i] = mul_add_fast(s[i], c->diag[i], sm);
+  }
+  for (int r = 0; r < 4; r++) {
+    for (int i = 0; i < 12; i++) s[i] = sbox_nc(add_canon_nc(s[i], c->rc_term[r][i]));
+    ext_linear_nc(s);
+  }
+#endif
 #pragma unroll
   for (int i = 0; i < 12; i++) s[i] = canon(s[i]);
 }
