@@ -99,7 +99,10 @@ int p3g_profile_read(p3g_ctx* ctx, const char* name, double* total_ms, uint64_t*
  *   independent chains of one VALU instruction (kind 0: v_add_u32, 1: v_mad_u64_u32,
  *   2: v_fma_f64, 3: v_mul_hi_u32, 4: a 64-bit add on a carry pair, v_add_co_u32 +
  *   v_addc_co_u32, 5: v_mov_b32, 6: v_cndmask_b32, 7: v_add_co_u32 with a carry-out SGPR
- *   pair, 8: v_lshl_add_u64, 9: v_add3_u32, 10: v_xor_b32, 11: v_alignbit_b32) for iters
+ *   pair, 8: v_lshl_add_u64, 9: v_add3_u32, 10: v_xor_b32, 11: v_alignbit_b32,
+ *   12: v_cndmask_b32_e32 (VOP2, mask in vcc), 13: v_mul_u32_u24_e32, and two mixed
+ *   streams: 14: v_mad_u64_u32 alternating with v_add_u32_e32, 15: v_mad_u64_u32
+ *   alternating with v_cndmask_b32_e32, each counting both) for iters
  *   iterations (a positive multiple of 4); *ms is the launch
  *   time (HIP events on the context's stream), *insts_per_wave the instructions of
  *   that class each wave issues.

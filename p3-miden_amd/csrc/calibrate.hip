@@ -214,6 +214,56 @@ __global__ __launch_bounds__(CAL_THREADS) void cal_misc_kernel(uint32_t iters, u
   out[t] = x;
 }
 
+// kinds 12..15: the pairing of VOP2 forms, alone and beside 64-bit multiply-adds
+//   12 v_cndmask_b32_e32 (VOP2, mask in vcc), 16 chains
+//   13 v_mul_u32_u24_e32 (VOP2), 16 chains
+//   14 v_mad_u64_u32 and v_add_u32_e32 alternating, 8 chains of each
+//   15 v_mad_u64_u32 and v_cndmask_b32_e32 alternating, 8 chains of each
+template <int KIND>
+__global__ __launch_bounds__(CAL_THREADS) void cal_pair_kernel(uint32_t iters, uint32_t k, uint64_t* out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t a[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) a[i] = t ^ (0x9e3779b9u * (i + 1));
+  uint64_t q[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) q[i] = (uint64_t(a[2 * i]) << 32) | a[2 * i + 1];
+  const uint64_t k64 = (uint64_t(k) << 32) | (k * 3u);
+  for (uint32_t it = 0; it < iters; it += CAL_UNROLL) {
+#pragma unroll
+    for (int u = 0; u < CAL_UNROLL; u++) {
+      if constexpr (KIND == 12) {
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+          asm volatile("s_mov_b64 vcc, %2\n\tv_cndmask_b32_e32 %0, %0, %1, vcc" : "+v"(a[i]) : "v"(k), "s"(k64) : "vcc");
+      } else if constexpr (KIND == 13) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) asm volatile("v_mul_u32_u24_e32 %0, %0, %1" : "+v"(a[i]) : "v"(k));
+      } else if constexpr (KIND == 14) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          asm volatile("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_add_u32_e32 %1, %1, %3"
+                       : "+v"(q[i]), "+v"(a[i])
+                       : "v"(a[8 + i]), "v"(k)
+                       : "vcc");
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          asm volatile("v_mad_u64_u32 %0, s[40:41], %2, %3, %0\n\tv_cndmask_b32_e32 %1, %1, %3, vcc"
+                       : "+v"(q[i]), "+v"(a[i])
+                       : "v"(a[8 + i]), "v"(k)
+                       : "s40", "s41");
+      }
+    }
+  }
+  uint64_t x = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) x ^= a[i];
+#pragma unroll
+  for (int i = 0; i < 8; i++) x ^= q[i];
+  out[t] = x;
+}
+
 __global__ __launch_bounds__(CAL_THREADS) void cal_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                               uint64_t n16) {
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
@@ -233,12 +283,13 @@ __global__ __launch_bounds__(CAL_THREADS) void cal_copy_kernel(const uint4* __re
 // Instructions of the measured class that one wave issues for `iters` (a
 // multiple of CAL_UNROLL) iterations.
 uint64_t calib_valu_insts_per_wave(unsigned kind, uint32_t iters) {
-  static const uint32_t per_iter[12] = {16, 8, 8, 16, 16, 16, 16, 16, 8, 16, 16, 16};
+  // kind 12 counts the selects only (each follows an s_mov_b64 to vcc)
+  static const uint32_t per_iter[16] = {16, 8, 8, 16, 16, 16, 16, 16, 8, 16, 16, 16, 16, 16, 16, 16};
   return uint64_t(iters) * per_iter[kind];
 }
 
 double calib_valu(Ctx& ctx, unsigned kind, uint32_t iters, uint32_t blocks) {
-  if (kind > 11) throw ApiError(E_INVAL, "calibrate_valu: kind is 0..11");
+  if (kind > 15) throw ApiError(E_INVAL, "calibrate_valu: kind is 0..15");
   if (iters == 0 || iters % CAL_UNROLL) throw ApiError(E_INVAL, "calibrate_valu: iters must be a positive multiple of 4");
   if (blocks == 0 || blocks > (1u << 20)) throw ApiError(E_INVAL, "calibrate_valu: blocks must be 1..2^20");
   hipStream_t s = ctx.stream;
@@ -256,7 +307,11 @@ double calib_valu(Ctx& ctx, unsigned kind, uint32_t iters, uint32_t blocks) {
       case 8: cal_misc_kernel<8><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
       case 9: cal_misc_kernel<9><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
       case 10: cal_misc_kernel<10><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
-      default: cal_misc_kernel<11><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 11: cal_misc_kernel<11><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 12: cal_pair_kernel<12><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 13: cal_pair_kernel<13><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      case 14: cal_pair_kernel<14><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
+      default: cal_pair_kernel<15><<<blocks, CAL_THREADS, 0, s>>>(n, 0x7u, out); break;
     }
     P3G_CHECK_HIP(hipGetLastError());
   };

@@ -48,7 +48,10 @@ struct LeafArgs {
   const P2Consts* p2;
 };
 
-__device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i) {
+// save: this thread's slots (stride T) of a block-wide LDS array for the
+// permutation's input (p2_permute_saved).
+template <int T>
+__device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i, uint64_t* save) {
   if (i >= a.h) return;
   uint64_t st[12];
   if (a.prev_states) {
@@ -84,7 +87,11 @@ __device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i) {
 #pragma unroll
       for (int k = 0; k < 8; k++) st[k] = (uint64_t)k < rem ? row[k] : 0;
     }
-    p2_permute_fast(st, a.p2);
+    // the constants pointer made opaque per chunk: the compiler would otherwise
+    // hoist the 118 round constants out of the loop into vector registers
+    const P2Consts* pc = a.p2;
+    asm volatile("" : "+s"(pc));
+    p2_permute_saved<T>(st, pc, save);
     j += 8;
   }
   if (a.out_digests) {
@@ -98,7 +105,8 @@ __device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i) {
 
 template <int T>
 __global__ void __launch_bounds__(T) p2_leaf_kernel(LeafArgs a) {
-  p2_leaf_body(a, (uint64_t)blockIdx.x * T + threadIdx.x);
+  __shared__ uint64_t save[12 * T];
+  p2_leaf_body<T>(a, (uint64_t)blockIdx.x * T + threadIdx.x, save + threadIdx.x);
 }
 
 // Poseidon2 leaf hashing of a short matrix group: one 16-lane row per leaf
@@ -175,9 +183,10 @@ __global__ void __launch_bounds__(T) keccak_leaf_kernel(LeafArgs a) {
 // (p2_leaf_many_kernel: 112 VGPRs, 4 waves per SIMD; 82 and 5 this way).
 __global__ void __launch_bounds__(128) p2_leaf_many_kernel(const LeafArgs* __restrict__ args,
                                                         const P2Consts* __restrict__ p2) {
+  __shared__ uint64_t save[12 * 128];
   LeafArgs a = args[blockIdx.y];
   a.p2 = p2;
-  p2_leaf_body(a, (uint64_t)blockIdx.x * 128 + threadIdx.x);
+  p2_leaf_body<128>(a, (uint64_t)blockIdx.x * 128 + threadIdx.x, save + threadIdx.x);
 }
 __global__ void __launch_bounds__(256) p2_leaf_coop_many_kernel(const LeafArgs* __restrict__ args,
                                                              const P2Consts* __restrict__ p2) {
@@ -219,9 +228,10 @@ __device__ __forceinline__ void compress_one(uint32_t hash, const uint64_t* l, c
   }
 }
 
+// save: this thread's slots (stride CMP_T) of a block-wide LDS array (Poseidon2)
 template <uint32_t HASH>
 __device__ __forceinline__ void compress_layer_body(const uint64_t* __restrict__ prev, uint64_t* __restrict__ next,
-                                                    uint64_t len, const P2Consts* p2, uint64_t i) {
+                                                    uint64_t len, const P2Consts* p2, uint64_t i, uint64_t* save) {
   if (i >= len) return;
   uint64_t l[4], r[4], o[4];
   const uint64_t* src = prev + i * 8;
@@ -230,7 +240,20 @@ __device__ __forceinline__ void compress_layer_body(const uint64_t* __restrict__
     l[k] = src[k];
     r[k] = src[4 + k];
   }
-  compress_one(HASH, l, r, o, p2);
+  if constexpr (HASH == HASH_P2) {
+    uint64_t st[12];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      st[k] = l[k];
+      st[4 + k] = r[k];
+      st[8 + k] = 0;
+    }
+    p2_permute_saved<CMP_T>(st, p2, save);
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = st[k];
+  } else {
+    compress_one(HASH, l, r, o, p2);
+  }
 #pragma unroll
   for (int k = 0; k < 4; k++) next[i * 4 + k] = o[k];
 }
@@ -239,7 +262,8 @@ template <uint32_t HASH>
 __global__ void __launch_bounds__(CMP_T) compress_layer_kernel(const uint64_t* __restrict__ prev,
                                                                uint64_t* __restrict__ next, uint64_t len,
                                                                const P2Consts* p2) {
-  compress_layer_body<HASH>(prev, next, len, p2, (uint64_t)blockIdx.x * CMP_T + threadIdx.x);
+  __shared__ uint64_t save[HASH == HASH_P2 ? 12 * CMP_T : 1];
+  compress_layer_body<HASH>(prev, next, len, p2, (uint64_t)blockIdx.x * CMP_T + threadIdx.x, save + threadIdx.x);
 }
 
 // one layer of a batch of same-shaped trees: job j reads layer[j].prev, writes layer[j].next
@@ -250,8 +274,9 @@ struct LayerJob {
 template <uint32_t HASH>
 __global__ void __launch_bounds__(CMP_T) compress_layer_many_kernel(const LayerJob* __restrict__ jobs, uint64_t len,
                                                                     const P2Consts* p2) {
+  __shared__ uint64_t save[HASH == HASH_P2 ? 12 * CMP_T : 1];
   const LayerJob j = jobs[blockIdx.y];
-  compress_layer_body<HASH>(j.prev, j.next, len, p2, (uint64_t)blockIdx.x * CMP_T + threadIdx.x);
+  compress_layer_body<HASH>(j.prev, j.next, len, p2, (uint64_t)blockIdx.x * CMP_T + threadIdx.x, save + threadIdx.x);
 }
 
 // Poseidon2 layer with few nodes: one 16-lane row per node (the cooperative

@@ -72,10 +72,9 @@ P3G_HD uint64_t sbox_nc(uint64_t x) {
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// a*b (+ s) mod p as some value < 2^64, in 11 (12) single-issue VALU
-// instructions instead of 16 (19): the carry-outs and 64-bit addends of
-// v_mad_u64_u32 do the carry propagation that otherwise takes add/addc pairs
-// and selects.  With a = a0 + 2^32 a1, b = b0 + 2^32 b1:
+// a*b (+ s) mod p as some value < 2^64 through v_mad_u64_u32 carry-outs and
+// 64-bit addends, which do the carry propagation that otherwise takes add/addc
+// pairs and selects.  With a = a0 + 2^32 a1, b = b0 + 2^32 b1:
 //   P + cp 2^64 = a0 b0 + s                (cp = 0 without s)
 //   M           = a1 b0 + P1 + 2^32 cp     (< 2^64: P1 <= 2^32 - 2 when cp = 1)
 //   N + cn 2^64 = a0 b1 + M
@@ -86,13 +85,22 @@ P3G_HD uint64_t sbox_nc(uint64_t x) {
 //   U - b 2^64 = T - R1 - cn               (borrow chain, borrow b)
 // and the result is U + (c - b) eps mod 2^64, which lies in [0, 2^64) in all
 // four cases of (c, b) (T < 2^64 - 2^33 when c = 1, U >= 2^64 - 2^32 when b = 1).
-// The fold is one 64-bit add of adj: eps when c & !b, -eps = {1, eps} when
-// b & !c, else 0 (the two masks are combined on the scalar unit).
+//
+// The borrow b needs T < R1 + cn < 2^32, which has probability about 2^-32, so
+// the products come in two forms:
+//  - EXACT: folds (c - b) eps (11 single-issue VALU instructions per product;
+//    the masks c & !b and b & !c are combined on the scalar unit);
+//  - TRY: folds c eps only (9 instructions) and ORs b into a wave-uniform flag.
+//    Its result is right in every lane whose borrow is clear; p2_permute_fast
+//    runs the permutation on TRY products and redoes it with EXACT ones when the
+//    flag is set, so the output is exact for every input.
 // The assembly is arithmetic only.  An SGPR that a VALU instruction writes is
 // read by VALU instructions two or more wait states later (s_nop), as gfx950
 // requires; VALU-to-SALU and SALU-to-VALU reads of an SGPR need none.  No SGPR
-// written here is read outside these statements.
-__device__ __forceinline__ uint64_t mul_tail(uint32_t a0, uint32_t a1, uint32_t b1, uint32_t P0, uint64_t M) {
+// written here is read outside these statements, except the TRY flag (SALU).
+template <bool TRY>
+__device__ __forceinline__ uint64_t mul_tail(uint32_t a0, uint32_t a1, uint32_t b1, uint32_t P0, uint64_t M,
+                                             uint64_t& flag) {
   uint64_t N, cn;
   asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(N), "=s"(cn) : "v"(a0), "v"(b1), "v"(M));
   const uint64_t R = (uint64_t)a1 * b1 + (N >> 32);
@@ -100,19 +108,33 @@ __device__ __forceinline__ uint64_t mul_tail(uint32_t a0, uint32_t a1, uint32_t 
   uint64_t T, c;
   asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(T), "=s"(c) : "v"((uint32_t)R), "v"(lo));
   uint32_t u0, u1, al, ah;
-  uint64_t sb, f, g;
-  asm("s_nop 1\n\t"
-      "v_subb_co_u32 %0, %4, %7, %9, %10\n\t"
-      "s_nop 1\n\t"
-      "v_subb_co_u32 %1, %4, %8, 0, %4\n\t"
-      "s_andn2_b64 %5, %4, %11\n\t"
-      "s_andn2_b64 %6, %11, %4\n\t"
-      "v_cndmask_b32 %2, 0, -1, %6\n\t"
-      "v_cndmask_b32 %3, 0, -1, %5\n\t"
-      "v_cndmask_b32 %2, %2, 1, %5"
-      : "=&v"(u0), "=&v"(u1), "=&v"(al), "=&v"(ah), "=&s"(sb), "=&s"(f), "=&s"(g)
-      : "v"((uint32_t)T), "v"((uint32_t)(T >> 32)), "v"((uint32_t)(R >> 32)), "s"(cn), "s"(c)
-      : "scc");
+  uint64_t sb;
+  if constexpr (TRY) {
+    asm("s_nop 1\n\t"
+        "v_subb_co_u32 %0, %3, %5, %7, %8\n\t"
+        "v_cndmask_b32 %2, 0, -1, %9\n\t"
+        "s_nop 0\n\t"
+        "v_subb_co_u32 %1, %3, %6, 0, %3\n\t"
+        "s_or_b64 %4, %4, %3"
+        : "=&v"(u0), "=&v"(u1), "=&v"(al), "=&s"(sb), "+s"(flag)
+        : "v"((uint32_t)T), "v"((uint32_t)(T >> 32)), "v"((uint32_t)(R >> 32)), "s"(cn), "s"(c)
+        : "scc");
+    ah = 0;
+  } else {
+    uint64_t f, g;
+    asm("s_nop 1\n\t"
+        "v_subb_co_u32 %0, %4, %7, %9, %10\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32 %1, %4, %8, 0, %4\n\t"
+        "s_andn2_b64 %5, %4, %11\n\t"
+        "s_andn2_b64 %6, %11, %4\n\t"
+        "v_cndmask_b32 %2, 0, -1, %6\n\t"
+        "v_cndmask_b32 %3, 0, -1, %5\n\t"
+        "v_cndmask_b32 %2, %2, 1, %5"
+        : "=&v"(u0), "=&v"(u1), "=&v"(al), "=&v"(ah), "=&s"(sb), "=&s"(f), "=&s"(g)
+        : "v"((uint32_t)T), "v"((uint32_t)(T >> 32)), "v"((uint32_t)(R >> 32)), "s"(cn), "s"(c)
+        : "scc");
+  }
   const uint64_t U = ((uint64_t)u1 << 32) | u0;
   const uint64_t adj = ((uint64_t)ah << 32) | al;
   uint64_t r;
@@ -120,14 +142,16 @@ __device__ __forceinline__ uint64_t mul_tail(uint32_t a0, uint32_t a1, uint32_t 
   return r;
 }
 
-__device__ __forceinline__ uint64_t mul_fast(uint64_t a, uint64_t b) {
+template <bool TRY>
+__device__ __forceinline__ uint64_t mul_t(uint64_t a, uint64_t b, uint64_t& flag) {
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
   const uint64_t P = (uint64_t)a0 * b0;
   const uint64_t M = (uint64_t)a1 * b0 + (P >> 32);
-  return mul_tail(a0, a1, b1, (uint32_t)P, M);
+  return mul_tail<TRY>(a0, a1, b1, (uint32_t)P, M, flag);
 }
 
-__device__ __forceinline__ uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_t s) {
+template <bool TRY>
+__device__ __forceinline__ uint64_t mul_add_t(uint64_t a, uint64_t b, uint64_t s, uint64_t& flag) {
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
   uint64_t P, cp;
   uint32_t cpv;
@@ -137,7 +161,24 @@ __device__ __forceinline__ uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_
       : "=&v"(P), "=&s"(cp), "=v"(cpv)
       : "v"(a0), "v"(b0), "v"(s));
   const uint64_t M = (uint64_t)a1 * b0 + (((uint64_t)cpv << 32) | (P >> 32));
-  return mul_tail(a0, a1, b1, (uint32_t)P, M);
+  return mul_tail<TRY>(a0, a1, b1, (uint32_t)P, M, flag);
+}
+
+template <bool TRY>
+__device__ __forceinline__ uint64_t sbox_t(uint64_t x, uint64_t& flag) {
+  const uint64_t x2 = mul_t<TRY>(x, x, flag);
+  const uint64_t x3 = mul_t<TRY>(x2, x, flag);
+  const uint64_t x4 = mul_t<TRY>(x2, x2, flag);
+  return mul_t<TRY>(x3, x4, flag);
+}
+
+__device__ __forceinline__ uint64_t mul_fast(uint64_t a, uint64_t b) {
+  uint64_t f = 0;
+  return mul_t<false>(a, b, f);
+}
+__device__ __forceinline__ uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_t s) {
+  uint64_t f = 0;
+  return mul_add_t<false>(a, b, s, f);
 }
 #else
 P3G_HD uint64_t mul_fast(uint64_t a, uint64_t b) { return mul_nc(a, b); }
@@ -149,13 +190,6 @@ P3G_HD uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_t s) {
   return red_nc(lo, hi + carry);
 }
 #endif
-
-P3G_HD uint64_t sbox_fast(uint64_t x) {
-  const uint64_t x2 = mul_fast(x, x);
-  const uint64_t x3 = mul_fast(x2, x);
-  const uint64_t x4 = mul_fast(x2, x2);
-  return mul_fast(x3, x4);
-}
 
 // External layer: y = circ(2 M4, M4, M4) x with M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]].
 P3G_HD void ext_linear_nc(uint64_t* s) {
@@ -274,35 +308,42 @@ __device__ __forceinline__ uint64_t sum12_fast(const uint64_t* s) {
 }
 #endif
 
-template <bool UNROLL_INTERNAL = true>
-P3G_HD void p2_permute_fast(uint64_t* s, const P2Consts* __restrict__ c) {
 #if defined(__HIP_DEVICE_COMPILE__)
+template <bool TRY>
+__device__ __forceinline__ void p2_permute_dev(uint64_t* s, const P2Consts* __restrict__ c, uint64_t& flag) {
   // Each external layer that precedes a full round also adds that round's
   // constants (in its reductions), so the full rounds start at the S-boxes.
   ext_linear_fast<true>(s, c->rc_init[0]);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = sbox_fast(s[i]);
+    for (int i = 0; i < 12; i++) s[i] = sbox_t<TRY>(s[i], flag);
     if (r < 3) ext_linear_fast<true>(s, c->rc_init[r + 1]);
     else ext_linear_fast<false>(s, nullptr);
   }
 #pragma unroll
   for (int r = 0; r < 22; r++) {
-    s[0] = sbox_fast(add_canon_nc(s[0], c->rc_int[r]));
+    s[0] = sbox_t<TRY>(add_canon_nc(s[0], c->rc_int[r]), flag);
     const uint64_t sm = sum12_fast(s);
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = mul_add_fast(s[i], c->diag[i], sm);
+    for (int i = 0; i < 12; i++) s[i] = mul_add_t<TRY>(s[i], c->diag[i], sm, flag);
   }
 #pragma unroll
   for (int i = 0; i < 12; i++) s[i] = add_canon_nc(s[i], c->rc_term[0][i]);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = sbox_fast(s[i]);
+    for (int i = 0; i < 12; i++) s[i] = sbox_t<TRY>(s[i], flag);
     if (r < 3) ext_linear_fast<true>(s, c->rc_term[r + 1]);
     else ext_linear_fast<false>(s, nullptr);
   }
+}
+#endif
+
+P3G_HD void p2_permute_fast(uint64_t* s, const P2Consts* __restrict__ c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t flag = 0;
+  p2_permute_dev<false>(s, c, flag);
 #else
   ext_linear_nc(s);
   for (int r = 0; r < 4; r++) {
@@ -324,5 +365,69 @@ P3G_HD void p2_permute_fast(uint64_t* s, const P2Consts* __restrict__ c) {
 #pragma unroll
   for (int i = 0; i < 12; i++) s[i] = canon(s[i]);
 }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// The canonical permutation of hash.hpp (p2_permute: canonical field
+// operations throughout) on a state held in LDS (v[k * STRIDE], k < 12), one
+// word at a time with every loop rolled: little code and few registers, for
+// the rare redo below, where its speed does not matter.
+template <int STRIDE>
+__device__ __forceinline__ void p2_ext_lds(uint64_t* v) {
+  uint64_t s[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) s[i] = v[i * STRIDE];
+  p2_external_linear(s);
+#pragma unroll
+  for (int i = 0; i < 12; i++) v[i * STRIDE] = s[i];
+}
+template <int STRIDE>
+__device__ __noinline__ void p2_permute_lds(uint64_t* v, const P2Consts* __restrict__ c) {
+  p2_ext_lds<STRIDE>(v);
+#pragma unroll 1
+  for (int r = 0; r < 8; r++) {
+    const uint64_t* rc = r < 4 ? c->rc_init[r] : c->rc_term[r - 4];
+#pragma unroll 1
+    for (int i = 0; i < 12; i++) v[i * STRIDE] = p2_sbox(gl_add(v[i * STRIDE], rc[i]));
+    p2_ext_lds<STRIDE>(v);
+    if (r == 3) {
+#pragma unroll 1
+      for (int q = 0; q < 22; q++) {
+        v[0] = p2_sbox(gl_add(v[0], c->rc_int[q]));
+        uint64_t sum = 0;
+#pragma unroll 1
+        for (int i = 0; i < 12; i++) sum = gl_add(sum, v[i * STRIDE]);
+#pragma unroll 1
+        for (int i = 0; i < 12; i++) v[i * STRIDE] = gl_add(gl_mul(v[i * STRIDE], c->diag[i]), sum);
+      }
+    }
+  }
+}
+
+// The permutation on TRY products, for the hashing kernels: the input is kept
+// in LDS (save[k * STRIDE], k < 12: this thread's slots in a block-wide
+// array), the permutation runs on TRY products, and the whole wave redoes it
+// from the saved input with the canonical permutation when any lane met a
+// product borrow (about 2^-32 per product).  The output equals p2_permute's
+// (canonical) for every input.
+template <int STRIDE>
+__device__ __forceinline__ void p2_permute_saved(uint64_t* s, const P2Consts* __restrict__ c, uint64_t* save) {
+#pragma unroll
+  for (int i = 0; i < 12; i++) save[i * STRIDE] = s[i];
+  uint64_t flag = 0;
+  p2_permute_dev<true>(s, c, flag);
+  if (__builtin_expect(flag != 0, 0)) {
+    p2_permute_lds<STRIDE>(save, c);
+#pragma unroll
+    for (int i = 0; i < 12; i++) s[i] = save[i * STRIDE];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 12; i++) s[i] = canon(s[i]);
+  }
+}
+#else
+// (the host compilation pass parses the kernels that call it)
+template <int STRIDE>
+__device__ void p2_permute_saved(uint64_t* s, const P2Consts* __restrict__ c, uint64_t* save);
+#endif
 
 }  // namespace p3g

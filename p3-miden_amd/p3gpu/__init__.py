@@ -360,7 +360,9 @@ class Device:
         return float(ms.value), int(n[0])
 
     VALU_KINDS = ("v_add_u32", "v_mad_u64_u32", "v_fma_f64", "v_mul_hi_u32", "add64_carry_pair", "v_mov_b32",
-                  "v_cndmask_b32", "v_add_co_u32_e64", "v_lshl_add_u64", "v_add3_u32", "v_xor_b32", "v_alignbit_b32")
+                  "v_cndmask_b32", "v_add_co_u32_e64", "v_lshl_add_u64", "v_add3_u32", "v_xor_b32", "v_alignbit_b32",
+                  "v_cndmask_b32_e32", "v_mul_u32_u24_e32", "v_mad_u64_u32+v_add_u32_e32",
+                  "v_mad_u64_u32+v_cndmask_b32_e32")
 
     def calibrate_valu(self, kind: int, iters: int = 16384, blocks: int = 8192) -> dict:
         """Sustained issue rate of one VALU instruction class (independent chains on every lane):
