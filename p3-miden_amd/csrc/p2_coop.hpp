@@ -7,8 +7,10 @@
 // row-rotation reduction, after which every lane multiplies its own word by
 // its diagonal entry.  The dependent chain is about a fifth of the
 // one-lane-per-permutation form, which is what the top levels of a tree (a
-// handful of permutations per level) are bound by.  Same arithmetic as
-// p2_permute_fast: the output equals the canonical permutation bit for bit.
+// handful of permutations per level) are bound by.  The products are the
+// EXACT ones of p2_fast.hpp; every value a DPP instruction reads comes from a
+// compiler-visible select, not straight from their assembly.  The output
+// equals the canonical permutation bit for bit.
 #pragma once
 #include "p2_fast.hpp"
 
@@ -52,16 +54,17 @@ __device__ __forceinline__ uint64_t coop_ext_linear(uint64_t x, uint32_t l) {
 // x: this lane's state word (zero in lanes 12..15); l: lane index within the row.
 __device__ __forceinline__ uint64_t p2_permute_coop(uint64_t x, uint32_t l, const P2Consts* __restrict__ c) {
   const uint32_t li = l < 12 ? l : 0;
+  uint64_t flag = 0;  // (EXACT products leave it untouched)
   x = coop_ext_linear(x, l);
 #pragma unroll 1
   for (int r = 0; r < 4; r++) {
-    x = sbox_nc(add_canon_nc(x, c->rc_init[r][li]));
+    x = sbox_t<false>(add_canon_nc(x, c->rc_init[r][li]), flag);
     x = coop_ext_linear(l < 12 ? x : 0, l);
   }
   const uint64_t dg = c->diag[li];
 #pragma unroll 1
   for (int r = 0; r < 22; r++) {
-    const uint64_t y0 = sbox_nc(add_canon_nc(x, c->rc_int[r]));
+    const uint64_t y0 = sbox_t<false>(add_canon_nc(x, c->rc_int[r]), flag);
     x = l == 0 ? y0 : x;
     Acc s = acc_of(l < 12 ? x : 0);
     s = acc_add(s, dpp_acc<DPP_ROW_ROR8>(s));
@@ -69,15 +72,12 @@ __device__ __forceinline__ uint64_t p2_permute_coop(uint64_t x, uint32_t l, cons
     s = acc_add(s, dpp_acc<DPP_ROW_ROR2>(s));
     s = acc_add(s, dpp_acc<DPP_ROW_ROR1>(s));
     const uint64_t sm = acc_fold(s);
-    uint64_t lo, hi;
-    mul128(x, dg, lo, hi);
-    unsigned int carry;
-    lo = add64c(lo, sm, carry);
-    x = l < 12 ? red_nc(lo, hi + carry) : 0;
+    const uint64_t y = mul_add_t<false>(x, dg, sm, flag);
+    x = l < 12 ? y : 0;
   }
 #pragma unroll 1
   for (int r = 0; r < 4; r++) {
-    x = sbox_nc(add_canon_nc(x, c->rc_term[r][li]));
+    x = sbox_t<false>(add_canon_nc(x, c->rc_term[r][li]), flag);
     x = coop_ext_linear(l < 12 ? x : 0, l);
   }
   return canon(x);

@@ -181,6 +181,7 @@ __device__ __forceinline__ uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_
   return mul_add_t<false>(a, b, s, f);
 }
 #else
+// host: the same values through 128-bit products
 P3G_HD uint64_t mul_fast(uint64_t a, uint64_t b) { return mul_nc(a, b); }
 P3G_HD uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_t s) {
   uint64_t lo, hi;
@@ -189,6 +190,12 @@ P3G_HD uint64_t mul_add_fast(uint64_t a, uint64_t b, uint64_t s) {
   lo = add64c(lo, s, carry);
   return red_nc(lo, hi + carry);
 }
+template <bool TRY>
+P3G_HD uint64_t mul_t(uint64_t a, uint64_t b, uint64_t&) { return mul_nc(a, b); }
+template <bool TRY>
+P3G_HD uint64_t mul_add_t(uint64_t a, uint64_t b, uint64_t s, uint64_t&) { return mul_add_fast(a, b, s); }
+template <bool TRY>
+P3G_HD uint64_t sbox_t(uint64_t x, uint64_t&) { return sbox_nc(x); }
 #endif
 
 // External layer: y = circ(2 M4, M4, M4) x with M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]].
