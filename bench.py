@@ -47,6 +47,9 @@ HBM_PEAK_GBS = 8000.0
 # FP32 vector rate / 2 flops per FMA) = 256 CUs x 128 lanes x 2.4 GHz; the measured peak replaces it
 # when the calibrate section runs
 VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
+# VALU issue slots: 1,024 SIMDs (256 CUs x 4), one wave64 VALU instruction per SIMD per 4 cycles at the
+# 2.4 GHz peak engine clock, in 10^9 quad-cycles/s
+ISSUE_PEAK_G = 1024 * 2.4e9 / 4 / 1e9
 
 
 def log(msg):
@@ -71,7 +74,8 @@ def read_profile_constants():
     path = os.path.join(ROOT, "profiles", "valu_issue_c2_pmc.md")
     if os.path.exists(path):
         for line in open(path):
-            for key in ("valu_insts_per_launch", "valu_issue_busy", "valu_paired_share", "hbm_bytes_per_launch"):
+            for key in ("valu_insts_per_launch", "valu_issue_cycles_per_launch", "valu_issue_busy",
+                        "valu_paired_share", "hbm_bytes_per_launch"):
                 if line.startswith(key + ":"):
                     out[key] = float(line.split(":")[1])
     return out
@@ -161,32 +165,36 @@ def bench_c2(G, W, dev, steps, warmup, peaks=None):
     pc = read_profile_constants()
     leaf_bytes = 8 * (n << b) * w + 32 * (n << b)  # read every LDE word once, write the 2^24 leaf digests
     peak_tops = (peaks or {}).get("valu_issue_peak_tops") or VALU_PEAK_TOPS
+    issue = pc.get("valu_issue_cycles_per_launch")
+    insts = pc.get("valu_insts_per_launch")
     roof = {
         "kernel": "p2_leaf (Poseidon2-GL12 LMCS leaf hashing of the 2^24 x 64 LDE: 2^24 leaves x 8 permutations)",
         "bound": "valu",
-        "achieved": None,
-        "peak": round(peak_tops, 2),
-        "unit": "T lane-ops/s",
-        "frac": None,
+        "achieved": round(issue / (leaf_ms * 1e-3) / 1e9, 1) if issue and leaf_ms else None,
+        "peak": ISSUE_PEAK_G,
+        "unit": "G VALU issue quad-cycles/s",
+        "frac": round(issue / (leaf_ms * 1e-3) / 1e9 / ISSUE_PEAK_G, 3) if issue and leaf_ms else None,
         "traffic": pc.get("hbm_bytes_per_launch"),
         "launch_ms": round(leaf_ms, 3),
         "permutations_per_s": round(perms / (leaf_ms * 1e-3), 1) if leaf_ms else None,
-        "peak_source": "measured on this GPU in this run (v_add_u32 issue rate, bench.py calibrate)" if peaks
-                       else "published FP32 vector rate / 2 (calibrate section not run)",
         "valu_issue_busy": pc.get("valu_issue_busy"),
         "valu_paired_share": pc.get("valu_paired_share"),
+        "valu_insts_per_permutation": round(insts * 64 / perms) if insts else None,
+        "valu_lane_ops_tops": round(insts * 64 / (leaf_ms * 1e-3) / 1e12, 2) if insts and leaf_ms else None,
+        "valu_lane_ops_peak_tops": round(peak_tops, 2),
+        "valu_lane_ops_peak_source": "measured v_add_u32 issue rate in this run (bench.py calibrate)" if peaks
+                                     else "published FP32 vector rate / 2 (calibrate section not run)",
         "hbm": {"algorithmic_bytes": leaf_bytes,
                 "achieved": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9, 1) if leaf_ms else None,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if leaf_ms else None,
                 "measured_copy_gbs": (peaks or {}).get("copy_kernel_gbs")},
-        "note": "VALU instructions per launch from SQ_INSTS_VALU (exact, calibrated) x 64 lanes / launch time; "
-                "valu_issue_busy = share of SIMD quad-cycles with a VALU issue (PMC), the kernel's real limit",
+        "note": "the VALU issue port: quad-cycles per launch in which a SIMD issued a VALU instruction "
+                "(SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2, rocprofv3 PMC) / the launch time from HIP events "
+                "in this run, against 1,024 SIMDs x 2.4 GHz peak engine clock / 4; valu_issue_busy = the same "
+                "share of the SIMDs' busy quad-cycles at the clock they ran (PMC); lane-ops = SQ_INSTS_VALU x 64 "
+                "/ launch time",
     }
-    if pc.get("valu_insts_per_launch") and leaf_ms:
-        ach = pc["valu_insts_per_launch"] * 64 / (leaf_ms * 1e-3) / 1e12
-        roof["achieved"] = round(ach, 2)
-        roof["frac"] = round(ach / peak_tops, 3)
     others = {
         "lde (inverse + forward passes)": {"ms": round(lde_ms, 3), "bytes": lde_bytes,
                                            "gbps": round(lde_bytes / (lde_ms * 1e-3) / 1e9, 1) if lde_ms else None,
