@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "internal.hpp"
+#include "ntt_ops.hpp"
 
 namespace p3g {
 
@@ -301,7 +302,7 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
       }
       __syncthreads();  // the previous coset's readers passed the tile barriers below
 #pragma unroll
-      for (int k = 0; k < 16; k++) r[k] = gl_mul(in[k], scl[17 * g + k]);
+      for (int k = 0; k < 16; k++) r[k] = mulc(in[k], scl[17 * g + k]);
     } else {
 #pragma unroll
       for (int k = 0; k < 16; k++) r[k] = in[k];
@@ -309,7 +310,7 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
     if (inverse) {
       // stages on x bits 0..3 (k bits), layout A
 #pragma unroll
-      for (int k = 1; k < 16; k++) r[k] = gl_mul(r[k], tfa[k]);
+      for (int k = 1; k < 16; k++) r[k] = mulc(r[k], tfa[k]);
 #pragma unroll
       for (int t = 0; t < 4; t++) {
 #pragma unroll
@@ -317,16 +318,19 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           if (k0 & (1 << t)) continue;
           const int k1 = k0 | (1 << t);
           const uint32_t jw = (uint32_t)(k0 & ((1 << t) - 1)) << (3 - t);
-          const uint64_t x = r[k0], y = gl_mul_w16(r[k1], jw, true);
-          r[k0] = gl_add(x, y);
-          r[k1] = gl_sub(x, y);
+          uint32_t E;
+          bool neg;
+          w16_pow2(jw, true, E, neg);  // omega^-jw = (-1)^neg 2^E
+          const uint64_t x = r[k0], y = mul_pow2c(r[k1], E);
+          r[k0] = neg ? gl_sub(x, y) : gl_add(x, y);
+          r[k1] = neg ? gl_add(x, y) : gl_sub(x, y);
         }
       }
       xchg_halves(tile, r, [&](int k) { return swz(16 * g + k, c); }, [&](int k) { return swz(g + G * k, c); });
       // stages on x bits 4..LR-1, layout B (x bit t is k bit t - (LR - 4))
 #pragma unroll
       for (int k = 0; k < 16; k++)
-        if (k >> KB0) r[k] = gl_mul(r[k], tfb[16 * g + k]);
+        if (k >> KB0) r[k] = mulc(r[k], tfb[16 * g + k]);
 #pragma unroll
       for (int t = 4; t < LR; t++) {
         const int kb = t - (LR - 4);
@@ -335,9 +339,12 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           if (k0 & (1 << kb)) continue;
           const int k1 = k0 | (1 << kb);
           const uint32_t jw = (uint32_t)((k0 >> KB0) & ((1 << (kb - KB0)) - 1)) << (3 - kb + KB0);
-          const uint64_t x = r[k0], y = gl_mul_w16(r[k1], jw, true);
-          r[k0] = gl_add(x, y);
-          r[k1] = gl_sub(x, y);
+          uint32_t E;
+          bool neg;
+          w16_pow2(jw, true, E, neg);  // omega^-jw = (-1)^neg 2^E
+          const uint64_t x = r[k0], y = mul_pow2c(r[k1], E);
+          r[k0] = neg ? gl_sub(x, y) : gl_add(x, y);
+          r[k1] = neg ? gl_add(x, y) : gl_sub(x, y);
         }
       }
       if (live) {
@@ -355,14 +362,17 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           if (k0 & (1 << kb)) continue;
           const int k1 = k0 | (1 << kb);
           const uint32_t jw = (uint32_t)((k0 >> KB0) & ((1 << (kb - KB0)) - 1)) << (3 - kb + KB0);
+          uint32_t E;
+          bool neg;
+          w16_pow2(jw, false, E, neg);  // omega^jw = (-1)^neg 2^E
           const uint64_t x = r[k0], y = r[k1];
           r[k0] = gl_add(x, y);
-          r[k1] = gl_mul_w16(gl_sub(x, y), jw, false);
+          r[k1] = mul_pow2c(neg ? gl_sub(y, x) : gl_sub(x, y), E);
         }
       }
 #pragma unroll
       for (int k = 0; k < 16; k++)
-        if (k >> KB0) r[k] = gl_mul(r[k], tfb[16 * g + k]);
+        if (k >> KB0) r[k] = mulc(r[k], tfb[16 * g + k]);
       __syncthreads();  // previous repetition's readers are done with the tile
       xchg_halves(tile, r, [&](int k) { return swz(g + G * k, c); }, [&](int k) { return swz(16 * g + k, c); });
 #pragma unroll
@@ -372,13 +382,16 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           if (k0 & (1 << q)) continue;
           const int k1 = k0 | (1 << q);
           const uint32_t jw = (uint32_t)(k0 & ((1 << q) - 1)) << (3 - q);
+          uint32_t E;
+          bool neg;
+          w16_pow2(jw, false, E, neg);  // omega^jw = (-1)^neg 2^E
           const uint64_t x = r[k0], y = r[k1];
           r[k0] = gl_add(x, y);
-          r[k1] = gl_mul_w16(gl_sub(x, y), jw, false);
+          r[k1] = mul_pow2c(neg ? gl_sub(y, x) : gl_sub(x, y), E);
         }
       }
 #pragma unroll
-      for (int k = 1; k < 16; k++) r[k] = gl_mul(r[k], tfa[k]);
+      for (int k = 1; k < 16; k++) r[k] = mulc(r[k], tfa[k]);
       const uint64_t ob = MODE == FWD_FIRST ? bitrev32((uint32_t)rep, a.logB) : (MODE == FWD ? blk : 0);
       // the row stride is made opaque inside the coset loop so that the 16 store
       // addresses are formed per coset instead of being held across it
