@@ -113,6 +113,24 @@ int p3g_calibrate_valu(p3g_ctx* ctx, uint32_t kind, uint32_t iters, uint32_t blo
                        uint64_t* insts_per_wave);
 int p3g_calibrate_copy(p3g_ctx* ctx, uint64_t bytes, uint32_t reps, double* kernel_ms, double* memcpy_ms);
 
+/* ---- device field operations on caller operands (instrumentation; no reference counterpart) ----
+ * For tests of the kernels' arithmetic on chosen operands, including the carry
+ * and borrow cases random data does not reach.  n operands (1..2^20) in host
+ * arrays a, b (and c, or NULL for zeros), one wave per 64; out[i] receives:
+ *   op 0: a*b (mod p, some value < 2^64) as the products of the proof's kernels
+ *         compute it in full;
+ *   op 1: the same with the hashing kernels' flagged shortcut: each wave's
+ *         64-bit lane mask of operands whose product took the rare borrow goes
+ *         to wave_flags[i / 64] (those lanes' out is a*b + (2^32 - 1) mod p);
+ *   op 2, 3: a*b + c, as op 0 and op 1;
+ *   op 4: a*b canonical (< p), for canonical a, b;
+ *   op 5: a * 2^e canonical, for canonical a and 0 <= e < 96;
+ *   op 6: (a + 2^32 b) mod p, some value < 2^64, for a, b < 2^40.
+ * wave_flags (ceil(n / 64) words, or NULL) is written for every op (zero
+ * except for ops 1 and 3). */
+int p3g_field_check(p3g_ctx* ctx, uint32_t op, uint32_t e, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                    uint64_t n, uint64_t* out, uint64_t* wave_flags);
+
 /* ---- device memory (resident inputs, benchmarks, serving) ---- */
 int p3g_alloc(p3g_ctx* ctx, uint64_t bytes, void** dptr);
 int p3g_free(p3g_ctx* ctx, void* dptr);

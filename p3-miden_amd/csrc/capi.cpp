@@ -16,6 +16,8 @@ void fill_random_device(Ctx& ctx, uint64_t seed, uint64_t n, uint64_t* out, hipS
 uint64_t calib_valu_insts_per_wave(unsigned kind, uint32_t iters);
 double calib_valu(Ctx& ctx, unsigned kind, uint32_t iters, uint32_t blocks);
 void calib_copy(Ctx& ctx, uint64_t bytes, uint32_t reps, double* kernel_ms, double* memcpy_ms);
+void field_check(Ctx& ctx, uint32_t op, uint32_t e, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                 uint64_t n, uint64_t* out, uint64_t* wave_flags);
 }
 
 static thread_local std::string g_last_error;
@@ -210,6 +212,15 @@ int p3g_calibrate_copy(p3g_ctx* ctx, uint64_t bytes, uint32_t reps, double* kern
   return guard([&] {
     P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
     calib_copy(ctx->c, bytes, reps, kernel_ms, memcpy_ms);
+  });
+}
+
+int p3g_field_check(p3g_ctx* ctx, uint32_t op, uint32_t e, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                    uint64_t n, uint64_t* out, uint64_t* wave_flags) {
+  if (!ctx || !a || !b || !out) return P3G_E_INVAL;
+  return guard([&] {
+    P3G_CHECK_HIP(hipSetDevice(ctx->c.device));
+    field_check(ctx->c, op, e, a, b, c, n, out, wave_flags);
   });
 }
 

@@ -91,7 +91,9 @@ P3G_HD uint64_t sbox_nc(uint64_t x) {
 //  - EXACT: folds (c - b) eps (11 single-issue VALU instructions per product;
 //    the masks c & !b and b & !c are combined on the scalar unit);
 //  - TRY: folds c eps only (9 instructions) and ORs b into a wave-uniform flag.
-//    Its result is right in every lane whose borrow is clear; p2_permute_fast
+//    Its result is right in every lane whose borrow is clear, and also where
+//    c = b = 1 (U >= 2^64 - 2^32 there, so U + eps wraps past 2^64, which is
+//    -eps again); the flag takes every borrow all the same.  p2_permute_fast
 //    runs the permutation on TRY products and redoes it with EXACT ones when the
 //    flag is set, so the output is exact for every input.
 // The assembly is arithmetic only.  An SGPR that a VALU instruction writes is
@@ -313,6 +315,9 @@ __device__ __forceinline__ uint64_t sum12_fast(const uint64_t* s) {
   }
   return red_halves(L, H);
 }
+#else
+// (the host compilation pass parses kernels that call it)
+__device__ uint64_t red_halves(uint64_t L, uint64_t H);
 #endif
 
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -168,6 +168,7 @@ def _load():
         "p3g_profile_reset": (C.c_int, [_vp]),
         "p3g_profile_read": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), _u64p]),
         "p3g_calibrate_valu": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double), _u64p]),
+        "p3g_field_check": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _u64p, _u64p, _u64p, C.c_uint64, _u64p, _u64p]),
         "p3g_calibrate_copy": (C.c_int, [_vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
         "p3g_alloc": (C.c_int, [_vp, C.c_uint64, C.POINTER(_vp)]),
@@ -374,6 +375,18 @@ class Device:
         insts = int(n[0]) * waves
         return {"kind": self.VALU_KINDS[kind], "ms": float(ms.value), "insts_per_wave": int(n[0]), "waves": waves,
                 "wave_insts_per_s": insts / (ms.value * 1e-3), "lane_ops_per_s": 64 * insts / (ms.value * 1e-3)}
+
+    def field_check(self, op: int, a, b, c=None, e: int = 0):
+        """Run one of the kernels' device field operations on operand arrays a, b (and c):
+        returns (out, wave_flags); see p3g_field_check in include/p3gpu.h."""
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        cc = None if c is None else np.ascontiguousarray(c, dtype=np.uint64)
+        out = np.zeros(a.size, np.uint64)
+        flags = np.zeros((a.size + 63) // 64, np.uint64)
+        _check(lib().p3g_field_check(self.h, op, e, _p(a), _p(b), None if cc is None else _p(cc), a.size, _p(out),
+                                     _p(flags)), "field_check")
+        return out, flags
 
     def calibrate_copy(self, nbytes: int = 4 << 30, reps: int = 5) -> dict:
         """Device-to-device copy bandwidth (read + write bytes per second) of a copy kernel and of
