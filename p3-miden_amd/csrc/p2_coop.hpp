@@ -8,7 +8,8 @@
 // its diagonal entry.  The dependent chain is about a fifth of the
 // one-lane-per-permutation form, which is what the top levels of a tree (a
 // handful of permutations per level) are bound by.  The products are the
-// EXACT ones of p2_fast.hpp; every value a DPP instruction reads comes from a
+// TRY ones of p2_fast.hpp, with an EXACT redo of the wave when one flags its
+// rare borrow; every value a DPP instruction reads comes from a
 // compiler-visible select, not straight from their assembly.  The output
 // equals the canonical permutation bit for bit.
 #pragma once
@@ -52,19 +53,21 @@ __device__ __forceinline__ uint64_t coop_ext_linear(uint64_t x, uint32_t l) {
 }
 
 // x: this lane's state word (zero in lanes 12..15); l: lane index within the row.
-__device__ __forceinline__ uint64_t p2_permute_coop(uint64_t x, uint32_t l, const P2Consts* __restrict__ c) {
+// TRY / EXACT products as in p2_fast.hpp; the result is not canonicalised.
+template <bool TRY>
+__device__ __forceinline__ uint64_t p2_permute_coop_body(uint64_t x, uint32_t l, const P2Consts* __restrict__ c,
+                                                         uint64_t& flag) {
   const uint32_t li = l < 12 ? l : 0;
-  uint64_t flag = 0;  // (EXACT products leave it untouched)
   x = coop_ext_linear(x, l);
 #pragma unroll 1
   for (int r = 0; r < 4; r++) {
-    x = sbox_t<false>(add_canon_nc(x, c->rc_init[r][li]), flag);
+    x = sbox_t<TRY>(add_canon_nc(x, c->rc_init[r][li]), flag);
     x = coop_ext_linear(l < 12 ? x : 0, l);
   }
   const uint64_t dg = c->diag[li];
 #pragma unroll 1
   for (int r = 0; r < 22; r++) {
-    const uint64_t y0 = sbox_t<false>(add_canon_nc(x, c->rc_int[r]), flag);
+    const uint64_t y0 = sbox_t<TRY>(add_canon_nc(x, c->rc_int[r]), flag);
     x = l == 0 ? y0 : x;
     Acc s = acc_of(l < 12 ? x : 0);
     s = acc_add(s, dpp_acc<DPP_ROW_ROR8>(s));
@@ -72,15 +75,25 @@ __device__ __forceinline__ uint64_t p2_permute_coop(uint64_t x, uint32_t l, cons
     s = acc_add(s, dpp_acc<DPP_ROW_ROR2>(s));
     s = acc_add(s, dpp_acc<DPP_ROW_ROR1>(s));
     const uint64_t sm = acc_fold(s);
-    const uint64_t y = mul_add_t<false>(x, dg, sm, flag);
+    const uint64_t y = mul_add_t<TRY>(x, dg, sm, flag);
     x = l < 12 ? y : 0;
   }
 #pragma unroll 1
   for (int r = 0; r < 4; r++) {
-    x = sbox_t<false>(add_canon_nc(x, c->rc_term[r][li]), flag);
+    x = sbox_t<TRY>(add_canon_nc(x, c->rc_term[r][li]), flag);
     x = coop_ext_linear(l < 12 ? x : 0, l);
   }
-  return canon(x);
+  return x;
+}
+
+// TRY products first; the whole wave (four rows) redoes the permutation with
+// EXACT products when any lane flagged a product borrow (about 2^-32 per
+// product): the one input word per lane is all that has to be kept.
+__device__ __forceinline__ uint64_t p2_permute_coop(uint64_t x, uint32_t l, const P2Consts* __restrict__ c) {
+  uint64_t flag = 0;
+  uint64_t y = p2_permute_coop_body<true>(x, l, c, flag);
+  if (__builtin_expect(flag != 0, 0)) y = p2_permute_coop_body<false>(x, l, c, flag);
+  return canon(y);
 }
 
 }  // namespace p3g

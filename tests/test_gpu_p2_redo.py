@@ -1,8 +1,9 @@
-"""The hashing kernels' Poseidon2 permutation takes a shortcut in its modular
+"""The hashing kernels' Poseidon2 permutations (one lane per permutation, and
+the lane-cooperative form of small trees) take a shortcut in their modular
 products that is wrong with probability about 2^-32 per product (a borrow it
-does not fold), flags it, and then redoes the whole wave's permutation with
-canonical arithmetic.  Random inputs never reach the redo, so these tests
-build leaf rows that do and check the trees against the oracle.
+does not fold), flag it, and then redo the whole wave's permutation exactly.
+Random inputs never reach the redo, so these tests build leaf rows that do and
+check the trees against the oracle.
 
 A product a*b is computed from a = a0 + 2^32 a1, b = b0 + 2^32 b1 as
   P = a0 b0, M = a1 b0 + P1, N + cn 2^64 = a0 b1 + M, R = a1 b1 + N1,
@@ -62,7 +63,9 @@ def test_borrow_construction():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("log_h,rows", [(15, [3, (1 << 14) + 5, (1 << 15) - 1]), (16, [0, 12345])])
+@pytest.mark.parametrize("log_h,rows", [(15, [3, (1 << 14) + 5, (1 << 15) - 1]), (16, [0, 12345]),
+                                        # trees of up to 2^14 leaves hash with the lane-cooperative permutation
+                                        (10, [1, 700]), (14, [(1 << 14) - 2])])
 def test_tree_with_redo_rows_matches_oracle(device, log_h, rows):
     h = 1 << log_h
     m = O.rand_matrix(404 + log_h, h, 8)
