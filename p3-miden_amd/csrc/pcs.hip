@@ -593,6 +593,7 @@ __global__ void __launch_bounds__(GRIND_T) grind_p2_kernel(const DevSponge* __re
   unsigned long long* __restrict__ wit = wits + blockIdx.y;
   __shared__ uint64_t S[12];
   __shared__ unsigned long long seen;
+  __shared__ uint64_t save[12 * GRIND_T];  // the permutation inputs (p2_permute_saved)
   const uint32_t t = threadIdx.x;
   const uint32_t k = (uint32_t)sp->in_len;
   if (t < 12) S[t] = t < k ? sp->in[t] : sp->st[t];
@@ -611,7 +612,7 @@ __global__ void __launch_bounds__(GRIND_T) grind_p2_kernel(const DevSponge* __re
       uint64_t st[12];
 #pragma unroll
       for (int q = 0; q < 12; q++) st[q] = (uint32_t)q == k ? i : S[q];
-      p2_permute_fast(st, p2);
+      p2_permute_saved<GRIND_T>(st, p2, save + t);
       if ((st[7] & mask) == 0) atomicMin(wit, (unsigned long long)i);
     }
   }
