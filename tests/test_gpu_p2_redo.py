@@ -91,3 +91,41 @@ def test_lifted_tree_with_redo_rows_matches_oracle(device):
     assert got.root().tolist() == want.root().tolist()
     for layer in range(17):
         assert np.array_equal(got.layer(layer), want.layer(layer)), layer
+
+
+def _trace_with_lde_row(seed: int, n: int, log_blowup: int, j: int, row) -> np.ndarray:
+    """An n x 8 trace whose coset LDE (shift 7, bit-reversed rows, as commit_traces
+    commits a single trace) has `row` at LDE row j: trace row 0 is moved along the
+    LDE of the unit vector e_0, which is nonzero at every coset point."""
+    t = O.rand_matrix(seed, n, 8)
+    base = O.coset_lde_batch(t, log_blowup, 7)
+    e0 = np.zeros((n, 1), dtype=np.uint64)
+    e0[0, 0] = 1
+    unit = int(O.coset_lde_batch(e0, log_blowup, 7)[j, 0])
+    inv = pow(unit, P - 2, P)
+    for c in range(8):
+        delta = (int(row[c]) - int(base[j, c])) * inv % P
+        t[0, c] = np.uint64((int(t[0, c]) + delta) % P)
+    assert np.array_equal(O.coset_lde_batch(t, log_blowup, 7)[j], row)
+    return t
+
+
+@pytest.mark.gpu
+def test_lockstep_proofs_with_redo_rows_match_oracle(device):
+    """Two lock-step jobs of 2^13 x 8 (LDE 2^14 rows each, so the batch hashes its
+    leaves with the job-indexed one-lane kernel): leaf rows of both LDEs take the
+    redo; the proofs equal the oracle's."""
+    n, log_blowup = 1 << 13, 1
+    rng = np.random.default_rng(21)
+    jobs = [[_trace_with_lde_row(500, n, log_blowup, 77, _borrow_row(rng))],
+            [_trace_with_lde_row(501, n, log_blowup, (1 << 14) - 1, _borrow_row(rng))]]
+    # the committed matrix of a single trace is its LDE with shift 7
+    lde = O.coset_lde_batch(jobs[0][0], log_blowup, 7)
+    assert (O.commit_traces(O.HASH_P2, jobs[0], log_blowup).root().tolist()
+            == O.Tree(O.HASH_P2, [lde], alignment=O.hash_alignment(O.HASH_P2)).root().tolist())
+    params = (log_blowup, 1, 3, 0, 0, 32, 0)
+    got = G.prove_many(G.LmcsConfig(device, G.HASH_POSEIDON2), G.PcsParams(*params), jobs)
+    for tr, (root, z, f, c) in zip(jobs, got):
+        root_o, z_o, f_o, c_o, _ = O.pcs_trace_prove(O.PcsParams(*params), O.HASH_P2, tr)
+        assert root.tolist() == root_o.tolist() and z == tuple(z_o)
+        assert np.array_equal(f, f_o) and np.array_equal(c, c_o)
