@@ -48,10 +48,16 @@ struct LeafArgs {
   const P2Consts* p2;
 };
 
-// save: this thread's slots (stride T) of a block-wide LDS array for the
-// permutation's input (p2_permute_saved).
+// LDS per wave (LEAF_WAVE_WORDS = 12 x 64 words): words 0..7 of its lanes are
+// a prefetch slot in the layout a direct global-to-LDS load writes (16 bytes
+// per lane per instruction: piece q of lane l at q * 128 + 2 l), words 8..11
+// the permutation's saved capacity (word k of lane l at k * 64 + l).  The
+// rare exact redo lays the whole input out as word k of lane l at k * 64 + l
+// over the same 768 words, so it needs no more LDS.
+constexpr uint32_t LEAF_WAVE_WORDS = 12 * 64;
+
 template <int T>
-__device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i, uint64_t* save) {
+__device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i, uint64_t* wave_lds, uint32_t lane) {
   if (i >= a.h) return;
   uint64_t st[12];
   if (a.prev_states) {
@@ -66,6 +72,12 @@ __device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i, uint
   // partial chunk, permute (a single permutation call site keeps the code small)
   uint32_t m = 0;
   uint64_t j = 0;
+  // A 128-byte line holds two chunks.  When a chunk starts a line and the next
+  // chunk is whole, the next chunk is loaded straight into the wave's LDS slot
+  // with the same fetch, so each line leaves the L2 once instead of twice, one
+  // permutation apart.
+  bool have_next = false;
+  const uint4* slot = reinterpret_cast<const uint4*>(wave_lds) + lane;  // piece q at slot[64 q]
   while (m < a.nmats) {
     const uint64_t w = a.mats[m].width;
     if (j >= w) {
@@ -75,7 +87,15 @@ __device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i, uint
     }
     const uint64_t* row = a.mats[m].ptr + i * w + j;
     const uint64_t rem = w - j;
-    if (rem >= 8 && ((((uintptr_t)row) & 15) == 0)) {
+    if (have_next) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint4 v = slot[64 * k];
+        st[2 * k] = ((uint64_t)v.y << 32) | v.x;
+        st[2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
+      }
+      have_next = false;
+    } else if (rem >= 8 && ((((uintptr_t)row) & 15) == 0)) {
       const uint4* r4 = reinterpret_cast<const uint4*>(row);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -83,15 +103,44 @@ __device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i, uint
         st[2 * k] = ((uint64_t)v.y << 32) | v.x;
         st[2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
       }
+      if (rem >= 16 && ((((uintptr_t)row) & 127) == 0)) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          __builtin_amdgcn_global_load_lds((const void*)(r4 + 4 + k),
+                                           (__attribute__((address_space(3))) void*)(wave_lds + 128 * k), 16, 0, 0);
+        have_next = true;
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < 8; k++) st[k] = (uint64_t)k < rem ? row[k] : 0;
     }
+#pragma unroll
+    for (int k = 8; k < 12; k++) wave_lds[k * 64 + lane] = st[k];
     // the constants pointer made opaque per chunk: the compiler would otherwise
     // hoist the 118 round constants out of the loop into vector registers
     const P2Consts* pc = a.p2;
     asm volatile("" : "+s"(pc));
-    p2_permute_saved<T>(st, pc, save);
+    // TRY products; the whole wave redoes the permutation exactly from its input
+    // (this chunk, re-read from the row, and the saved capacity) when any lane
+    // flagged a product borrow (about 2^-32 per product)
+    uint64_t flag = 0;
+    p2_permute_dev<true>(st, pc, flag);
+    if (__builtin_expect(flag != 0, 0)) {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // a prefetch into the slot has landed
+      // the chunk's address formed again here, so that it is not held across the permutation
+      uint64_t ii = i;
+      asm volatile("" : "+v"(ii));
+      const uint64_t* row2 = a.mats[m].ptr + ii * w + j;
+#pragma unroll
+      for (int k = 0; k < 8; k++) wave_lds[k * 64 + lane] = (uint64_t)k < w - j ? row2[k] : 0;
+      p2_permute_lds<64>(wave_lds + lane, pc);
+#pragma unroll
+      for (int k = 0; k < 12; k++) st[k] = wave_lds[k * 64 + lane];
+      have_next = false;  // the slot now holds the redo's state
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) st[k] = canon(st[k]);
+    }
     j += 8;
   }
   if (a.out_digests) {
@@ -104,9 +153,10 @@ __device__ __forceinline__ void p2_leaf_body(const LeafArgs& a, uint64_t i, uint
 }
 
 template <int T>
-__global__ void __launch_bounds__(T) p2_leaf_kernel(LeafArgs a) {
-  __shared__ uint64_t save[12 * T];
-  p2_leaf_body<T>(a, (uint64_t)blockIdx.x * T + threadIdx.x, save + threadIdx.x);
+__global__ void __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5))) p2_leaf_kernel(LeafArgs a) {
+  __shared__ uint64_t lds[(T / 64) * LEAF_WAVE_WORDS];
+  p2_leaf_body<T>(a, (uint64_t)blockIdx.x * T + threadIdx.x, lds + (threadIdx.x >> 6) * LEAF_WAVE_WORDS,
+                  threadIdx.x & 63);
 }
 
 // Poseidon2 leaf hashing of a short matrix group: one 16-lane row per leaf
@@ -183,10 +233,11 @@ __global__ void __launch_bounds__(T) keccak_leaf_kernel(LeafArgs a) {
 // (p2_leaf_many_kernel: 112 VGPRs, 4 waves per SIMD; 82 and 5 this way).
 __global__ void __launch_bounds__(128) p2_leaf_many_kernel(const LeafArgs* __restrict__ args,
                                                         const P2Consts* __restrict__ p2) {
-  __shared__ uint64_t save[12 * 128];
+  __shared__ uint64_t lds[2 * LEAF_WAVE_WORDS];
   LeafArgs a = args[blockIdx.y];
   a.p2 = p2;
-  p2_leaf_body<128>(a, (uint64_t)blockIdx.x * 128 + threadIdx.x, save + threadIdx.x);
+  p2_leaf_body<128>(a, (uint64_t)blockIdx.x * 128 + threadIdx.x, lds + (threadIdx.x >> 6) * LEAF_WAVE_WORDS,
+                    threadIdx.x & 63);
 }
 __global__ void __launch_bounds__(256) p2_leaf_coop_many_kernel(const LeafArgs* __restrict__ args,
                                                              const P2Consts* __restrict__ p2) {

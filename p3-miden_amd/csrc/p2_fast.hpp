@@ -437,9 +437,13 @@ __device__ __forceinline__ void p2_permute_saved(uint64_t* s, const P2Consts* __
   }
 }
 #else
-// (the host compilation pass parses the kernels that call it)
+// (the host compilation pass parses the kernels that call them)
 template <int STRIDE>
 __device__ void p2_permute_saved(uint64_t* s, const P2Consts* __restrict__ c, uint64_t* save);
+template <bool TRY>
+__device__ void p2_permute_dev(uint64_t* s, const P2Consts* __restrict__ c, uint64_t& flag);
+template <int STRIDE>
+__device__ void p2_permute_lds(uint64_t* v, const P2Consts* __restrict__ c);
 #endif
 
 }  // namespace p3g
