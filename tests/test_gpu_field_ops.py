@@ -139,3 +139,12 @@ def test_half_word_sums(device):
     out, _ = device.field_check(6, np.array(L, dtype=np.uint64), np.array(H, dtype=np.uint64))
     for lo, hi, r in zip(L, H, out.tolist()):
         assert r % P == (lo + (hi << 32)) % P
+
+
+@pytest.mark.gpu
+def test_field_check_rejects_bad_arguments(device):
+    import p3gpu as G
+    one = np.ones(4, np.uint64)
+    for op, e, n in [(7, 0, 4), (5, 96, 4), (0, 0, 0)]:
+        with pytest.raises(G.P3GError):
+            device.field_check(op, one[:n], one[:n], e=e)
