@@ -125,7 +125,10 @@ int p3g_calibrate_copy(p3g_ctx* ctx, uint64_t bytes, uint32_t reps, double* kern
  *   op 2, 3: a*b + c, as op 0 and op 1;
  *   op 4: a*b canonical (< p), for canonical a, b;
  *   op 5: a * 2^e canonical, for canonical a and 0 <= e < 96;
- *   op 6: (a + 2^32 b) mod p, some value < 2^64, for a, b < 2^40.
+ *   op 6: (a + 2^32 b) mod p, some value < 2^64, for a, b < 2^40;
+ *   op 7, 8: a + b and a - b canonical, for canonical a, b (the NTT passes'
+ *         butterflies);
+ *   op 9: a mod p for any 64-bit a.
  * wave_flags (ceil(n / 64) words, or NULL) is written for every op (zero
  * except for ops 1 and 3). */
 int p3g_field_check(p3g_ctx* ctx, uint32_t op, uint32_t e, const uint64_t* a, const uint64_t* b, const uint64_t* c,

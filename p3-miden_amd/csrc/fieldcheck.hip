@@ -16,6 +16,7 @@ namespace {
 // op 0: EXACT a*b        1: TRY a*b (wave flag)     2: EXACT a*b + c
 //    3: TRY a*b + c      4: canonical a*b (mulc)    5: canonical a*2^e (mul_pow2c)
 //    6: (a + 2^32 b) mod p from 64-bit sums a, b < 2^40 (red_halves)
+//    7: canonical a + b (add_c)   8: canonical a - b (sub_c)   9: a mod p (canon3)
 __global__ void __launch_bounds__(64) field_check_kernel(uint32_t op, uint32_t e, const uint64_t* __restrict__ a,
                                                          const uint64_t* __restrict__ b,
                                                          const uint64_t* __restrict__ c, uint64_t n,
@@ -31,7 +32,10 @@ __global__ void __launch_bounds__(64) field_check_kernel(uint32_t op, uint32_t e
     case 3: r = mul_add_t<true>(x, y, z, f); break;
     case 4: r = mulc(x, y); break;
     case 5: r = mul_pow2c(x, e); break;
-    default: r = red_halves(x, y); break;
+    case 6: r = red_halves(x, y); break;
+    case 7: r = add_c(x, y); break;
+    case 8: r = sub_c(x, y); break;
+    default: r = canon3(x); break;
   }
   if (i < n) out[i] = r;
   if (threadIdx.x == 0) flags[blockIdx.x] = f;
@@ -41,7 +45,7 @@ __global__ void __launch_bounds__(64) field_check_kernel(uint32_t op, uint32_t e
 
 void field_check(Ctx& ctx, uint32_t op, uint32_t e, const uint64_t* a, const uint64_t* b, const uint64_t* c,
                  uint64_t n, uint64_t* out, uint64_t* wave_flags) {
-  if (op > 6) throw ApiError(E_INVAL, "field_check: op is 0..6");
+  if (op > 9) throw ApiError(E_INVAL, "field_check: op is 0..9");
   if (op == 5 && e >= 96) throw ApiError(E_INVAL, "field_check: e is 0..95");
   if (n == 0 || n > (1u << 20)) throw ApiError(E_INVAL, "field_check: n is 1..2^20");
   hipStream_t s = ctx.stream;

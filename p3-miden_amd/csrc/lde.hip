@@ -209,8 +209,37 @@ __device__ __forceinline__ void xchg_halves(uint32_t* tile, uint64_t (&r)[16], W
   for (int k = 0; k < 16; k++) r[k] = ((uint64_t)tile[rpos(k)] << 32) | lo[k];
 }
 
+// Field operations of a pass.  add_c / sub_c / mulc / mul_pow2c take the
+// fewest issue slots; two passes keep some carry-chain forms for their
+// register budget:
+//  - the coset-expanding first forward pass holds its input tile across the
+//    coset loop and spills with add_c's wider live ranges (gl_add there);
+//  - the inverse first pass gathers bit-reversed rows and is latency-bound:
+//    the carry-chain forms compile it to 80 VGPRs (6 waves per SIMD) where the
+//    others take 99 (4 waves), which measured 13% slower.
+template <uint32_t MODE>
+struct NttOps {
+  static __device__ __forceinline__ uint64_t add(uint64_t x, uint64_t y) {
+    if constexpr (MODE == FWD_FIRST || MODE == INV_FIRST) return gl_add(x, y);
+    else return add_c(x, y);
+  }
+  static __device__ __forceinline__ uint64_t sub(uint64_t x, uint64_t y) {
+    if constexpr (MODE == INV_FIRST) return gl_sub(x, y);
+    else return sub_c(x, y);
+  }
+  static __device__ __forceinline__ uint64_t mul(uint64_t x, uint64_t y) {
+    if constexpr (MODE == INV_FIRST) return mulc_cc(x, y);
+    else return mulc(x, y);
+  }
+  static __device__ __forceinline__ uint64_t pow2(uint64_t x, uint32_t E) {
+    if constexpr (MODE == INV_FIRST) return mul_pow2c_cc(x, E);
+    else return mul_pow2c(x, E);
+  }
+};
+
 template <uint32_t MODE, int LR>
 __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
+  using Ops = NttOps<MODE>;
   constexpr uint32_t T = 1u << LR;      // threads = rows per tile
   constexpr uint32_t G = 1u << (LR - 4);  // row groups in layout A / row stride in layout B
   constexpr int KB0 = 8 - LR;             // layout B: k bits below KB0 belong to layout-A stages
@@ -302,7 +331,7 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
       }
       __syncthreads();  // the previous coset's readers passed the tile barriers below
 #pragma unroll
-      for (int k = 0; k < 16; k++) r[k] = mulc(in[k], scl[17 * g + k]);
+      for (int k = 0; k < 16; k++) r[k] = Ops::mul(in[k], scl[17 * g + k]);
     } else {
 #pragma unroll
       for (int k = 0; k < 16; k++) r[k] = in[k];
@@ -310,7 +339,7 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
     if (inverse) {
       // stages on x bits 0..3 (k bits), layout A
 #pragma unroll
-      for (int k = 1; k < 16; k++) r[k] = mulc(r[k], tfa[k]);
+      for (int k = 1; k < 16; k++) r[k] = Ops::mul(r[k], tfa[k]);
 #pragma unroll
       for (int t = 0; t < 4; t++) {
 #pragma unroll
@@ -321,16 +350,16 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           uint32_t E;
           bool neg;
           w16_pow2(jw, true, E, neg);  // omega^-jw = (-1)^neg 2^E
-          const uint64_t x = r[k0], y = mul_pow2c(r[k1], E);
-          r[k0] = neg ? gl_sub(x, y) : gl_add(x, y);
-          r[k1] = neg ? gl_add(x, y) : gl_sub(x, y);
+          const uint64_t x = r[k0], y = Ops::pow2(r[k1], E);
+          r[k0] = neg ? Ops::sub(x, y) : Ops::add(x, y);
+          r[k1] = neg ? Ops::add(x, y) : Ops::sub(x, y);
         }
       }
       xchg_halves(tile, r, [&](int k) { return swz(16 * g + k, c); }, [&](int k) { return swz(g + G * k, c); });
       // stages on x bits 4..LR-1, layout B (x bit t is k bit t - (LR - 4))
 #pragma unroll
       for (int k = 0; k < 16; k++)
-        if (k >> KB0) r[k] = mulc(r[k], tfb[16 * g + k]);
+        if (k >> KB0) r[k] = Ops::mul(r[k], tfb[16 * g + k]);
 #pragma unroll
       for (int t = 4; t < LR; t++) {
         const int kb = t - (LR - 4);
@@ -342,9 +371,9 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           uint32_t E;
           bool neg;
           w16_pow2(jw, true, E, neg);  // omega^-jw = (-1)^neg 2^E
-          const uint64_t x = r[k0], y = mul_pow2c(r[k1], E);
-          r[k0] = neg ? gl_sub(x, y) : gl_add(x, y);
-          r[k1] = neg ? gl_add(x, y) : gl_sub(x, y);
+          const uint64_t x = r[k0], y = Ops::pow2(r[k1], E);
+          r[k0] = neg ? Ops::sub(x, y) : Ops::add(x, y);
+          r[k1] = neg ? Ops::add(x, y) : Ops::sub(x, y);
         }
       }
       if (live) {
@@ -366,13 +395,13 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           bool neg;
           w16_pow2(jw, false, E, neg);  // omega^jw = (-1)^neg 2^E
           const uint64_t x = r[k0], y = r[k1];
-          r[k0] = gl_add(x, y);
-          r[k1] = mul_pow2c(neg ? gl_sub(y, x) : gl_sub(x, y), E);
+          r[k0] = Ops::add(x, y);
+          r[k1] = Ops::pow2(neg ? Ops::sub(y, x) : Ops::sub(x, y), E);
         }
       }
 #pragma unroll
       for (int k = 0; k < 16; k++)
-        if (k >> KB0) r[k] = mulc(r[k], tfb[16 * g + k]);
+        if (k >> KB0) r[k] = Ops::mul(r[k], tfb[16 * g + k]);
       __syncthreads();  // previous repetition's readers are done with the tile
       xchg_halves(tile, r, [&](int k) { return swz(g + G * k, c); }, [&](int k) { return swz(16 * g + k, c); });
 #pragma unroll
@@ -386,12 +415,12 @@ __device__ __forceinline__ void ntt_reg_body(const NttArgs& a, uint64_t bid) {
           bool neg;
           w16_pow2(jw, false, E, neg);  // omega^jw = (-1)^neg 2^E
           const uint64_t x = r[k0], y = r[k1];
-          r[k0] = gl_add(x, y);
-          r[k1] = mul_pow2c(neg ? gl_sub(y, x) : gl_sub(x, y), E);
+          r[k0] = Ops::add(x, y);
+          r[k1] = Ops::pow2(neg ? Ops::sub(y, x) : Ops::sub(x, y), E);
         }
       }
 #pragma unroll
-      for (int k = 1; k < 16; k++) r[k] = mulc(r[k], tfa[k]);
+      for (int k = 1; k < 16; k++) r[k] = Ops::mul(r[k], tfa[k]);
       const uint64_t ob = MODE == FWD_FIRST ? bitrev32((uint32_t)rep, a.logB) : (MODE == FWD ? blk : 0);
       // the row stride is made opaque inside the coset loop so that the 16 store
       // addresses are formed per coset instead of being held across it

@@ -121,6 +121,13 @@ def test_canonical_products(device):
 def test_powers_of_two(device, e):
     rng = np.random.default_rng(100 + e)
     vals = [x for x in EDGES if x < P] + [int(v) for v in rng.integers(0, P, 500, dtype=np.uint64)]
+    if 1 <= e <= 32:
+        # lo + eps hi in [p, 2^64) without a carry (hi = 1, lo = 2^64 - 2^e j): the reduction
+        # after the mad
+        for j in sorted({EPS // (1 << e) + 1, 2 * EPS // (1 << e)}):
+            v = (1 << (65 - e)) - j
+            if v < P and EPS < (j << e) <= 2 * EPS:
+                vals.append(v)
     if 33 <= e <= 63:
         # v = h1 2^(96 - e): low product word 0 and h0 = 0, so the borrow fold runs
         vals += [h1 << (96 - e) for h1 in sorted({1, 2, 3, (1 << (e - 32)) - 1}) if h1 < 1 << (e - 32)]
@@ -142,9 +149,35 @@ def test_half_word_sums(device):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("op", [7, 8])
+def test_canonical_sums_and_differences(device, op):
+    """The NTT butterflies' add_c (64-bit adds and one compare) and sub_c (a borrow
+    chain whose borrow adds p back as +1 on the low word and -1 + carry on the
+    high word), on every pair of edge values and random canonical words."""
+    rng = np.random.default_rng(20 + op)
+    can = [x for x in EDGES if x < P] + [(1 << 32) - 2, P - EPS, EPS << 1]
+    a = [x for x in can for _ in can] + [int(v) for v in rng.integers(0, P, 4000, dtype=np.uint64)]
+    b = [y for _ in can for y in can] + [int(v) for v in rng.integers(0, P, 4000, dtype=np.uint64)]
+    out, _ = device.field_check(op, np.array(a, dtype=np.uint64), np.array(b, dtype=np.uint64))
+    for x, y, r in zip(a, b, out.tolist()):
+        assert r == ((x + y) if op == 7 else (x - y)) % P, (op, x, y)
+
+
+@pytest.mark.gpu
+def test_canonical_reduction(device):
+    rng = np.random.default_rng(31)
+    vals = EDGES + [P + k for k in range(5)] + [M64 - k for k in range(5)]
+    vals += [int(v) for v in rng.integers(P, M64, 500, dtype=np.uint64, endpoint=True)]
+    vals += [int(v) for v in rng.integers(0, M64, 2000, dtype=np.uint64, endpoint=True)]
+    out, _ = device.field_check(9, np.array(vals, dtype=np.uint64), np.zeros(len(vals), np.uint64))
+    for x, r in zip(vals, out.tolist()):
+        assert r == x % P, x
+
+
+@pytest.mark.gpu
 def test_field_check_rejects_bad_arguments(device):
     import p3gpu as G
     one = np.ones(4, np.uint64)
-    for op, e, n in [(7, 0, 4), (5, 96, 4), (0, 0, 0)]:
+    for op, e, n in [(10, 0, 4), (5, 96, 4), (0, 0, 0)]:
         with pytest.raises(G.P3GError):
             device.field_check(op, one[:n], one[:n], e=e)
