@@ -186,10 +186,24 @@ __global__ void __launch_bounds__(256) p2_leaf_coop_kernel(LeafArgs a) {
   p2_leaf_coop_body(a, ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4, threadIdx.x & 15);
 }
 
-__device__ __forceinline__ void keccak_leaf_body(const LeafArgs& a, uint64_t i) {
-  if (i >= a.h) return;
+// One rate block (17 words) of each of a wave's 64 rows.
+constexpr uint32_t KECCAK_WAVE_WORDS = 64 * 17;
+
+// Row i of the leaf matrices, hashed by lane `lane` of its wave (rows
+// i - lane .. i - lane + 63).  Each Keccak-f absorbs the next 17 words of the
+// row (overwrite mode; a partial last block is zero-filled).  The wave loads
+// its 64 rows' next blocks together: load t (0..16) gives lane l word
+// q = 64 t + l of the wave's block, word q % 17 of row q / 17, so each load
+// instruction reads a few contiguous row segments instead of 64 words 8 bytes
+// wide from 64 rows one row width apart.  The block goes through the wave's
+// LDS slot `wl` (written as q, read back as row l's 17 words) and the next
+// block's loads are in flight during the current permutation.  Rows past the
+// height read zeros and store nothing.
+__device__ __forceinline__ void keccak_leaf_body(const LeafArgs& a, uint64_t i, uint64_t* wl, uint32_t lane) {
+  const uint64_t i0 = i - lane;
+  const bool live = i < a.h;
   uint64_t st[25];
-  if (a.prev_states) {
+  if (a.prev_states && live) {
     const uint64_t* ps = a.prev_states + (i >> a.prev_shift) * 25;
 #pragma unroll
     for (int k = 0; k < 25; k++) st[k] = ps[k];
@@ -199,20 +213,28 @@ __device__ __forceinline__ void keccak_leaf_body(const LeafArgs& a, uint64_t i) 
   }
   for (uint32_t m = 0; m < a.nmats; m++) {
     const uint64_t w = a.mats[m].width;
-    const uint64_t* row = a.mats[m].ptr + i * w;
-    uint64_t j = 0;
-    for (; j + 17 <= w; j += 17) {
+    const uint64_t* base = a.mats[m].ptr + i0 * w;
+    uint64_t v[17];
+    auto load = [&](uint64_t j) {
 #pragma unroll
-      for (int k = 0; k < 17; k++) st[k] = row[j + k];
-      keccak_f(st);
-    }
-    if (j < w) {
-      uint64_t rem = w - j;
+      for (int t = 0; t < 17; t++) {
+        const uint32_t q = 64u * t + lane, r = q / 17u, k = q - 17u * r;
+        v[t] = (i0 + r < a.h && j + k < w) ? base[(uint64_t)r * w + j + k] : 0;
+      }
+    };
+    if (w) load(0);
+    for (uint64_t j = 0; j < w; j += 17) {
 #pragma unroll
-      for (int k = 0; k < 17; k++) st[k] = (uint64_t)k < rem ? row[j + k] : 0;
+      for (int t = 0; t < 17; t++) wl[64 * t + lane] = v[t];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < 17; k++) st[k] = wl[17 * lane + k];
+      __builtin_amdgcn_wave_barrier();
+      if (j + 17 < w) load(j + 17);
       keccak_f(st);
     }
   }
+  if (!live) return;
   if (a.out_digests) {
 #pragma unroll
     for (int k = 0; k < 4; k++) a.out_digests[i * 4 + k] = st[k];
@@ -224,7 +246,9 @@ __device__ __forceinline__ void keccak_leaf_body(const LeafArgs& a, uint64_t i) 
 
 template <int T>
 __global__ void __launch_bounds__(T) keccak_leaf_kernel(LeafArgs a) {
-  keccak_leaf_body(a, (uint64_t)blockIdx.x * T + threadIdx.x);
+  __shared__ uint64_t lds[(T / 64) * KECCAK_WAVE_WORDS];
+  keccak_leaf_body(a, (uint64_t)blockIdx.x * T + threadIdx.x, lds + (threadIdx.x >> 6) * KECCAK_WAVE_WORDS,
+                   threadIdx.x & 63);
 }
 
 // ---- job-indexed forms for batches of same-shaped trees (blockIdx.y = job).
@@ -246,7 +270,9 @@ __global__ void __launch_bounds__(256) p2_leaf_coop_many_kernel(const LeafArgs* 
   p2_leaf_coop_body(a, ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 4, threadIdx.x & 15);
 }
 __global__ void __launch_bounds__(128) keccak_leaf_many_kernel(const LeafArgs* __restrict__ args) {
-  keccak_leaf_body(args[blockIdx.y], (uint64_t)blockIdx.x * 128 + threadIdx.x);
+  __shared__ uint64_t lds[2 * KECCAK_WAVE_WORDS];
+  keccak_leaf_body(args[blockIdx.y], (uint64_t)blockIdx.x * 128 + threadIdx.x,
+                   lds + (threadIdx.x >> 6) * KECCAK_WAVE_WORDS, threadIdx.x & 63);
 }
 
 constexpr int CMP_T = 128;
