@@ -215,11 +215,20 @@ __device__ __forceinline__ void keccak_leaf_body(const LeafArgs& a, uint64_t i, 
     const uint64_t w = a.mats[m].width;
     const uint64_t* base = a.mats[m].ptr + i0 * w;
     uint64_t v[17];
+    const bool all_rows = i0 + 64 <= a.h;  // wave-uniform
     auto load = [&](uint64_t j) {
+      if (all_rows && j + 17 <= w) {
 #pragma unroll
-      for (int t = 0; t < 17; t++) {
-        const uint32_t q = 64u * t + lane, r = q / 17u, k = q - 17u * r;
-        v[t] = (i0 + r < a.h && j + k < w) ? base[(uint64_t)r * w + j + k] : 0;
+        for (int t = 0; t < 17; t++) {
+          const uint32_t q = 64u * t + lane, r = q / 17u, k = q - 17u * r;
+          v[t] = base[(uint64_t)r * w + j + k];
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 17; t++) {
+          const uint32_t q = 64u * t + lane, r = q / 17u, k = q - 17u * r;
+          v[t] = (i0 + r < a.h && j + k < w) ? base[(uint64_t)r * w + j + k] : 0;
+        }
       }
     };
     if (w) load(0);
