@@ -409,6 +409,7 @@ def bench_c5(G, W, dev, n_requests, workers):
     # warm every worker context and the shared block pool with the whole set
     for r in [srv.submit(tr, G.PcsParams(*p)) for tr, p, _ in reqs]:
         srv.result(r)
+    st_warm = srv.stats()
 
     log(f"[c5] saturation: {n_requests} requests submitted at once")
     t0 = time.perf_counter()
@@ -457,7 +458,11 @@ def bench_c5(G, W, dev, n_requests, workers):
         "peak_inflight_gb": round(st["peak_inflight_bytes"] / 1e9, 2),
         "peak_running": st["peak_running"],
         "allocator": {"hipMalloc_calls": st["n_malloc"], "hipMalloc_ms": round(st["malloc_ms"], 1),
-                      "hipFree_calls": st["n_free"], "hipFree_ms": round(st["free_ms"], 1)},
+                      "hipFree_calls": st["n_free"], "hipFree_ms": round(st["free_ms"], 1),
+                      "in_warm_pass": {"hipMalloc_calls": st_warm["n_malloc"],
+                                       "hipMalloc_ms": round(st_warm["malloc_ms"], 1)},
+                      "in_timed_passes": {"hipMalloc_calls": st["n_malloc"] - st_warm["n_malloc"],
+                                          "hipMalloc_ms": round(st["malloc_ms"] - st_warm["malloc_ms"], 1)}},
     }
 
 
