@@ -33,6 +33,27 @@ def test_coset_lde_matches_oracle(device, logn, w, bits):
     assert np.array_equal(got, want)
 
 
+def _lde_sweep_cases(n=24, seed=77):
+    """Seeded shapes across the register-tile passes: 5..16 stages (so every pass
+    size 5..8 and both first-pass modes), widths around the 16-column tile edges
+    (masked columns), blowups 2..8."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        logn = int(rng.integers(5, 17))
+        w = int(rng.choice([8, 9, 11, 15, 16, 17, 23, 24, 31, 33, 40]))
+        bits = int(rng.integers(1, 4))
+        out.append((logn, w, bits))
+    return out
+
+
+@pytest.mark.parametrize("logn,w,bits", _lde_sweep_cases())
+def test_coset_lde_sweep_matches_oracle(device, logn, w, bits):
+    tr = O.rand_matrix(3000 + 31 * logn + w, 1 << logn, w)
+    shift = O.gl_pow(7, 1 + (logn + w) % 5)
+    assert np.array_equal(G.Dft(device).coset_lde_batch(tr, bits, shift), O.coset_lde_batch(tr, bits, shift))
+
+
 def test_coset_lde_natural_order(device):
     tr = O.rand_matrix(5, 256, 7)
     want = O.coset_lde_batch(tr, 2, 7, bit_reverse=False)
