@@ -200,6 +200,22 @@ def bench_c2(G, W, dev, steps, warmup, peaks=None):
                                            "gbps": round(lde_bytes / (lde_ms * 1e-3) / 1e9, 1) if lde_ms else None,
                                            "peak": HBM_PEAK_GBS},
     }
+    cs_ms = per["deep_colsum"][0]
+    if cs_ms:
+        cs_bytes = 8 * n * w  # each trace's first n rows of its LDE
+        others["deep_colsum"] = {"ms": round(cs_ms, 3), "bytes": cs_bytes,
+                                 "gbps": round(cs_bytes / (cs_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS}
+    ff_ms = per["fri_fold"][0]
+    if ff_ms:
+        # every round reads its m extension elements and writes m / a of them
+        lb, la = W.C2_PARAMS[0], W.C2_PARAMS[1]
+        m, ff_bytes = n << lb, 0
+        for _ in range(params.num_rounds((n << lb).bit_length() - 1)):
+            ff_bytes += 16 * m + 16 * (m >> la)
+            m >>= la
+        others["fri_fold (all rounds)"] = {"ms": round(ff_ms, 3), "bytes": ff_bytes,
+                                           "gbps": round(ff_bytes / (ff_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                           "note": "rounds below ~2^16 elements are launch-latency bound"}
     dq_ms = per["deep_quotient"][0]
     if dq_ms:
         dq_bytes = 8 * (n << b) * w + 16 * (n << b)
