@@ -54,6 +54,25 @@ def test_coset_lde_sweep_matches_oracle(device, logn, w, bits):
     assert np.array_equal(G.Dft(device).coset_lde_batch(tr, bits, shift), O.coset_lde_batch(tr, bits, shift))
 
 
+@pytest.mark.parametrize("logn,w,bits", [(7, 16, 1), (12, 17, 2), (15, 8, 3)])
+def test_coset_lde_edge_values_match_oracle(device, logn, w, bits):
+    """Columns of p - 1, of 0 / p - 1 alternating, of 2^k and of p - 2^k: butterfly
+    sums landing on p and above, differences borrowing from zero, shift twiddles
+    of words with the top bits set."""
+    n = 1 << logn
+    P = (1 << 64) - (1 << 32) + 1
+    r = np.arange(n, dtype=np.uint64)
+    cols = [np.full(n, P - 1, np.uint64), np.where(r % 2 == 0, 0, P - 1).astype(np.uint64),
+            (np.uint64(1) << (r % np.uint64(64))) % np.uint64(P),
+            (np.uint64(P) - (np.uint64(1) << (r % np.uint64(63)))).astype(np.uint64),
+            np.where(r % 3 == 0, P - 1, 1).astype(np.uint64)]
+    tr = np.stack([cols[c % len(cols)] if c < 2 * len(cols) else O.rand_matrix(c, n, 1)[:, 0]
+                   for c in range(w)], axis=1).astype(np.uint64)
+    assert (tr < np.uint64(P)).all()
+    shift = O.gl_pow(7, 2)
+    assert np.array_equal(G.Dft(device).coset_lde_batch(tr, bits, shift), O.coset_lde_batch(tr, bits, shift))
+
+
 def test_coset_lde_natural_order(device):
     tr = O.rand_matrix(5, 256, 7)
     want = O.coset_lde_batch(tr, 2, 7, bit_reverse=False)
