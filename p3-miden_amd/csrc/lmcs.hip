@@ -441,33 +441,49 @@ constexpr uint64_t TOPC_MAX_LEN = 2 * (TOPC_T / 16);
 __host__ __device__ constexpr uint64_t top_max_len(uint32_t hash) {
   return hash == HASH_P2 ? TOPC_MAX_LEN : 2 * TOP_T;
 }
-// len <= TOPC_MAX_LEN digests at layer offset off
-__device__ __forceinline__ void compress_top_p2_body(uint64_t* digests, uint64_t off, uint64_t len,
-                                                     const P2Consts* p2) {
+// Levels above a layer of `len` digests at layer offset `off`, in slices of
+// `slen` digests (a power of two, at most TOPC_MAX_LEN): workgroup b compresses
+// digests b*slen .. b*slen + slen - 1 up log2(slen) levels, staged through LDS,
+// and writes every level where the tree stores it (layers lie one after
+// another, so the layer above starts at off + len).  One slice (slen == len)
+// is the tree's top; many slices take the levels of a wider layer in one
+// launch instead of one launch per level, each of which costs a kernel gap.
+__device__ __forceinline__ void compress_slice_p2_body(uint64_t* digests, uint64_t off, uint64_t len, uint64_t slen,
+                                                       uint64_t b, const P2Consts* p2) {
   __shared__ uint64_t buf[2][TOPC_MAX_LEN * 4];
   const uint32_t tid = threadIdx.x;
   const uint32_t l = tid & 15, node = tid >> 4;
-  for (uint64_t e = tid; e < len * 4; e += TOPC_T) buf[0][e] = digests[off * 4 + e];
+  for (uint64_t e = tid; e < slen * 4; e += TOPC_T) buf[0][e] = digests[(off + b * slen) * 4 + e];
   __syncthreads();
   int cur = 0;
-  while (len > 1) {
-    const uint64_t nl = len / 2;
+  while (slen > 1) {
+    const uint64_t nl = slen / 2;
     const uint64_t noff = off + len;
     if (node < nl) {  // uniform over the 16-lane row
       uint64_t x = l < 8 ? buf[cur][node * 8 + l] : 0;
       x = p2_permute_coop(x, l, p2);
       if (l < 4) {
         buf[cur ^ 1][node * 4 + l] = x;
-        digests[(noff + node) * 4 + l] = x;
+        digests[(noff + b * nl + node) * 4 + l] = x;
       }
     }
     __syncthreads();
     cur ^= 1;
     off = noff;
-    len = nl;
+    len /= 2;
+    slen = nl;
   }
 }
+// len <= TOPC_MAX_LEN digests at layer offset off
+__device__ __forceinline__ void compress_top_p2_body(uint64_t* digests, uint64_t off, uint64_t len,
+                                                     const P2Consts* p2) {
+  compress_slice_p2_body(digests, off, len, len, 0, p2);
+}
 
+__global__ void __launch_bounds__(TOPC_T) compress_slices_p2_kernel(uint64_t* digests, uint64_t off, uint64_t len,
+                                                                    uint64_t slen, const P2Consts* p2) {
+  compress_slice_p2_body(digests, off, len, slen, blockIdx.x, p2);
+}
 __global__ void __launch_bounds__(TOPC_T) compress_top_p2_kernel(uint64_t* digests, uint64_t off, uint64_t len,
                                                                  const P2Consts* p2) {
   compress_top_p2_body(digests, off, len, p2);
@@ -527,6 +543,27 @@ static void compress_layers(Ctx* ctx, uint32_t hash, uint64_t* digests, const st
   unsigned l = 0;
   while (len > top_max_len(hash)) {
     uint64_t nl = len / 2;
+    if (hash == HASH_P2 && nl <= COOP_LAYER_MAX) {
+      // the cooperative levels in slices: down to TOPC_MAX_LEN digests in at
+      // most two launches (a layer of <= 2^14 nodes has <= 2^15 digests)
+      const uint64_t slen = std::min<uint64_t>(TOPC_MAX_LEN, std::max<uint64_t>(2, len / TOPC_MAX_LEN));
+      auto slices = [&] {
+        hipLaunchKernelGGL(compress_slices_p2_kernel, dim3((uint32_t)(len / slen)), dim3(TOPC_T), 0, s, digests,
+                           layer_off[l], len, slen, p2);
+      };
+      if (ctx) {
+        ProfScope ps(*ctx, "compress_layer", s);
+        slices();
+      } else {
+        slices();
+      }
+      P3G_CHECK_HIP(hipGetLastError());
+      for (uint64_t t = slen; t > 1; t /= 2) {
+        len /= 2;
+        l++;
+      }
+      continue;
+    }
     uint32_t grid = (uint32_t)((nl + CMP_T - 1) / CMP_T);
     const uint64_t* prev = digests + layer_off[l] * 4;
     uint64_t* next = digests + layer_off[l + 1] * 4;
