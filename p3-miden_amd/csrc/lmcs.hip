@@ -1007,9 +1007,13 @@ size_t prove_batch_emit(const Tree& t, const std::vector<std::pair<const uint64_
     uint64_t w = pieces[p].second;
     const bool is_salt = t.salt.ptr && (p - first) % per == per - 1;
     uint64_t aw = is_salt ? w : aligned_len(w, t.alignment);
-    padded.assign(raw + off, raw + off + w);
-    padded.resize(aw, 0);
-    tx.hint_fields(padded.data(), aw);
+    if (aw == w) {
+      tx.hint_fields(raw + off, w);  // no padding: straight from the gathered words
+    } else {
+      padded.assign(raw + off, raw + off + w);
+      padded.resize(aw, 0);
+      tx.hint_fields(padded.data(), aw);
+    }
     off += w;
   }
   for (size_t p = first + nrows; p < last; p++) {

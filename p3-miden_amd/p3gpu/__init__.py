@@ -566,10 +566,12 @@ class ProverTranscript:
         """TranscriptData: (fields [n], commitments [m, 4])."""
         nf = lib().p3g_transcript_num_fields(self.h)
         nc = lib().p3g_transcript_num_commitments(self.h)
-        f = np.zeros(max(1, nf), np.uint64)
-        c = np.zeros(max(1, 4 * nc), np.uint64)
+        # filled in place: a wide proof's fields run to megabytes, and every extra
+        # pass over them shows in the proof rate
+        f = np.empty(max(1, nf), np.uint64)
+        c = np.empty(max(1, 4 * nc), np.uint64)
         _check(lib().p3g_transcript_data(self.h, _p(f), _p(c)), "transcript_data")
-        return f[:nf].copy(), c[: 4 * nc].reshape(nc, 4).copy()
+        return f[:nf], c[: 4 * nc].reshape(nc, 4)
 
 
 def proof_bytes(fields: np.ndarray, commitments: np.ndarray) -> bytes:
